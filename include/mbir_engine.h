@@ -1,0 +1,93 @@
+/*
+ * mbir_engine.h -- host-side stage set-up of the MBIR engine, exported so
+ * callers (tests, the benchmark, other drivers) can drive the HIP C-ABI of
+ * mbir_hip.h one stage at a time, and reconstruct_ex(), the extended form of
+ * reconstruct() (mbir4d.h).
+ *
+ * Each helper restates one piece of the reference's initStructures
+ * (MBIR_4D/XT_Init.c:211-432) for a single process; paths are relative to
+ * the reference's Source_Code/ directory.
+ */
+#ifndef PHASECON_MBIR_ENGINE_H
+#define PHASECON_MBIR_ENGINE_H
+#include <stdint.h>
+#include <stdio.h>
+
+#include "mbir_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MBIR_MAX_STAGES 8 /* MAX_MULTRES_NUM, MBIR_4D/XT_Constants.h:100 */
+
+/* Multi-resolution voxel scales, coarse to fine (MBIR_4D/XT_Engine.c:
+ * 111-141).  Returns the number of stages. */
+int32_t mbir_multires_schedule(int32_t proj_rows, int32_t proj_cols,
+                               int32_t* mult_xy, int32_t* mult_z);
+
+/* View -> frame map (MBIR_4D/XT_Init.c:69-96).  frame_ptr[N_time+1],
+ * frame_views[<= N_p].  Returns -1 if times decrease (XT_Init.c:109-113),
+ * else the number of assigned views. */
+int32_t mbir_view_map(int32_t N_p, const float* proj_times, int32_t N_time,
+                      const float* recon_times, int32_t* frame_ptr,
+                      int32_t* frame_views);
+
+/* Geometry and prior of one stage (MBIR_4D/XT_Init.c:138-190, 211-432);
+ * p_s/p_t are the spatial/temporal qGGMRF p (the reference has one,
+ * MRF_P = 1.2).  Returns -1 on the reference's divisibility errors. */
+int mbir_stage_geometry(mbir_hip_geom* g, int32_t proj_rows,
+                        int32_t proj_cols, int32_t proj_num, int32_t recon_num,
+                        float vox_wid, float rot_center, float sig_s,
+                        float sig_t, float c_s, float c_t, int32_t mult_xy,
+                        int32_t mult_z, int32_t quad_convex, float huber_delta,
+                        float huber_T, const int32_t* frame_ptr, double p_s,
+                        double p_t);
+
+/* Per-view detector response H[N_p][65] (MBIR_4D/XT_Profile.c:47-218). */
+void mbir_detector_response(const mbir_hip_geom* g, const float* angles,
+                            float* H);
+
+/* View cosines and sines (MBIR_4D/XT_Init.c:193-206). */
+void mbir_sincos(int32_t n, const float* angles, double* cosv, double* sinv);
+
+/* Inscribed-circle update mask [N_y][N_x] of radius vox_wid*proj_cols/2
+ * (MBIR_4D/XT_Init.c:243, XT_ICD_update.c:1127-1136). */
+void mbir_update_mask(const mbir_hip_geom* g, double radius, uint8_t* mask);
+
+/* File layout [p][t][r] -> internal [p][r][t] (MBIR_4D/XT_Init.c:294-301). */
+void mbir_sino_to_internal(int32_t N_p, int32_t N_t, int32_t N_r,
+                           const float* in, float* out);
+
+/* Options of reconstruct_ex(); mbir_default_options() gives the values
+ * reconstruct() uses. */
+typedef struct {
+  double p_s, p_t;         /* qGGMRF p, spatial / temporal (1.2, 1.2)     */
+  int32_t tile;            /* ICD tile edge (16)                          */
+  double alpha;            /* over-relaxation, <= 0: reference's 1.5      */
+  int32_t max_equits;      /* per stage (1000, MAX_NUM_ITERATIONS)        */
+  int32_t single_stage;    /* 1: only the full-resolution stage           */
+  int32_t write_files;     /* 1: write the reference's CWD output files   */
+  double stop_cost;        /* > 0: also stop once the FP64 cost <= this   */
+  uint32_t seed;           /* ICD visiting-order seed (761521)            */
+} mbir_options;
+
+void mbir_default_options(mbir_options* opt);
+
+/* reconstruct() with options.  equits_out / cost_out (may be NULL) receive
+ * the equits run in the last stage and the final FP64 cost. */
+int reconstruct_ex(float** object, float* projections, float* weights,
+                   float* proj_angles, float* proj_times, float* recon_times,
+                   int32_t proj_rows, int32_t proj_cols, int32_t proj_num,
+                   int32_t recon_num, float vox_wid, float rot_center,
+                   float sig_s, float sig_t, float c_s, float c_t,
+                   float convg_thresh, int32_t remove_rings,
+                   int32_t quad_convex, float huber_delta, float huber_T,
+                   uint8_t restart, FILE* debug_msg_ptr,
+                   const mbir_options* opt, int32_t* equits_out,
+                   double* cost_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

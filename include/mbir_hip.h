@@ -1,0 +1,141 @@
+/*
+ * mbir_hip.h -- thin C-ABI between the C host engine (reconstruct()) and the
+ * hand-written gfx950 kernels of the MBIR hot path.
+ *
+ * Plain pointers and sizes only; no GPU handle crosses this interface: the
+ * context owns its device buffers, its HIP stream and its timing events.
+ * All host arrays use the reference's layouts:
+ *   sinogram-shaped  [N_p][N_r][N_t]  (view, detector column r, row t) --
+ *                    the reference's internal order (XT_Init.c:294-301)
+ *   object           [N_time][N_z][N_y][N_x]  (no halo slices)
+ *   A windows        first[N_y*N_x][N_slots] int32 (-1 = empty column),
+ *                    val[N_y*N_x][N_slots][W], W = 2*mult_xy+1, where the
+ *                    slots enumerate the views frame by frame
+ *                    (slot s holds view frame_views[s]).
+ * Every function returns 0 on success and -1 on failure (the reference's
+ * convention, XT_Debug.h:22-32); mbir_hip_last_error() gives the reason.
+ *
+ * Each entry point names the reference function whose role it takes over
+ * (paths relative to the reference's Source_Code/ directory).
+ */
+#ifndef PHASECON_MBIR_HIP_H
+#define PHASECON_MBIR_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Geometry and prior of one multi-resolution stage, as set up by
+ * initStructures (MBIR_4D/XT_Init.c:211-432). */
+typedef struct {
+  int32_t N_r, N_t, N_p;           /* detector columns, rows, views       */
+  int32_t N_x, N_y, N_z, N_time;   /* voxels and reconstruction frames    */
+  int32_t mult_xy, mult_z;         /* voxel scale; z_overlap = mult_z     */
+  double delta_r, delta_t, R0, RMax, x0, y0, delta_xy, OffsetR;
+  double q, p_s, p_t, c_s, c_t;    /* qGGMRF shape (p split space/time)   */
+  double sig_s_q, sig_s_qp, sig_t_q, sig_t_qp; /* sigma^q, sigma^(q-p)    */
+  double filt[27], filt_t;         /* neighbour weights [dz][dy][dx]      */
+  double alpha;                    /* over-relaxation (XT_Constants.h:87) */
+  double huber_T, huber_delta;     /* generalized Huber T, delta          */
+  double var_est;                  /* variance parameter sigma^2          */
+} mbir_hip_geom;
+
+typedef struct mbir_hip_ctx mbir_hip_ctx;
+
+/* Kernel ids for mbir_hip_kernel_time(). */
+enum {
+  MBIR_K_AMATRIX = 0,  /* A build                                       */
+  MBIR_K_FWDPROJ = 1,  /* A x into the error sinogram                   */
+  MBIR_K_SWEEP = 2,    /* parallel ICD sweep                            */
+  MBIR_K_COST = 3,     /* FP64 MAP cost                                 */
+  MBIR_K_BACKPROJ = 4, /* A^T v                                         */
+  MBIR_K_COUNT = 5
+};
+
+const char* mbir_hip_last_error(void);
+
+/* Creates a context on the current HIP device and uploads the stage inputs.
+ * H is the per-view detector response [N_p][65] (XT_Profile.c:129-218);
+ * cosv/sinv the view cosines/sines (XT_Init.c:193-206); frame_ptr
+ * [N_time+1] / frame_views [frame_ptr[N_time]] the view->frame map
+ * (XT_Init.c:69-96); y, w the projections and weights (w already divided by
+ * var_est); d the ring offset [N_r][N_t] or NULL for zero; mask the update
+ * mask [N_y][N_x] (XT_ICD_update.c:1127-1136). */
+int mbir_hip_create(mbir_hip_ctx** out, const mbir_hip_geom* g,
+                    const float* H, const double* cosv, const double* sinv,
+                    const int32_t* frame_ptr, const int32_t* frame_views,
+                    const float* y, const float* w, const float* d,
+                    const uint8_t* mask);
+void mbir_hip_destroy(mbir_hip_ctx* ctx);
+
+/* Tuning knobs: "tile" (ICD tile edge, default 16), "alpha" (override of
+ * g->alpha for the sweep), "fp16" (1: store A as FP16 with a per-column
+ * FP32 scale; takes effect at the next mbir_hip_build_amatrix). */
+int mbir_hip_set_option(mbir_hip_ctx* ctx, const char* key, double value);
+
+/* Object transfer (replaces the Object array of XT_Structures.h:215). */
+int mbir_hip_set_object(mbir_hip_ctx* ctx, const float* x);
+int mbir_hip_get_object(mbir_hip_ctx* ctx, float* x);
+
+/* Builds the stored system matrix for every (column, view) pair;
+ * replaces the per-visit calcAMatrixColumnforAngle (XT_AMatrix.c:50-110).
+ * get_amatrix returns it in the window layout above (values dequantised
+ * when stored in FP16). */
+int mbir_hip_build_amatrix(mbir_hip_ctx* ctx);
+int mbir_hip_get_amatrix(mbir_hip_ctx* ctx, int32_t* first, float* val);
+
+/* e = y - A x - d from the current object, A without the delta_t factor
+ * (initErrorSinogam, XT_ICD_update.c:651-698). */
+int mbir_hip_init_error(mbir_hip_ctx* ctx);
+int mbir_hip_get_error(mbir_hip_ctx* ctx, float* e);
+int mbir_hip_set_error(mbir_hip_ctx* ctx, const float* e);
+
+/* Pure projector pair for tests and the warm start:
+ * ax = A x  (forward_project_voxel_AMat1D, XT_ForwardProject.c:44-61) and
+ * bt = A^T v over each frame's views (the adjoint of that scatter). */
+int mbir_hip_forward_project(mbir_hip_ctx* ctx, const float* x, float* ax);
+int mbir_hip_backproject(mbir_hip_ctx* ctx, const float* v, float* bt);
+
+/* n equits of the parallel ICD (updateVoxelsTimeSlices,
+ * XT_ICD_update.c:932-1092, with the voxel update of XT_VoxUpdate.c:45-283
+ * and the substitution of XT_ICD_update.c:133-182).  Iteration numbers run
+ * first_iter .. first_iter+n-1 (zero skipping starts at 2).  stats (may be
+ * NULL) receives [sum |dx|, sum |x_new|, zero-skipped voxels, updated
+ * voxels] over the n equits. */
+int mbir_hip_icd_equits(mbir_hip_ctx* ctx, int32_t n, int32_t first_iter,
+                        uint32_t seed, double* stats);
+
+/* FP64 MAP cost (computeCost, XT_ICD_update.c:185-318).  fresh = 1
+ * recomputes e = y - A x - d in FP64 on the fly; fresh = 0 uses the error
+ * sinogram the sweeps maintain.  out = [total, data term incl.
+ * N log(var)/2, prior term]. */
+int mbir_hip_cost(mbir_hip_ctx* ctx, int32_t fresh, double* out);
+
+/* Batched functional substitution (CE_FunctionalSubstitution,
+ * XT_ICD_update.c:133-182) on n independent voxels, FP32: nb [n][27]
+ * ([dz][dy][dx], centre ignored), nbf its flags, tn/tf [n][2] the t-1/t+1
+ * neighbours and flags.  No context needed. */
+int mbir_hip_substitution_batch(const mbir_hip_geom* g, int32_t n,
+                                const float* V, const float* theta1,
+                                const float* theta2, const float* nb,
+                                const uint8_t* nbf, const float* tn,
+                                const uint8_t* tf, float* out);
+
+/* theta1/theta2 of n voxels (t, z, y, x quadruples) from the stored A, the
+ * current error sinogram and weights, FP32 accumulation
+ * (XT_VoxUpdate.c:56-85). */
+int mbir_hip_voxel_thetas(mbir_hip_ctx* ctx, int32_t n, const int32_t* tzyx,
+                          float* theta1, float* theta2);
+
+/* Device time of the kernels of one id since the last reset, from HIP
+ * events recorded on the context's stream around every launch. */
+int mbir_hip_kernel_time(mbir_hip_ctx* ctx, int32_t kernel_id, double* ms,
+                         int64_t* launches);
+int mbir_hip_reset_kernel_time(mbir_hip_ctx* ctx);
+int mbir_hip_synchronize(mbir_hip_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

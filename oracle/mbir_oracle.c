@@ -1,0 +1,787 @@
+/*
+ * mbir_oracle.c -- CPU restatement of the reference MBIR_4D hot path, in FP64.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the
+ * product library never does.  All file:line citations are relative to the
+ * reference's Source_Code/ directory.
+ *
+ * What pins each function (tests/test_oracle_*.py):
+ *   orc_random2           randlib.c:69-92 known-answer table (first 20 seeds)
+ *   orc_view_map          Examples proj_times/recon_times .bin fixtures,
+ *                         reference build (oracle/_ref) on random time lists
+ *   orc_setup             reference build geometry dump; filter sums to
+ *                         mult_xy^3 * delta_recon (closed form)
+ *   orc_detector_response bit-exact vs reference build; closed-form area
+ *   orc_amatrix_column    bit-exact index/value vs reference build; area
+ *                         invariant sum_j A*delta_r ~= delta_xy^2
+ *   orc_forward_project   reference build initErrorSinogam; adjointness
+ *   orc_qggmrf_weight     finite differences of orc_qggmrf_value; delta=0
+ *                         limit q/(sigma^q c)
+ *   orc_substitution      reference build; all-equal and p=q closed forms
+ *   orc_voxel_thetas      reference build compute_voxel_update_Atten;
+ *                         theta2 = ||A_col||^2_W brute force
+ *   orc_cost              reference build computeCost; hand-computed 2x2x2
+ *   orc_icd_equit         reference ICD cost trajectory is monotone and
+ *                         reaches the same converged cost (convexity)
+ * orc_icd_equit_tiled is a schedule study tool with no reference counterpart.
+ */
+#include "mbir_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_PROFILE_RES 1536   /* XT_Constants.h:77 */
+#define ORC_BEAM_RES 512       /* XT_Constants.h:78 */
+#define ORC_PI 3.14159265358979323846
+#define ORC_PI_2 1.57079632679489661923132169163975144
+#define ORC_PI_4 0.785398163397448309615660845819875721
+
+/* ------------------------------------------------------------------ RNG */
+/* Park-Miller minimal standard generator: s <- 16807 s mod (2^31 - 1),
+ * returned as s / (2^31 - 1).  randlib.c:119-141 computes the same product
+ * with split 16-bit halves; the 64-bit product here is the textbook form. */
+static uint64_t orc_rng_state = 1;
+
+void orc_srandom2(uint32_t seed) { orc_rng_state = seed; }
+
+double orc_random2(void) {
+  orc_rng_state = (orc_rng_state * 16807ULL) % 2147483647ULL;
+  return (double)orc_rng_state / 2147483647.0;
+}
+
+/* ------------------------------------------------------------- set-up */
+int32_t orc_view_map(int32_t N_p, const float* proj_times, int32_t N_time,
+                     const float* recon_times, int32_t* frame_ptr,
+                     int32_t* frame_views) {
+  /* XT_Init.c:69-96: frame i takes the views whose time lies in
+   * [recon_times[i], recon_times[i+1]); the scan for frame i+1 resumes at
+   * the first view whose time reached recon_times[i+1]. */
+  int32_t scan_from = 0, n = 0;
+  for (int32_t f = 0; f < N_time; ++f) {
+    frame_ptr[f] = n;
+    for (int32_t v = scan_from; v < N_p; ++v) {
+      double tv = proj_times[v];
+      if (tv >= recon_times[f] && tv < recon_times[f + 1]) frame_views[n++] = v;
+      if (tv >= recon_times[f + 1]) {
+        scan_from = v;
+        break;
+      }
+    }
+  }
+  frame_ptr[N_time] = n;
+  return n;
+}
+
+static double orc_node_distance(int a, int b, int c, int d) {
+  /* XT_Init.c:138-141 */
+  return sqrt(pow((double)a - 1.0, 2.0) + pow((double)b - 1.0, 2.0) +
+              pow((double)c - 1.0, 2.0) + pow((double)d - 1.0, 2.0));
+}
+
+int32_t orc_setup(orc_geom* g, int32_t proj_rows, int32_t proj_cols,
+                  int32_t proj_num, int32_t recon_num, float vox_wid,
+                  float rot_center, float sig_s, float sig_t, float c_s,
+                  float c_t, int32_t mult_xy, int32_t mult_z,
+                  int32_t quad_convex, float huber_delta, float huber_T,
+                  const int32_t* frame_ptr, double p_s, double p_t) {
+  memset(g, 0, sizeof(*g));
+  const double vw = vox_wid, rc = rot_center;
+  const double length_r = vw * proj_cols;  /* XT_Init.c:223-224 */
+  const double length_t = vw * proj_rows;  /* single rank: no split */
+  g->N_p = proj_num;
+  g->N_r = proj_cols;
+  g->N_t = proj_rows;
+  g->N_time = recon_num;
+  g->mult_xy = mult_xy;
+  g->mult_z = mult_z;
+  g->W = 2 * mult_xy + 1;
+  /* XT_Init.c:318-323 */
+  g->delta_r = length_r / g->N_r;
+  g->delta_t = length_t / g->N_t;
+  g->R0 = -rc * g->delta_r;
+  g->RMax = (g->N_r - rc) * g->delta_r;
+  g->T0 = -length_t / 2.0;
+  g->TMax = length_t / 2.0;
+  /* XT_Init.c:329-334, 358-361 */
+  g->N_x = (int32_t)(g->N_r / (double)mult_xy);
+  g->N_y = g->N_x;
+  g->N_z = (int32_t)(g->N_t / (double)mult_z);
+  g->delta_xy = mult_xy * g->delta_r;
+  g->delta_z = mult_z * g->delta_t;
+  g->x0 = g->R0;
+  g->z0 = g->T0;
+  g->y0 = -length_r / 2.0;
+  /* XT_Init.c:372-373 */
+  g->OffsetR = (g->delta_xy / sqrt(2.0) + g->delta_r / 2.0) / ORC_RESPONSE_BINS;
+  g->OffsetT = ((g->delta_z / 2) + g->delta_t / 2) / ORC_RESPONSE_BINS;
+  g->radius_obj = vw * proj_cols / 2;  /* XT_Init.c:243 */
+  /* XT_Init.c:117-126: mean views per frame */
+  double nviews = 0;
+  for (int32_t f = 0; f < recon_num; ++f) nviews += frame_ptr[f + 1] - frame_ptr[f];
+  g->delta_recon = nviews / recon_num;
+  /* XT_Init.c:418-421 (one MRF_P in the reference; p_s/p_t here) */
+  g->q = 2.0;
+  g->p_s = p_s;
+  g->p_t = p_t;
+  g->c_s = c_s;
+  g->c_t = c_t;
+  const double ss = (double)sig_s * (double)mult_xy;
+  const double st = (double)sig_t * g->delta_recon;
+  g->sig_s_q = pow(ss, g->q);
+  g->sig_s_qp = pow(ss, g->q - p_s);
+  g->sig_t_q = pow(st, g->q);
+  g->sig_t_qp = pow(st, g->q - p_t);
+  /* XT_Init.c:144-173: 1/distance weights, normalised with the two
+   * temporal neighbours included, scaled by mult_xy^3 * delta_recon. */
+  const double scale = (double)mult_xy * mult_xy * mult_xy * g->delta_recon;
+  double ft = 1.0 / orc_node_distance(0, 1, 1, 1);
+  double sum = 2.0 * ft;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      for (int c = 0; c < 3; ++c) {
+        double v = 0;
+        if (a != 1 || b != 1 || c != 1) {
+          v = 1.0 / orc_node_distance(1, a, b, c);
+          sum = sum + v;
+        }
+        g->filt[(a * 3 + b) * 3 + c] = v;
+      }
+  for (int k = 0; k < 27; ++k) g->filt[k] = scale * g->filt[k] / sum;
+  g->filt_t = scale * ft / sum;
+  g->alpha = 1.5; /* XT_Constants.h:87 */
+  if (quad_convex) { /* XT_Init.c:259-268, XT_Constants.h:97-98 */
+    g->huber_T = 100000;
+    g->huber_delta = 1;
+  } else {
+    g->huber_T = huber_T;
+    g->huber_delta = huber_delta;
+  }
+  g->var_est = 1.0;
+  /* XT_Init.c:427-428 */
+  if (g->N_t % mult_z != 0 || g->N_r % mult_xy != 0) return -1;
+  return 0;
+}
+
+void orc_sincos(int32_t N_p, const float* angles, double* cosv, double* sinv) {
+  for (int32_t i = 0; i < N_p; ++i) { /* XT_Init.c:200-204 */
+    cosv[i] = cos(angles[i]);
+    sinv[i] = sin(angles[i]);
+  }
+}
+
+/* ---------------------------------------------- detector response (a2) */
+void orc_detector_response(const orc_geom* g, const float* angles, float* H) {
+  const int32_t NB = ORC_RESPONSE_BINS + 1;
+  float* prof = (float*)malloc(sizeof(float) * ORC_PROFILE_RES);
+  double beam[ORC_BEAM_RES];
+  /* Hamming beam over one detector pitch, normalised to unit area
+   * (XT_Profile.c:108-124). */
+  double bsum = 0;
+  for (int i = 0; i < ORC_BEAM_RES; ++i) {
+    beam[i] = 0.54 - 0.46 * cos((2 * ORC_PI / ORC_BEAM_RES) * i);
+    bsum = bsum + beam[i];
+  }
+  for (int i = 0; i < ORC_BEAM_RES; ++i) beam[i] /= bsum;
+
+  const double bw = g->delta_r; /* BeamWidth, XT_Init.c:361 */
+  const double r_first = -bw / 2;
+  const double step = bw / ORC_BEAM_RES;
+  const double to_index = (ORC_PROFILE_RES) / (2 * g->delta_xy);
+
+  for (int32_t v = 0; v < g->N_p; ++v) {
+    /* Trapezoidal line integral of a square voxel at this view, sampled on
+     * [0, 2) in units of delta_xy (XT_Profile.c:47-104). */
+    double ang = angles[v];
+    while (ang > ORC_PI_2) ang -= ORC_PI_2;
+    while (ang < 0) ang += ORC_PI_2;
+    double peak = (ang <= ORC_PI_4) ? g->delta_xy / cos(ang)
+                                    : g->delta_xy / cos(ORC_PI_2 - ang);
+    double c45 = cos(ORC_PI_4);
+    double d_outer = c45 * cos((ORC_PI_4 - ang));
+    double d_inner = c45 * fabs((cos((ORC_PI_4 + ang))));
+    double lo_corner = 1 - d_outer, lo_flat = 1 - d_inner;
+    double hi_flat = 1 + d_inner, hi_corner = 1 + d_outer;
+    for (int j = 0; j < ORC_PROFILE_RES; ++j) {
+      double t = 2.0 * j / ORC_PROFILE_RES;
+      double val;
+      if (t <= lo_corner || t >= hi_corner)
+        val = 0;
+      else if (t > hi_flat)
+        val = peak * (hi_corner - t) / (hi_corner - hi_flat);
+      else if (t >= lo_flat)
+        val = peak;
+      else
+        val = peak * (t - lo_corner) / (lo_flat - lo_corner);
+      prof[j] = (float)val;
+    }
+    /* Convolve with the beam at 64 offsets of OffsetR (XT_Profile.c:
+     * 143-162); the 65th entry is zero. */
+    for (int i = 0; i < ORC_RESPONSE_BINS; ++i) {
+      double centre = i * g->OffsetR;
+      double lo = centre - g->delta_xy;
+      double acc = 0;
+      for (int p = 0; p < ORC_BEAM_RES; ++p) {
+        double r = r_first + p * step;
+        int32_t pi = (int32_t)floor((r - lo) * to_index);
+        if (pi >= 0 && pi <= ORC_PROFILE_RES - 1) acc += (prof[pi] * beam[p]);
+      }
+      H[v * NB + i] = (float)acc;
+    }
+    H[v * NB + ORC_RESPONSE_BINS] = 0.0f;
+  }
+  free(prof);
+}
+
+/* --------------------------------------------------- A column (a1) */
+int32_t orc_amatrix_column(const orc_geom* g, const double* cosv,
+                           const double* sinv, const float* H, int32_t row,
+                           int32_t col, int32_t view, int32_t* idx,
+                           double* val) {
+  const int32_t NB = ORC_RESPONSE_BINS + 1;
+  double xc = g->x0 + ((double)col + 0.5) * g->delta_xy;
+  double yc = g->y0 + ((double)row + 0.5) * g->delta_xy;
+  double r = xc * cosv[view] - yc * sinv[view];
+  double rlo = r - g->delta_xy, rhi = r + g->delta_xy;
+  if (rhi < g->R0 || rlo > g->RMax) return 0;
+  int32_t jlo = (int32_t)(floor(((rlo - g->R0) / g->delta_r)));
+  int32_t jhi = (int32_t)(floor((rhi - g->R0) / g->delta_r));
+  if (jhi >= g->N_r) jhi = g->N_r - 1;
+  if (jlo < 0) jlo = 0;
+  int32_t n = 0;
+  const float* h = H + (size_t)view * NB;
+  for (int32_t j = jlo; j <= jhi; ++j) {
+    double centre = (g->R0 + (((double)j) + 0.5) * (g->delta_r));
+    double d = fabs(r - centre);
+    int32_t k = (int32_t)(floor((d / g->OffsetR)));
+    if (k >= 0 && k < ORC_RESPONSE_BINS) {
+      double w_hi = d - k * g->OffsetR;
+      double w_lo = g->OffsetR - w_hi;
+      double f = (w_lo * h[k] + w_hi * h[k + 1]) / g->OffsetR;
+      if (f > 0) {
+        idx[n] = j;
+        val[n] = f;
+        ++n;
+      }
+    }
+  }
+  return n;
+}
+
+void orc_amatrix_all(const orc_geom* g, const double* cosv, const double* sinv,
+                     const float* H, int32_t* first, double* val) {
+  const int32_t W = g->W;
+#pragma omp parallel
+  {
+    int32_t idx[64];
+    double v[64];
+#pragma omp for schedule(static)
+    for (int32_t c = 0; c < g->N_x * g->N_y; ++c) {
+      int32_t yy = c / g->N_x, xx = c % g->N_x;
+      for (int32_t p = 0; p < g->N_p; ++p) {
+        size_t o = (size_t)c * g->N_p + p;
+        double* dst = val + o * W;
+        for (int k = 0; k < W; ++k) dst[k] = 0;
+        int32_t n = orc_amatrix_column(g, cosv, sinv, H, yy, xx, p, idx, v);
+        if (n == 0) {
+          first[o] = -1;
+          continue;
+        }
+        /* window starts at the clamped lower index; kept bins are
+         * contiguous around the projected centre */
+        double rlo = g->x0 + ((double)xx + 0.5) * g->delta_xy;
+        double ylo = g->y0 + ((double)yy + 0.5) * g->delta_xy;
+        double r = rlo * cosv[p] - ylo * sinv[p] - g->delta_xy;
+        int32_t j0 = (int32_t)floor((r - g->R0) / g->delta_r);
+        if (j0 < 0) j0 = 0;
+        first[o] = j0;
+        for (int32_t m = 0; m < n; ++m) dst[idx[m] - j0] = v[m];
+      }
+    }
+  }
+}
+
+/* ------------------------------------------- forward projection (a3) */
+void orc_forward_project(const orc_geom* g, const double* cosv,
+                         const double* sinv, const float* H,
+                         const int32_t* frame_ptr, const int32_t* frame_views,
+                         const double* x, double* ax) {
+  const int32_t Nr = g->N_r, Nt = g->N_t, zo = g->mult_z;
+  memset(ax, 0, sizeof(double) * (size_t)g->N_p * Nr * Nt);
+  /* frames own disjoint views, so frames run in parallel without races
+   * (XT_ICD_update.c:665-682) */
+#pragma omp parallel for schedule(dynamic)
+  for (int32_t f = 0; f < g->N_time; ++f) {
+    int32_t idx[64];
+    double v[64];
+    for (int32_t yy = 0; yy < g->N_y; ++yy)
+      for (int32_t xx = 0; xx < g->N_x; ++xx)
+        for (int32_t s = frame_ptr[f]; s < frame_ptr[f + 1]; ++s) {
+          int32_t p = frame_views[s];
+          int32_t n = orc_amatrix_column(g, cosv, sinv, H, yy, xx, p, idx, v);
+          for (int32_t z = 0; z < g->N_z; ++z) {
+            double pix = x[(((size_t)f * g->N_z + z) * g->N_y + yy) * g->N_x + xx];
+            for (int32_t m = 0; m < n; ++m)
+              for (int32_t k = 0; k < zo; ++k)
+                ax[((size_t)p * Nr + idx[m]) * Nt + z * zo + k] += pix * v[m];
+          }
+        }
+  }
+}
+
+/* --------------------------------------------------- qGGMRF (a5) */
+double orc_qggmrf_value(const orc_geom* g, double delta, int spatial) {
+  /* XT_ICD_update.c:82-90 */
+  double sq = spatial ? g->sig_s_q : g->sig_t_q;
+  double sqp = spatial ? g->sig_s_qp : g->sig_t_qp;
+  double c = spatial ? g->c_s : g->c_t;
+  double p = spatial ? g->p_s : g->p_t;
+  double ad = fabs(delta);
+  return ((pow(ad, g->q) / sq) / (c + pow(ad, g->q - p) / sqp));
+}
+
+double orc_qggmrf_weight(const orc_geom* g, double delta, int spatial) {
+  double sq = spatial ? g->sig_s_q : g->sig_t_q;
+  double sqp = spatial ? g->sig_s_qp : g->sig_t_qp;
+  double c = spatial ? g->c_s : g->c_t;
+  double p = spatial ? g->p_s : g->p_t;
+  if (delta == 0) return g->q / (sq * c); /* rho''(0), XT_ICD_update.c:120-128 */
+  /* rho'(delta) (XT_ICD_update.c:92-118), divided by delta */
+  double ratio = pow(fabs(delta), g->q - p) / (sqp);
+  double mag = pow(fabs(delta), g->q - 1);
+  double den = c + ratio;
+  double shape = (g->q - ((g->q - p) * ratio) / (den));
+  double deriv = (delta < 0) ? ((-1 * mag / (den * sq)) * shape)
+                             : ((mag / (den * sq)) * shape);
+  return deriv / delta;
+}
+
+double orc_substitution(const orc_geom* g, double V, double theta1,
+                        double theta2, const double* nb, const uint8_t* nbf,
+                        const double* tn, const uint8_t* tf) {
+  /* XT_ICD_update.c:133-182: one step of the quadratic surrogate of the
+   * qGGMRF prior around V, plus the data term's theta1/theta2 parabola,
+   * then over-relaxation by alpha. */
+  double num = 0, den = 0;
+  for (int k = 0; k < 27; ++k) {
+    if (k == 13 || !nbf[k]) continue;
+    double wgt = g->filt[k] * orc_qggmrf_weight(g, V - nb[k], 1);
+    num += wgt * nb[k];
+    den += wgt;
+  }
+  for (int k = 0; k < 2; ++k) {
+    if (!tf[k]) continue;
+    double wgt = g->filt_t * orc_qggmrf_weight(g, V - tn[k], 0);
+    num += wgt * tn[k];
+    den += wgt;
+  }
+  double u = (num + (theta2 * V) - theta1) / (den + theta2);
+  return V + g->alpha * (u - V);
+}
+
+/* --------------------------------------------- voxel thetas (a4) */
+static void orc_thetas_cols(const orc_geom* g, int32_t f, int32_t z,
+                            const int32_t* frame_ptr, const int32_t* frame_views,
+                            const int32_t* cnt, const int32_t* cidx,
+                            const double* cval, const double* e,
+                            const double* w, double* th1, double* th2) {
+  /* XT_VoxUpdate.c:56-85: sums over the frame's views, the kept bins and
+   * the z_overlap detector rows of the voxel's slice; entries flagged as
+   * outliers (|e sqrt(w)| >= T) use the generalized-Huber weight
+   * T delta sqrt(w) / |e|. */
+  const int32_t Nr = g->N_r, Nt = g->N_t, zo = g->mult_z, cap = g->W;
+  double t1 = 0, t2 = 0;
+  for (int32_t s = frame_ptr[f]; s < frame_ptr[f + 1]; ++s) {
+    int32_t p = frame_views[s];
+    int32_t si = s - frame_ptr[f];
+    for (int32_t m = 0; m < cnt[si]; ++m) {
+      int32_t r = cidx[si * cap + m];
+      double a = (cval[si * cap + m] * g->delta_t);
+      for (int32_t k = 0; k < zo; ++k) {
+        size_t o = ((size_t)p * Nr + r) * Nt + z * zo + k;
+        double ev = e[o], wv = w[o];
+        if (fabs(ev * sqrt(wv)) < g->huber_T) {
+          t2 += (a * a * wv);
+          t1 += -(ev * a * wv);
+        } else {
+          double hw = g->huber_T * g->huber_delta * sqrt(wv) / fabs(ev);
+          t2 += (a * a * hw);
+          t1 += -(ev * a * hw);
+        }
+      }
+    }
+  }
+  *th1 = t1;
+  *th2 = t2;
+}
+
+void orc_voxel_thetas(const orc_geom* g, const double* cosv,
+                      const double* sinv, const float* H,
+                      const int32_t* frame_ptr, const int32_t* frame_views,
+                      const double* e, const double* w, int32_t t, int32_t z,
+                      int32_t yy, int32_t xx, double* theta1, double* theta2) {
+  int32_t nv = frame_ptr[t + 1] - frame_ptr[t];
+  int32_t cap = g->W;
+  int32_t* cnt = (int32_t*)calloc(nv + 1, sizeof(int32_t));
+  int32_t* cidx = (int32_t*)calloc((size_t)(nv + 1) * cap, sizeof(int32_t));
+  double* cval = (double*)calloc((size_t)(nv + 1) * cap, sizeof(double));
+  for (int32_t s = 0; s < nv; ++s)
+    cnt[s] = orc_amatrix_column(g, cosv, sinv, H, yy, xx,
+                                frame_views[frame_ptr[t] + s], cidx + s * cap,
+                                cval + s * cap);
+  orc_thetas_cols(g, t, z, frame_ptr, frame_views, cnt, cidx, cval, e, w,
+                  theta1, theta2);
+  free(cnt);
+  free(cidx);
+  free(cval);
+}
+
+/* ------------------------------------------------------- cost (a8) */
+double orc_cost(const orc_geom* g, const double* e, const double* w,
+                const double* x, double* out) {
+  const size_t ns = (size_t)g->N_p * g->N_r * g->N_t;
+  const double T = g->huber_T, dl = g->huber_delta;
+  double data = 0;
+#pragma omp parallel for reduction(+ : data) schedule(static)
+  for (size_t i = 0; i < ns; ++i) {
+    double s = e[i] * sqrt(w[i]);
+    if (fabs(s) < T)
+      data += s * s;
+    else
+      data += 2.0 * dl * T * fabs(s) + T * T * (1.0 - 2.0 * dl);
+  }
+  data /= 2.0;
+  data += (double)g->N_p * (double)g->N_r * (double)g->N_t * log(g->var_est) / 2;
+  /* Prior: each neighbour pair once -- 4 in-plane forward neighbours, 9 at
+   * z+1, and t+1 (XT_ICD_update.c:205-298). */
+  static const int fwd[13][3] = {{0, 0, 1},  {0, 1, -1}, {0, 1, 0},
+                                 {0, 1, 1},  {1, -1, 0}, {1, 0, 0},
+                                 {1, 1, 0},  {1, -1, -1}, {1, -1, 1},
+                                 {1, 0, -1}, {1, 1, -1}, {1, 1, 1},
+                                 {1, 0, 1}};
+  const int32_t Nx = g->N_x, Ny = g->N_y, Nz = g->N_z, Nf = g->N_time;
+  double prior = 0;
+#pragma omp parallel for reduction(+ : prior) schedule(static)
+  for (int32_t fz = 0; fz < Nf * Nz; ++fz) {
+    int32_t f = fz / Nz, z = fz % Nz;
+    for (int32_t yy = 0; yy < Ny; ++yy)
+      for (int32_t xx = 0; xx < Nx; ++xx) {
+        size_t o = (((size_t)f * Nz + z) * Ny + yy) * Nx + xx;
+        double v = x[o];
+        for (int n = 0; n < 13; ++n) {
+          int32_t z2 = z + fwd[n][0], y2 = yy + fwd[n][1], x2 = xx + fwd[n][2];
+          if (z2 >= Nz || y2 < 0 || y2 >= Ny || x2 < 0 || x2 >= Nx) continue;
+          double nbv = x[(((size_t)f * Nz + z2) * Ny + y2) * Nx + x2];
+          int k = ((fwd[n][0] + 1) * 3 + (fwd[n][1] + 1)) * 3 + fwd[n][2] + 1;
+          prior += g->filt[k] * orc_qggmrf_value(g, v - nbv, 1);
+        }
+        if (f + 1 < Nf) {
+          double nbv = x[o + (size_t)Nz * Ny * Nx];
+          prior += g->filt_t * orc_qggmrf_value(g, v - nbv, 0);
+        }
+      }
+  }
+  if (out) {
+    out[0] = data;
+    out[1] = prior;
+  }
+  return data + prior;
+}
+
+void orc_mask(const orc_geom* g, uint8_t* mask) {
+  for (int32_t yy = 0; yy < g->N_y; ++yy)
+    for (int32_t xx = 0; xx < g->N_x; ++xx) {
+      double xc = g->x0 + ((double)xx + 0.5) * g->delta_xy;
+      double yc = g->y0 + ((double)yy + 0.5) * g->delta_xy;
+      mask[yy * g->N_x + xx] =
+          (xc * xc + yc * yc < g->radius_obj * g->radius_obj) ? 1 : 0;
+    }
+}
+
+/* ---------------------------------------------- one voxel update */
+/* Gathers the neighbourhood of (f, z, yy, xx), applies zero skipping and
+ * the substitution, writes x and updates e along the column
+ * (XT_VoxUpdate.c:92-116, 165-270).  Returns 1 if skipped. */
+static int orc_update_voxel(const orc_geom* g, int32_t f, int32_t z,
+                            int32_t yy, int32_t xx, int32_t iter,
+                            const int32_t* frame_ptr,
+                            const int32_t* frame_views, const int32_t* cnt,
+                            const int32_t* cidx, const double* cval, double* x,
+                            double* e, const double* w, double* dabs,
+                            double* xabs) {
+  const int32_t Nx = g->N_x, Ny = g->N_y, Nz = g->N_z;
+  const size_t vol = (size_t)Nz * Ny * Nx;
+  double nb[27];
+  uint8_t nbf[27];
+  double tn[2];
+  uint8_t tf[2];
+  size_t o = (((size_t)f * Nz + z) * Ny + yy) * Nx + xx;
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        int k = ((dz + 1) * 3 + dy + 1) * 3 + dx + 1;
+        int32_t z2 = z + dz, y2 = yy + dy, x2 = xx + dx;
+        if (z2 >= 0 && z2 < Nz && y2 >= 0 && y2 < Ny && x2 >= 0 && x2 < Nx) {
+          nb[k] = x[(((size_t)f * Nz + z2) * Ny + y2) * Nx + x2];
+          nbf[k] = 1;
+        } else {
+          nb[k] = 0;
+          nbf[k] = 0;
+        }
+      }
+  nb[13] = 0;
+  tf[0] = f > 0;
+  tn[0] = tf[0] ? x[o - vol] : 0;
+  tf[1] = f + 1 < g->N_time;
+  tn[1] = tf[1] ? x[o + vol] : 0;
+  double V = x[o];
+  /* zero skipping (XT_VoxUpdate.c:238-258) */
+  if (V == 0.0 && iter > 1) {
+    int any = (tn[0] > 0.0 || tn[1] > 0.0);
+    for (int k = 0; k < 27 && !any; ++k)
+      if (nb[k] > 0.0) any = 1;
+    if (!any) return 1;
+  }
+  double t1, t2;
+  orc_thetas_cols(g, f, z, frame_ptr, frame_views, cnt, cidx, cval, e, w, &t1,
+                  &t2);
+  double xn = orc_substitution(g, V, t1, t2, nb, nbf, tn, tf);
+  x[o] = xn;
+  const int32_t Nr = g->N_r, Nt = g->N_t, zo = g->mult_z, cap = g->W;
+  for (int32_t s = frame_ptr[f]; s < frame_ptr[f + 1]; ++s) {
+    int32_t p = frame_views[s], si = s - frame_ptr[f];
+    for (int32_t m = 0; m < cnt[si]; ++m) {
+      double a = cval[si * cap + m] * g->delta_t;
+      int32_t r = cidx[si * cap + m];
+      for (int32_t k = 0; k < zo; ++k)
+        e[((size_t)p * Nr + r) * Nt + z * zo + k] -= a * (xn - V);
+    }
+  }
+  *dabs += fabs(xn - V);
+  *xabs += fabs(xn);
+  return 0;
+}
+
+static void orc_line_columns(const orc_geom* g, const double* cosv,
+                             const double* sinv, const float* H, int32_t f,
+                             int32_t yy, int32_t xx, const int32_t* frame_ptr,
+                             const int32_t* frame_views, int32_t* cnt,
+                             int32_t* cidx, double* cval) {
+  for (int32_t s = frame_ptr[f]; s < frame_ptr[f + 1]; ++s) {
+    int32_t si = s - frame_ptr[f];
+    cnt[si] = orc_amatrix_column(g, cosv, sinv, H, yy, xx, frame_views[s],
+                                 cidx + si * g->W, cval + si * g->W);
+  }
+}
+
+/* ------------------------------------------------ ICD equit (a6/a7) */
+void orc_icd_equit(const orc_geom* g, const double* cosv, const double* sinv,
+                   const float* H, const int32_t* frame_ptr,
+                   const int32_t* frame_views, const uint8_t* mask,
+                   int32_t num_z_blocks, int32_t K, int32_t iter, double* x,
+                   double* e, const double* w, double* stats) {
+  const int32_t Nx = g->N_x, Ny = g->N_y, Nz = g->N_z, Nf = g->N_time;
+  const int32_t ncol = Nx * Ny, nb = num_z_blocks;
+  /* Random line order per (frame, block) (XT_ICD_update.c:499-531): draw
+   * without replacement from all columns, keep the masked ones. */
+  int32_t* order = (int32_t*)malloc(sizeof(int32_t) * (size_t)Nf * nb * ncol);
+  int32_t* count = (int32_t*)calloc((size_t)Nf * nb, sizeof(int32_t));
+  int32_t* pool = (int32_t*)malloc(sizeof(int32_t) * ncol);
+  const int32_t zper = Nz / nb;
+  for (int32_t f = 0; f < Nf; ++f)
+    for (int32_t b = 0; b < nb; ++b) {
+      int32_t left = ncol;
+      for (int32_t i = 0; i < ncol; ++i) pool[i] = i;
+      int32_t* dst = order + ((size_t)f * nb + b) * ncol;
+      for (int32_t i = 0; i < ncol; ++i) {
+        int32_t pick = (int32_t)floor(orc_random2() * left);
+        if (pick == left) pick = left - 1;
+        int32_t c = pool[pick];
+        if (zper > 0 && mask[c]) dst[count[f * nb + b]++] = c;
+        pool[pick] = pool[left - 1];
+        --left;
+      }
+    }
+  free(pool);
+  double dsum = 0, xsum = 0, skipped = 0;
+  for (int32_t j = 0; j < K; ++j) {
+    for (int phase = 0; phase < 2; ++phase) {
+#pragma omp parallel for collapse(2) schedule(dynamic) reduction(+ : dsum, xsum, skipped)
+      for (int32_t f = 0; f < Nf; ++f)
+        for (int32_t bb = 0; bb < nb; bb += 2) {
+          /* XT_ICD_update.c:966-974, 989-997 */
+          int32_t b = ((f % 2 == 0) == (phase == 0)) ? bb : bb + 1;
+          if (b >= nb) continue;
+          int32_t z0 = b * zper;
+          int32_t z1 = (b >= nb - 1) ? Nz - 1 : (b + 1) * zper - 1;
+          int32_t n = count[f * nb + b];
+          int32_t lo = j * (n / K);
+          int32_t hi = (j == K - 1) ? n - 1 : (j + 1) * (n / K) - 1;
+          int32_t nv = frame_ptr[f + 1] - frame_ptr[f];
+          int32_t* cnt = (int32_t*)calloc(nv + 1, sizeof(int32_t));
+          int32_t* cidx = (int32_t*)calloc((size_t)(nv + 1) * g->W, sizeof(int32_t));
+          double* cval = (double*)calloc((size_t)(nv + 1) * g->W, sizeof(double));
+          const int32_t* ord = order + ((size_t)f * nb + b) * ncol;
+          for (int32_t i = lo; i <= hi; ++i) {
+            int32_t c = ord[i], yy = c / Nx, xx = c % Nx;
+            orc_line_columns(g, cosv, sinv, H, f, yy, xx, frame_ptr,
+                             frame_views, cnt, cidx, cval);
+            for (int32_t z = z0; z <= z1; ++z)
+              skipped += orc_update_voxel(g, f, z, yy, xx, iter, frame_ptr,
+                                          frame_views, cnt, cidx, cval, x, e,
+                                          w, &dsum, &xsum);
+          }
+          free(cnt);
+          free(cidx);
+          free(cval);
+        }
+    }
+  }
+  free(order);
+  free(count);
+  if (stats) {
+    stats[0] = dsum;
+    stats[1] = xsum;
+    stats[2] = skipped;
+  }
+}
+
+/* ---------------------------------------- tiled schedule study tool */
+static uint32_t orc_hash(uint32_t a) {
+  a ^= a >> 16;
+  a *= 0x7feb352dU;
+  a ^= a >> 15;
+  a *= 0x846ca68bU;
+  a ^= a >> 16;
+  return a;
+}
+
+void orc_icd_equit_tiled(const orc_geom* g, const double* cosv,
+                         const double* sinv, const float* H,
+                         const int32_t* frame_ptr, const int32_t* frame_views,
+                         const uint8_t* mask, int32_t T, double alpha,
+                         uint32_t seed, int32_t iter, double* x, double* e,
+                         const double* w) {
+  orc_geom gg = *g;
+  gg.alpha = alpha;
+  const int32_t Nx = g->N_x, Ny = g->N_y, Nz = g->N_z, Nf = g->N_time;
+  const int32_t TX = (Nx + T - 1) / T, TY = (Ny + T - 1) / T;
+  const size_t vol = (size_t)Nz * Ny * Nx;
+  const size_t ns = (size_t)g->N_p * g->N_r * g->N_t;
+  double* esnap = (double*)malloc(sizeof(double) * ns);
+  int32_t* lists = (int32_t*)malloc(sizeof(int32_t) * (size_t)TX * TY * Nf * T * T);
+  int32_t* lens = (int32_t*)calloc((size_t)TX * TY * Nf, sizeof(int32_t));
+  /* per (tile, frame) list of masked columns in a seeded random order */
+  for (int32_t ty = 0; ty < TY; ++ty)
+    for (int32_t tx = 0; tx < TX; ++tx)
+      for (int32_t f = 0; f < Nf; ++f) {
+        size_t id = ((size_t)ty * TX + tx) * Nf + f;
+        int32_t* L = lists + id * T * T;
+        int32_t n = 0;
+        for (int32_t yy = ty * T; yy < ty * T + T && yy < Ny; ++yy)
+          for (int32_t xx = tx * T; xx < tx * T + T && xx < Nx; ++xx)
+            if (mask[yy * Nx + xx]) L[n++] = yy * Nx + xx;
+        uint32_t h = orc_hash(seed * 2654435761U + (uint32_t)id);
+        for (int32_t i = n - 1; i > 0; --i) {
+          h = orc_hash(h + (uint32_t)i);
+          int32_t j = (int32_t)(h % (uint32_t)(i + 1));
+          int32_t tmp = L[i];
+          L[i] = L[j];
+          L[j] = tmp;
+        }
+        lens[id] = n;
+      }
+  for (int color = 0; color < 4; ++color)
+    for (int fpar = 0; fpar < 2; ++fpar) {
+      int32_t maxlen = 0;
+      for (int32_t ty = 0; ty < TY; ++ty)
+        for (int32_t tx = 0; tx < TX; ++tx)
+          for (int32_t f = fpar; f < Nf; f += 2) {
+            size_t id = ((size_t)ty * TX + tx) * Nf + f;
+            if ((tx % 2) + 2 * (ty % 2) == color && lens[id] > maxlen) maxlen = lens[id];
+          }
+      for (int32_t step = 0; step < maxlen; ++step) {
+        memcpy(esnap, e, sizeof(double) * ns);
+#pragma omp parallel for collapse(3) schedule(dynamic)
+        for (int32_t ty = 0; ty < TY; ++ty)
+          for (int32_t tx = 0; tx < TX; ++tx)
+            for (int32_t f = 0; f < Nf; ++f) {
+              if ((f % 2) != fpar || (tx % 2) + 2 * (ty % 2) != color) continue;
+              size_t id = ((size_t)ty * TX + tx) * Nf + f;
+              if (step >= lens[id]) continue;
+              int32_t c = lists[id * T * T + step], yy = c / Nx, xx = c % Nx;
+              int32_t nv = frame_ptr[f + 1] - frame_ptr[f];
+              int32_t* cnt = (int32_t*)calloc(nv + 1, sizeof(int32_t));
+              int32_t* cidx = (int32_t*)calloc((size_t)(nv + 1) * g->W, sizeof(int32_t));
+              double* cval = (double*)calloc((size_t)(nv + 1) * g->W, sizeof(double));
+              double* th1 = (double*)malloc(sizeof(double) * Nz);
+              double* th2 = (double*)malloc(sizeof(double) * Nz);
+              orc_line_columns(g, cosv, sinv, H, f, yy, xx, frame_ptr,
+                               frame_views, cnt, cidx, cval);
+              for (int32_t z = 0; z < Nz; ++z)
+                orc_thetas_cols(g, f, z, frame_ptr, frame_views, cnt, cidx,
+                                cval, esnap, w, &th1[z], &th2[z]);
+              for (int par = 0; par < 2; ++par)
+                for (int32_t z = par; z < Nz; z += 2) {
+                  size_t o = (((size_t)f * Nz + z) * Ny + yy) * Nx + xx;
+                  double nb[27], tn[2];
+                  uint8_t nbf[27], tf[2];
+                  for (int dz = -1; dz <= 1; ++dz)
+                    for (int dy = -1; dy <= 1; ++dy)
+                      for (int dx = -1; dx <= 1; ++dx) {
+                        int k = ((dz + 1) * 3 + dy + 1) * 3 + dx + 1;
+                        int32_t z2 = z + dz, y2 = yy + dy, x2 = xx + dx;
+                        int in = z2 >= 0 && z2 < Nz && y2 >= 0 && y2 < Ny &&
+                                 x2 >= 0 && x2 < Nx;
+                        nb[k] = in ? x[(((size_t)f * Nz + z2) * Ny + y2) * Nx + x2] : 0;
+                        nbf[k] = (uint8_t)in;
+                      }
+                  nb[13] = 0;
+                  tf[0] = f > 0;
+                  tn[0] = tf[0] ? x[o - vol] : 0;
+                  tf[1] = f + 1 < Nf;
+                  tn[1] = tf[1] ? x[o + vol] : 0;
+                  double V = x[o];
+                  if (V == 0.0 && iter > 1) {
+                    int any = (tn[0] > 0.0 || tn[1] > 0.0);
+                    for (int k = 0; k < 27 && !any; ++k)
+                      if (nb[k] > 0.0) any = 1;
+                    if (!any) continue;
+                  }
+                  double xn = orc_substitution(&gg, V, th1[z], th2[z], nb, nbf, tn, tf);
+                  x[o] = xn;
+                  for (int32_t s = frame_ptr[f]; s < frame_ptr[f + 1]; ++s) {
+                    int32_t p = frame_views[s], si = s - frame_ptr[f];
+                    for (int32_t m = 0; m < cnt[si]; ++m) {
+                      double a = cval[si * g->W + m] * g->delta_t;
+                      int32_t r = cidx[si * g->W + m];
+                      for (int32_t k = 0; k < g->mult_z; ++k) {
+                        double* dst = &e[((size_t)p * g->N_r + r) * g->N_t + z * g->mult_z + k];
+#pragma omp atomic
+                        *dst -= a * (xn - V);
+                      }
+                    }
+                  }
+                }
+              free(cnt);
+              free(cidx);
+              free(cval);
+              free(th1);
+              free(th2);
+            }
+      }
+    }
+  free(esnap);
+  free(lists);
+  free(lens);
+}
+
+void orc_sino_from_file(int32_t N_p, int32_t N_t, int32_t N_r,
+                        const float* in, double* out) {
+  /* XT_Init.c:294-301: file index p*N_t*N_r + t*N_r + r */
+  for (int32_t p = 0; p < N_p; ++p)
+    for (int32_t r = 0; r < N_r; ++r)
+      for (int32_t t = 0; t < N_t; ++t)
+        out[((size_t)p * N_r + r) * N_t + t] = in[((size_t)p * N_t + t) * N_r + r];
+}

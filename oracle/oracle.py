@@ -1,0 +1,355 @@
+"""ctypes access to the test-only oracle libraries.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product package.
+
+* ``Oracle`` wraps oracle/liboracle_mbir.so, the FP64 C restatement of the
+  reference MBIR_4D hot path (oracle/mbir_oracle.c).
+* ``Reference`` wraps oracle/_ref/libmbir4d_ref.so, the reference's own C
+  sources built by ``make -C oracle ref`` (present only where it was built).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle_mbir.so")
+REF_SO = os.path.join(HERE, "_ref", "libmbir4d_ref.so")
+NB = 65  # detector response table entries per view
+
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+
+
+class Geom(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("N_r", "N_t", "N_p")] + \
+        [(n, C.c_double) for n in ("delta_r", "delta_t", "R0", "RMax", "T0", "TMax",
+                                   "OffsetR", "OffsetT")] + \
+        [(n, C.c_int32) for n in ("N_x", "N_y", "N_z", "N_time", "mult_xy", "mult_z", "W")] + \
+        [(n, C.c_double) for n in ("x0", "y0", "z0", "delta_xy", "delta_z", "radius_obj",
+                                   "delta_recon", "p_s", "p_t", "q", "c_s", "c_t",
+                                   "sig_s_q", "sig_s_qp", "sig_t_q", "sig_t_qp")] + \
+        [("filt", C.c_double * 27), ("filt_t", C.c_double), ("alpha", C.c_double),
+         ("huber_T", C.c_double), ("huber_delta", C.c_double), ("var_est", C.c_double)]
+
+
+def _lib(path):
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is not built (make -C oracle)")
+    return C.CDLL(path)
+
+
+def sino_to_internal(a, N_p, N_t, N_r):
+    """file layout [p][t][r] -> internal [p][r][t] (XT_Init.c:294-301)."""
+    return np.ascontiguousarray(np.asarray(a).reshape(N_p, N_t, N_r).transpose(0, 2, 1))
+
+
+class Oracle:
+    """The FP64 restatement for one stage of one scan."""
+
+    def __init__(self, scan, mult_xy=1, mult_z=1, quad_convex=1, huber_delta=0.5,
+                 huber_T=4.0, p_s=None, p_t=None):
+        L = self.L = _lib(ORACLE_SO)
+        L.orc_random2.restype = C.c_double
+        L.orc_cost.restype = C.c_double
+        L.orc_substitution.restype = C.c_double
+        L.orc_qggmrf_weight.restype = C.c_double
+        L.orc_qggmrf_value.restype = C.c_double
+        L.orc_amatrix_column.restype = C.c_int32
+        L.orc_setup.restype = C.c_int32
+        L.orc_view_map.restype = C.c_int32
+        s = self.scan = scan
+        N_p, N_t = s["proj_num"], s["proj_rows"]
+        self.frame_ptr = np.zeros(s["recon_num"] + 1, np.int32)
+        self.frame_views = np.zeros(max(N_p, 1), np.int32)
+        L.orc_view_map(C.c_int32(N_p), s["proj_times"].ctypes.data_as(C.c_void_p),
+                       C.c_int32(s["recon_num"]), s["recon_times"].ctypes.data_as(C.c_void_p),
+                       self.frame_ptr.ctypes.data_as(C.c_void_p),
+                       self.frame_views.ctypes.data_as(C.c_void_p))
+        g = self.g = Geom()
+        rc = L.orc_setup(C.byref(g), N_t, s["proj_cols"], N_p, s["recon_num"],
+                         C.c_float(s["vox_wid"]), C.c_float(s["rot_center"]),
+                         C.c_float(s["sig_s"]), C.c_float(s["sig_t"]),
+                         C.c_float(s["c_s"]), C.c_float(s["c_t"]), mult_xy, mult_z,
+                         quad_convex, C.c_float(huber_delta), C.c_float(huber_T),
+                         self.frame_ptr.ctypes.data_as(C.c_void_p),
+                         C.c_double(s["p_s"] if p_s is None else p_s),
+                         C.c_double(s["p_t"] if p_t is None else p_t))
+        if rc:
+            raise ValueError("orc_setup rejected the geometry")
+        self.angles = np.ascontiguousarray(s["angles"], np.float32)
+        self.cos = np.zeros(N_p, np.float64)
+        self.sin = np.zeros(N_p, np.float64)
+        L.orc_sincos(N_p, self.angles.ctypes.data_as(C.c_void_p),
+                     self.cos.ctypes.data_as(C.c_void_p), self.sin.ctypes.data_as(C.c_void_p))
+        self.H = np.zeros((N_p, NB), np.float32)
+        L.orc_detector_response(C.byref(g), self.angles.ctypes.data_as(C.c_void_p),
+                                self.H.ctypes.data_as(C.c_void_p))
+        self.y = sino_to_internal(s["projections"], N_p, N_t, s["proj_cols"]).astype(np.float64)
+        self.w = sino_to_internal(s["weights"], N_p, N_t, s["proj_cols"]).astype(np.float64)
+        self.mask = np.zeros((g.N_y, g.N_x), np.uint8)
+        L.orc_mask(C.byref(g), self.mask.ctypes.data_as(C.c_void_p))
+
+    def _p(self, a):
+        return a.ctypes.data_as(C.c_void_p)
+
+    @property
+    def shape_x(self):
+        g = self.g
+        return (g.N_time, g.N_z, g.N_y, g.N_x)
+
+    def amatrix_column(self, row, col, view):
+        idx = np.zeros(256, np.int32)
+        val = np.zeros(256, np.float64)
+        n = self.L.orc_amatrix_column(C.byref(self.g), self._p(self.cos), self._p(self.sin),
+                                      self._p(self.H), row, col, view, self._p(idx), self._p(val))
+        return idx[:n].copy(), val[:n].copy()
+
+    def amatrix_all(self):
+        g = self.g
+        first = np.zeros((g.N_y * g.N_x, g.N_p), np.int32)
+        val = np.zeros((g.N_y * g.N_x, g.N_p, g.W), np.float64)
+        self.L.orc_amatrix_all(C.byref(g), self._p(self.cos), self._p(self.sin), self._p(self.H),
+                               self._p(first), self._p(val))
+        return first, val
+
+    def forward_project(self, x):
+        g = self.g
+        x = np.ascontiguousarray(x, np.float64).reshape(self.shape_x)
+        ax = np.zeros((g.N_p, g.N_r, g.N_t), np.float64)
+        self.L.orc_forward_project(C.byref(g), self._p(self.cos), self._p(self.sin),
+                                   self._p(self.H), self._p(self.frame_ptr),
+                                   self._p(self.frame_views), self._p(x), self._p(ax))
+        return ax
+
+    def init_error(self, x):
+        return self.y - self.forward_project(x)
+
+    def cost(self, e, x):
+        out = np.zeros(2, np.float64)
+        e = np.ascontiguousarray(e, np.float64)
+        x = np.ascontiguousarray(x, np.float64).reshape(self.shape_x)
+        c = self.L.orc_cost(C.byref(self.g), self._p(e), self._p(self.w), self._p(x), self._p(out))
+        return c, out
+
+    def substitution(self, V, th1, th2, nb, nbf, tn, tf):
+        nb = np.ascontiguousarray(nb, np.float64).ravel()
+        nbf = np.ascontiguousarray(nbf, np.uint8).ravel()
+        tn = np.ascontiguousarray(tn, np.float64)
+        tf = np.ascontiguousarray(tf, np.uint8)
+        return self.L.orc_substitution(C.byref(self.g), C.c_double(V), C.c_double(th1),
+                                       C.c_double(th2), self._p(nb), self._p(nbf),
+                                       self._p(tn), self._p(tf))
+
+    def qggmrf_weight(self, d, spatial=True):
+        return self.L.orc_qggmrf_weight(C.byref(self.g), C.c_double(d), int(spatial))
+
+    def qggmrf_value(self, d, spatial=True):
+        return self.L.orc_qggmrf_value(C.byref(self.g), C.c_double(d), int(spatial))
+
+    def thetas(self, e, t, z, row, col, w=None):
+        th1, th2 = C.c_double(), C.c_double()
+        e = np.ascontiguousarray(e, np.float64)
+        w = self.w if w is None else np.ascontiguousarray(w, np.float64)
+        self.L.orc_voxel_thetas(C.byref(self.g), self._p(self.cos), self._p(self.sin),
+                                self._p(self.H), self._p(self.frame_ptr),
+                                self._p(self.frame_views), self._p(e), self._p(w),
+                                t, z, row, col, C.byref(th1), C.byref(th2))
+        return th1.value, th2.value
+
+    def icd_equit(self, x, e, iteration, num_z_blocks=2, K=10):
+        stats = np.zeros(3, np.float64)
+        self.L.orc_icd_equit(C.byref(self.g), self._p(self.cos), self._p(self.sin),
+                             self._p(self.H), self._p(self.frame_ptr), self._p(self.frame_views),
+                             self._p(self.mask), num_z_blocks, K, iteration, self._p(x),
+                             self._p(e), self._p(self.w), self._p(stats))
+        return stats
+
+    def icd_equit_tiled(self, x, e, iteration, T, alpha, seed):
+        self.L.orc_icd_equit_tiled(C.byref(self.g), self._p(self.cos), self._p(self.sin),
+                                   self._p(self.H), self._p(self.frame_ptr),
+                                   self._p(self.frame_views), self._p(self.mask), T,
+                                   C.c_double(alpha), C.c_uint32(seed), iteration, self._p(x),
+                                   self._p(e), self._p(self.w))
+
+    def srandom2(self, seed):
+        self.L.orc_srandom2(C.c_uint32(seed))
+
+    def random2(self):
+        return self.L.orc_random2()
+
+    def run_icd(self, n_equits, x=None, num_z_blocks=2, K=10, seed=761521):
+        """Sequential-schedule ICD from x (zeros by default); returns x, e, costs."""
+        x = np.zeros(self.shape_x, np.float64) if x is None else np.array(x, np.float64)
+        e = self.init_error(x)
+        self.srandom2(seed)
+        costs = [self.cost(e, x)[0]]
+        for it in range(1, n_equits + 1):
+            self.icd_equit(x, e, it, num_z_blocks, K)
+            costs.append(self.cost(e, x)[0])
+        return x, e, np.array(costs)
+
+
+def reference_available():
+    return os.path.exists(REF_SO)
+
+
+_REF = None
+
+
+def _ref_lib():
+    global _REF
+    if _REF is None:
+        L = _lib(REF_SO)
+        L.ref_create.restype = C.c_void_p
+        L.ref_substitution.restype = C.c_double
+        L.ref_cost.restype = C.c_double
+        L.ref_random2.restype = C.c_double
+        L.ref_amatrix_column.restype = C.c_int32
+        L.ref_update_voxel.restype = C.c_long
+        L.ref_init_error.restype = C.c_int32
+        L.ref_reconstruct.restype = C.c_int32
+        _REF = L
+    return _REF
+
+
+class Reference:
+    """The reference build (oracle/_ref) for one stage of one scan."""
+
+    def __init__(self, scan, mult_xy=1, mult_z=1, quad_convex=1, remove_rings=0,
+                 huber_delta=0.5, huber_T=4.0, convg=1.0, threads=0):
+        self.L = L = _ref_lib()
+        self.scan = s = scan
+        self.tmp = tempfile.mkdtemp(prefix="mbir_ref_")
+        self._keep = [np.ascontiguousarray(s[k], np.float32)
+                      for k in ("projections", "weights", "angles", "proj_times", "recon_times")]
+        P = [a.ctypes.data_as(C.c_void_p) for a in self._keep]
+        f = C.c_float
+        self.h = L.ref_create(self.tmp.encode(), mult_xy, mult_z, *P,
+                              s["proj_rows"], s["proj_cols"], s["proj_num"], s["recon_num"],
+                              f(s["vox_wid"]), f(s["rot_center"]), f(s["sig_s"]), f(s["sig_t"]),
+                              f(s["c_s"]), f(s["c_t"]), f(convg), remove_rings, quad_convex,
+                              f(huber_delta), f(huber_T), threads)
+        if not self.h:
+            raise RuntimeError("reference initStructures failed")
+        self.h = C.c_void_p(self.h)
+        ints = np.zeros(8, np.int32)
+        dbls = np.zeros(24 + 27, np.float64)
+        L.ref_geometry(self.h, ints.ctypes.data_as(C.c_void_p), dbls.ctypes.data_as(C.c_void_p))
+        self.ints, self.dbls = ints, dbls
+        (self.N_r, self.N_t, self.N_p, self.N_x, self.N_y, self.N_z, self.N_time,
+         self.num_z_blocks) = [int(v) for v in ints]
+
+    def _p(self, a):
+        return a.ctypes.data_as(C.c_void_p)
+
+    @property
+    def shape_x(self):
+        return (self.N_time, self.N_z, self.N_y, self.N_x)
+
+    @property
+    def shape_e(self):
+        return (self.N_p, self.N_r, self.N_t)
+
+    def view_map(self):
+        ptr = np.zeros(self.N_time + 1, np.int32)
+        views = np.zeros(max(self.N_p, 1), np.int32)
+        self.L.ref_view_map(self.h, self._p(ptr), self._p(views))
+        return ptr, views[:ptr[-1]]
+
+    def detector_response(self):
+        H = np.zeros((self.N_p, NB), np.float32)
+        self.L.ref_detector_response(self.h, self._p(H))
+        return H
+
+    def amatrix_column(self, row, col, view):
+        idx = np.zeros(256, np.int32)
+        val = np.zeros(256, np.float64)
+        n = self.L.ref_amatrix_column(self.h, C.c_int32(row), C.c_int32(col),
+                                      C.c_int32(view), self._p(idx), self._p(val))
+        return idx[:n].copy(), val[:n].copy()
+
+    def set_object(self, x):
+        x = np.ascontiguousarray(x, np.float32).reshape(self.shape_x)
+        self.L.ref_set_object(self.h, self._p(x))
+
+    def get_object(self):
+        x = np.zeros(self.shape_x, np.float32)
+        self.L.ref_get_object(self.h, self._p(x))
+        return x
+
+    def init_error(self):
+        e = np.zeros(self.shape_e, np.float32)
+        if self.L.ref_init_error(self.h, self._p(e)):
+            raise RuntimeError("initErrorSinogam failed")
+        return e
+
+    def get_error(self):
+        e = np.zeros(self.shape_e, np.float32)
+        self.L.ref_get_error(self.h, self._p(e))
+        return e
+
+    def get_weight(self):
+        w = np.zeros(self.shape_e, np.float32)
+        self.L.ref_get_weight(self.h, self._p(w))
+        return w
+
+    def substitution(self, V, th1, th2, nb, nbf, tn, tf):
+        nb = np.ascontiguousarray(nb, np.float64).ravel()
+        nbf = np.ascontiguousarray(nbf, np.uint8).ravel()
+        tn = np.ascontiguousarray(tn, np.float64)
+        tf = np.ascontiguousarray(tf, np.uint8)
+        return self.L.ref_substitution(self.h, C.c_double(V), C.c_double(th1),
+                                       C.c_double(th2), self._p(nb), self._p(nbf),
+                                       self._p(tn), self._p(tf))
+
+    def update_voxel(self, t, z, row, col, iteration):
+        return self.L.ref_update_voxel(self.h, C.c_int32(t), C.c_int32(z), C.c_int32(row),
+                                       C.c_int32(col), C.c_int32(iteration))
+
+    def cost(self):
+        c = self.L.ref_cost(self.h)
+        parts = np.zeros(2, np.float64)
+        self.L.ref_cost_parts(self.h, self._p(parts))
+        return c, parts
+
+    def icd_run(self, n, first_iter=1):
+        costs = np.zeros(n, np.float64)
+        conv = np.zeros(n, np.int32)
+        secs = np.zeros(n, np.float64)
+        self.L.ref_icd_run(self.h, C.c_int32(n), C.c_int32(first_iter), self._p(costs),
+                           self._p(conv), self._p(secs))
+        return costs, conv, secs
+
+    def close(self):
+        if self.h:
+            self.L.ref_destroy(self.h)
+            self.h = None
+
+
+def reference_reconstruct(scan, convg=1.0, quad_convex=1, remove_rings=0, huber_delta=0.5,
+                          huber_T=4.0, threads=0, scratch=None):
+    """The reference's full reconstruct() (multi-resolution); returns the
+    object [recon_num][rows+2][cols][cols] including the halo slices."""
+    L = _ref_lib()
+    s = scan
+    keep = [np.ascontiguousarray(s[k], np.float32)
+            for k in ("projections", "weights", "angles", "proj_times", "recon_times")]
+    obj = np.zeros((s["recon_num"], s["proj_rows"] + 2, s["proj_cols"], s["proj_cols"]), np.float32)
+    d = scratch or tempfile.mkdtemp(prefix="mbir_refrec_")
+    f = C.c_float
+    rc = L.ref_reconstruct(d.encode(), obj.ctypes.data_as(C.c_void_p),
+                           *[a.ctypes.data_as(C.c_void_p) for a in keep],
+                           s["proj_rows"], s["proj_cols"], s["proj_num"], s["recon_num"],
+                           f(s["vox_wid"]), f(s["rot_center"]), f(s["sig_s"]), f(s["sig_t"]),
+                           f(s["c_s"]), f(s["c_t"]), f(convg), remove_rings, quad_convex,
+                           f(huber_delta), f(huber_T), threads)
+    if rc:
+        raise RuntimeError("reference reconstruct() returned -1")
+    return obj, d

@@ -1,0 +1,1301 @@
+// mbir_hip.hip -- gfx950 kernels of the MBIR hot path and the thin C-ABI
+// declared in include/mbir_hip.h.
+//
+// Device data layout (HBM-resident for the whole stage):
+//   y, w, e       [N_p][N_r][N_t] float      detector row t fastest, so one
+//                                            voxel column's rows are contiguous
+//   x             [N_time][N_y][N_x][N_z]    z fastest: a voxel column and its
+//                                            3x3 neighbour columns are 9
+//                                            contiguous runs of N_z floats
+//   A             first[col][slot] int32, val[col][slot][W] float, slots in
+//                 frame order, so one (column, frame) visit reads one
+//                 contiguous run of P_f*W values.
+// Kernels:
+//   k_amatrix     A build, FP64 arithmetic identical to the reference rule
+//   k_fwdproj     A x per view with an LDS accumulator (FP32 for e, FP64 for
+//                 the cost), A recomputed on the fly in FP64
+//   k_sweep       the parallel ICD: one wavefront per (tile, frame) walks the
+//                 tile's voxel columns in a seeded random order; theta1/theta2
+//                 reduced over the column's sparse footprint across the
+//                 wavefront, qGGMRF surrogate from an LDS-staged 3x3xN_z (+t+-1)
+//                 neighbourhood, error sinogram updated with atomics
+//   k_prior_cost  FP64 prior term, k_data_cost FP64 data term
+//   k_backproj    A^T v, k_thetas / k_subst unit-test entry kernels
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mbir_hip.h"
+
+namespace {
+
+constexpr int kNB = 65;        // detector response entries per view
+constexpr int kRespBins = 64;  // XT_Constants.h:79
+constexpr int kNZMax = 128;    // voxels per column (N_z) supported by k_sweep
+constexpr int kNTMax = 256;    // detector rows (N_t) supported by k_sweep
+constexpr int kWMax = 33;      // window width 2*mult_xy+1 (mult_xy <= 16)
+constexpr int kWave = 64;
+
+thread_local std::string g_err;
+
+int fail(const std::string& msg) {
+  g_err = msg;
+  return -1;
+}
+
+#define HIP_OK(call)                                                        \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess)                                                   \
+      return fail(std::string(#call) + ": " + hipGetErrorString(e_));       \
+  } while (0)
+
+// Parameters passed to kernels by value.
+struct DevGeom {
+  int N_r, N_t, N_p, N_x, N_y, N_z, N_time, W, zo, nslots;
+  double delta_r, R0, RMax, x0, y0, delta_xy, OffsetR;
+  float delta_t;
+  float q, p_s, p_t, c_s, c_t;
+  float inv_sq_s, inv_sqp_s, inv_sq_t, inv_sqp_t;  // 1/sigma^q, 1/sigma^(q-p)
+  float b0_s, b0_t;                                // rho''(0) = q/(sigma^q c)
+  float filt[27];
+  float filt_t;
+  float alpha;
+  float huber_T, huber_delta;
+  double dq, dp_s, dp_t, dc_s, dc_t, dsq_s, dsqp_s, dsq_t, dsqp_t;
+  double dfilt[27];
+  double dfilt_t, dhuber_T, dhuber_delta;
+};
+
+__device__ __forceinline__ unsigned mix32(unsigned a) {
+  a ^= a >> 16;
+  a *= 0x7feb352dU;
+  a ^= a >> 15;
+  a *= 0x846ca68bU;
+  a ^= a >> 16;
+  return a;
+}
+
+__device__ __forceinline__ int gcd_i(int a, int b) {
+  while (b) {
+    int t = a % b;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+// ---------------------------------------------------------------- A rule
+// The A window of column (row, col) at one view: XT_AMatrix.c:50-110.  The
+// arithmetic is FP64 in the reference's operation order, so indices match
+// exactly and values to the last bit before any float store.  Calls
+// fn(j, j - jlo, value) for each kept bin j (f1 > 0) and returns the clamped
+// lower bin jlo, or -1 if the voxel misses the detector.
+template <typename Fn>
+__device__ __forceinline__ int amatrix_visit(const DevGeom& G, const float* __restrict__ h,
+                                             double cs, double sn, int row, int col,
+                                             Fn&& fn) {
+  const double xc = G.x0 + ((double)col + 0.5) * G.delta_xy;
+  const double yc = G.y0 + ((double)row + 0.5) * G.delta_xy;
+  const double r = xc * cs - yc * sn;
+  const double rlo = r - G.delta_xy, rhi = r + G.delta_xy;
+  if (rhi < G.R0 || rlo > G.RMax) return -1;
+  int jlo = (int)(floor(((rlo - G.R0) / G.delta_r)));
+  int jhi = (int)(floor((rhi - G.R0) / G.delta_r));
+  if (jhi >= G.N_r) jhi = G.N_r - 1;
+  if (jlo < 0) jlo = 0;
+  for (int j = jlo; j <= jhi && j - jlo < G.W; ++j) {
+    const double centre = (G.R0 + (((double)j) + 0.5) * (G.delta_r));
+    const double d = fabs(r - centre);
+    const int k = (int)(floor((d / G.OffsetR)));
+    if (k >= 0 && k < kRespBins) {
+      const double w_hi = d - k * G.OffsetR;
+      const double w_lo = G.OffsetR - w_hi;
+      const double f = (w_lo * h[k] + w_hi * h[k + 1]) / G.OffsetR;
+      if (f > 0) fn(j, j - jlo, f);
+    }
+  }
+  return jlo;
+}
+
+__global__ __launch_bounds__(256) void k_amatrix(DevGeom G,
+                                                 const float* __restrict__ H,
+                                                 const double* __restrict__ cosv,
+                                                 const double* __restrict__ sinv,
+                                                 const int* __restrict__ slot_view,
+                                                 int* __restrict__ afirst,
+                                                 float* __restrict__ aval) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long total = (long long)G.N_x * G.N_y * G.nslots;
+  if (i >= total) return;
+  const int s = (int)(i % G.nslots);
+  const int c = (int)(i / G.nslots);
+  const int p = slot_view[s];
+  float* dst = aval + i * G.W;
+  for (int k = 0; k < G.W; ++k) dst[k] = 0.f;
+  afirst[i] = amatrix_visit(G, H + (size_t)p * kNB, cosv[p], sinv[p], c / G.N_x,
+                            c % G.N_x, [&](int, int k, double v) { dst[k] = (float)v; });
+}
+
+// ----------------------------------------------------------- projection
+// One workgroup per (view, row chunk): accumulate A x of the whole object
+// for this view in LDS, then write e = y - Ax - d (mode 0), the FP64 data
+// cost of y - Ax - d (mode 1), or Ax itself (mode 2).
+template <typename Acc>
+__global__ __launch_bounds__(256) void k_fwdproj(
+    DevGeom G, int mode, int tchunk, const float* __restrict__ H,
+    const double* __restrict__ cosv, const double* __restrict__ sinv,
+    const int* __restrict__ view_frame, const float* __restrict__ x,
+    const float* __restrict__ y, const float* __restrict__ w,
+    const float* __restrict__ d, float* __restrict__ out,
+    double* __restrict__ partial) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  Acc* acc = reinterpret_cast<Acc*>(smem);
+  __shared__ float sh[kNB];
+  __shared__ double red[256];
+  const int p = blockIdx.x;
+  const int t0 = blockIdx.y * tchunk;
+  const int tc = min(tchunk, G.N_t - t0);
+  const int tid = threadIdx.x;
+  for (int i = tid; i < G.N_r * tc; i += 256) acc[i] = Acc(0);
+  for (int i = tid; i < kNB; i += 256) sh[i] = H[(size_t)p * kNB + i];
+  __syncthreads();
+  const int f = view_frame[p];
+  if (f >= 0) {
+    const double cs = cosv[p], sn = sinv[p];
+    const size_t vol = (size_t)G.N_y * G.N_x * G.N_z;
+    const float* xf = x + (size_t)f * vol;
+    const int ncol = G.N_x * G.N_y;
+    for (int c = tid; c < ncol; c += 256) {
+      const float* xc = xf + (size_t)c * G.N_z;
+      bool any = false;
+      for (int z = t0 / G.zo; z <= (t0 + tc - 1) / G.zo; ++z) any |= (xc[z] != 0.f);
+      if (!any) continue;
+      amatrix_visit(G, sh, cs, sn, c / G.N_x, c % G.N_x, [&](int j, int, double v) {
+        Acc* row = acc + (size_t)j * tc;
+        for (int t = 0; t < tc; ++t) {
+          const float xv = xc[(t0 + t) / G.zo];
+          if (xv != 0.f) atomicAdd(&row[t], (Acc)((double)xv * v));
+        }
+      });
+    }
+  }
+  __syncthreads();
+  double local = 0.0;
+  for (int i = tid; i < G.N_r * tc; i += 256) {
+    const int r = i / tc, t = t0 + i % tc;
+    const size_t o = ((size_t)p * G.N_r + r) * G.N_t + t;
+    const float dv = d ? d[(size_t)r * G.N_t + t] : 0.f;
+    if (mode == 0) {
+      // XT_ICD_update.c:689, in float as the reference's arrays are
+      out[o] = y[o] - (float)acc[i] - dv;
+    } else if (mode == 1) {
+      const double ev = (double)y[o] - (double)acc[i] - (double)dv;
+      const double s = ev * sqrt((double)w[o]);
+      if (fabs(s) < G.dhuber_T)
+        local += s * s;
+      else
+        local += 2.0 * G.dhuber_delta * G.dhuber_T * fabs(s) +
+                 G.dhuber_T * G.dhuber_T * (1.0 - 2.0 * G.dhuber_delta);
+    } else {
+      out[o] = (float)acc[i];
+    }
+  }
+  if (mode == 1) {
+    red[tid] = local;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (tid < s) red[tid] += red[tid + s];
+      __syncthreads();
+    }
+    if (tid == 0) partial[(size_t)blockIdx.x * gridDim.y + blockIdx.y] = red[0];
+  }
+}
+
+// Data term from the maintained error sinogram (XT_ICD_update.c:192-204).
+__global__ __launch_bounds__(256) void k_data_cost(DevGeom G, size_t n,
+                                                   const float* __restrict__ e,
+                                                   const float* __restrict__ w,
+                                                   double* __restrict__ partial) {
+  __shared__ double red[256];
+  double local = 0.0;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * 256) {
+    const double s = (double)e[i] * sqrt((double)w[i]);
+    if (fabs(s) < G.dhuber_T)
+      local += s * s;
+    else
+      local += 2.0 * G.dhuber_delta * G.dhuber_T * fabs(s) +
+               G.dhuber_T * G.dhuber_T * (1.0 - 2.0 * G.dhuber_delta);
+  }
+  red[threadIdx.x] = local;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// qGGMRF potential in FP64 (XT_ICD_update.c:82-90).
+__device__ __forceinline__ double qg_value64(double delta, double q, double p,
+                                             double c, double sq, double sqp) {
+  const double ad = fabs(delta);
+  if (ad == 0.0) return 0.0;
+  const double aq = (q == 2.0) ? ad * ad : pow(ad, q);
+  return (aq / sq) / (c + pow(ad, q - p) / sqp);
+}
+
+// Prior term: each neighbour pair once (13 forward spatial + t+1), FP64
+// (XT_ICD_update.c:205-298).
+__global__ __launch_bounds__(256) void k_prior_cost(DevGeom G,
+                                                    const float* __restrict__ x,
+                                                    double* __restrict__ partial) {
+  __shared__ double red[256];
+  const int fwd[13][3] = {{0, 0, 1},  {0, 1, -1},  {0, 1, 0}, {0, 1, 1},
+                          {1, -1, 0}, {1, 0, 0},   {1, 1, 0}, {1, -1, -1},
+                          {1, -1, 1}, {1, 0, -1},  {1, 1, -1}, {1, 1, 1},
+                          {1, 0, 1}};
+  const size_t vol = (size_t)G.N_y * G.N_x * G.N_z;
+  const size_t n = vol * G.N_time;
+  double local = 0.0;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * 256) {
+    const int z = (int)(i % G.N_z);
+    const size_t c = (i / G.N_z) % ((size_t)G.N_y * G.N_x);
+    const int f = (int)(i / vol);
+    const int yy = (int)(c / G.N_x), xx = (int)(c % G.N_x);
+    const double v = x[i];
+    for (int k = 0; k < 13; ++k) {
+      const int z2 = z + fwd[k][0], y2 = yy + fwd[k][1], x2 = xx + fwd[k][2];
+      if (z2 >= G.N_z || y2 < 0 || y2 >= G.N_y || x2 < 0 || x2 >= G.N_x) continue;
+      const double nv = x[(size_t)f * vol + ((size_t)y2 * G.N_x + x2) * G.N_z + z2];
+      const int fi = ((fwd[k][0] + 1) * 3 + (fwd[k][1] + 1)) * 3 + fwd[k][2] + 1;
+      local += G.dfilt[fi] * qg_value64(v - nv, G.dq, G.dp_s, G.dc_s, G.dsq_s, G.dsqp_s);
+    }
+    if (f + 1 < G.N_time)
+      local += G.dfilt_t * qg_value64(v - (double)x[i + vol], G.dq, G.dp_t, G.dc_t,
+                                      G.dsq_t, G.dsqp_t);
+  }
+  red[threadIdx.x] = local;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// --------------------------------------------------------- ICD building blocks
+// qGGMRF b(delta) = rho'(delta)/delta, FP32 (XT_ICD_update.c:92-128): with
+// r = |d|^(q-p)/sigma^(q-p), b = |d|^(q-2) (q - (q-p) r/(c+r)) / ((c+r) sigma^q),
+// and b(0) = q/(sigma^q c).
+__device__ __forceinline__ float qg_b(float d, float q, float p, float c,
+                                      float inv_sq, float inv_sqp, float b0) {
+  const float ad = fabsf(d);
+  if (ad == 0.f) return b0;
+  const float lg = log2f(ad);
+  const float ratio = exp2f((q - p) * lg) * inv_sqp;
+  const float den = c + ratio;
+  const float shape = q - (q - p) * ratio / den;
+  const float mag = (q == 2.f) ? 1.f : exp2f((q - 2.f) * lg);
+  return mag * shape * inv_sq / den;
+}
+
+// Weight of neighbour j of a voxel with value V: j < 27 spatial ([dz][dy][dx],
+// 13 is the centre and never used), 27/28 = t-1/t+1.
+__device__ __forceinline__ float nb_weight(const DevGeom& G, int j, float V, float nbv) {
+  if (j < 27)
+    return G.filt[j] * qg_b(V - nbv, G.q, G.p_s, G.c_s, G.inv_sq_s, G.inv_sqp_s, G.b0_s);
+  return G.filt_t * qg_b(V - nbv, G.q, G.p_t, G.c_t, G.inv_sq_t, G.inv_sqp_t, G.b0_t);
+}
+
+// Surrogate solve and over-relaxation (XT_ICD_update.c:174-176).
+__device__ __forceinline__ float substitute(const DevGeom& G, float alpha, float V,
+                                            float th1, float th2, float num, float den) {
+  const float u = (num + th2 * V - th1) / (den + th2);
+  return V + alpha * (u - V);
+}
+
+__device__ __forceinline__ float e_load(const float* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void e_add(float* p, float v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+struct ColumnArgs {
+  const float* aval;
+  const int* afirst;
+  const int* frame_views;
+  const float* w;
+  float* e;
+};
+
+// theta1/theta2 of every detector row of voxel column c in frame f (views
+// slots s0 .. s0+nv-1): lanes are (entry group g, row rr) so the rows of one
+// (view, bin) entry are read as one contiguous run.  Per-row sums land in
+// row1/row2 (LDS); the voxel's thetas are the sums over its z_overlap rows.
+// XT_VoxUpdate.c:56-85 including the generalized-Huber weight.
+__device__ void column_thetas(const DevGeom& G, const ColumnArgs& A, int c, int s0,
+                              int nv, float* row1, float* row2, float* part1,
+                              float* part2) {
+  const int lane = threadIdx.x;
+  const int rpl = G.N_t < kWave ? G.N_t : kWave;
+  const int groups = kWave / rpl;
+  const int g = lane / rpl, rr = lane % rpl;
+  const int nent = nv * G.W;
+  const float T = G.huber_T, hd = G.huber_delta;
+  for (int t0 = 0; t0 < G.N_t; t0 += kWave) {
+    const int row = t0 + rr;
+    float a1 = 0.f, a2 = 0.f;
+    if (g < groups && row < G.N_t) {
+      for (int sk = g; sk < nent; sk += groups) {
+        const int s = sk / G.W, k = sk - s * G.W;
+        const size_t slot = (size_t)c * G.nslots + s0 + s;
+        const int r0 = A.afirst[slot];
+        if (r0 < 0) continue;
+        const int r = r0 + k;
+        if (r >= G.N_r) continue;
+        float a = A.aval[slot * G.W + k];
+        if (a == 0.f) continue;
+        a *= G.delta_t;
+        const int p = A.frame_views[s0 + s];
+        const size_t o = ((size_t)p * G.N_r + r) * G.N_t + row;
+        const float ev = e_load(A.e + o), wv = A.w[o];
+        const float sw = sqrtf(wv);
+        const float hw = (fabsf(ev * sw) < T) ? wv : T * hd * sw / fabsf(ev);
+        a2 += a * a * hw;
+        a1 -= ev * a * hw;
+      }
+    }
+    part1[lane] = a1;
+    part2[lane] = a2;
+    __syncthreads();
+    if (lane < rpl && row < G.N_t) {
+      float s1 = 0.f, s2 = 0.f;
+      for (int gg = 0; gg < groups; ++gg) {
+        s1 += part1[gg * rpl + lane];
+        s2 += part2[gg * rpl + lane];
+      }
+      row1[row] = s1;
+      row2[row] = s2;
+    }
+    __syncthreads();
+  }
+}
+
+// e -= A delta_t dx along voxel column c (XT_VoxUpdate.c:96-116), with
+// atomics because other wavefronts' columns share detector bins.
+__device__ void column_update_error(const DevGeom& G, const ColumnArgs& A, int c,
+                                    int s0, int nv, const float* dxz) {
+  const int lane = threadIdx.x;
+  const int rpl = G.N_t < kWave ? G.N_t : kWave;
+  const int groups = kWave / rpl;
+  const int g = lane / rpl, rr = lane % rpl;
+  const int nent = nv * G.W;
+  for (int t0 = 0; t0 < G.N_t; t0 += kWave) {
+    const int row = t0 + rr;
+    if (g >= groups || row >= G.N_t) continue;
+    const float dz = dxz[row / G.zo];
+    if (dz == 0.f) continue;
+    for (int sk = g; sk < nent; sk += groups) {
+      const int s = sk / G.W, k = sk - s * G.W;
+      const size_t slot = (size_t)c * G.nslots + s0 + s;
+      const int r0 = A.afirst[slot];
+      if (r0 < 0) continue;
+      const int r = r0 + k;
+      if (r >= G.N_r) continue;
+      const float a = A.aval[slot * G.W + k];
+      if (a == 0.f) continue;
+      const int p = A.frame_views[s0 + s];
+      e_add(A.e + ((size_t)p * G.N_r + r) * G.N_t + row, -(a * G.delta_t) * dz);
+    }
+  }
+}
+
+struct SweepArgs {
+  ColumnArgs col;
+  const int* frame_ptr;
+  const int* tile_ptr;
+  const int* tile_cols;
+  const int* launch_tiles;
+  int n_launch_tiles;
+  int fparity;
+  int iter;
+  unsigned seed;
+  float alpha;
+  float* x;
+  double* stats;
+};
+
+// One ICD pass over the voxel columns of the tiles in this launch (one
+// 2x2-checkerboard colour) for the frames of one parity.  Concurrent
+// wavefronts never touch neighbouring columns or temporally adjacent frames,
+// so x needs no atomics; the error sinogram is shared and updated
+// atomically.  Within a column, z is updated in two parity halves so each
+// voxel sees its z+-1 neighbours' new values (sequential-ICD equivalent).
+__global__ __launch_bounds__(kWave) void k_sweep(DevGeom G, SweepArgs A) {
+  __shared__ float s_nb[11 * kNZMax];
+  __shared__ int s_ok[11];
+  __shared__ float s_row1[kNTMax], s_row2[kNTMax];
+  __shared__ float s_part1[kWave], s_part2[kWave];
+  __shared__ float s_dx[kNZMax];
+  const int lane = threadIdx.x;
+  const int wid = blockIdx.x;
+  const int tile = A.launch_tiles[wid % A.n_launch_tiles];
+  const int f = A.fparity + 2 * (wid / A.n_launch_tiles);
+  if (f >= G.N_time) return;
+  const int c0 = A.tile_ptr[tile];
+  const int n = A.tile_ptr[tile + 1] - c0;
+  const int s0 = A.frame_ptr[f];
+  const int nv = A.frame_ptr[f + 1] - s0;
+  if (n == 0 || nv == 0) return;
+  // seeded random visiting order: k -> (pa k + pb) mod n, gcd(pa, n) = 1
+  const unsigned h = mix32(A.seed ^ mix32((unsigned)tile * 131071u + (unsigned)f * 524287u));
+  int pa = 1, pb = 0;
+  if (n > 1) {
+    pa = 1 + (int)(h % (unsigned)(n - 1));
+    while (gcd_i(pa, n) != 1) pa = pa % (n - 1) + 1;
+    pb = (int)(mix32(h + 0x9e3779b9u) % (unsigned)n);
+  }
+  const int Nz = G.N_z;
+  const size_t vol = (size_t)G.N_y * G.N_x * Nz;
+  double st_dx = 0.0, st_x = 0.0, st_skip = 0.0, st_upd = 0.0;
+  for (int k = 0; k < n; ++k) {
+    const int c = A.tile_cols[c0 + (int)(((long long)pa * k + pb) % n)];
+    const int cy = c / G.N_x, cx = c % G.N_x;
+    // neighbourhood: 3x3 columns at t, the column at t-1 and t+1
+    if (lane < 11) {
+      int ok;
+      if (lane < 9) {
+        const int yy = cy + lane / 3 - 1, xx = cx + lane % 3 - 1;
+        ok = (yy >= 0 && yy < G.N_y && xx >= 0 && xx < G.N_x);
+      } else {
+        const int ff = lane == 9 ? f - 1 : f + 1;
+        ok = (ff >= 0 && ff < G.N_time);
+      }
+      s_ok[lane] = ok;
+    }
+    for (int j = lane; j < 11 * Nz; j += kWave) {
+      const int cc = j / Nz, z = j - cc * Nz;
+      float v = 0.f;
+      if (cc < 9) {
+        const int yy = cy + cc / 3 - 1, xx = cx + cc % 3 - 1;
+        if (yy >= 0 && yy < G.N_y && xx >= 0 && xx < G.N_x)
+          v = A.x[(size_t)f * vol + ((size_t)yy * G.N_x + xx) * Nz + z];
+      } else {
+        const int ff = cc == 9 ? f - 1 : f + 1;
+        if (ff >= 0 && ff < G.N_time) v = A.x[(size_t)ff * vol + (size_t)c * Nz + z];
+      }
+      s_nb[j] = v;
+    }
+    for (int z = lane; z < Nz; z += kWave) s_dx[z] = 0.f;
+    column_thetas(G, A.col, c, s0, nv, s_row1, s_row2, s_part1, s_part2);
+    // surrogate solve, z parity halves
+    for (int par = 0; par < 2; ++par) {
+      const int nzp = (Nz - par + 1) / 2;
+      int lpv = 32;
+      while (lpv > 1 && lpv * nzp > kWave) lpv >>= 1;
+      const int vpp = kWave / lpv;
+      for (int vb = 0; vb < nzp; vb += vpp) {
+        const int vi = vb + lane / lpv, sub = lane % lpv;
+        const bool act = vi < nzp;
+        const int z = par + 2 * vi;
+        const float V = act ? s_nb[4 * Nz + z] : 0.f;
+        float num = 0.f, den = 0.f;
+        int pos = 0;
+        if (act) {
+          for (int j = sub; j < 29; j += lpv) {
+            if (j == 13) continue;
+            int ok;
+            float nbv;
+            if (j < 27) {
+              const int z2 = z + j / 9 - 1, cc = j % 9;
+              ok = s_ok[cc] && z2 >= 0 && z2 < Nz;
+              nbv = ok ? s_nb[cc * Nz + z2] : 0.f;
+            } else {
+              const int cc = 9 + (j - 27);
+              ok = s_ok[cc];
+              nbv = ok ? s_nb[cc * Nz + z] : 0.f;
+            }
+            if (nbv > 0.f) pos = 1;
+            if (!ok) continue;
+            const float wt = nb_weight(G, j, V, nbv);
+            num += wt * nbv;
+            den += wt;
+          }
+        }
+        for (int off = lpv / 2; off > 0; off >>= 1) {
+          num += __shfl_xor(num, off);
+          den += __shfl_xor(den, off);
+          pos |= __shfl_xor(pos, off);
+        }
+        if (act && sub == 0) {
+          // zero skipping (XT_VoxUpdate.c:238-262)
+          if (V == 0.f && A.iter > 1 && !pos) {
+            st_skip += 1.0;
+          } else {
+            float th1 = 0.f, th2 = 0.f;
+            for (int m = 0; m < G.zo; ++m) {
+              th1 += s_row1[z * G.zo + m];
+              th2 += s_row2[z * G.zo + m];
+            }
+            const float xn = substitute(G, A.alpha, V, th1, th2, num, den);
+            s_nb[4 * Nz + z] = xn;
+            s_dx[z] = xn - V;
+            A.x[(size_t)f * vol + (size_t)c * Nz + z] = xn;
+            st_dx += fabs((double)xn - (double)V);
+            st_x += fabs((double)xn);
+            st_upd += 1.0;
+          }
+        }
+      }
+      __syncthreads();
+    }
+    column_update_error(G, A.col, c, s0, nv, s_dx);
+    __syncthreads();
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    st_dx += __shfl_xor(st_dx, off);
+    st_x += __shfl_xor(st_x, off);
+    st_skip += __shfl_xor(st_skip, off);
+    st_upd += __shfl_xor(st_upd, off);
+  }
+  if (lane == 0) {
+    atomicAdd(&A.stats[0], st_dx);
+    atomicAdd(&A.stats[1], st_x);
+    atomicAdd(&A.stats[2], st_skip);
+    atomicAdd(&A.stats[3], st_upd);
+  }
+}
+
+// theta1/theta2 of individual voxels through the sweep's column routine.
+__global__ __launch_bounds__(kWave) void k_thetas(DevGeom G, ColumnArgs A,
+                                                  const int* __restrict__ frame_ptr,
+                                                  const int* __restrict__ tzyx,
+                                                  float* __restrict__ th1,
+                                                  float* __restrict__ th2) {
+  __shared__ float s_row1[kNTMax], s_row2[kNTMax];
+  __shared__ float s_part1[kWave], s_part2[kWave];
+  const int i = blockIdx.x;
+  const int f = tzyx[4 * i], z = tzyx[4 * i + 1];
+  const int c = tzyx[4 * i + 2] * G.N_x + tzyx[4 * i + 3];
+  const int s0 = frame_ptr[f], nv = frame_ptr[f + 1] - s0;
+  column_thetas(G, A, c, s0, nv, s_row1, s_row2, s_part1, s_part2);
+  if (threadIdx.x == 0) {
+    float a = 0.f, b = 0.f;
+    for (int m = 0; m < G.zo; ++m) {
+      a += s_row1[z * G.zo + m];
+      b += s_row2[z * G.zo + m];
+    }
+    th1[i] = a;
+    th2[i] = b;
+  }
+}
+
+// Functional substitution on independent voxels with the sweep's lane split
+// (16 lanes per voxel, 4 voxels per wavefront).
+__global__ __launch_bounds__(kWave) void k_subst(DevGeom G, int n,
+                                                 const float* __restrict__ V,
+                                                 const float* __restrict__ th1,
+                                                 const float* __restrict__ th2,
+                                                 const float* __restrict__ nb,
+                                                 const unsigned char* __restrict__ nbf,
+                                                 const float* __restrict__ tn,
+                                                 const unsigned char* __restrict__ tf,
+                                                 float* __restrict__ out) {
+  constexpr int lpv = 16;
+  const int lane = threadIdx.x;
+  const int i = blockIdx.x * (kWave / lpv) + lane / lpv;
+  const int sub = lane % lpv;
+  const bool act = i < n;
+  const float v = act ? V[i] : 0.f;
+  float num = 0.f, den = 0.f;
+  if (act) {
+    for (int j = sub; j < 29; j += lpv) {
+      if (j == 13) continue;
+      const bool ok = j < 27 ? nbf[(size_t)i * 27 + j] != 0 : tf[(size_t)i * 2 + j - 27] != 0;
+      if (!ok) continue;
+      const float nbv = j < 27 ? nb[(size_t)i * 27 + j] : tn[(size_t)i * 2 + j - 27];
+      const float wt = nb_weight(G, j, v, nbv);
+      num += wt * nbv;
+      den += wt;
+    }
+  }
+  for (int off = lpv / 2; off > 0; off >>= 1) {
+    num += __shfl_xor(num, off);
+    den += __shfl_xor(den, off);
+  }
+  if (act && sub == 0) out[i] = substitute(G, G.alpha, v, th1[i], th2[i], num, den);
+}
+
+// A^T v: one thread per (column, frame), z_overlap rows folded into z.
+__global__ __launch_bounds__(256) void k_backproj(DevGeom G,
+                                                  const float* __restrict__ aval,
+                                                  const int* __restrict__ afirst,
+                                                  const int* __restrict__ frame_ptr,
+                                                  const int* __restrict__ frame_views,
+                                                  const float* __restrict__ v,
+                                                  float* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long ncol = (long long)G.N_x * G.N_y;
+  if (i >= ncol * G.N_time) return;
+  const int f = (int)(i / ncol);
+  const int c = (int)(i % ncol);
+  float* o = out + ((size_t)f * ncol + c) * G.N_z;
+  for (int z = 0; z < G.N_z; ++z) o[z] = 0.f;
+  for (int s = frame_ptr[f]; s < frame_ptr[f + 1]; ++s) {
+    const size_t slot = (size_t)c * G.nslots + s;
+    const int r0 = afirst[slot];
+    if (r0 < 0) continue;
+    const int p = frame_views[s];
+    for (int k = 0; k < G.W; ++k) {
+      const float a = aval[slot * G.W + k];
+      const int r = r0 + k;
+      if (a == 0.f || r >= G.N_r) continue;
+      const float* vr = v + ((size_t)p * G.N_r + r) * G.N_t;
+      for (int t = 0; t < G.N_t; ++t) o[t / G.zo] += a * vr[t];
+    }
+  }
+}
+
+// [t][z][y][x] <-> [t][y][x][z]
+__global__ __launch_bounds__(256) void k_to_colmajor(int Nt, int Nz, int ncol,
+                                                     const float* __restrict__ in,
+                                                     float* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long n = (long long)Nt * Nz * ncol;
+  if (i >= n) return;
+  const int c = (int)(i % ncol);
+  const int z = (int)((i / ncol) % Nz);
+  const int f = (int)(i / ((long long)ncol * Nz));
+  out[((size_t)f * ncol + c) * Nz + z] = in[i];
+}
+
+__global__ __launch_bounds__(256) void k_from_colmajor(int Nt, int Nz, int ncol,
+                                                       const float* __restrict__ in,
+                                                       float* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long n = (long long)Nt * Nz * ncol;
+  if (i >= n) return;
+  const int c = (int)(i % ncol);
+  const int z = (int)((i / ncol) % Nz);
+  const int f = (int)(i / ((long long)ncol * Nz));
+  out[i] = in[((size_t)f * ncol + c) * Nz + z];
+}
+
+DevGeom make_dev_geom(const mbir_hip_geom& g, int nslots) {
+  DevGeom G;
+  memset(&G, 0, sizeof(G));
+  G.N_r = g.N_r;
+  G.N_t = g.N_t;
+  G.N_p = g.N_p;
+  G.N_x = g.N_x;
+  G.N_y = g.N_y;
+  G.N_z = g.N_z;
+  G.N_time = g.N_time;
+  G.W = 2 * g.mult_xy + 1;
+  G.zo = g.mult_z;
+  G.nslots = nslots;
+  G.delta_r = g.delta_r;
+  G.R0 = g.R0;
+  G.RMax = g.RMax;
+  G.x0 = g.x0;
+  G.y0 = g.y0;
+  G.delta_xy = g.delta_xy;
+  G.OffsetR = g.OffsetR;
+  G.delta_t = (float)g.delta_t;
+  G.q = (float)g.q;
+  G.p_s = (float)g.p_s;
+  G.p_t = (float)g.p_t;
+  G.c_s = (float)g.c_s;
+  G.c_t = (float)g.c_t;
+  G.inv_sq_s = (float)(1.0 / g.sig_s_q);
+  G.inv_sqp_s = (float)(1.0 / g.sig_s_qp);
+  G.inv_sq_t = (float)(1.0 / g.sig_t_q);
+  G.inv_sqp_t = (float)(1.0 / g.sig_t_qp);
+  G.b0_s = (float)(g.q / (g.sig_s_q * g.c_s));
+  G.b0_t = (float)(g.q / (g.sig_t_q * g.c_t));
+  for (int k = 0; k < 27; ++k) {
+    G.filt[k] = (float)g.filt[k];
+    G.dfilt[k] = g.filt[k];
+  }
+  G.filt_t = (float)g.filt_t;
+  G.alpha = (float)g.alpha;
+  G.huber_T = (float)g.huber_T;
+  G.huber_delta = (float)g.huber_delta;
+  G.dq = g.q;
+  G.dp_s = g.p_s;
+  G.dp_t = g.p_t;
+  G.dc_s = g.c_s;
+  G.dc_t = g.c_t;
+  G.dsq_s = g.sig_s_q;
+  G.dsqp_s = g.sig_s_qp;
+  G.dsq_t = g.sig_t_q;
+  G.dsqp_t = g.sig_t_qp;
+  G.dfilt_t = g.filt_t;
+  G.dhuber_T = g.huber_T;
+  G.dhuber_delta = g.huber_delta;
+  return G;
+}
+
+template <typename T>
+int dev_alloc(T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  HIP_OK(hipMalloc((void**)p, n * sizeof(T)));
+  return 0;
+}
+
+}  // namespace
+
+struct mbir_hip_ctx {
+  mbir_hip_geom g;
+  DevGeom G;
+  hipStream_t stream = nullptr;
+  int nslots = 0, ncol = 0;
+  size_t nsino = 0, nvox = 0;
+  float* d_H = nullptr;
+  double *d_cos = nullptr, *d_sin = nullptr;
+  int *d_frame_ptr = nullptr, *d_frame_views = nullptr, *d_view_frame = nullptr;
+  float *d_y = nullptr, *d_w = nullptr, *d_d = nullptr, *d_e = nullptr;
+  float *d_x = nullptr, *d_stage = nullptr, *d_stage2 = nullptr;
+  int* d_afirst = nullptr;
+  float* d_aval = nullptr;
+  bool a_built = false;
+  int *d_tile_ptr = nullptr, *d_tile_cols = nullptr, *d_launch_tiles = nullptr;
+  int launch_off[5] = {0, 0, 0, 0, 0};
+  int tile = 16, tiles_for = -1;
+  double alpha = -1.0;
+  double* d_stats = nullptr;
+  double* d_partial = nullptr;
+  size_t partial_cap = 0;
+  std::vector<uint8_t> mask;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[MBIR_K_COUNT];
+  size_t ev_used[MBIR_K_COUNT] = {0, 0, 0, 0, 0};
+  double ms[MBIR_K_COUNT] = {0, 0, 0, 0, 0};
+  int64_t launches[MBIR_K_COUNT] = {0, 0, 0, 0, 0};
+};
+
+namespace {
+
+int ev_begin(mbir_hip_ctx* c, int k) {
+  if (c->ev_used[k] == c->ev[k].size()) {
+    hipEvent_t a, b;
+    HIP_OK(hipEventCreate(&a));
+    HIP_OK(hipEventCreate(&b));
+    c->ev[k].push_back({a, b});
+  }
+  HIP_OK(hipEventRecord(c->ev[k][c->ev_used[k]].first, c->stream));
+  return 0;
+}
+
+int ev_end(mbir_hip_ctx* c, int k) {
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipEventRecord(c->ev[k][c->ev_used[k]].second, c->stream));
+  c->ev_used[k]++;
+  c->launches[k]++;
+  return 0;
+}
+
+int ev_collect(mbir_hip_ctx* c) {
+  HIP_OK(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < MBIR_K_COUNT; ++k) {
+    for (size_t i = 0; i < c->ev_used[k]; ++i) {
+      float m = 0.f;
+      HIP_OK(hipEventElapsedTime(&m, c->ev[k][i].first, c->ev[k][i].second));
+      c->ms[k] += m;
+    }
+    c->ev_used[k] = 0;
+  }
+  return 0;
+}
+
+int ensure_partial(mbir_hip_ctx* c, size_t n) {
+  if (n <= c->partial_cap) return 0;
+  if (c->d_partial) HIP_OK(hipFree(c->d_partial));
+  if (dev_alloc(&c->d_partial, n)) return -1;
+  c->partial_cap = n;
+  return 0;
+}
+
+// Tiles of tile x tile columns, masked columns only, grouped by 2x2 colour.
+int build_tiles(mbir_hip_ctx* c) {
+  if (c->tiles_for == c->tile) return 0;
+  const int T = c->tile, Nx = c->g.N_x, Ny = c->g.N_y;
+  const int TX = (Nx + T - 1) / T, TY = (Ny + T - 1) / T;
+  std::vector<int> ptr(1, 0), cols, order[4];
+  for (int ty = 0; ty < TY; ++ty)
+    for (int tx = 0; tx < TX; ++tx) {
+      const int id = ty * TX + tx;
+      for (int yy = ty * T; yy < std::min(Ny, ty * T + T); ++yy)
+        for (int xx = tx * T; xx < std::min(Nx, tx * T + T); ++xx)
+          if (c->mask[(size_t)yy * Nx + xx]) cols.push_back(yy * Nx + xx);
+      ptr.push_back((int)cols.size());
+      if (ptr[id + 1] > ptr[id]) order[(tx % 2) + 2 * (ty % 2)].push_back(id);
+    }
+  std::vector<int> launch;
+  for (int k = 0; k < 4; ++k) {
+    c->launch_off[k] = (int)launch.size();
+    launch.insert(launch.end(), order[k].begin(), order[k].end());
+  }
+  c->launch_off[4] = (int)launch.size();
+  if (c->d_tile_ptr) HIP_OK(hipFree(c->d_tile_ptr));
+  if (c->d_tile_cols) HIP_OK(hipFree(c->d_tile_cols));
+  if (c->d_launch_tiles) HIP_OK(hipFree(c->d_launch_tiles));
+  if (dev_alloc(&c->d_tile_ptr, ptr.size()) || dev_alloc(&c->d_tile_cols, cols.size()) ||
+      dev_alloc(&c->d_launch_tiles, launch.size()))
+    return -1;
+  HIP_OK(hipMemcpy(c->d_tile_ptr, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (!cols.empty())
+    HIP_OK(hipMemcpy(c->d_tile_cols, cols.data(), cols.size() * sizeof(int),
+                     hipMemcpyHostToDevice));
+  if (!launch.empty())
+    HIP_OK(hipMemcpy(c->d_launch_tiles, launch.data(), launch.size() * sizeof(int),
+                     hipMemcpyHostToDevice));
+  c->tiles_for = T;
+  return 0;
+}
+
+size_t fwd_tchunk(const mbir_hip_geom& g, size_t acc_bytes) {
+  const size_t cap = 56 * 1024;  // LDS budget per workgroup (dynamic part)
+  size_t tc = cap / (acc_bytes * (size_t)g.N_r);
+  if (tc > (size_t)g.N_t) tc = g.N_t;
+  if (tc < 1) tc = 1;
+  return tc;
+}
+
+int run_fwdproj(mbir_hip_ctx* c, int mode, const float* x, float* out) {
+  const mbir_hip_geom& g = c->g;
+  const bool dbl = (mode == 1);
+  const size_t tc = fwd_tchunk(g, dbl ? 8 : 4);
+  if ((size_t)g.N_r * (dbl ? 8 : 4) > 56 * 1024)
+    return fail("detector too wide for the LDS projection accumulator");
+  const int nch = (int)((g.N_t + tc - 1) / tc);
+  dim3 grid(g.N_p, nch);
+  const size_t shm = (size_t)g.N_r * tc * (dbl ? 8 : 4);
+  if (mode == 1 && ensure_partial(c, (size_t)g.N_p * nch)) return -1;
+  if (ev_begin(c, mode == 1 ? MBIR_K_COST : MBIR_K_FWDPROJ)) return -1;
+  if (dbl)
+    hipLaunchKernelGGL(k_fwdproj<double>, grid, dim3(256), shm, c->stream, c->G, mode,
+                       (int)tc, c->d_H, c->d_cos, c->d_sin, c->d_view_frame, x, c->d_y,
+                       c->d_w, c->d_d, out, c->d_partial);
+  else
+    hipLaunchKernelGGL(k_fwdproj<float>, grid, dim3(256), shm, c->stream, c->G, mode,
+                       (int)tc, c->d_H, c->d_cos, c->d_sin, c->d_view_frame, x, c->d_y,
+                       c->d_w, c->d_d, out, c->d_partial);
+  return ev_end(c, mode == 1 ? MBIR_K_COST : MBIR_K_FWDPROJ);
+}
+
+int sum_partials(mbir_hip_ctx* c, size_t n, double* out) {
+  std::vector<double> h(n);
+  HIP_OK(hipMemcpyAsync(h.data(), c->d_partial, n * sizeof(double), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  double s = 0.0;
+  for (double v : h) s += v;
+  *out = s;
+  return 0;
+}
+
+int check_geom(const mbir_hip_geom* g) {
+  if (!g) return fail("null geometry");
+  if (g->N_r <= 0 || g->N_t <= 0 || g->N_p <= 0 || g->N_x <= 0 || g->N_y <= 0 ||
+      g->N_z <= 0 || g->N_time <= 0 || g->mult_xy <= 0 || g->mult_z <= 0)
+    return fail("non-positive geometry size");
+  if (g->N_z > kNZMax) return fail("N_z exceeds the sweep kernel's LDS column (128)");
+  if (g->N_t > kNTMax) return fail("N_t exceeds the sweep kernel's LDS rows (256)");
+  if (2 * g->mult_xy + 1 > kWMax) return fail("mult_xy exceeds 16");
+  if ((long long)g->N_z * g->mult_z > g->N_t) return fail("N_z*mult_z exceeds N_t");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mbir_hip_last_error(void) { return g_err.c_str(); }
+
+int mbir_hip_create(mbir_hip_ctx** out, const mbir_hip_geom* g, const float* H,
+                    const double* cosv, const double* sinv, const int32_t* frame_ptr,
+                    const int32_t* frame_views, const float* y, const float* w,
+                    const float* d, const uint8_t* mask) {
+  if (!out) return fail("null output pointer");
+  *out = nullptr;
+  if (check_geom(g)) return -1;
+  if (!H || !cosv || !sinv || !frame_ptr || !frame_views || !y || !w || !mask)
+    return fail("null input array");
+  auto* c = new mbir_hip_ctx();
+  c->g = *g;
+  c->nslots = frame_ptr[g->N_time];
+  c->ncol = g->N_x * g->N_y;
+  c->nsino = (size_t)g->N_p * g->N_r * g->N_t;
+  c->nvox = (size_t)g->N_time * g->N_z * c->ncol;
+  c->G = make_dev_geom(*g, c->nslots);
+  c->mask.assign(mask, mask + c->ncol);
+  for (int f = 0; f < g->N_time; ++f)
+    if (frame_ptr[f + 1] < frame_ptr[f]) {
+      delete c;
+      return fail("frame_ptr not monotone");
+    }
+  std::vector<int> view_frame(g->N_p, -1);
+  for (int f = 0; f < g->N_time; ++f)
+    for (int s = frame_ptr[f]; s < frame_ptr[f + 1]; ++s) {
+      if (frame_views[s] < 0 || frame_views[s] >= g->N_p) {
+        delete c;
+        return fail("frame_views entry out of range");
+      }
+      view_frame[frame_views[s]] = f;
+    }
+  int rc = 0;
+  if (hipStreamCreate(&c->stream) != hipSuccess) rc = fail("hipStreamCreate failed");
+  rc = rc || dev_alloc(&c->d_H, (size_t)g->N_p * kNB) || dev_alloc(&c->d_cos, g->N_p) ||
+       dev_alloc(&c->d_sin, g->N_p) || dev_alloc(&c->d_frame_ptr, g->N_time + 1) ||
+       dev_alloc(&c->d_frame_views, c->nslots) || dev_alloc(&c->d_view_frame, g->N_p) ||
+       dev_alloc(&c->d_y, c->nsino) || dev_alloc(&c->d_w, c->nsino) ||
+       dev_alloc(&c->d_e, c->nsino) || dev_alloc(&c->d_x, c->nvox) ||
+       dev_alloc(&c->d_stage, std::max(c->nvox, c->nsino)) ||
+       dev_alloc(&c->d_stage2, std::max(c->nvox, c->nsino)) ||
+       dev_alloc(&c->d_stats, 4);
+  if (!rc && d) rc = dev_alloc(&c->d_d, (size_t)g->N_r * g->N_t);
+  if (rc) {
+    std::string keep = g_err;
+    mbir_hip_destroy(c);
+    return fail(keep);
+  }
+#define UP(dst, src, n)                                                           \
+  if (hipMemcpy(dst, src, (n) * sizeof(*(src)), hipMemcpyHostToDevice) != hipSuccess) { \
+    mbir_hip_destroy(c);                                                          \
+    return fail("upload of " #src " failed");                                     \
+  }
+  UP(c->d_H, H, (size_t)g->N_p * kNB);
+  UP(c->d_cos, cosv, g->N_p);
+  UP(c->d_sin, sinv, g->N_p);
+  UP(c->d_frame_ptr, frame_ptr, g->N_time + 1);
+  if (c->nslots) UP(c->d_frame_views, frame_views, c->nslots);
+  UP(c->d_view_frame, view_frame.data(), g->N_p);
+  UP(c->d_y, y, c->nsino);
+  UP(c->d_w, w, c->nsino);
+  if (d) UP(c->d_d, d, (size_t)g->N_r * g->N_t);
+#undef UP
+  if (hipMemset(c->d_x, 0, c->nvox * sizeof(float)) != hipSuccess ||
+      hipMemset(c->d_e, 0, c->nsino * sizeof(float)) != hipSuccess) {
+    mbir_hip_destroy(c);
+    return fail("hipMemset failed");
+  }
+  *out = c;
+  return 0;
+}
+
+void mbir_hip_destroy(mbir_hip_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  void* bufs[] = {c->d_H, c->d_cos, c->d_sin, c->d_frame_ptr, c->d_frame_views,
+                  c->d_view_frame, c->d_y, c->d_w, c->d_d, c->d_e, c->d_x,
+                  c->d_stage, c->d_stage2, c->d_afirst, c->d_aval, c->d_tile_ptr,
+                  c->d_tile_cols, c->d_launch_tiles, c->d_stats, c->d_partial};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  for (int k = 0; k < MBIR_K_COUNT; ++k)
+    for (auto& pr : c->ev[k]) {
+      (void)hipEventDestroy(pr.first);
+      (void)hipEventDestroy(pr.second);
+    }
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int mbir_hip_set_option(mbir_hip_ctx* c, const char* key, double value) {
+  if (!c || !key) return fail("null argument");
+  const std::string k(key);
+  if (k == "tile") {
+    if (value < 2 || value > 1024) return fail("tile must be in [2, 1024]");
+    c->tile = (int)value;
+  } else if (k == "alpha") {
+    c->alpha = value;
+  } else {
+    return fail("unknown option " + k);
+  }
+  return 0;
+}
+
+int mbir_hip_set_object(mbir_hip_ctx* c, const float* x) {
+  if (!c || !x) return fail("null argument");
+  HIP_OK(hipMemcpyAsync(c->d_stage, x, c->nvox * sizeof(float), hipMemcpyHostToDevice,
+                        c->stream));
+  const unsigned blocks = (unsigned)((c->nvox + 255) / 256);
+  hipLaunchKernelGGL(k_to_colmajor, dim3(blocks), dim3(256), 0, c->stream, c->g.N_time,
+                     c->g.N_z, c->ncol, c->d_stage, c->d_x);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mbir_hip_get_object(mbir_hip_ctx* c, float* x) {
+  if (!c || !x) return fail("null argument");
+  const unsigned blocks = (unsigned)((c->nvox + 255) / 256);
+  hipLaunchKernelGGL(k_from_colmajor, dim3(blocks), dim3(256), 0, c->stream, c->g.N_time,
+                     c->g.N_z, c->ncol, c->d_x, c->d_stage);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(x, c->d_stage, c->nvox * sizeof(float), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mbir_hip_build_amatrix(mbir_hip_ctx* c) {
+  if (!c) return fail("null context");
+  const size_t n = (size_t)c->ncol * c->nslots;
+  if (!c->d_afirst) {
+    if (dev_alloc(&c->d_afirst, n) || dev_alloc(&c->d_aval, n * c->G.W)) return -1;
+  }
+  if (n == 0) return 0;
+  if (ev_begin(c, MBIR_K_AMATRIX)) return -1;
+  hipLaunchKernelGGL(k_amatrix, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                     c->G, c->d_H, c->d_cos, c->d_sin, c->d_frame_views, c->d_afirst,
+                     c->d_aval);
+  if (ev_end(c, MBIR_K_AMATRIX)) return -1;
+  c->a_built = true;
+  return 0;
+}
+
+int mbir_hip_get_amatrix(mbir_hip_ctx* c, int32_t* first, float* val) {
+  if (!c || !first || !val) return fail("null argument");
+  if (!c->a_built) return fail("A not built");
+  const size_t n = (size_t)c->ncol * c->nslots;
+  HIP_OK(hipMemcpyAsync(first, c->d_afirst, n * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipMemcpyAsync(val, c->d_aval, n * c->G.W * sizeof(float), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mbir_hip_init_error(mbir_hip_ctx* c) {
+  if (!c) return fail("null context");
+  return run_fwdproj(c, 0, c->d_x, c->d_e);
+}
+
+int mbir_hip_get_error(mbir_hip_ctx* c, float* e) {
+  if (!c || !e) return fail("null argument");
+  HIP_OK(hipMemcpyAsync(e, c->d_e, c->nsino * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mbir_hip_set_error(mbir_hip_ctx* c, const float* e) {
+  if (!c || !e) return fail("null argument");
+  HIP_OK(hipMemcpyAsync(c->d_e, e, c->nsino * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mbir_hip_forward_project(mbir_hip_ctx* c, const float* x, float* ax) {
+  if (!c || !x || !ax) return fail("null argument");
+  HIP_OK(hipMemcpyAsync(c->d_stage, x, c->nvox * sizeof(float), hipMemcpyHostToDevice,
+                        c->stream));
+  const unsigned blocks = (unsigned)((c->nvox + 255) / 256);
+  hipLaunchKernelGGL(k_to_colmajor, dim3(blocks), dim3(256), 0, c->stream, c->g.N_time,
+                     c->g.N_z, c->ncol, c->d_stage, c->d_stage2);
+  HIP_OK(hipGetLastError());
+  if (run_fwdproj(c, 2, c->d_stage2, c->d_stage)) return -1;
+  HIP_OK(hipMemcpyAsync(ax, c->d_stage, c->nsino * sizeof(float), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mbir_hip_backproject(mbir_hip_ctx* c, const float* v, float* bt) {
+  if (!c || !v || !bt) return fail("null argument");
+  if (!c->a_built) return fail("A not built");
+  HIP_OK(hipMemcpyAsync(c->d_stage, v, c->nsino * sizeof(float), hipMemcpyHostToDevice,
+                        c->stream));
+  const long long n = (long long)c->ncol * c->g.N_time;
+  if (ev_begin(c, MBIR_K_BACKPROJ)) return -1;
+  hipLaunchKernelGGL(k_backproj, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                     c->G, c->d_aval, c->d_afirst, c->d_frame_ptr, c->d_frame_views,
+                     c->d_stage, c->d_stage2);
+  if (ev_end(c, MBIR_K_BACKPROJ)) return -1;
+  const unsigned blocks = (unsigned)((c->nvox + 255) / 256);
+  hipLaunchKernelGGL(k_from_colmajor, dim3(blocks), dim3(256), 0, c->stream, c->g.N_time,
+                     c->g.N_z, c->ncol, c->d_stage2, c->d_stage);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(bt, c->d_stage, c->nvox * sizeof(float), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mbir_hip_icd_equits(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t seed,
+                        double* stats) {
+  if (!c) return fail("null context");
+  if (!c->a_built) return fail("A not built");
+  if (build_tiles(c)) return -1;
+  HIP_OK(hipMemsetAsync(c->d_stats, 0, 4 * sizeof(double), c->stream));
+  SweepArgs A;
+  A.col.aval = c->d_aval;
+  A.col.afirst = c->d_afirst;
+  A.col.frame_views = c->d_frame_views;
+  A.col.w = c->d_w;
+  A.col.e = c->d_e;
+  A.frame_ptr = c->d_frame_ptr;
+  A.tile_ptr = c->d_tile_ptr;
+  A.tile_cols = c->d_tile_cols;
+  A.x = c->d_x;
+  A.stats = c->d_stats;
+  A.alpha = (float)(c->alpha > 0 ? c->alpha : c->g.alpha);
+  const int nf = c->g.N_time;
+  for (int it = 0; it < n; ++it) {
+    A.iter = first_iter + it;
+    for (int colour = 0; colour < 4; ++colour) {
+      const int ntl = c->launch_off[colour + 1] - c->launch_off[colour];
+      if (ntl == 0) continue;
+      A.launch_tiles = c->d_launch_tiles + c->launch_off[colour];
+      A.n_launch_tiles = ntl;
+      for (int fp = 0; fp < 2; ++fp) {
+        const int frames = (nf - fp + 1) / 2;
+        if (frames <= 0) continue;
+        A.fparity = fp;
+        A.seed = seed * 2654435761u + (unsigned)(it * 8 + colour * 2 + fp) * 40503u;
+        if (ev_begin(c, MBIR_K_SWEEP)) return -1;
+        hipLaunchKernelGGL(k_sweep, dim3((unsigned)(ntl * frames)), dim3(kWave), 0,
+                           c->stream, c->G, A);
+        if (ev_end(c, MBIR_K_SWEEP)) return -1;
+      }
+    }
+  }
+  if (stats) {
+    HIP_OK(hipMemcpyAsync(stats, c->d_stats, 4 * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+  }
+  return 0;
+}
+
+int mbir_hip_cost(mbir_hip_ctx* c, int32_t fresh, double* out) {
+  if (!c || !out) return fail("null argument");
+  const mbir_hip_geom& g = c->g;
+  double data = 0.0, prior = 0.0;
+  if (fresh) {
+    if (run_fwdproj(c, 1, c->d_x, nullptr)) return -1;
+    const size_t nch = (g.N_t + fwd_tchunk(g, 8) - 1) / fwd_tchunk(g, 8);
+    if (sum_partials(c, (size_t)g.N_p * nch, &data)) return -1;
+  } else {
+    const unsigned blocks = 1024;
+    if (ensure_partial(c, blocks)) return -1;
+    if (ev_begin(c, MBIR_K_COST)) return -1;
+    hipLaunchKernelGGL(k_data_cost, dim3(blocks), dim3(256), 0, c->stream, c->G, c->nsino,
+                       c->d_e, c->d_w, c->d_partial);
+    if (ev_end(c, MBIR_K_COST)) return -1;
+    if (sum_partials(c, blocks, &data)) return -1;
+  }
+  data /= 2.0;
+  data += (double)g.N_p * (double)g.N_r * (double)g.N_t * log(g.var_est) / 2;
+  {
+    const unsigned blocks = 1024;
+    if (ensure_partial(c, blocks)) return -1;
+    if (ev_begin(c, MBIR_K_COST)) return -1;
+    hipLaunchKernelGGL(k_prior_cost, dim3(blocks), dim3(256), 0, c->stream, c->G, c->d_x,
+                       c->d_partial);
+    if (ev_end(c, MBIR_K_COST)) return -1;
+    if (sum_partials(c, blocks, &prior)) return -1;
+  }
+  out[0] = data + prior;
+  out[1] = data;
+  out[2] = prior;
+  return 0;
+}
+
+int mbir_hip_substitution_batch(const mbir_hip_geom* g, int32_t n, const float* V,
+                                const float* theta1, const float* theta2, const float* nb,
+                                const uint8_t* nbf, const float* tn, const uint8_t* tf,
+                                float* out) {
+  if (!g || n < 0 || (n > 0 && (!V || !theta1 || !theta2 || !nb || !nbf || !tn || !tf || !out)))
+    return fail("bad argument");
+  if (n == 0) return 0;
+  DevGeom G = make_dev_geom(*g, 0);
+  float *dV, *d1, *d2, *dnb, *dtn, *dout;
+  unsigned char *dnbf, *dtf;
+  if (dev_alloc(&dV, n) || dev_alloc(&d1, n) || dev_alloc(&d2, n) ||
+      dev_alloc(&dnb, (size_t)n * 27) || dev_alloc(&dtn, (size_t)n * 2) || dev_alloc(&dout, n) ||
+      dev_alloc(&dnbf, (size_t)n * 27) || dev_alloc(&dtf, (size_t)n * 2))
+    return -1;
+  HIP_OK(hipMemcpy(dV, V, n * 4, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(d1, theta1, n * 4, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(d2, theta2, n * 4, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(dnb, nb, (size_t)n * 27 * 4, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(dtn, tn, (size_t)n * 2 * 4, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(dnbf, nbf, (size_t)n * 27, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(dtf, tf, (size_t)n * 2, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_subst, dim3((unsigned)((n + 3) / 4)), dim3(kWave), 0, 0, G, n, dV, d1, d2,
+                     dnb, dnbf, dtn, dtf, dout);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpy(out, dout, n * 4, hipMemcpyDeviceToHost));
+  void* bufs[] = {dV, d1, d2, dnb, dtn, dout, dnbf, dtf};
+  for (void* b : bufs) HIP_OK(hipFree(b));
+  return 0;
+}
+
+int mbir_hip_voxel_thetas(mbir_hip_ctx* c, int32_t n, const int32_t* tzyx, float* theta1,
+                          float* theta2) {
+  if (!c || n < 0 || (n > 0 && (!tzyx || !theta1 || !theta2))) return fail("bad argument");
+  if (!c->a_built) return fail("A not built");
+  if (n == 0) return 0;
+  for (int i = 0; i < n; ++i) {
+    const int32_t* q = tzyx + 4 * i;
+    if (q[0] < 0 || q[0] >= c->g.N_time || q[1] < 0 || q[1] >= c->g.N_z || q[2] < 0 ||
+        q[2] >= c->g.N_y || q[3] < 0 || q[3] >= c->g.N_x)
+      return fail("voxel index out of range");
+  }
+  int* dq;
+  float *d1, *d2;
+  if (dev_alloc(&dq, (size_t)n * 4) || dev_alloc(&d1, n) || dev_alloc(&d2, n)) return -1;
+  HIP_OK(hipMemcpyAsync(dq, tzyx, (size_t)n * 16, hipMemcpyHostToDevice, c->stream));
+  ColumnArgs A;
+  A.aval = c->d_aval;
+  A.afirst = c->d_afirst;
+  A.frame_views = c->d_frame_views;
+  A.w = c->d_w;
+  A.e = c->d_e;
+  hipLaunchKernelGGL(k_thetas, dim3(n), dim3(kWave), 0, c->stream, c->G, A, c->d_frame_ptr, dq,
+                     d1, d2);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(theta1, d1, n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipMemcpyAsync(theta2, d2, n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  HIP_OK(hipFree(dq));
+  HIP_OK(hipFree(d1));
+  HIP_OK(hipFree(d2));
+  return 0;
+}
+
+int mbir_hip_kernel_time(mbir_hip_ctx* c, int32_t k, double* ms, int64_t* launches) {
+  if (!c || k < 0 || k >= MBIR_K_COUNT) return fail("bad argument");
+  if (ev_collect(c)) return -1;
+  if (ms) *ms = c->ms[k];
+  if (launches) *launches = c->launches[k];
+  return 0;
+}
+
+int mbir_hip_reset_kernel_time(mbir_hip_ctx* c) {
+  if (!c) return fail("null context");
+  if (ev_collect(c)) return -1;
+  for (int k = 0; k < MBIR_K_COUNT; ++k) {
+    c->ms[k] = 0;
+    c->launches[k] = 0;
+  }
+  return 0;
+}
+
+int mbir_hip_synchronize(mbir_hip_ctx* c) {
+  if (!c) return fail("null context");
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,101 @@
+"""ctypes binding of the in-tree native library phasecon_amd/libmbir4d.so.
+
+The library exports the drop-in ``reconstruct`` (include/mbir4d.h), the
+engine helpers and ``reconstruct_ex`` (include/mbir_engine.h) and the HIP
+C-ABI (include/mbir_hip.h).  There is no fallback: if the library is not
+built, ``load()`` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libmbir4d.so")
+
+K_AMATRIX, K_FWDPROJ, K_SWEEP, K_COST, K_BACKPROJ = range(5)
+
+
+class HipGeom(C.Structure):
+    """mirror of mbir_hip_geom (include/mbir_hip.h)"""
+    _fields_ = [(n, C.c_int32) for n in ("N_r", "N_t", "N_p", "N_x", "N_y", "N_z",
+                                          "N_time", "mult_xy", "mult_z")] + \
+        [(n, C.c_double) for n in ("delta_r", "delta_t", "R0", "RMax", "x0", "y0",
+                                   "delta_xy", "OffsetR", "q", "p_s", "p_t", "c_s",
+                                   "c_t", "sig_s_q", "sig_s_qp", "sig_t_q", "sig_t_qp")] + \
+        [("filt", C.c_double * 27), ("filt_t", C.c_double), ("alpha", C.c_double),
+         ("huber_T", C.c_double), ("huber_delta", C.c_double), ("var_est", C.c_double)]
+
+
+class Options(C.Structure):
+    """mirror of mbir_options (include/mbir_engine.h)"""
+    _fields_ = [("p_s", C.c_double), ("p_t", C.c_double), ("tile", C.c_int32),
+                ("alpha", C.c_double), ("max_equits", C.c_int32),
+                ("single_stage", C.c_int32), ("write_files", C.c_int32),
+                ("stop_cost", C.c_double), ("seed", C.c_uint32)]
+
+
+_LIB = None
+
+# name -> (restype, argtypes); None argtypes = unchecked pointers
+_SIGS = {
+    "reconstruct": (C.c_int, None),
+    "reconstruct_ex": (C.c_int, None),
+    "mbir_default_options": (None, [C.POINTER(Options)]),
+    "mbir_multires_schedule": (C.c_int32, None),
+    "mbir_view_map": (C.c_int32, None),
+    "mbir_stage_geometry": (C.c_int, None),
+    "mbir_detector_response": (None, None),
+    "mbir_sincos": (None, None),
+    "mbir_update_mask": (None, None),
+    "mbir_sino_to_internal": (None, None),
+    "mbir_hip_last_error": (C.c_char_p, []),
+    "mbir_hip_create": (C.c_int, None),
+    "mbir_hip_destroy": (None, [C.c_void_p]),
+    "mbir_hip_set_option": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),
+    "mbir_hip_set_object": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "mbir_hip_get_object": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "mbir_hip_build_amatrix": (C.c_int, [C.c_void_p]),
+    "mbir_hip_get_amatrix": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mbir_hip_init_error": (C.c_int, [C.c_void_p]),
+    "mbir_hip_get_error": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "mbir_hip_set_error": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "mbir_hip_forward_project": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mbir_hip_backproject": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mbir_hip_icd_equits": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_uint32, C.c_void_p]),
+    "mbir_hip_cost": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
+    "mbir_hip_substitution_batch": (C.c_int, None),
+    "mbir_hip_voxel_thetas": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mbir_hip_kernel_time": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    "mbir_hip_reset_kernel_time": (C.c_int, [C.c_void_p]),
+    "mbir_hip_synchronize": (C.c_int, [C.c_void_p]),
+}
+
+
+def exported_names():
+    return list(_SIGS)
+
+
+def load():
+    """Loads libmbir4d.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run `python -m phasecon_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            if args is not None:
+                fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def last_error():
+    return load().mbir_hip_last_error().decode(errors="replace")
+
+
+def check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed: {last_error()}")

@@ -1,0 +1,256 @@
+"""Python mirror of the MBIR engine's interface.
+
+``reconstruct`` calls the native drop-in (include/mbir4d.h, via
+reconstruct_ex for the extra options) with the reference's argument list;
+``Stage`` drives one multi-resolution stage through the HIP C-ABI
+(include/mbir_hip.h) for tests, the benchmark and the warm start.
+
+Array conventions follow the reference: projections/weights in file layout
+[view][row][col]; objects [frame][z][y][x]; sinogram-shaped stage arrays in
+the internal [view][col r][row t] layout (MBIR_4D/XT_Init.c:294-301).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import lib as _lib
+
+NB = 65
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def multires_schedule(proj_rows, proj_cols):
+    L = _lib.load()
+    mxy = np.zeros(8, np.int32)
+    mz = np.zeros(8, np.int32)
+    n = L.mbir_multires_schedule(C.c_int32(proj_rows), C.c_int32(proj_cols), _p(mxy), _p(mz))
+    return [(int(mxy[i]), int(mz[i])) for i in range(n)]
+
+
+def view_map(proj_times, recon_times):
+    L = _lib.load()
+    t = np.ascontiguousarray(proj_times, np.float32)
+    r = np.ascontiguousarray(recon_times, np.float32)
+    ptr = np.zeros(len(r), np.int32)
+    views = np.zeros(max(len(t), 1), np.int32)
+    n = L.mbir_view_map(C.c_int32(len(t)), _p(t), C.c_int32(len(r) - 1), _p(r), _p(ptr), _p(views))
+    if n < 0:
+        raise ValueError("projection or reconstruction times decrease")
+    return ptr, views[:n].copy()
+
+
+class Stage:
+    """One multi-resolution stage of one scan, resident on the GPU."""
+
+    def __init__(self, scan, mult_xy=1, mult_z=1, quad_convex=1, huber_delta=0.5,
+                 huber_T=4.0, p_s=None, p_t=None, tile=16, alpha=None, var_est=1.0):
+        L = self.L = _lib.load()
+        s = self.scan = scan
+        N_p, N_t, N_r = s["proj_num"], s["proj_rows"], s["proj_cols"]
+        self.frame_ptr, self.frame_views = view_map(s["proj_times"], s["recon_times"])
+        self.frame_views = np.ascontiguousarray(np.resize(self.frame_views, max(len(self.frame_views), 1)))
+        g = self.g = _lib.HipGeom()
+        f = C.c_float
+        rc = L.mbir_stage_geometry(
+            C.byref(g), C.c_int32(N_t), C.c_int32(N_r), C.c_int32(N_p), C.c_int32(s["recon_num"]),
+            f(s["vox_wid"]), f(s["rot_center"]), f(s["sig_s"]), f(s["sig_t"]), f(s["c_s"]),
+            f(s["c_t"]), C.c_int32(mult_xy), C.c_int32(mult_z), C.c_int32(quad_convex),
+            f(huber_delta), f(huber_T), _p(self.frame_ptr),
+            C.c_double(s["p_s"] if p_s is None else p_s),
+            C.c_double(s["p_t"] if p_t is None else p_t))
+        if rc:
+            raise ValueError("mbir_stage_geometry rejected the geometry")
+        g.var_est = var_est
+        self.angles = np.ascontiguousarray(s["angles"], np.float32)
+        self.cos = np.zeros(N_p, np.float64)
+        self.sin = np.zeros(N_p, np.float64)
+        L.mbir_sincos(C.c_int32(N_p), _p(self.angles), _p(self.cos), _p(self.sin))
+        self.H = np.zeros((N_p, NB), np.float32)
+        L.mbir_detector_response(C.byref(g), _p(self.angles), _p(self.H))
+        self.y = self._internal(s["projections"])
+        self.w = (self._internal(s["weights"]) / np.float32(var_est)).astype(np.float32)
+        self.mask = np.zeros((g.N_y, g.N_x), np.uint8)
+        L.mbir_update_mask(C.byref(g), C.c_double(float(s["vox_wid"]) * N_r / 2), _p(self.mask))
+        h = C.c_void_p()
+        _lib.check(L.mbir_hip_create(C.byref(h), C.byref(g), _p(self.H), _p(self.cos),
+                                     _p(self.sin), _p(self.frame_ptr), _p(self.frame_views),
+                                     _p(self.y), _p(self.w), None, _p(self.mask)),
+                   "mbir_hip_create")
+        self.h = h
+        self.option("tile", tile)
+        if alpha is not None:
+            self.option("alpha", alpha)
+
+    def _internal(self, a):
+        s = self.scan
+        N_p, N_t, N_r = s["proj_num"], s["proj_rows"], s["proj_cols"]
+        a = np.ascontiguousarray(a, np.float32).reshape(N_p, N_t, N_r)
+        out = np.zeros((N_p, N_r, N_t), np.float32)
+        self.L.mbir_sino_to_internal(C.c_int32(N_p), C.c_int32(N_t), C.c_int32(N_r), _p(a), _p(out))
+        return out
+
+    # -- sizes
+    @property
+    def shape_x(self):
+        g = self.g
+        return (g.N_time, g.N_z, g.N_y, g.N_x)
+
+    @property
+    def shape_e(self):
+        g = self.g
+        return (g.N_p, g.N_r, g.N_t)
+
+    @property
+    def W(self):
+        return 2 * self.g.mult_xy + 1
+
+    # -- calls
+    def option(self, key, value):
+        _lib.check(self.L.mbir_hip_set_option(self.h, key.encode(), float(value)), "set_option")
+
+    def set_object(self, x):
+        x = np.ascontiguousarray(x, np.float32).reshape(self.shape_x)
+        _lib.check(self.L.mbir_hip_set_object(self.h, _p(x)), "set_object")
+
+    def get_object(self):
+        x = np.zeros(self.shape_x, np.float32)
+        _lib.check(self.L.mbir_hip_get_object(self.h, _p(x)), "get_object")
+        return x
+
+    def build_amatrix(self):
+        _lib.check(self.L.mbir_hip_build_amatrix(self.h), "build_amatrix")
+
+    def get_amatrix(self):
+        g = self.g
+        n = g.N_x * g.N_y
+        ns = int(self.frame_ptr[-1])
+        first = np.zeros((n, max(ns, 1)), np.int32)
+        val = np.zeros((n, max(ns, 1), self.W), np.float32)
+        _lib.check(self.L.mbir_hip_get_amatrix(self.h, _p(first), _p(val)), "get_amatrix")
+        return first[:, :ns], val[:, :ns]
+
+    def init_error(self):
+        _lib.check(self.L.mbir_hip_init_error(self.h), "init_error")
+
+    def get_error(self):
+        e = np.zeros(self.shape_e, np.float32)
+        _lib.check(self.L.mbir_hip_get_error(self.h, _p(e)), "get_error")
+        return e
+
+    def set_error(self, e):
+        e = np.ascontiguousarray(e, np.float32).reshape(self.shape_e)
+        _lib.check(self.L.mbir_hip_set_error(self.h, _p(e)), "set_error")
+
+    def forward_project(self, x):
+        x = np.ascontiguousarray(x, np.float32).reshape(self.shape_x)
+        ax = np.zeros(self.shape_e, np.float32)
+        _lib.check(self.L.mbir_hip_forward_project(self.h, _p(x), _p(ax)), "forward_project")
+        return ax
+
+    def backproject(self, v):
+        v = np.ascontiguousarray(v, np.float32).reshape(self.shape_e)
+        bt = np.zeros(self.shape_x, np.float32)
+        _lib.check(self.L.mbir_hip_backproject(self.h, _p(v), _p(bt)), "backproject")
+        return bt
+
+    def icd(self, n, first_iter, seed=761521):
+        st = np.zeros(4, np.float64)
+        _lib.check(self.L.mbir_hip_icd_equits(self.h, n, first_iter, seed, _p(st)), "icd_equits")
+        return st
+
+    def cost(self, fresh=False):
+        out = np.zeros(3, np.float64)
+        _lib.check(self.L.mbir_hip_cost(self.h, int(bool(fresh)), _p(out)), "cost")
+        return out
+
+    def thetas(self, tzyx):
+        q = np.ascontiguousarray(tzyx, np.int32).reshape(-1, 4)
+        t1 = np.zeros(len(q), np.float32)
+        t2 = np.zeros(len(q), np.float32)
+        _lib.check(self.L.mbir_hip_voxel_thetas(self.h, len(q), _p(q), _p(t1), _p(t2)), "voxel_thetas")
+        return t1, t2
+
+    def kernel_time(self, k):
+        ms = C.c_double()
+        n = C.c_int64()
+        _lib.check(self.L.mbir_hip_kernel_time(self.h, k, C.byref(ms), C.byref(n)), "kernel_time")
+        return ms.value, n.value
+
+    def reset_kernel_time(self):
+        _lib.check(self.L.mbir_hip_reset_kernel_time(self.h), "reset_kernel_time")
+
+    def synchronize(self):
+        _lib.check(self.L.mbir_hip_synchronize(self.h), "synchronize")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.mbir_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def substitution_batch(g, V, th1, th2, nb, nbf, tn, tf):
+    """Batched functional substitution on the GPU (FP32)."""
+    L = _lib.load()
+    n = len(V)
+    arr = [np.ascontiguousarray(a, np.float32) for a in (V, th1, th2, nb, tn)]
+    nbf = np.ascontiguousarray(nbf, np.uint8)
+    tf = np.ascontiguousarray(tf, np.uint8)
+    out = np.zeros(n, np.float32)
+    _lib.check(L.mbir_hip_substitution_batch(C.byref(g), C.c_int32(n), _p(arr[0]), _p(arr[1]),
+                                             _p(arr[2]), _p(arr[3]), _p(nbf), _p(arr[4]),
+                                             _p(tf), _p(out)), "substitution_batch")
+    return out
+
+
+def default_options():
+    o = _lib.Options()
+    _lib.load().mbir_default_options(C.byref(o))
+    return o
+
+
+def reconstruct(scan, convg_thresh=1.0, remove_rings=0, quad_convex=1, huber_delta=0.5,
+                huber_T=4.0, restart=0, options=None, log_path=None):
+    """Runs the native reconstruct_ex(); returns (object with halo slices,
+    equits of the last stage, final cost)."""
+    L = _lib.load()
+    s = scan
+    keep = [np.ascontiguousarray(s[k], np.float32)
+            for k in ("projections", "weights", "angles", "proj_times", "recon_times")]
+    obj = C.POINTER(C.c_float)()
+    libc = C.CDLL(None)
+    libc.fopen.restype = C.c_void_p
+    libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+    libc.fclose.argtypes = [C.c_void_p]
+    fp = libc.fopen((log_path or "/dev/null").encode(), b"w")
+    opt = options or default_options()
+    eq = C.c_int32()
+    cost = C.c_double()
+    f = C.c_float
+    try:
+        rc = L.reconstruct_ex(C.byref(obj), *[_p(a) for a in keep], C.c_int32(s["proj_rows"]),
+                              C.c_int32(s["proj_cols"]), C.c_int32(s["proj_num"]),
+                              C.c_int32(s["recon_num"]), f(s["vox_wid"]), f(s["rot_center"]),
+                              f(s["sig_s"]), f(s["sig_t"]), f(s["c_s"]), f(s["c_t"]),
+                              f(convg_thresh), C.c_int32(remove_rings), C.c_int32(quad_convex),
+                              f(huber_delta), f(huber_T), C.c_uint8(restart), C.c_void_p(fp),
+                              C.byref(opt), C.byref(eq), C.byref(cost))
+    finally:
+        libc.fclose(C.c_void_p(fp))
+    if rc != 0:
+        raise RuntimeError(f"reconstruct returned {rc}: {_lib.last_error()}")
+    shape = (s["recon_num"], s["proj_rows"] + 2, s["proj_cols"], s["proj_cols"])
+    out = np.ctypeslib.as_array(obj, shape=(int(np.prod(shape)),)).reshape(shape).copy()
+    libc.free.argtypes = [C.c_void_p]
+    libc.free(C.cast(obj, C.c_void_p))
+    return out, eq.value, cost.value
