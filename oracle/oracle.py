@@ -108,7 +108,8 @@ class Oracle:
         idx = np.zeros(256, np.int32)
         val = np.zeros(256, np.float64)
         n = self.L.orc_amatrix_column(C.byref(self.g), self._p(self.cos), self._p(self.sin),
-                                      self._p(self.H), row, col, view, self._p(idx), self._p(val))
+                                      self._p(self.H), int(row), int(col), int(view), self._p(idx),
+                                      self._p(val))
         return idx[:n].copy(), val[:n].copy()
 
     def amatrix_all(self):

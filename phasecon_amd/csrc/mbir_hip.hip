@@ -137,8 +137,14 @@ __global__ __launch_bounds__(256) void k_amatrix(DevGeom G,
   const int p = slot_view[s];
   float* dst = aval + i * G.W;
   for (int k = 0; k < G.W; ++k) dst[k] = 0.f;
-  afirst[i] = amatrix_visit(G, H + (size_t)p * kNB, cosv[p], sinv[p], c / G.N_x,
-                            c % G.N_x, [&](int, int k, double v) { dst[k] = (float)v; });
+  int kept = 0;
+  const int jlo = amatrix_visit(G, H + (size_t)p * kNB, cosv[p], sinv[p], c / G.N_x, c % G.N_x,
+                                [&](int, int k, double v) {
+                                  dst[k] = (float)v;
+                                  ++kept;
+                                });
+  // a column with no kept bin is empty (count 0 in XT_AMatrix.c:109)
+  afirst[i] = kept ? jlo : -1;
 }
 
 // ----------------------------------------------------------- projection

@@ -1,0 +1,343 @@
+"""Pins the FP64 oracle (oracle/mbir_oracle.c) before it is trusted.
+
+Three kinds of check, per SURVEY-listed function:
+  * golden vectors written by tests/golden/make_golden.py from the reference
+    build (bit-exact where the reference is exact, float-rounding otherwise);
+  * the live reference build in oracle/_ref when present (skipped otherwise);
+  * closed forms and invariants that a plausible mistake would break.
+CPU only.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, rel_err
+
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+# randlib.c:69-92: the first 20 states of the generator from seed 1
+RANDLIB_KAT = [16807, 282475249, 1622650073, 984943658, 1144108930, 470211272,
+               101027544, 1457850878, 1458777923, 2007237709, 823564440, 1115438165,
+               1784484492, 74243042, 114807987, 1137522503, 1441282327, 16531729,
+               823378840, 143542612]
+
+
+@pytest.fixture(scope="module")
+def gold3d():
+    return dict(np.load(os.path.join(GOLD, "mbir_tiny3d.npz")))
+
+
+@pytest.fixture(scope="module")
+def gold4d():
+    return dict(np.load(os.path.join(GOLD, "mbir_ch4d_small.npz")))
+
+
+@pytest.fixture(scope="module")
+def orc3d(tiny_scan):
+    from oracle.oracle import Oracle
+    return Oracle(tiny_scan)
+
+
+@pytest.fixture(scope="module")
+def orc4d(small4d_scan):
+    from oracle.oracle import Oracle
+    return Oracle(small4d_scan)
+
+
+def _geom_vector(o):
+    g = o.g
+    return np.array([g.delta_r, g.delta_t, g.R0, g.RMax, g.T0, g.TMax, g.OffsetR, g.OffsetT,
+                     g.x0, g.y0, g.z0, g.delta_xy, g.delta_z, g.radius_obj, g.delta_recon,
+                     g.sig_s_q, g.sig_s_qp, g.sig_t_q, g.sig_t_qp, g.filt_t, g.alpha,
+                     g.huber_T, g.huber_delta, g.var_est] + list(g.filt))
+
+
+def test_random2_known_answers():
+    from oracle.oracle import Oracle, _lib, ORACLE_SO
+    import ctypes as C
+    L = _lib(ORACLE_SO)
+    L.orc_random2.restype = C.c_double
+    L.orc_srandom2(C.c_uint32(1))
+    got = [round(L.orc_random2() * 2147483647) for _ in range(20)]
+    assert got == RANDLIB_KAT
+
+
+@pytest.mark.parametrize("which", ["3d", "4d"])
+def test_geometry_matches_reference(which, gold3d, gold4d, orc3d, orc4d):
+    gold, o = (gold3d, orc3d) if which == "3d" else (gold4d, orc4d)
+    g = o.g
+    assert [g.N_r, g.N_t, g.N_p, g.N_x, g.N_y, g.N_z, g.N_time] == list(gold["ints"][:7])
+    assert np.array_equal(_geom_vector(o), gold["dbls"])
+
+
+def test_filter_closed_form(orc4d):
+    g = orc4d.g
+    # weights 1/distance normalised over 26 spatial + 2 temporal neighbours,
+    # times mult_xy^3 * mean views per frame
+    total = sum(g.filt) + 2 * g.filt_t
+    assert abs(total - g.mult_xy ** 3 * g.delta_recon) < 1e-12 * total
+    f = np.array(g.filt).reshape(3, 3, 3)
+    assert f[1, 1, 1] == 0 and abs(f[0, 0, 0] * np.sqrt(3) - f[1, 1, 0]) < 1e-15
+
+
+@pytest.mark.parametrize("which", ["3d", "4d"])
+def test_view_map_matches_reference(which, gold3d, gold4d, orc3d, orc4d):
+    gold, o = (gold3d, orc3d) if which == "3d" else (gold4d, orc4d)
+    n = gold["frame_ptr"][-1]
+    assert np.array_equal(o.frame_ptr, gold["frame_ptr"])
+    assert np.array_equal(o.frame_views[:n], gold["frame_views"][:n])
+
+
+def test_view_map_examples():
+    """Reference example metadata: half-open frames drop the last Shepp-Logan
+    view (time 256 = last edge); the Cahn-Hilliard run has 32 views/frame."""
+    import ctypes as C
+    from oracle.oracle import _lib, ORACLE_SO
+    L = _lib(ORACLE_SO)
+    ex = os.path.join(GOLD, "examples")
+    for pre, nframes, expect in (("shepp", 1, [255]), ("ch", 32, [32] * 32)):
+        t = np.fromfile(os.path.join(ex, f"{pre}_proj_times.bin"), np.float32)
+        r = np.fromfile(os.path.join(ex, f"{pre}_recon_times.bin"), np.float32)
+        assert len(r) == nframes + 1
+        ptr = np.zeros(nframes + 1, np.int32)
+        views = np.zeros(len(t), np.int32)
+        L.orc_view_map(C.c_int32(len(t)), t.ctypes.data_as(C.c_void_p), C.c_int32(nframes),
+                       r.ctypes.data_as(C.c_void_p), ptr.ctypes.data_as(C.c_void_p),
+                       views.ctypes.data_as(C.c_void_p))
+        assert np.diff(ptr).tolist() == expect
+
+
+@pytest.mark.parametrize("which", ["3d", "4d"])
+def test_detector_response_bit_exact(which, gold3d, gold4d, orc3d, orc4d):
+    gold, o = (gold3d, orc3d) if which == "3d" else (gold4d, orc4d)
+    assert np.array_equal(o.H, gold["H"])
+
+
+def test_detector_response_area(orc3d):
+    """The tabulated response integrates (in r) to the voxel area."""
+    o = orc3d
+    g = o.g
+    for p in range(g.N_p):
+        h = o.H[p, :64].astype(np.float64)
+        r = np.arange(64) * g.OffsetR
+        area = 2 * np.trapezoid(h, r)
+        assert abs(area - g.delta_xy ** 2) < 2e-2 * g.delta_xy ** 2
+
+
+@pytest.mark.parametrize("which", ["3d", "4d"])
+def test_amatrix_bit_exact(which, gold3d, gold4d, orc3d, orc4d):
+    gold, o = (gold3d, orc3d) if which == "3d" else (gold4d, orc4d)
+    g = o.g
+    for i, c in enumerate(gold["a_cols"]):
+        for p in range(g.N_p):
+            idx, val = o.amatrix_column(int(c // g.N_x), int(c % g.N_x), p)
+            n = len(idx)
+            assert np.array_equal(idx, gold["a_idx"][i, p, :n])
+            assert np.all(gold["a_idx"][i, p, n:] == -1)
+            assert np.array_equal(val, gold["a_val"][i, p, :n])
+
+
+def test_amatrix_area_invariant(orc3d):
+    """sum_j A_ij * delta_r is the voxel area delta_xy^2 on average over the
+    sub-pixel positions of the voxels in the field (each view), and never
+    off by more than the sampling of a 2*delta_xy-wide profile at pitch
+    delta_r allows."""
+    o = orc3d
+    g = o.g
+    first, val = o.amatrix_all()
+    s = val.sum(axis=2)[o.mask.ravel().astype(bool)] * g.delta_r
+    assert np.all(np.abs(s.mean(axis=0) - g.delta_xy ** 2) < 5e-3)
+    assert s.min() > 0.5 and s.max() < 1.15
+
+
+@pytest.mark.parametrize("which", ["3d", "4d"])
+def test_forward_projection_matches_reference(which, gold3d, gold4d, orc3d, orc4d):
+    gold, o = (gold3d, orc3d) if which == "3d" else (gold4d, orc4d)
+    e = o.init_error(gold["x"].astype(np.float64))
+    # the reference accumulates A x in float
+    assert rel_err(e, gold["e"]) < 2e-6
+
+
+def test_forward_projection_adjoint(orc4d):
+    """<A x, v> = <x, A^T v> with A^T built from the oracle's own columns."""
+    o = orc4d
+    g = o.g
+    rng = np.random.default_rng(0)
+    x = rng.random(o.shape_x)
+    v = rng.standard_normal((g.N_p, g.N_r, g.N_t))
+    lhs = np.sum(o.forward_project(x) * v)
+    atv = np.zeros(o.shape_x)
+    for f in range(g.N_time):
+        for s in range(o.frame_ptr[f], o.frame_ptr[f + 1]):
+            p = o.frame_views[s]
+            for yy in range(g.N_y):
+                for xx in range(g.N_x):
+                    idx, val = o.amatrix_column(yy, xx, p)
+                    for z in range(g.N_z):
+                        atv[f, z, yy, xx] += np.sum(val * v[p, idx, z])
+    rhs = np.sum(x * atv)
+    assert abs(lhs - rhs) < 1e-10 * abs(lhs)
+
+
+@pytest.mark.parametrize("which", ["3d", "4d"])
+def test_cost_matches_reference(which, gold3d, gold4d, orc3d, orc4d):
+    gold, o = (gold3d, orc3d) if which == "3d" else (gold4d, orc4d)
+    c, parts = o.cost(gold["e"].astype(np.float64), gold["x"].astype(np.float64))
+    # the reference differences neighbours in float: 1e-8 relative
+    assert abs(c - gold["cost"][0]) < 1e-8 * gold["cost"][0]
+    assert abs(parts[0] - gold["cost"][1]) < 1e-10 * gold["cost"][1]
+
+
+def test_cost_hand_computed():
+    """2x2x2 object, one frame, e = 0: the prior alone, each pair once."""
+    from oracle.oracle import Oracle
+    from phasecon_amd.synth import make_scan
+    s = make_scan("tiny3d", seed=0)
+    o = Oracle(s)
+    g = o.g
+    x = np.zeros(o.shape_x)
+    x[0, 0, 0, 0] = 0.01
+    e = np.zeros((g.N_p, g.N_r, g.N_t))
+    c, parts = o.cost(e, x)
+    assert parts[0] == 0.0
+    # the single non-zero voxel at a corner has 7 neighbours at z=0,1 in
+    # the forward/backward sense: all 7 pairs involve it exactly once
+    f = np.array(g.filt).reshape(3, 3, 3)
+    rho = o.qggmrf_value(0.01)
+    expect = rho * (f[1, 1, 2] + f[1, 2, 1] + f[1, 2, 2] + f[2, 1, 1] + f[2, 1, 2] + f[2, 2, 1] + f[2, 2, 2])
+    assert abs(parts[1] - expect) < 1e-14 * expect
+
+
+@pytest.mark.parametrize("which", ["3d", "4d"])
+def test_substitution_matches_reference(which, gold3d, gold4d, orc3d, orc4d):
+    gold, o = (gold3d, orc3d) if which == "3d" else (gold4d, orc4d)
+    si, sf = gold["sub_in"], gold["sub_flags"]
+    for i in range(len(si)):
+        got = o.substitution(si[i, 0], si[i, 1], si[i, 2], si[i, 3:30], sf[i, :27],
+                             si[i, 30:32], sf[i, 27:29])
+        assert got == gold["sub_out"][i]
+
+
+def test_substitution_closed_forms(orc3d):
+    o = orc3d
+    g = o.g
+    V, th1, th2 = 0.011, 3.0, 900.0
+    nb = np.full(27, V)
+    flags = np.ones(27, np.uint8)
+    # all neighbours equal V: every b = rho''(0) = q / (sigma^q c)
+    b0 = g.q / (g.sig_s_q * g.c_s)
+    den = b0 * (sum(g.filt) - g.filt[13])
+    u = V - th1 / (den + th2)
+    got = o.substitution(V, th1, th2, nb, flags, np.zeros(2), np.zeros(2, np.uint8))
+    assert abs(got - (V + g.alpha * (u - V))) < 1e-15
+    # no neighbours: a pure data-term Newton step
+    got = o.substitution(V, th1, th2, nb, np.zeros(27, np.uint8), np.zeros(2), np.zeros(2, np.uint8))
+    assert abs(got - (V - g.alpha * th1 / th2)) < 1e-15
+
+
+def test_qggmrf_weight_is_derivative_over_delta(orc3d):
+    o = orc3d
+    for d in (1e-4, 3e-3, -2e-3, 0.02):
+        h = abs(d) * 1e-6
+        fd = (o.qggmrf_value(d + h) - o.qggmrf_value(d - h)) / (2 * h)
+        assert abs(o.qggmrf_weight(d) * d - fd) < 1e-6 * abs(fd)
+    g = o.g
+    assert o.qggmrf_weight(0.0) == g.q / (g.sig_s_q * g.c_s)
+
+
+@pytest.mark.parametrize("which", ["3d", "4d"])
+def test_voxel_updates_match_reference(which, gold3d, gold4d, orc3d, orc4d):
+    """Sequential single-voxel updates through the oracle reproduce the
+    reference's voxel-line driver (new values and the final error
+    sinogram)."""
+    gold, o = (gold3d, orc3d) if which == "3d" else (gold4d, orc4d)
+    g = o.g
+    x = gold["x"].astype(np.float64)
+    e = gold["e"].astype(np.float64)
+    for k, (t, z, yy, xx) in enumerate(gold["vox"]):
+        th1, th2 = o.thetas(e, int(t), int(z), int(yy), int(xx))
+        nb = np.zeros(27)
+        nbf = np.zeros(27, np.uint8)
+        for dz in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    i = ((dz + 1) * 3 + dy + 1) * 3 + dx + 1
+                    z2, y2, x2 = z + dz, yy + dy, xx + dx
+                    if 0 <= z2 < g.N_z and 0 <= y2 < g.N_y and 0 <= x2 < g.N_x:
+                        nb[i], nbf[i] = x[t, z2, y2, x2], 1
+        nb[13] = 0
+        tn = np.array([x[t - 1, z, yy, xx] if t > 0 else 0, x[t + 1, z, yy, xx] if t + 1 < g.N_time else 0])
+        tf = np.array([t > 0, t + 1 < g.N_time], np.uint8)
+        V = x[t, z, yy, xx]
+        xn = o.substitution(V, th1, th2, nb, nbf, tn, tf)
+        # the reference stores x in float before updating e
+        xn32 = float(np.float32(xn))
+        assert xn32 == float(gold["vox_x"][k])
+        idx_all = []
+        for s in range(o.frame_ptr[t], o.frame_ptr[t + 1]):
+            p = o.frame_views[s]
+            idx, val = o.amatrix_column(int(yy), int(xx), p)
+            for r, a in zip(idx, val):
+                for m in range(g.mult_z):
+                    e[p, r, z * g.mult_z + m] -= a * g.delta_t * (xn32 - V)
+        x[t, z, yy, xx] = xn32
+    assert rel_err(e, gold["vox_e_final"]) < 2e-6
+
+
+def test_theta2_brute_force(orc3d):
+    o = orc3d
+    g = o.g
+    rng = np.random.default_rng(3)
+    e = rng.standard_normal((g.N_p, g.N_r, g.N_t)) * 0.01
+    for (t, z, yy, xx) in [(0, 1, 10, 12), (0, 3, 20, 5)]:
+        th1, th2 = o.thetas(e, t, z, yy, xx)
+        b1 = b2 = 0.0
+        for p in range(g.N_p):
+            idx, val = o.amatrix_column(yy, xx, p)
+            a = val * g.delta_t
+            b2 += np.sum(a * a * o.w[p, idx, z])
+            b1 -= np.sum(a * o.w[p, idx, z] * e[p, idx, z])
+        assert abs(th2 - b2) < 1e-12 * b2 and abs(th1 - b1) < 1e-12 * abs(b2)
+
+
+def test_oracle_icd_reaches_reference_converged_image(gold3d, orc3d):
+    """The reference's converged image, re-evaluated in FP64, has the same
+    MAP cost as the oracle's converged ICD (convex problem)."""
+    o = orc3d
+    x, e, costs = o.run_icd(300)
+    assert np.all(np.diff(costs) <= 1e-12 * costs[1:])
+    xr = gold3d["icd_x"].astype(np.float64)
+    cr = o.cost(o.init_error(xr), xr)[0]
+    assert abs(cr - costs[-1]) < 1e-9 * costs[-1]
+    rmse = np.sqrt(np.mean((xr - x) ** 2)) / np.sqrt(np.mean(x ** 2))
+    assert rmse < 1e-4
+
+
+def test_reference_cost_trajectory_monotone_to_fp32(gold3d):
+    """The reference's own ICD trajectory decreases until float e drift."""
+    c = gold3d["icd_costs"]
+    assert c[-1] < c[0] * 0.1
+    assert np.all(np.diff(c[:40]) <= 0)
+    assert abs(c[-1] - c[60]) < 1e-6 * c[-1]
+
+
+# ---------------------------------------------------------- live reference
+ref_needed = pytest.mark.skipif(
+    not os.path.exists(os.path.join(ROOT, "oracle", "_ref", "libmbir4d_ref.so")),
+    reason="reference build oracle/_ref not present")
+
+
+@ref_needed
+def test_live_reference_random_states(tiny_scan):
+    from oracle.oracle import Oracle, Reference
+    o = Oracle(tiny_scan)
+    r = Reference(tiny_scan, threads=1)
+    rng = np.random.default_rng(77)
+    x = (rng.random(o.shape_x) * 0.03).astype(np.float32)
+    r.set_object(x)
+    er = r.init_error()
+    assert rel_err(o.init_error(x.astype(np.float64)), er) < 2e-6
+    c = r.cost()[0]
+    assert abs(o.cost(er.astype(np.float64), x.astype(np.float64))[0] - c) < 1e-8 * c
+    r.close()
