@@ -663,6 +663,17 @@ void orc_icd_equit_tiled(const orc_geom* g, const double* cosv,
                          const uint8_t* mask, int32_t T, double alpha,
                          uint32_t seed, int32_t iter, double* x, double* e,
                          const double* w) {
+  orc_icd_equit_tiled_sync(g, cosv, sinv, H, frame_ptr, frame_views, mask, T,
+                           alpha, seed, iter, 1, x, e, w);
+}
+
+void orc_icd_equit_tiled_sync(const orc_geom* g, const double* cosv,
+                              const double* sinv, const float* H,
+                              const int32_t* frame_ptr,
+                              const int32_t* frame_views, const uint8_t* mask,
+                              int32_t T, double alpha, uint32_t seed,
+                              int32_t iter, int32_t m, double* x, double* e,
+                              const double* w) {
   orc_geom gg = *g;
   gg.alpha = alpha;
   const int32_t Nx = g->N_x, Ny = g->N_y, Nz = g->N_z, Nf = g->N_time;
@@ -701,75 +712,91 @@ void orc_icd_equit_tiled(const orc_geom* g, const double* cosv,
             size_t id = ((size_t)ty * TX + tx) * Nf + f;
             if ((tx % 2) + 2 * (ty % 2) == color && lens[id] > maxlen) maxlen = lens[id];
           }
-      for (int32_t step = 0; step < maxlen; ++step) {
+      for (int32_t step0 = 0; step0 < maxlen; step0 += m) {
         memcpy(esnap, e, sizeof(double) * ns);
-#pragma omp parallel for collapse(3) schedule(dynamic)
-        for (int32_t ty = 0; ty < TY; ++ty)
-          for (int32_t tx = 0; tx < TX; ++tx)
-            for (int32_t f = 0; f < Nf; ++f) {
-              if ((f % 2) != fpar || (tx % 2) + 2 * (ty % 2) != color) continue;
-              size_t id = ((size_t)ty * TX + tx) * Nf + f;
-              if (step >= lens[id]) continue;
-              int32_t c = lists[id * T * T + step], yy = c / Nx, xx = c % Nx;
-              int32_t nv = frame_ptr[f + 1] - frame_ptr[f];
-              int32_t* cnt = (int32_t*)calloc(nv + 1, sizeof(int32_t));
-              int32_t* cidx = (int32_t*)calloc((size_t)(nv + 1) * g->W, sizeof(int32_t));
-              double* cval = (double*)calloc((size_t)(nv + 1) * g->W, sizeof(double));
-              double* th1 = (double*)malloc(sizeof(double) * Nz);
-              double* th2 = (double*)malloc(sizeof(double) * Nz);
-              orc_line_columns(g, cosv, sinv, H, f, yy, xx, frame_ptr,
-                               frame_views, cnt, cidx, cval);
-              for (int32_t z = 0; z < Nz; ++z)
-                orc_thetas_cols(g, f, z, frame_ptr, frame_views, cnt, cidx,
-                                cval, esnap, w, &th1[z], &th2[z]);
-              for (int par = 0; par < 2; ++par)
-                for (int32_t z = par; z < Nz; z += 2) {
-                  size_t o = (((size_t)f * Nz + z) * Ny + yy) * Nx + xx;
-                  double nb[27], tn[2];
-                  uint8_t nbf[27], tf[2];
-                  for (int dz = -1; dz <= 1; ++dz)
-                    for (int dy = -1; dy <= 1; ++dy)
-                      for (int dx = -1; dx <= 1; ++dx) {
-                        int k = ((dz + 1) * 3 + dy + 1) * 3 + dx + 1;
-                        int32_t z2 = z + dz, y2 = yy + dy, x2 = xx + dx;
-                        int in = z2 >= 0 && z2 < Nz && y2 >= 0 && y2 < Ny &&
-                                 x2 >= 0 && x2 < Nx;
-                        nb[k] = in ? x[(((size_t)f * Nz + z2) * Ny + y2) * Nx + x2] : 0;
-                        nbf[k] = (uint8_t)in;
+#pragma omp parallel
+        {
+          double* ew = (double*)malloc(sizeof(double) * ns);
+          double* dacc = (double*)malloc(sizeof(double) * ns);
+#pragma omp for collapse(3) schedule(dynamic)
+          for (int32_t ty = 0; ty < TY; ++ty)
+            for (int32_t tx = 0; tx < TX; ++tx)
+              for (int32_t f = 0; f < Nf; ++f) {
+                if ((f % 2) != fpar || (tx % 2) + 2 * (ty % 2) != color) continue;
+                size_t id = ((size_t)ty * TX + tx) * Nf + f;
+                if (step0 >= lens[id]) continue;
+                memcpy(ew, esnap, sizeof(double) * ns);
+                memset(dacc, 0, sizeof(double) * ns);
+                int32_t nv = frame_ptr[f + 1] - frame_ptr[f];
+                int32_t* cnt = (int32_t*)calloc(nv + 1, sizeof(int32_t));
+                int32_t* cidx = (int32_t*)calloc((size_t)(nv + 1) * g->W, sizeof(int32_t));
+                double* cval = (double*)calloc((size_t)(nv + 1) * g->W, sizeof(double));
+                double* th1 = (double*)malloc(sizeof(double) * Nz);
+                double* th2 = (double*)malloc(sizeof(double) * Nz);
+                for (int32_t step = step0; step < step0 + m && step < lens[id]; ++step) {
+                  int32_t c = lists[id * T * T + step], yy = c / Nx, xx = c % Nx;
+                  orc_line_columns(g, cosv, sinv, H, f, yy, xx, frame_ptr,
+                                   frame_views, cnt, cidx, cval);
+                  for (int32_t z = 0; z < Nz; ++z)
+                    orc_thetas_cols(g, f, z, frame_ptr, frame_views, cnt, cidx,
+                                    cval, ew, w, &th1[z], &th2[z]);
+                  for (int par = 0; par < 2; ++par)
+                    for (int32_t z = par; z < Nz; z += 2) {
+                      size_t o = (((size_t)f * Nz + z) * Ny + yy) * Nx + xx;
+                      double nb[27], tn[2];
+                      uint8_t nbf[27], tf[2];
+                      for (int dz = -1; dz <= 1; ++dz)
+                        for (int dy = -1; dy <= 1; ++dy)
+                          for (int dx = -1; dx <= 1; ++dx) {
+                            int k = ((dz + 1) * 3 + dy + 1) * 3 + dx + 1;
+                            int32_t z2 = z + dz, y2 = yy + dy, x2 = xx + dx;
+                            int in = z2 >= 0 && z2 < Nz && y2 >= 0 && y2 < Ny &&
+                                     x2 >= 0 && x2 < Nx;
+                            nb[k] = in ? x[(((size_t)f * Nz + z2) * Ny + y2) * Nx + x2] : 0;
+                            nbf[k] = (uint8_t)in;
+                          }
+                      nb[13] = 0;
+                      tf[0] = f > 0;
+                      tn[0] = tf[0] ? x[o - vol] : 0;
+                      tf[1] = f + 1 < Nf;
+                      tn[1] = tf[1] ? x[o + vol] : 0;
+                      double V = x[o];
+                      if (V == 0.0 && iter > 1) {
+                        int any = (tn[0] > 0.0 || tn[1] > 0.0);
+                        for (int k = 0; k < 27 && !any; ++k)
+                          if (nb[k] > 0.0) any = 1;
+                        if (!any) continue;
                       }
-                  nb[13] = 0;
-                  tf[0] = f > 0;
-                  tn[0] = tf[0] ? x[o - vol] : 0;
-                  tf[1] = f + 1 < Nf;
-                  tn[1] = tf[1] ? x[o + vol] : 0;
-                  double V = x[o];
-                  if (V == 0.0 && iter > 1) {
-                    int any = (tn[0] > 0.0 || tn[1] > 0.0);
-                    for (int k = 0; k < 27 && !any; ++k)
-                      if (nb[k] > 0.0) any = 1;
-                    if (!any) continue;
-                  }
-                  double xn = orc_substitution(&gg, V, th1[z], th2[z], nb, nbf, tn, tf);
-                  x[o] = xn;
-                  for (int32_t s = frame_ptr[f]; s < frame_ptr[f + 1]; ++s) {
-                    int32_t p = frame_views[s], si = s - frame_ptr[f];
-                    for (int32_t m = 0; m < cnt[si]; ++m) {
-                      double a = cval[si * g->W + m] * g->delta_t;
-                      int32_t r = cidx[si * g->W + m];
-                      for (int32_t k = 0; k < g->mult_z; ++k) {
-                        double* dst = &e[((size_t)p * g->N_r + r) * g->N_t + z * g->mult_z + k];
-#pragma omp atomic
-                        *dst -= a * (xn - V);
+                      double xn = orc_substitution(&gg, V, th1[z], th2[z], nb, nbf, tn, tf);
+                      x[o] = xn;
+                      for (int32_t s2 = frame_ptr[f]; s2 < frame_ptr[f + 1]; ++s2) {
+                        int32_t p = frame_views[s2], si = s2 - frame_ptr[f];
+                        for (int32_t q = 0; q < cnt[si]; ++q) {
+                          double a = cval[si * g->W + q] * g->delta_t;
+                          int32_t r = cidx[si * g->W + q];
+                          for (int32_t k = 0; k < g->mult_z; ++k) {
+                            size_t eo = ((size_t)p * g->N_r + r) * g->N_t + z * g->mult_z + k;
+                            ew[eo] -= a * (xn - V);
+                            dacc[eo] -= a * (xn - V);
+                          }
+                        }
                       }
                     }
-                  }
                 }
-              free(cnt);
-              free(cidx);
-              free(cval);
-              free(th1);
-              free(th2);
-            }
+                for (size_t i = 0; i < ns; ++i)
+                  if (dacc[i] != 0.0) {
+#pragma omp atomic
+                    e[i] += dacc[i];
+                  }
+                free(cnt);
+                free(cidx);
+                free(cval);
+                free(th1);
+                free(th2);
+              }
+          free(ew);
+          free(dacc);
+        }
       }
     }
   free(esnap);

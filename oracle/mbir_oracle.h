@@ -146,6 +146,18 @@ void orc_icd_equit_tiled(const orc_geom* g, const double* cosv,
                          uint32_t seed, int32_t iter, double* x, double* e,
                          const double* w);
 
+/* As orc_icd_equit_tiled, but each concurrent wavefront works on a private
+ * copy of the error sinogram for m consecutive columns of its tile and the
+ * copies' updates are merged every m columns (m = 1 is the lock-step
+ * model above).  Small problems only: one sinogram copy per thread. */
+void orc_icd_equit_tiled_sync(const orc_geom* g, const double* cosv,
+                              const double* sinv, const float* H,
+                              const int32_t* frame_ptr,
+                              const int32_t* frame_views, const uint8_t* mask,
+                              int32_t T, double alpha, uint32_t seed,
+                              int32_t iter, int32_t m, double* x, double* e,
+                              const double* w);
+
 /* File-layout <-> internal sinogram transpose (XT_Init.c:294-301):
  * file [p][t][r] -> internal [p][r][t]. */
 void orc_sino_from_file(int32_t N_p, int32_t N_t, int32_t N_r,

@@ -27,6 +27,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "mbir_hip.h"
@@ -39,6 +40,7 @@ constexpr int kNZMax = 128;    // voxels per column (N_z) supported by k_sweep
 constexpr int kNTMax = 256;    // detector rows (N_t) supported by k_sweep
 constexpr int kWMax = 33;      // window width 2*mult_xy+1 (mult_xy <= 16)
 constexpr int kWave = 64;
+constexpr size_t kSweepXMax = 48 * 1024;  // LDS bytes for a sweep tile slab
 
 thread_local std::string g_err;
 
@@ -159,7 +161,7 @@ __global__ __launch_bounds__(256) void k_fwdproj(
     const float* __restrict__ y, const float* __restrict__ w,
     const float* __restrict__ d, float* __restrict__ out,
     double* __restrict__ partial) {
-  extern __shared__ __align__(16) unsigned char smem[];
+  extern __shared__ __align__(16) float smem[];
   Acc* acc = reinterpret_cast<Acc*>(smem);
   __shared__ float sh[kNB];
   __shared__ double red[256];
@@ -252,8 +254,9 @@ __device__ __forceinline__ double qg_value64(double delta, double q, double p,
                                              double c, double sq, double sqp) {
   const double ad = fabs(delta);
   if (ad == 0.0) return 0.0;
-  const double aq = (q == 2.0) ? ad * ad : pow(ad, q);
-  return (aq / sq) / (c + pow(ad, q - p) / sqp);
+  const double lg = log2(ad);
+  const double aq = (q == 2.0) ? ad * ad : exp2(q * lg);
+  return (aq / sq) / (c + exp2((q - p) * lg) / sqp);
 }
 
 // Prior term: each neighbour pair once (13 forward spatial + t+1), FP64
@@ -343,51 +346,132 @@ struct ColumnArgs {
   float* e;
 };
 
-// theta1/theta2 of every detector row of voxel column c in frame f (views
-// slots s0 .. s0+nv-1): lanes are (entry group g, row rr) so the rows of one
-// (view, bin) entry are read as one contiguous run.  Per-row sums land in
-// row1/row2 (LDS); the voxel's thetas are the sums over its z_overlap rows.
-// XT_VoxUpdate.c:56-85 including the generalized-Huber weight.
-__device__ void column_thetas(const DevGeom& G, const ColumnArgs& A, int c, int s0,
-                              int nv, float* row1, float* row2, float* part1,
-                              float* part2) {
+constexpr int kMaxViews = 512;  // views per frame held in LDS
+constexpr int kMaxAV = 1024;    // A values per (column, frame) held in LDS
+constexpr int kECh = 6;         // sinogram entries per lane per load batch
+constexpr int kPreA = kMaxAV / kWave;
+constexpr int kPreI = kMaxViews / kWave;
+
+// One (column, frame) A window in LDS: first bin per view slot, W values each.
+struct AWin {
+  int* a0;
+  float* av;
+};
+
+// Dynamic LDS of the sweep: the tile slab, two A windows, the view list.
+struct SweepLds {
+  float* x;
+  AWin win[2];
+  int* views;
+  __device__ SweepLds(float* base, int xfloats, int nvmax, int W) {
+    x = base;
+    float* p = base + xfloats;
+    for (int b = 0; b < 2; ++b) {
+      win[b].a0 = reinterpret_cast<int*>(p);
+      p += nvmax;
+      win[b].av = p;
+      p += nvmax * W;
+    }
+    views = reinterpret_cast<int*>(p);
+  }
+};
+
+// Register staging of the next column's A window (prefetch).
+struct APre {
+  int a0[kPreI];
+  float av[kPreA];
+};
+
+__device__ __forceinline__ void awin_fetch(const DevGeom& G, const ColumnArgs& A, int c,
+                                           int s0, int nv, APre& pre) {
   const int lane = threadIdx.x;
-  const int rpl = G.N_t < kWave ? G.N_t : kWave;
-  const int groups = kWave / rpl;
-  const int g = lane / rpl, rr = lane % rpl;
+  const size_t base = (size_t)c * G.nslots + s0;
+#pragma unroll
+  for (int i = 0; i < kPreI; ++i) {
+    const int j = lane + i * kWave;
+    pre.a0[i] = j < nv ? A.afirst[base + j] : -1;
+  }
+  const int nav = nv * G.W;
+#pragma unroll
+  for (int i = 0; i < kPreA; ++i) {
+    const int j = lane + i * kWave;
+    pre.av[i] = j < nav ? A.aval[base * G.W + j] : 0.f;
+  }
+}
+
+__device__ __forceinline__ void awin_store(const APre& pre, int nv, int nav, const AWin& win) {
+  const int lane = threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < kPreI; ++i) {
+    const int j = lane + i * kWave;
+    if (j < nv) win.a0[j] = pre.a0[i];
+  }
+#pragma unroll
+  for (int i = 0; i < kPreA; ++i) {
+    const int j = lane + i * kWave;
+    if (j < nav) win.av[j] = pre.av[i];
+  }
+}
+
+// Lane layout over a column's sinogram footprint: lane = (entry group g,
+// detector row rr); each entry is one (view slot, bin) pair of the window,
+// whose rows are contiguous in e and w.
+struct Lanes {
+  int rpl, groups, g, rr;
+  __device__ Lanes(int N_t) {
+    rpl = N_t < kWave ? N_t : kWave;
+    groups = kWave / rpl;
+    g = threadIdx.x / rpl;
+    rr = threadIdx.x % rpl;
+  }
+};
+
+// theta1/theta2 per detector row of one voxel column (XT_VoxUpdate.c:56-85,
+// generalized-Huber weight included): each lane issues its batch of e/w
+// loads back to back, then accumulates; per-row sums land in row1/row2.
+__device__ void window_thetas(const DevGeom& G, const ColumnArgs& A, const AWin& win,
+                              const int* views, int nv, float* row1, float* row2,
+                              float* part1, float* part2) {
+  const int lane = threadIdx.x;
+  const Lanes L(G.N_t);
   const int nent = nv * G.W;
   const float T = G.huber_T, hd = G.huber_delta;
   for (int t0 = 0; t0 < G.N_t; t0 += kWave) {
-    const int row = t0 + rr;
+    const int row = t0 + L.rr;
+    const bool lane_ok = L.g < L.groups && row < G.N_t;
     float a1 = 0.f, a2 = 0.f;
-    if (g < groups && row < G.N_t) {
-      for (int sk = g; sk < nent; sk += groups) {
-        const int s = sk / G.W, k = sk - s * G.W;
-        const size_t slot = (size_t)c * G.nslots + s0 + s;
-        const int r0 = A.afirst[slot];
-        if (r0 < 0) continue;
-        const int r = r0 + k;
-        if (r >= G.N_r) continue;
-        float a = A.aval[slot * G.W + k];
-        if (a == 0.f) continue;
-        a *= G.delta_t;
-        const int p = A.frame_views[s0 + s];
-        const size_t o = ((size_t)p * G.N_r + r) * G.N_t + row;
-        const float ev = e_load(A.e + o), wv = A.w[o];
-        const float sw = sqrtf(wv);
-        const float hw = (fabsf(ev * sw) < T) ? wv : T * hd * sw / fabsf(ev);
-        a2 += a * a * hw;
-        a1 -= ev * a * hw;
+    for (int base = 0; base < nent; base += L.groups * kECh) {
+      float ev[kECh], wv[kECh], av[kECh];
+#pragma unroll
+      for (int i = 0; i < kECh; ++i) {
+        const int sk = base + L.g + i * L.groups;
+        bool ok = lane_ok && sk < nent;
+        const int s = ok ? sk / G.W : 0;
+        const int r0 = ok ? win.a0[s] : -1;
+        const float a = ok ? win.av[sk] : 0.f;
+        const int r = r0 + (sk - s * G.W);
+        ok = ok && r0 >= 0 && a != 0.f && r < G.N_r;
+        const size_t o = ok ? ((size_t)views[s] * G.N_r + r) * G.N_t + row : 0;
+        ev[i] = e_load(A.e + o);
+        wv[i] = ok ? A.w[o] : 0.f;
+        av[i] = ok ? a * G.delta_t : 0.f;
+      }
+#pragma unroll
+      for (int i = 0; i < kECh; ++i) {
+        const float sw = sqrtf(wv[i]);
+        const float hw = (fabsf(ev[i] * sw) < T) ? wv[i] : T * hd * sw / fabsf(ev[i]);
+        a2 += av[i] * av[i] * hw;
+        a1 -= ev[i] * av[i] * hw;
       }
     }
     part1[lane] = a1;
     part2[lane] = a2;
     __syncthreads();
-    if (lane < rpl && row < G.N_t) {
+    if (lane < L.rpl && row < G.N_t) {
       float s1 = 0.f, s2 = 0.f;
-      for (int gg = 0; gg < groups; ++gg) {
-        s1 += part1[gg * rpl + lane];
-        s2 += part2[gg * rpl + lane];
+      for (int gg = 0; gg < L.groups; ++gg) {
+        s1 += part1[gg * L.rpl + lane];
+        s2 += part2[gg * L.rpl + lane];
       }
       row1[row] = s1;
       row2[row] = s2;
@@ -396,31 +480,24 @@ __device__ void column_thetas(const DevGeom& G, const ColumnArgs& A, int c, int 
   }
 }
 
-// e -= A delta_t dx along voxel column c (XT_VoxUpdate.c:96-116), with
+// e -= A delta_t dx along one voxel column (XT_VoxUpdate.c:96-116), with
 // atomics because other wavefronts' columns share detector bins.
-__device__ void column_update_error(const DevGeom& G, const ColumnArgs& A, int c,
-                                    int s0, int nv, const float* dxz) {
-  const int lane = threadIdx.x;
-  const int rpl = G.N_t < kWave ? G.N_t : kWave;
-  const int groups = kWave / rpl;
-  const int g = lane / rpl, rr = lane % rpl;
+__device__ void window_update_error(const DevGeom& G, const ColumnArgs& A, const AWin& win,
+                                    const int* views, int nv, const float* dxz) {
+  const Lanes L(G.N_t);
   const int nent = nv * G.W;
   for (int t0 = 0; t0 < G.N_t; t0 += kWave) {
-    const int row = t0 + rr;
-    if (g >= groups || row >= G.N_t) continue;
+    const int row = t0 + L.rr;
+    if (L.g >= L.groups || row >= G.N_t) continue;
     const float dz = dxz[row / G.zo];
     if (dz == 0.f) continue;
-    for (int sk = g; sk < nent; sk += groups) {
-      const int s = sk / G.W, k = sk - s * G.W;
-      const size_t slot = (size_t)c * G.nslots + s0 + s;
-      const int r0 = A.afirst[slot];
-      if (r0 < 0) continue;
-      const int r = r0 + k;
-      if (r >= G.N_r) continue;
-      const float a = A.aval[slot * G.W + k];
-      if (a == 0.f) continue;
-      const int p = A.frame_views[s0 + s];
-      e_add(A.e + ((size_t)p * G.N_r + r) * G.N_t + row, -(a * G.delta_t) * dz);
+    for (int sk = L.g; sk < nent; sk += L.groups) {
+      const int s = sk / G.W;
+      const int r0 = win.a0[s];
+      const float a = win.av[sk];
+      const int r = r0 + (sk - s * G.W);
+      if (r0 < 0 || a == 0.f || r >= G.N_r) continue;
+      e_add(A.e + ((size_t)views[s] * G.N_r + r) * G.N_t + row, -(a * G.delta_t) * dz);
     }
   }
 }
@@ -432,6 +509,9 @@ struct SweepArgs {
   const int* tile_cols;
   const int* launch_tiles;
   int n_launch_tiles;
+  int tiles_x;  // tiles per row of the tile grid
+  int tile;     // tile edge T
+  int nvmax;    // most views in one frame (sizes the LDS windows)
   int fparity;
   int iter;
   unsigned seed;
@@ -441,14 +521,17 @@ struct SweepArgs {
 };
 
 // One ICD pass over the voxel columns of the tiles in this launch (one
-// 2x2-checkerboard colour) for the frames of one parity.  Concurrent
-// wavefronts never touch neighbouring columns or temporally adjacent frames,
-// so x needs no atomics; the error sinogram is shared and updated
-// atomically.  Within a column, z is updated in two parity halves so each
-// voxel sees its z+-1 neighbours' new values (sequential-ICD equivalent).
+// 2x2-checkerboard colour) for the frames of one parity, one wavefront per
+// (tile, frame).  The tile's object slab plus a one-column halo lives in LDS
+// for the whole pass (neighbouring tiles and adjacent frames are not updated
+// in the same launch, so the halo and the t+-1 values are constant); the
+// next column's A window is prefetched into a second LDS buffer while the
+// current column is solved; the error sinogram is shared with the other
+// wavefronts and updated with atomics.  Within a column, z is solved in two
+// parity halves so each voxel sees its z+-1 neighbours' new values.
 __global__ __launch_bounds__(kWave) void k_sweep(DevGeom G, SweepArgs A) {
-  __shared__ float s_nb[11 * kNZMax];
-  __shared__ int s_ok[11];
+  extern __shared__ __align__(16) float smem[];
+  __shared__ float s_tn[2 * kNZMax];
   __shared__ float s_row1[kNTMax], s_row2[kNTMax];
   __shared__ float s_part1[kWave], s_part2[kWave];
   __shared__ float s_dx[kNZMax];
@@ -462,6 +545,14 @@ __global__ __launch_bounds__(kWave) void k_sweep(DevGeom G, SweepArgs A) {
   const int s0 = A.frame_ptr[f];
   const int nv = A.frame_ptr[f + 1] - s0;
   if (n == 0 || nv == 0) return;
+  const int Nz = G.N_z, T = A.tile, BW = T + 2;
+  const SweepLds lds(smem, BW * BW * Nz, A.nvmax, G.W);
+  float* s_x = lds.x;
+  const AWin win0 = lds.win[0], win1 = lds.win[1];
+  int* s_views = lds.views;
+  const int oy = (tile / A.tiles_x) * T - 1, ox = (tile % A.tiles_x) * T - 1;
+  const size_t vol = (size_t)G.N_y * G.N_x * Nz;
+  float* xf = A.x + (size_t)f * vol;
   // seeded random visiting order: k -> (pa k + pb) mod n, gcd(pa, n) = 1
   const unsigned h = mix32(A.seed ^ mix32((unsigned)tile * 131071u + (unsigned)f * 524287u));
   int pa = 1, pb = 0;
@@ -470,39 +561,37 @@ __global__ __launch_bounds__(kWave) void k_sweep(DevGeom G, SweepArgs A) {
     while (gcd_i(pa, n) != 1) pa = pa % (n - 1) + 1;
     pb = (int)(mix32(h + 0x9e3779b9u) % (unsigned)n);
   }
-  const int Nz = G.N_z;
-  const size_t vol = (size_t)G.N_y * G.N_x * Nz;
+  // the tile's slab with halo, the frame's view list, the first A window
+  for (int j = lane; j < BW * BW * Nz; j += kWave) {
+    const int z = j % Nz, b = j / Nz;
+    const int yy = oy + b / BW, xx = ox + b % BW;
+    s_x[j] = (yy >= 0 && yy < G.N_y && xx >= 0 && xx < G.N_x)
+                 ? xf[((size_t)yy * G.N_x + xx) * Nz + z]
+                 : 0.f;
+  }
+  for (int j = lane; j < nv; j += kWave) s_views[j] = A.col.frame_views[s0 + j];
+  const int nav = nv * G.W;
+  APre pre;
+  awin_fetch(G, A.col, A.tile_cols[c0 + (int)((long long)pb % n)], s0, nv, pre);
+  awin_store(pre, nv, nav, win0);
+  __syncthreads();
   double st_dx = 0.0, st_x = 0.0, st_skip = 0.0, st_upd = 0.0;
   for (int k = 0; k < n; ++k) {
+    const bool odd = (k & 1) != 0;
+    const AWin wcur = odd ? win1 : win0, wnext = odd ? win0 : win1;
     const int c = A.tile_cols[c0 + (int)(((long long)pa * k + pb) % n)];
     const int cy = c / G.N_x, cx = c % G.N_x;
-    // neighbourhood: 3x3 columns at t, the column at t-1 and t+1
-    if (lane < 11) {
-      int ok;
-      if (lane < 9) {
-        const int yy = cy + lane / 3 - 1, xx = cx + lane % 3 - 1;
-        ok = (yy >= 0 && yy < G.N_y && xx >= 0 && xx < G.N_x);
-      } else {
-        const int ff = lane == 9 ? f - 1 : f + 1;
-        ok = (ff >= 0 && ff < G.N_time);
-      }
-      s_ok[lane] = ok;
-    }
-    for (int j = lane; j < 11 * Nz; j += kWave) {
-      const int cc = j / Nz, z = j - cc * Nz;
-      float v = 0.f;
-      if (cc < 9) {
-        const int yy = cy + cc / 3 - 1, xx = cx + cc % 3 - 1;
-        if (yy >= 0 && yy < G.N_y && xx >= 0 && xx < G.N_x)
-          v = A.x[(size_t)f * vol + ((size_t)yy * G.N_x + xx) * Nz + z];
-      } else {
-        const int ff = cc == 9 ? f - 1 : f + 1;
-        if (ff >= 0 && ff < G.N_time) v = A.x[(size_t)ff * vol + (size_t)c * Nz + z];
-      }
-      s_nb[j] = v;
+    const int ly = cy - oy, lx = cx - ox;
+    // t-1 / t+1 columns (read-only in this launch)
+    for (int j = lane; j < 2 * Nz; j += kWave) {
+      const int ff = j < Nz ? f - 1 : f + 1;
+      s_tn[j] = (ff >= 0 && ff < G.N_time) ? A.x[(size_t)ff * vol + (size_t)c * Nz + j % Nz] : 0.f;
     }
     for (int z = lane; z < Nz; z += kWave) s_dx[z] = 0.f;
-    column_thetas(G, A.col, c, s0, nv, s_row1, s_row2, s_part1, s_part2);
+    if (k + 1 < n)
+      awin_fetch(G, A.col, A.tile_cols[c0 + (int)(((long long)pa * (k + 1) + pb) % n)], s0,
+                 nv, pre);
+    window_thetas(G, A.col, wcur, s_views, nv, s_row1, s_row2, s_part1, s_part2);
     // surrogate solve, z parity halves
     for (int par = 0; par < 2; ++par) {
       const int nzp = (Nz - par + 1) / 2;
@@ -513,7 +602,8 @@ __global__ __launch_bounds__(kWave) void k_sweep(DevGeom G, SweepArgs A) {
         const int vi = vb + lane / lpv, sub = lane % lpv;
         const bool act = vi < nzp;
         const int z = par + 2 * vi;
-        const float V = act ? s_nb[4 * Nz + z] : 0.f;
+        const int centre = ((ly * BW + lx) * Nz) + z;
+        const float V = act ? s_x[centre] : 0.f;
         float num = 0.f, den = 0.f;
         int pos = 0;
         if (act) {
@@ -522,13 +612,14 @@ __global__ __launch_bounds__(kWave) void k_sweep(DevGeom G, SweepArgs A) {
             int ok;
             float nbv;
             if (j < 27) {
-              const int z2 = z + j / 9 - 1, cc = j % 9;
-              ok = s_ok[cc] && z2 >= 0 && z2 < Nz;
-              nbv = ok ? s_nb[cc * Nz + z2] : 0.f;
+              const int dz = j / 9 - 1, dy = (j / 3) % 3 - 1, dx = j % 3 - 1;
+              const int z2 = z + dz, y2 = cy + dy, x2 = cx + dx;
+              ok = z2 >= 0 && z2 < Nz && y2 >= 0 && y2 < G.N_y && x2 >= 0 && x2 < G.N_x;
+              nbv = ok ? s_x[((ly + dy) * BW + lx + dx) * Nz + z2] : 0.f;
             } else {
-              const int cc = 9 + (j - 27);
-              ok = s_ok[cc];
-              nbv = ok ? s_nb[cc * Nz + z] : 0.f;
+              const int ff = j == 27 ? f - 1 : f + 1;
+              ok = ff >= 0 && ff < G.N_time;
+              nbv = ok ? s_tn[(j - 27) * Nz + z] : 0.f;
             }
             if (nbv > 0.f) pos = 1;
             if (!ok) continue;
@@ -553,9 +644,8 @@ __global__ __launch_bounds__(kWave) void k_sweep(DevGeom G, SweepArgs A) {
               th2 += s_row2[z * G.zo + m];
             }
             const float xn = substitute(G, A.alpha, V, th1, th2, num, den);
-            s_nb[4 * Nz + z] = xn;
+            s_x[centre] = xn;
             s_dx[z] = xn - V;
-            A.x[(size_t)f * vol + (size_t)c * Nz + z] = xn;
             st_dx += fabs((double)xn - (double)V);
             st_x += fabs((double)xn);
             st_upd += 1.0;
@@ -564,8 +654,17 @@ __global__ __launch_bounds__(kWave) void k_sweep(DevGeom G, SweepArgs A) {
       }
       __syncthreads();
     }
-    column_update_error(G, A.col, c, s0, nv, s_dx);
+    window_update_error(G, A.col, wcur, s_views, nv, s_dx);
+    if (k + 1 < n) awin_store(pre, nv, nav, wnext);
     __syncthreads();
+  }
+  // write the tile back (interior of the box, inside the object)
+  for (int j = lane; j < BW * BW * Nz; j += kWave) {
+    const int z = j % Nz, b = j / Nz;
+    const int by = b / BW, bx = b % BW;
+    if (by == 0 || bx == 0 || by == BW - 1 || bx == BW - 1) continue;
+    const int yy = oy + by, xx = ox + bx;
+    if (yy < G.N_y && xx < G.N_x) xf[((size_t)yy * G.N_x + xx) * Nz + z] = s_x[j];
   }
   for (int off = 32; off > 0; off >>= 1) {
     st_dx += __shfl_xor(st_dx, off);
@@ -581,19 +680,28 @@ __global__ __launch_bounds__(kWave) void k_sweep(DevGeom G, SweepArgs A) {
   }
 }
 
-// theta1/theta2 of individual voxels through the sweep's column routine.
+// theta1/theta2 of individual voxels through the sweep's window routine.
 __global__ __launch_bounds__(kWave) void k_thetas(DevGeom G, ColumnArgs A,
                                                   const int* __restrict__ frame_ptr,
                                                   const int* __restrict__ tzyx,
                                                   float* __restrict__ th1,
                                                   float* __restrict__ th2) {
+  extern __shared__ __align__(16) float smem[];
   __shared__ float s_row1[kNTMax], s_row2[kNTMax];
   __shared__ float s_part1[kWave], s_part2[kWave];
+  const SweepLds lds(smem, 0, kMaxViews, G.W);
+  const AWin s_win = lds.win[0];
+  int* s_views = lds.views;
   const int i = blockIdx.x;
   const int f = tzyx[4 * i], z = tzyx[4 * i + 1];
   const int c = tzyx[4 * i + 2] * G.N_x + tzyx[4 * i + 3];
   const int s0 = frame_ptr[f], nv = frame_ptr[f + 1] - s0;
-  column_thetas(G, A, c, s0, nv, s_row1, s_row2, s_part1, s_part2);
+  for (int j = threadIdx.x; j < nv; j += kWave) s_views[j] = A.frame_views[s0 + j];
+  APre pre;
+  awin_fetch(G, A, c, s0, nv, pre);
+  awin_store(pre, nv, nv * G.W, s_win);
+  __syncthreads();
+  window_thetas(G, A, s_win, s_views, nv, s_row1, s_row2, s_part1, s_part2);
   if (threadIdx.x == 0) {
     float a = 0.f, b = 0.f;
     for (int m = 0; m < G.zo; ++m) {
@@ -765,7 +873,7 @@ struct mbir_hip_ctx {
   mbir_hip_geom g;
   DevGeom G;
   hipStream_t stream = nullptr;
-  int nslots = 0, ncol = 0;
+  int nslots = 0, ncol = 0, nvmax = 0;
   size_t nsino = 0, nvox = 0;
   float* d_H = nullptr;
   double *d_cos = nullptr, *d_sin = nullptr;
@@ -950,6 +1058,12 @@ int mbir_hip_create(mbir_hip_ctx** out, const mbir_hip_geom* g, const float* H,
       delete c;
       return fail("frame_ptr not monotone");
     }
+  for (int f = 0; f < g->N_time; ++f)
+    c->nvmax = std::max(c->nvmax, (int)(frame_ptr[f + 1] - frame_ptr[f]));
+  if (c->nvmax > kMaxViews || c->nvmax * (2 * g->mult_xy + 1) > kMaxAV) {
+    delete c;
+    return fail("too many views per frame for the sweep's LDS window (512 views, 1024 A values)");
+  }
   std::vector<int> view_frame(g->N_p, -1);
   for (int f = 0; f < g->N_time; ++f)
     for (int s = frame_ptr[f]; s < frame_ptr[f + 1]; ++s) {
@@ -1155,6 +1269,14 @@ int mbir_hip_icd_equits(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t
   A.x = c->d_x;
   A.stats = c->d_stats;
   A.alpha = (float)(c->alpha > 0 ? c->alpha : c->g.alpha);
+  A.tile = c->tile;
+  A.tiles_x = (c->g.N_x + c->tile - 1) / c->tile;
+  A.nvmax = c->nvmax;
+  const size_t xfl = (size_t)(c->tile + 2) * (c->tile + 2) * c->g.N_z;
+  if (xfl * sizeof(float) > kSweepXMax)
+    return fail("tile too large for N_z: the tile slab must fit 48 KiB of LDS");
+  const size_t xbytes =
+      sizeof(float) * (xfl + 2 * ((size_t)c->nvmax + (size_t)c->nvmax * c->G.W) + c->nvmax);
   const int nf = c->g.N_time;
   for (int it = 0; it < n; ++it) {
     A.iter = first_iter + it;
@@ -1169,7 +1291,7 @@ int mbir_hip_icd_equits(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t
         A.fparity = fp;
         A.seed = seed * 2654435761u + (unsigned)(it * 8 + colour * 2 + fp) * 40503u;
         if (ev_begin(c, MBIR_K_SWEEP)) return -1;
-        hipLaunchKernelGGL(k_sweep, dim3((unsigned)(ntl * frames)), dim3(kWave), 0,
+        hipLaunchKernelGGL(k_sweep, dim3((unsigned)(ntl * frames)), dim3(kWave), xbytes,
                            c->stream, c->G, A);
         if (ev_end(c, MBIR_K_SWEEP)) return -1;
       }
@@ -1268,7 +1390,8 @@ int mbir_hip_voxel_thetas(mbir_hip_ctx* c, int32_t n, const int32_t* tzyx, float
   A.frame_views = c->d_frame_views;
   A.w = c->d_w;
   A.e = c->d_e;
-  hipLaunchKernelGGL(k_thetas, dim3(n), dim3(kWave), 0, c->stream, c->G, A, c->d_frame_ptr, dq,
+  const size_t shm = sizeof(float) * (size_t)(2 * (kMaxViews + kMaxViews * c->G.W) + kMaxViews);
+  hipLaunchKernelGGL(k_thetas, dim3(n), dim3(kWave), shm, c->stream, c->G, A, c->d_frame_ptr, dq,
                      d1, d2);
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(theta1, d1, n * 4, hipMemcpyDeviceToHost, c->stream));
