@@ -63,8 +63,10 @@ def algorithmic_bytes_per_equit(st):
     return total
 
 
-def run_once(st, target, max_equits, seed):
-    """One timed reconstruction from zero; returns (seconds, equits, cost)."""
+def run_once(st, target, max_equits, seed, refresh):
+    """One timed reconstruction from zero; returns (seconds, equits, cost).
+    Every `refresh` equits the error sinogram is recomputed exactly (FP64
+    accumulation) so FP32 drift cannot bias the fixed point or the test."""
     st.set_object(np.zeros(st.shape_x, np.float32))
     st.synchronize()
     t0 = time.perf_counter()
@@ -75,6 +77,8 @@ def run_once(st, target, max_equits, seed):
     while it < max_equits:
         it += 1
         st.icd(1, it, seed)
+        if refresh and it % refresh == 0:
+            st.init_error()
         cost = st.cost()[0]
         if cost <= target:
             fresh = st.cost(fresh=True)[0]   # FP64 check from a recomputed e
@@ -124,6 +128,7 @@ def main():
     ap.add_argument("--tile", type=int, default=8)
     ap.add_argument("--max-equits", type=int, default=3000)
     ap.add_argument("--seed", type=int, default=761521)
+    ap.add_argument("--refresh", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-equits", type=int, default=2)
     ap.add_argument("--longrun", type=int, default=0)
@@ -149,6 +154,8 @@ def main():
         st.init_error()
         for it in range(1, args.longrun + 1):
             st.icd(1, it, args.seed)
+            if args.refresh and it % args.refresh == 0:
+                st.init_error()
             if it % 500 == 0:
                 print(f"[bench] longrun {it}: cost {st.cost()[0]:.12g}", file=sys.stderr, flush=True)
         c = st.cost(fresh=True)[0]
@@ -174,12 +181,12 @@ def main():
     target = cstar * (1.0 + REL_TOL)
 
     for _ in range(args.warmup):
-        run_once(st, target, args.max_equits, args.seed)
+        run_once(st, target, args.max_equits, args.seed, args.refresh)
     st.synchronize()
     st.reset_kernel_time()
     times, equits, costs = [], [], []
     for k in range(args.steps):
-        dt, n, c = run_once(st, target, args.max_equits, args.seed + k)
+        dt, n, c = run_once(st, target, args.max_equits, args.seed + k, args.refresh)
         times.append(dt)
         equits.append(n)
         costs.append(c)
@@ -217,6 +224,7 @@ def main():
                          f"frames, {w['views']} views" + (f" interlaced K={w['K']}" if w['K'] else "")
                          + f", I0={w['I0']:g}, qGGMRF p_s={w['p_s']} p_t={w['p_t']} q=2, from zero"),
             "tile": args.tile,
+            "refresh_every": args.refresh,
             "cstar": cstar,
             "cstar_source": csrc,
             "target_rel": REL_TOL,

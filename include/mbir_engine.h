@@ -63,13 +63,14 @@ void mbir_sino_to_internal(int32_t N_p, int32_t N_t, int32_t N_r,
  * reconstruct() uses. */
 typedef struct {
   double p_s, p_t;         /* qGGMRF p, spatial / temporal (1.2, 1.2)     */
-  int32_t tile;            /* ICD tile edge (16)                          */
+  int32_t tile;            /* ICD tile edge (8)                           */
   double alpha;            /* over-relaxation, <= 0: reference's 1.5      */
   int32_t max_equits;      /* per stage (1000, MAX_NUM_ITERATIONS)        */
   int32_t single_stage;    /* 1: only the full-resolution stage           */
   int32_t write_files;     /* 1: write the reference's CWD output files   */
   double stop_cost;        /* > 0: also stop once the FP64 cost <= this   */
   uint32_t seed;           /* ICD visiting-order seed (761521)            */
+  int32_t refresh_every;   /* exact e = y - A x recompute period (10)     */
 } mbir_options;
 
 void mbir_default_options(mbir_options* opt);

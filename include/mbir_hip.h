@@ -69,9 +69,10 @@ int mbir_hip_create(mbir_hip_ctx** out, const mbir_hip_geom* g,
                     const uint8_t* mask);
 void mbir_hip_destroy(mbir_hip_ctx* ctx);
 
-/* Tuning knobs: "tile" (ICD tile edge, default 16), "alpha" (override of
- * g->alpha for the sweep), "fp16" (1: store A as FP16 with a per-column
- * FP32 scale; takes effect at the next mbir_hip_build_amatrix). */
+/* Tuning knobs: "tile" (ICD tile edge, default 8), "alpha" (override of
+ * g->alpha for the sweep), "sv" (1: hold each tile's error/weight footprint
+ * in LDS when it fits, default 0), "sync" (columns between the LDS
+ * footprint's refreshes from the shared error sinogram, default 8). */
 int mbir_hip_set_option(mbir_hip_ctx* ctx, const char* key, double value);
 
 /* Object transfer (replaces the Object array of XT_Structures.h:215). */
@@ -86,7 +87,9 @@ int mbir_hip_build_amatrix(mbir_hip_ctx* ctx);
 int mbir_hip_get_amatrix(mbir_hip_ctx* ctx, int32_t* first, float* val);
 
 /* e = y - A x - d from the current object, A without the delta_t factor
- * (initErrorSinogam, XT_ICD_update.c:651-698). */
+ * (initErrorSinogam, XT_ICD_update.c:651-698), accumulated in FP64 and
+ * rounded once.  Calling it again between sweeps removes the rounding drift
+ * the sweeps' FP32 atomic updates accumulate in e. */
 int mbir_hip_init_error(mbir_hip_ctx* ctx);
 int mbir_hip_get_error(mbir_hip_ctx* ctx, float* e);
 int mbir_hip_set_error(mbir_hip_ctx* ctx, const float* e);

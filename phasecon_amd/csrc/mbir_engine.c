@@ -241,13 +241,14 @@ void mbir_sino_to_internal(int32_t N_p, int32_t N_t, int32_t N_r,
 void mbir_default_options(mbir_options* o) {
   o->p_s = 1.2; /* MRF_P, XT_Constants.h:62 */
   o->p_t = 1.2;
-  o->tile = 16;
+  o->tile = 8;
   o->alpha = -1.0;
   o->max_equits = 1000; /* MAX_NUM_ITERATIONS, XT_Constants.h:86 */
   o->single_stage = 0;
   o->write_files = 1;
   o->stop_cost = 0.0;
   o->seed = 761521; /* XT_Engine.c:99 */
+  o->refresh_every = 10;
 }
 
 /* ------------------------------------------------------------ files */
@@ -581,6 +582,10 @@ int reconstruct_ex(float** object, float* projections, float* weights,
       for (it = 1; it <= opt.max_equits; ++it) {
         double stats[4];
         if (mbir_hip_icd_equits(ctx, 1, it, opt.seed + (uint32_t)st * 7919u, stats)) goto gpu_fail;
+        /* the sweeps update e with FP32 atomics; recompute it exactly now
+         * and then so rounding drift cannot shift the fixed point */
+        if (opt.refresh_every > 0 && it % opt.refresh_every == 0 && mbir_hip_init_error(ctx))
+          goto gpu_fail;
         /* update magnitude relative to the mean object magnitude of one of
          * the reference's NHICD_Iterations line subsets (XT_ICD_update.c:
          * 963, 1058, 1083-1090) */

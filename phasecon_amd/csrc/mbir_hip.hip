@@ -41,6 +41,7 @@ constexpr int kNTMax = 256;    // detector rows (N_t) supported by k_sweep
 constexpr int kWMax = 33;      // window width 2*mult_xy+1 (mult_xy <= 16)
 constexpr int kWave = 64;
 constexpr size_t kSweepXMax = 48 * 1024;  // LDS bytes for a sweep tile slab
+constexpr size_t kSvLdsMax = 60 * 1024;   // dynamic LDS of the footprint sweep
 
 thread_local std::string g_err;
 
@@ -199,8 +200,8 @@ __global__ __launch_bounds__(256) void k_fwdproj(
     const size_t o = ((size_t)p * G.N_r + r) * G.N_t + t;
     const float dv = d ? d[(size_t)r * G.N_t + t] : 0.f;
     if (mode == 0) {
-      // XT_ICD_update.c:689, in float as the reference's arrays are
-      out[o] = y[o] - (float)acc[i] - dv;
+      // XT_ICD_update.c:689, rounded once to the float array
+      out[o] = (float)((double)y[o] - (double)acc[i] - (double)dv);
     } else if (mode == 1) {
       const double ev = (double)y[o] - (double)acc[i] - (double)dv;
       const double s = ev * sqrt((double)w[o]);
@@ -680,6 +681,337 @@ __global__ __launch_bounds__(kWave) void k_sweep(DevGeom G, SweepArgs A) {
   }
 }
 
+// ------------------------------------------------- LDS-footprint sweep
+// Wave-scope ordering: one wavefront per workgroup, and a wavefront's LDS
+// operations execute in issue order, so a compiler fence at wavefront scope
+// is enough between LDS phases (no s_waitcnt on in-flight global loads).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct SvArgs {
+  SweepArgs b;
+  int sync_every;  // columns between refreshes of the LDS error footprint
+  int fp_bins;     // capacity in detector bins per view of a tile footprint
+  int* overflow;   // set if a footprint exceeded fp_bins (host sizing bug)
+};
+
+// Dynamic LDS of the footprint sweep.
+struct SvLds {
+  float* x;
+  AWin win[2];
+  int *views, *fp_lo, *fp_off;
+  float *e, *w;
+  __device__ SvLds(float* base, int xfloats, int nvmax, int W, int fp_cap) {
+    x = base;
+    float* p = base + xfloats;
+    for (int b = 0; b < 2; ++b) {
+      win[b].a0 = reinterpret_cast<int*>(p);
+      p += nvmax;
+      win[b].av = p;
+      p += nvmax * W;
+    }
+    views = reinterpret_cast<int*>(p);
+    p += nvmax;
+    fp_lo = reinterpret_cast<int*>(p);
+    p += nvmax;
+    fp_off = reinterpret_cast<int*>(p);
+    p += nvmax + 1;
+    e = p;
+    p += fp_cap;
+    w = p;
+  }
+};
+
+// Loads the footprint's error (and, if with_w, weight) entries from global
+// memory: slot s covers bins fp_lo[s].. of view views[s], all rows, which is
+// one contiguous run of e.  Loads go out in batches of 8 per lane.
+__device__ void sv_load_footprint(const DevGeom& G, const ColumnArgs& A, const SvLds& L,
+                                  int nv, bool with_w) {
+  const int lane = threadIdx.x;
+  const int F = L.fp_off[nv];
+  int s = 0;
+  for (int i0 = lane; i0 < F; i0 += 8 * kWave) {
+    size_t o[8];
+    float ev[8], wv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = i0 + u * kWave;
+      o[u] = 0;
+      if (i < F) {
+        while (i >= L.fp_off[s + 1]) ++s;
+        const int j = i - L.fp_off[s];
+        o[u] = ((size_t)L.views[s] * G.N_r + L.fp_lo[s]) * G.N_t + j;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      ev[u] = e_load(A.e + o[u]);
+      wv[u] = with_w ? A.w[o[u]] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = i0 + u * kWave;
+      if (i < F) {
+        L.e[i] = ev[u];
+        if (with_w) L.w[i] = wv[u];
+      }
+    }
+  }
+}
+
+// The LDS-footprint variant of k_sweep.  The tile's error and weight
+// footprint (every detector bin any of its columns touches, in the frame's
+// views, all rows) is held in LDS: each column's theta1/theta2 and its error
+// update run from LDS, the update is also sent to the shared error sinogram
+// with atomics (fire and forget), and every sync_every columns the LDS copy
+// is reloaded from global memory after this wavefront's atomics drained, so
+// the other tiles' updates are seen at most sync_every columns late.
+__global__ __launch_bounds__(kWave) void k_sweep_sv(DevGeom G, SvArgs S) {
+  extern __shared__ __align__(16) float smem[];
+  __shared__ float s_tn[2 * kNZMax];
+  __shared__ float s_row1[kNTMax], s_row2[kNTMax];
+  __shared__ float s_part1[kWave], s_part2[kWave];
+  __shared__ float s_dx[kNZMax];
+  const SweepArgs& A = S.b;
+  const int lane = threadIdx.x;
+  const int wid = blockIdx.x;
+  const int tile = A.launch_tiles[wid % A.n_launch_tiles];
+  const int f = A.fparity + 2 * (wid / A.n_launch_tiles);
+  if (f >= G.N_time) return;
+  const int c0 = A.tile_ptr[tile];
+  const int n = A.tile_ptr[tile + 1] - c0;
+  const int s0 = A.frame_ptr[f];
+  const int nv = A.frame_ptr[f + 1] - s0;
+  if (n == 0 || nv == 0) return;
+  const int Nz = G.N_z, T = A.tile, BW = T + 2;
+  const SvLds L(smem, BW * BW * Nz, A.nvmax, G.W, A.nvmax * S.fp_bins * G.N_t);
+  const int oy = (tile / A.tiles_x) * T - 1, ox = (tile % A.tiles_x) * T - 1;
+  const size_t vol = (size_t)G.N_y * G.N_x * Nz;
+  float* xf = A.x + (size_t)f * vol;
+  const unsigned h = mix32(A.seed ^ mix32((unsigned)tile * 131071u + (unsigned)f * 524287u));
+  int pa = 1, pb = 0;
+  if (n > 1) {
+    pa = 1 + (int)(h % (unsigned)(n - 1));
+    while (gcd_i(pa, n) != 1) pa = pa % (n - 1) + 1;
+    pb = (int)(mix32(h + 0x9e3779b9u) % (unsigned)n);
+  }
+  // slab with halo, view list, first A window, footprint bounds
+  for (int j = lane; j < BW * BW * Nz; j += kWave) {
+    const int z = j % Nz, b = j / Nz;
+    const int yy = oy + b / BW, xx = ox + b % BW;
+    L.x[j] = (yy >= 0 && yy < G.N_y && xx >= 0 && xx < G.N_x)
+                 ? xf[((size_t)yy * G.N_x + xx) * Nz + z]
+                 : 0.f;
+  }
+  for (int j = lane; j < nv; j += kWave) {
+    L.views[j] = A.col.frame_views[s0 + j];
+    L.fp_lo[j] = 0x7fffffff;
+    L.fp_off[j] = -1;  // holds the highest bin until the prefix sum
+  }
+  APre pre;
+  awin_fetch(G, A.col, A.tile_cols[c0 + (int)((long long)pb % n)], s0, nv, pre);
+  wave_sync();
+  for (int j = lane; j < n * nv; j += kWave) {
+    const int i = j / nv, sl = j - i * nv;
+    const int c = A.tile_cols[c0 + i];
+    const int a0 = A.col.afirst[(size_t)c * G.nslots + s0 + sl];
+    if (a0 >= 0) {
+      atomicMin(&L.fp_lo[sl], a0);
+      atomicMax(&L.fp_off[sl], min(a0 + G.W - 1, G.N_r - 1));
+    }
+  }
+  awin_store(pre, nv, nv * G.W, L.win[0]);
+  wave_sync();
+  __shared__ int s_over;
+  if (lane == 0) {
+    int acc = 0, over = 0;
+    for (int sl = 0; sl < nv; ++sl) {
+      const int hi = L.fp_off[sl];
+      int nb = 0;
+      if (hi >= 0) {
+        nb = hi - L.fp_lo[sl] + 1;
+      } else {
+        L.fp_lo[sl] = 0;
+      }
+      if (nb > S.fp_bins) {
+        over = 1;
+        nb = S.fp_bins;
+      }
+      L.fp_off[sl] = acc;
+      acc += nb * G.N_t;
+    }
+    L.fp_off[nv] = acc;
+    s_over = over;
+    if (over) atomicOr(S.overflow, 1);
+  }
+  wave_sync();
+  if (s_over) return;  // the host sized fp_bins too small: nothing is touched
+  sv_load_footprint(G, A.col, L, nv, true);
+  wave_sync();
+  const int nent = nv * G.W;
+  const Lanes LN(G.N_t);
+  const float Th = G.huber_T, hd = G.huber_delta;
+  double st_dx = 0.0, st_x = 0.0, st_skip = 0.0, st_upd = 0.0;
+  for (int k = 0; k < n; ++k) {
+    const bool odd = (k & 1) != 0;
+    const AWin wcur = odd ? L.win[1] : L.win[0], wnext = odd ? L.win[0] : L.win[1];
+    const int c = A.tile_cols[c0 + (int)(((long long)pa * k + pb) % n)];
+    const int cy = c / G.N_x, cx = c % G.N_x;
+    const int ly = cy - oy, lx = cx - ox;
+    for (int j = lane; j < 2 * Nz; j += kWave) {
+      const int ff = j < Nz ? f - 1 : f + 1;
+      s_tn[j] = (ff >= 0 && ff < G.N_time) ? A.x[(size_t)ff * vol + (size_t)c * Nz + j % Nz] : 0.f;
+    }
+    for (int z = lane; z < Nz; z += kWave) s_dx[z] = 0.f;
+    if (k + 1 < n)
+      awin_fetch(G, A.col, A.tile_cols[c0 + (int)(((long long)pa * (k + 1) + pb) % n)], s0,
+                 nv, pre);
+    // theta1/theta2 per row from the LDS footprint
+    for (int t0 = 0; t0 < G.N_t; t0 += kWave) {
+      const int row = t0 + LN.rr;
+      float a1 = 0.f, a2 = 0.f;
+      if (LN.g < LN.groups && row < G.N_t) {
+        for (int sk = LN.g; sk < nent; sk += LN.groups) {
+          const int sl = sk / G.W;
+          const int r0 = wcur.a0[sl];
+          const float a0v = wcur.av[sk];
+          if (r0 < 0 || a0v == 0.f) continue;
+          const int r = r0 + (sk - sl * G.W);
+          if (r >= G.N_r) continue;
+          const int loc = L.fp_off[sl] + (r - L.fp_lo[sl]) * G.N_t + row;
+          const float ev = L.e[loc], wv = L.w[loc];
+          const float a = a0v * G.delta_t;
+          const float sw = sqrtf(wv);
+          const float hw = (fabsf(ev * sw) < Th) ? wv : Th * hd * sw / fabsf(ev);
+          a2 += a * a * hw;
+          a1 -= ev * a * hw;
+        }
+      }
+      s_part1[lane] = a1;
+      s_part2[lane] = a2;
+      wave_sync();
+      if (lane < LN.rpl && row < G.N_t) {
+        float q1 = 0.f, q2 = 0.f;
+        for (int gg = 0; gg < LN.groups; ++gg) {
+          q1 += s_part1[gg * LN.rpl + lane];
+          q2 += s_part2[gg * LN.rpl + lane];
+        }
+        s_row1[row] = q1;
+        s_row2[row] = q2;
+      }
+      wave_sync();
+    }
+    // surrogate solve, z parity halves
+    for (int par = 0; par < 2; ++par) {
+      const int nzp = (Nz - par + 1) / 2;
+      int lpv = 32;
+      while (lpv > 1 && lpv * nzp > kWave) lpv >>= 1;
+      const int vpp = kWave / lpv;
+      for (int vb = 0; vb < nzp; vb += vpp) {
+        const int vi = vb + lane / lpv, sub = lane % lpv;
+        const bool act = vi < nzp;
+        const int z = par + 2 * vi;
+        const int centre = ((ly * BW + lx) * Nz) + z;
+        const float V = act ? L.x[centre] : 0.f;
+        float num = 0.f, den = 0.f;
+        int pos = 0;
+        if (act) {
+          for (int j = sub; j < 29; j += lpv) {
+            if (j == 13) continue;
+            int ok;
+            float nbv;
+            if (j < 27) {
+              const int dz = j / 9 - 1, dy = (j / 3) % 3 - 1, dx = j % 3 - 1;
+              const int z2 = z + dz, y2 = cy + dy, x2 = cx + dx;
+              ok = z2 >= 0 && z2 < Nz && y2 >= 0 && y2 < G.N_y && x2 >= 0 && x2 < G.N_x;
+              nbv = ok ? L.x[((ly + dy) * BW + lx + dx) * Nz + z2] : 0.f;
+            } else {
+              const int ff = j == 27 ? f - 1 : f + 1;
+              ok = ff >= 0 && ff < G.N_time;
+              nbv = ok ? s_tn[(j - 27) * Nz + z] : 0.f;
+            }
+            if (nbv > 0.f) pos = 1;
+            if (!ok) continue;
+            const float wt = nb_weight(G, j, V, nbv);
+            num += wt * nbv;
+            den += wt;
+          }
+        }
+        for (int off = lpv / 2; off > 0; off >>= 1) {
+          num += __shfl_xor(num, off);
+          den += __shfl_xor(den, off);
+          pos |= __shfl_xor(pos, off);
+        }
+        if (act && sub == 0) {
+          if (V == 0.f && A.iter > 1 && !pos) {
+            st_skip += 1.0;
+          } else {
+            float th1 = 0.f, th2 = 0.f;
+            for (int m = 0; m < G.zo; ++m) {
+              th1 += s_row1[z * G.zo + m];
+              th2 += s_row2[z * G.zo + m];
+            }
+            const float xn = substitute(G, A.alpha, V, th1, th2, num, den);
+            L.x[centre] = xn;
+            s_dx[z] = xn - V;
+            st_dx += fabs((double)xn - (double)V);
+            st_x += fabs((double)xn);
+            st_upd += 1.0;
+          }
+        }
+      }
+      wave_sync();
+    }
+    // error update: LDS copy and the shared sinogram
+    for (int t0 = 0; t0 < G.N_t; t0 += kWave) {
+      const int row = t0 + LN.rr;
+      if (LN.g >= LN.groups || row >= G.N_t) continue;
+      const float dz = s_dx[row / G.zo];
+      if (dz == 0.f) continue;
+      for (int sk = LN.g; sk < nent; sk += LN.groups) {
+        const int sl = sk / G.W;
+        const int r0 = wcur.a0[sl];
+        const float a0v = wcur.av[sk];
+        const int r = r0 + (sk - sl * G.W);
+        if (r0 < 0 || a0v == 0.f || r >= G.N_r) continue;
+        const float d = -(a0v * G.delta_t) * dz;
+        L.e[L.fp_off[sl] + (r - L.fp_lo[sl]) * G.N_t + row] += d;
+        e_add(A.col.e + ((size_t)L.views[sl] * G.N_r + r) * G.N_t + row, d);
+      }
+    }
+    if (k + 1 < n) awin_store(pre, nv, nv * G.W, wnext);
+    wave_sync();
+    if (S.sync_every > 0 && (k + 1) % S.sync_every == 0 && k + 1 < n) {
+      // this wavefront's atomics are complete before the reload
+      __builtin_amdgcn_s_waitcnt(0);
+      sv_load_footprint(G, A.col, L, nv, false);
+      wave_sync();
+    }
+  }
+  for (int j = lane; j < BW * BW * Nz; j += kWave) {
+    const int z = j % Nz, b = j / Nz;
+    const int by = b / BW, bx = b % BW;
+    if (by == 0 || bx == 0 || by == BW - 1 || bx == BW - 1) continue;
+    const int yy = oy + by, xx = ox + bx;
+    if (yy < G.N_y && xx < G.N_x) xf[((size_t)yy * G.N_x + xx) * Nz + z] = L.x[j];
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    st_dx += __shfl_xor(st_dx, off);
+    st_x += __shfl_xor(st_x, off);
+    st_skip += __shfl_xor(st_skip, off);
+    st_upd += __shfl_xor(st_upd, off);
+  }
+  if (lane == 0) {
+    atomicAdd(&A.stats[0], st_dx);
+    atomicAdd(&A.stats[1], st_x);
+    atomicAdd(&A.stats[2], st_skip);
+    atomicAdd(&A.stats[3], st_upd);
+  }
+}
+
 // theta1/theta2 of individual voxels through the sweep's window routine.
 __global__ __launch_bounds__(kWave) void k_thetas(DevGeom G, ColumnArgs A,
                                                   const int* __restrict__ frame_ptr,
@@ -885,8 +1217,10 @@ struct mbir_hip_ctx {
   bool a_built = false;
   int *d_tile_ptr = nullptr, *d_tile_cols = nullptr, *d_launch_tiles = nullptr;
   int launch_off[5] = {0, 0, 0, 0, 0};
-  int tile = 16, tiles_for = -1;
+  int tile = 8, tiles_for = -1;
+  int sync_every = 8, use_sv = 0;
   double alpha = -1.0;
+  int* d_overflow = nullptr;
   double* d_stats = nullptr;
   double* d_partial = nullptr;
   size_t partial_cap = 0;
@@ -987,7 +1321,9 @@ size_t fwd_tchunk(const mbir_hip_geom& g, size_t acc_bytes) {
 
 int run_fwdproj(mbir_hip_ctx* c, int mode, const float* x, float* out) {
   const mbir_hip_geom& g = c->g;
-  const bool dbl = (mode == 1);
+  // FP64 accumulation for the error sinogram and the cost, so e is the
+  // correctly rounded y - A x - d however long the sweeps have run
+  const bool dbl = (mode == 0 || mode == 1);
   const size_t tc = fwd_tchunk(g, dbl ? 8 : 4);
   if ((size_t)g.N_r * (dbl ? 8 : 4) > 56 * 1024)
     return fail("detector too wide for the LDS projection accumulator");
@@ -1119,7 +1455,8 @@ void mbir_hip_destroy(mbir_hip_ctx* c) {
   void* bufs[] = {c->d_H, c->d_cos, c->d_sin, c->d_frame_ptr, c->d_frame_views,
                   c->d_view_frame, c->d_y, c->d_w, c->d_d, c->d_e, c->d_x,
                   c->d_stage, c->d_stage2, c->d_afirst, c->d_aval, c->d_tile_ptr,
-                  c->d_tile_cols, c->d_launch_tiles, c->d_stats, c->d_partial};
+                  c->d_tile_cols, c->d_launch_tiles, c->d_stats, c->d_partial,
+                  c->d_overflow};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (int k = 0; k < MBIR_K_COUNT; ++k)
@@ -1139,6 +1476,11 @@ int mbir_hip_set_option(mbir_hip_ctx* c, const char* key, double value) {
     c->tile = (int)value;
   } else if (k == "alpha") {
     c->alpha = value;
+  } else if (k == "sync") {
+    if (value < 0) return fail("sync must be >= 0");
+    c->sync_every = (int)value;
+  } else if (k == "sv") {
+    c->use_sv = value != 0;
   } else {
     return fail("unknown option " + k);
   }
@@ -1277,6 +1619,19 @@ int mbir_hip_icd_equits(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t
     return fail("tile too large for N_z: the tile slab must fit 48 KiB of LDS");
   const size_t xbytes =
       sizeof(float) * (xfl + 2 * ((size_t)c->nvmax + (size_t)c->nvmax * c->G.W) + c->nvmax);
+  // footprint variant: bins a T x T tile can touch in one view
+  SvArgs S;
+  S.sync_every = c->sync_every;
+  S.fp_bins = (int)(((c->tile - 1) * 1.41422 + 2.0) * c->g.mult_xy) + 3;
+  const size_t fp_cap = (size_t)c->nvmax * S.fp_bins * c->g.N_t;
+  const size_t svbytes = sizeof(float) * (xfl + 2 * ((size_t)c->nvmax + (size_t)c->nvmax * c->G.W) +
+                                          3 * (size_t)c->nvmax + 1 + 2 * fp_cap);
+  const bool sv = c->use_sv && svbytes <= kSvLdsMax;
+  if (sv && !c->d_overflow) {
+    if (dev_alloc(&c->d_overflow, 1)) return -1;
+    HIP_OK(hipMemset(c->d_overflow, 0, sizeof(int)));
+  }
+  S.overflow = c->d_overflow;
   const int nf = c->g.N_time;
   for (int it = 0; it < n; ++it) {
     A.iter = first_iter + it;
@@ -1291,8 +1646,14 @@ int mbir_hip_icd_equits(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t
         A.fparity = fp;
         A.seed = seed * 2654435761u + (unsigned)(it * 8 + colour * 2 + fp) * 40503u;
         if (ev_begin(c, MBIR_K_SWEEP)) return -1;
-        hipLaunchKernelGGL(k_sweep, dim3((unsigned)(ntl * frames)), dim3(kWave), xbytes,
-                           c->stream, c->G, A);
+        if (sv) {
+          S.b = A;
+          hipLaunchKernelGGL(k_sweep_sv, dim3((unsigned)(ntl * frames)), dim3(kWave), svbytes,
+                             c->stream, c->G, S);
+        } else {
+          hipLaunchKernelGGL(k_sweep, dim3((unsigned)(ntl * frames)), dim3(kWave), xbytes,
+                             c->stream, c->G, A);
+        }
         if (ev_end(c, MBIR_K_SWEEP)) return -1;
       }
     }
@@ -1301,6 +1662,12 @@ int mbir_hip_icd_equits(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t
     HIP_OK(hipMemcpyAsync(stats, c->d_stats, 4 * sizeof(double), hipMemcpyDeviceToHost,
                           c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
+  }
+  if (sv) {
+    int over = 0;
+    HIP_OK(hipMemcpyAsync(&over, c->d_overflow, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    if (over) return fail("tile footprint exceeded its LDS capacity");
   }
   return 0;
 }

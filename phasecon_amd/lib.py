@@ -32,7 +32,7 @@ class Options(C.Structure):
     _fields_ = [("p_s", C.c_double), ("p_t", C.c_double), ("tile", C.c_int32),
                 ("alpha", C.c_double), ("max_equits", C.c_int32),
                 ("single_stage", C.c_int32), ("write_files", C.c_int32),
-                ("stop_cost", C.c_double), ("seed", C.c_uint32)]
+                ("stop_cost", C.c_double), ("seed", C.c_uint32), ("refresh_every", C.c_int32)]
 
 
 _LIB = None
