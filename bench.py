@@ -17,6 +17,13 @@ over a bounded sample of equits of the same workload.
 
     python bench.py --longrun N --dump-x PATH   runs N equits and saves the
                                                 object (C* warm start)
+    python bench.py --profile-equits N          A build, error sinogram and N
+                                                equits only, for counter runs
+                                                under rocprofv3 (also runs the
+                                                object upload, whose transpose
+                                                kernel k_to_colmajor reads and
+                                                writes 4 B per voxel: the
+                                                traffic calibration copy)
 """
 from __future__ import annotations
 
@@ -133,6 +140,7 @@ def main():
     ap.add_argument("--cpu-equits", type=int, default=2)
     ap.add_argument("--longrun", type=int, default=0)
     ap.add_argument("--dump-x", default=None)
+    ap.add_argument("--profile-equits", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs one process on one GPU; multi-GPU (N > 1) is not "
@@ -148,6 +156,20 @@ def main():
     print(f"[bench] scan {args.workload} generated in {time.time() - t0:.1f}s", file=sys.stderr,
           flush=True)
     st = Stage(scan, tile=args.tile)
+
+    if args.profile_equits:
+        st.set_object(np.zeros(st.shape_x, np.float32))
+        st.build_amatrix()
+        st.init_error()
+        st.reset_kernel_time()
+        for it in range(1, args.profile_equits + 1):
+            st.icd(1, it, args.seed)
+        ms, n = st.kernel_time(K_SWEEP)
+        bytes_equit = algorithmic_bytes_per_equit(st)
+        print(json.dumps({"profile_equits": args.profile_equits, "sweep_launches": n,
+                          "sweep_avg_ms": ms / n, "bytes_per_launch": bytes_equit * args.profile_equits / n,
+                          "calibration_copy_bytes": 4 * int(np.prod(st.shape_x))}))
+        return
 
     if args.longrun:
         st.build_amatrix()

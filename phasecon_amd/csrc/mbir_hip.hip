@@ -303,31 +303,38 @@ __global__ __launch_bounds__(256) void k_prior_cost(DevGeom G,
 // --------------------------------------------------------- ICD building blocks
 // qGGMRF b(delta) = rho'(delta)/delta, FP32 (XT_ICD_update.c:92-128): with
 // r = |d|^(q-p)/sigma^(q-p), b = |d|^(q-2) (q - (q-p) r/(c+r)) / ((c+r) sigma^q),
-// and b(0) = q/(sigma^q c).
+// and b(0) = q/(sigma^q c).  |d|^(q-p) is one v_log_f32 / v_exp_f32 pair and
+// the divisions are v_rcp_f32 (each within about 1 ulp).
 __device__ __forceinline__ float qg_b(float d, float q, float p, float c,
                                       float inv_sq, float inv_sqp, float b0) {
   const float ad = fabsf(d);
   if (ad == 0.f) return b0;
-  const float lg = log2f(ad);
-  const float ratio = exp2f((q - p) * lg) * inv_sqp;
-  const float den = c + ratio;
-  const float shape = q - (q - p) * ratio / den;
-  const float mag = (q == 2.f) ? 1.f : exp2f((q - 2.f) * lg);
-  return mag * shape * inv_sq / den;
+  const float lg = __builtin_amdgcn_logf(ad);
+  const float ratio = __builtin_amdgcn_exp2f((q - p) * lg) * inv_sqp;
+  const float rden = __builtin_amdgcn_rcpf(c + ratio);
+  const float shape = q - (q - p) * ratio * rden;
+  const float mag = (q == 2.f) ? 1.f : __builtin_amdgcn_exp2f((q - 2.f) * lg);
+  return mag * shape * inv_sq * rden;
 }
 
 // Weight of neighbour j of a voxel with value V: j < 27 spatial ([dz][dy][dx],
-// 13 is the centre and never used), 27/28 = t-1/t+1.
-__device__ __forceinline__ float nb_weight(const DevGeom& G, int j, float V, float nbv) {
+// 13 is the centre and never used), 27/28 = t-1/t+1; filt[0..27] holds the
+// 27 spatial weights and the temporal one (in LDS).
+__device__ __forceinline__ float nb_weight(const DevGeom& G, const float* filt, int j,
+                                           float V, float nbv) {
   if (j < 27)
-    return G.filt[j] * qg_b(V - nbv, G.q, G.p_s, G.c_s, G.inv_sq_s, G.inv_sqp_s, G.b0_s);
-  return G.filt_t * qg_b(V - nbv, G.q, G.p_t, G.c_t, G.inv_sq_t, G.inv_sqp_t, G.b0_t);
+    return filt[j] * qg_b(V - nbv, G.q, G.p_s, G.c_s, G.inv_sq_s, G.inv_sqp_s, G.b0_s);
+  return filt[27] * qg_b(V - nbv, G.q, G.p_t, G.c_t, G.inv_sq_t, G.inv_sqp_t, G.b0_t);
+}
+
+__device__ __forceinline__ void load_filter(const DevGeom& G, float* filt) {
+  for (int j = threadIdx.x; j < 28; j += blockDim.x) filt[j] = j < 27 ? G.filt[j] : G.filt_t;
 }
 
 // Surrogate solve and over-relaxation (XT_ICD_update.c:174-176).
 __device__ __forceinline__ float substitute(const DevGeom& G, float alpha, float V,
                                             float th1, float th2, float num, float den) {
-  const float u = (num + th2 * V - th1) / (den + th2);
+  const float u = (num + th2 * V - th1) * __builtin_amdgcn_rcpf(den + th2);
   return V + alpha * (u - V);
 }
 
@@ -459,8 +466,11 @@ __device__ void window_thetas(const DevGeom& G, const ColumnArgs& A, const AWin&
       }
 #pragma unroll
       for (int i = 0; i < kECh; ++i) {
-        const float sw = sqrtf(wv[i]);
-        const float hw = (fabsf(ev[i] * sw) < T) ? wv[i] : T * hd * sw / fabsf(ev[i]);
+        // |e sqrt(w)| < T tested as e^2 w < T^2; the generalized-Huber weight
+        // T delta sqrt(w)/|e| only where an entry is an outlier
+        float hw = wv[i];
+        if (ev[i] * ev[i] * wv[i] >= T * T)
+          hw = T * hd * sqrtf(wv[i]) * __builtin_amdgcn_rcpf(fabsf(ev[i]));
         a2 += av[i] * av[i] * hw;
         a1 -= ev[i] * av[i] * hw;
       }
@@ -536,8 +546,10 @@ __global__ __launch_bounds__(kWave) void k_sweep(DevGeom G, SweepArgs A) {
   __shared__ float s_row1[kNTMax], s_row2[kNTMax];
   __shared__ float s_part1[kWave], s_part2[kWave];
   __shared__ float s_dx[kNZMax];
+  __shared__ float s_filt[28];
   const int lane = threadIdx.x;
   const int wid = blockIdx.x;
+  load_filter(G, s_filt);
   const int tile = A.launch_tiles[wid % A.n_launch_tiles];
   const int f = A.fparity + 2 * (wid / A.n_launch_tiles);
   if (f >= G.N_time) return;
@@ -573,14 +585,17 @@ __global__ __launch_bounds__(kWave) void k_sweep(DevGeom G, SweepArgs A) {
   for (int j = lane; j < nv; j += kWave) s_views[j] = A.col.frame_views[s0 + j];
   const int nav = nv * G.W;
   APre pre;
-  awin_fetch(G, A.col, A.tile_cols[c0 + (int)((long long)pb % n)], s0, nv, pre);
+  int pos = pb;  // visiting order: pos_k = (pb + k pa) mod n
+  awin_fetch(G, A.col, A.tile_cols[c0 + pos], s0, nv, pre);
   awin_store(pre, nv, nav, win0);
   __syncthreads();
   double st_dx = 0.0, st_x = 0.0, st_skip = 0.0, st_upd = 0.0;
   for (int k = 0; k < n; ++k) {
     const bool odd = (k & 1) != 0;
     const AWin wcur = odd ? win1 : win0, wnext = odd ? win0 : win1;
-    const int c = A.tile_cols[c0 + (int)(((long long)pa * k + pb) % n)];
+    const int c = A.tile_cols[c0 + pos];
+    pos += pa;
+    if (pos >= n) pos -= n;
     const int cy = c / G.N_x, cx = c % G.N_x;
     const int ly = cy - oy, lx = cx - ox;
     // t-1 / t+1 columns (read-only in this launch)
@@ -589,9 +604,7 @@ __global__ __launch_bounds__(kWave) void k_sweep(DevGeom G, SweepArgs A) {
       s_tn[j] = (ff >= 0 && ff < G.N_time) ? A.x[(size_t)ff * vol + (size_t)c * Nz + j % Nz] : 0.f;
     }
     for (int z = lane; z < Nz; z += kWave) s_dx[z] = 0.f;
-    if (k + 1 < n)
-      awin_fetch(G, A.col, A.tile_cols[c0 + (int)(((long long)pa * (k + 1) + pb) % n)], s0,
-                 nv, pre);
+    if (k + 1 < n) awin_fetch(G, A.col, A.tile_cols[c0 + pos], s0, nv, pre);
     window_thetas(G, A.col, wcur, s_views, nv, s_row1, s_row2, s_part1, s_part2);
     // surrogate solve, z parity halves
     for (int par = 0; par < 2; ++par) {
@@ -624,7 +637,7 @@ __global__ __launch_bounds__(kWave) void k_sweep(DevGeom G, SweepArgs A) {
             }
             if (nbv > 0.f) pos = 1;
             if (!ok) continue;
-            const float wt = nb_weight(G, j, V, nbv);
+            const float wt = nb_weight(G, s_filt, j, V, nbv);
             num += wt * nbv;
             den += wt;
           }
@@ -775,8 +788,10 @@ __global__ __launch_bounds__(kWave) void k_sweep_sv(DevGeom G, SvArgs S) {
   __shared__ float s_row1[kNTMax], s_row2[kNTMax];
   __shared__ float s_part1[kWave], s_part2[kWave];
   __shared__ float s_dx[kNZMax];
+  __shared__ float s_filt[28];
   const SweepArgs& A = S.b;
   const int lane = threadIdx.x;
+  load_filter(G, s_filt);
   const int wid = blockIdx.x;
   const int tile = A.launch_tiles[wid % A.n_launch_tiles];
   const int f = A.fparity + 2 * (wid / A.n_launch_tiles);
@@ -812,7 +827,8 @@ __global__ __launch_bounds__(kWave) void k_sweep_sv(DevGeom G, SvArgs S) {
     L.fp_off[j] = -1;  // holds the highest bin until the prefix sum
   }
   APre pre;
-  awin_fetch(G, A.col, A.tile_cols[c0 + (int)((long long)pb % n)], s0, nv, pre);
+  int pos = pb;  // visiting order: pos_k = (pb + k pa) mod n
+  awin_fetch(G, A.col, A.tile_cols[c0 + pos], s0, nv, pre);
   wave_sync();
   for (int j = lane; j < n * nv; j += kWave) {
     const int i = j / nv, sl = j - i * nv;
@@ -858,7 +874,9 @@ __global__ __launch_bounds__(kWave) void k_sweep_sv(DevGeom G, SvArgs S) {
   for (int k = 0; k < n; ++k) {
     const bool odd = (k & 1) != 0;
     const AWin wcur = odd ? L.win[1] : L.win[0], wnext = odd ? L.win[0] : L.win[1];
-    const int c = A.tile_cols[c0 + (int)(((long long)pa * k + pb) % n)];
+    const int c = A.tile_cols[c0 + pos];
+    pos += pa;
+    if (pos >= n) pos -= n;
     const int cy = c / G.N_x, cx = c % G.N_x;
     const int ly = cy - oy, lx = cx - ox;
     for (int j = lane; j < 2 * Nz; j += kWave) {
@@ -866,9 +884,7 @@ __global__ __launch_bounds__(kWave) void k_sweep_sv(DevGeom G, SvArgs S) {
       s_tn[j] = (ff >= 0 && ff < G.N_time) ? A.x[(size_t)ff * vol + (size_t)c * Nz + j % Nz] : 0.f;
     }
     for (int z = lane; z < Nz; z += kWave) s_dx[z] = 0.f;
-    if (k + 1 < n)
-      awin_fetch(G, A.col, A.tile_cols[c0 + (int)(((long long)pa * (k + 1) + pb) % n)], s0,
-                 nv, pre);
+    if (k + 1 < n) awin_fetch(G, A.col, A.tile_cols[c0 + pos], s0, nv, pre);
     // theta1/theta2 per row from the LDS footprint
     for (int t0 = 0; t0 < G.N_t; t0 += kWave) {
       const int row = t0 + LN.rr;
@@ -884,8 +900,8 @@ __global__ __launch_bounds__(kWave) void k_sweep_sv(DevGeom G, SvArgs S) {
           const int loc = L.fp_off[sl] + (r - L.fp_lo[sl]) * G.N_t + row;
           const float ev = L.e[loc], wv = L.w[loc];
           const float a = a0v * G.delta_t;
-          const float sw = sqrtf(wv);
-          const float hw = (fabsf(ev * sw) < Th) ? wv : Th * hd * sw / fabsf(ev);
+          float hw = wv;
+          if (ev * ev * wv >= Th * Th) hw = Th * hd * sqrtf(wv) * __builtin_amdgcn_rcpf(fabsf(ev));
           a2 += a * a * hw;
           a1 -= ev * a * hw;
         }
@@ -935,7 +951,7 @@ __global__ __launch_bounds__(kWave) void k_sweep_sv(DevGeom G, SvArgs S) {
             }
             if (nbv > 0.f) pos = 1;
             if (!ok) continue;
-            const float wt = nb_weight(G, j, V, nbv);
+            const float wt = nb_weight(G, s_filt, j, V, nbv);
             num += wt * nbv;
             den += wt;
           }
@@ -1057,6 +1073,9 @@ __global__ __launch_bounds__(kWave) void k_subst(DevGeom G, int n,
                                                  const unsigned char* __restrict__ tf,
                                                  float* __restrict__ out) {
   constexpr int lpv = 16;
+  __shared__ float s_filt[28];
+  load_filter(G, s_filt);
+  __syncthreads();
   const int lane = threadIdx.x;
   const int i = blockIdx.x * (kWave / lpv) + lane / lpv;
   const int sub = lane % lpv;
@@ -1069,7 +1088,7 @@ __global__ __launch_bounds__(kWave) void k_subst(DevGeom G, int n,
       const bool ok = j < 27 ? nbf[(size_t)i * 27 + j] != 0 : tf[(size_t)i * 2 + j - 27] != 0;
       if (!ok) continue;
       const float nbv = j < 27 ? nb[(size_t)i * 27 + j] : tn[(size_t)i * 2 + j - 27];
-      const float wt = nb_weight(G, j, v, nbv);
+      const float wt = nb_weight(G, s_filt, j, v, nbv);
       num += wt * nbv;
       den += wt;
     }
