@@ -143,8 +143,8 @@ def main():
     ap.add_argument("--profile-equits", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs one process on one GPU; multi-GPU (N > 1) is not "
-              "part of it (see DESIGN.md for how the path would shard).", file=sys.stderr)
+        print("bench.py: the MBIR sweep runs as one process on one GPU; --gpus must be 1 "
+              "(multi-GPU runs are not supported).", file=sys.stderr)
         sys.exit(2)
 
     from phasecon_amd.synth import make_scan, WORKLOADS

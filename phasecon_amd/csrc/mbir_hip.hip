@@ -41,7 +41,6 @@ constexpr int kNTMax = 256;    // detector rows (N_t) supported by k_sweep
 constexpr int kWMax = 33;      // window width 2*mult_xy+1 (mult_xy <= 16)
 constexpr int kWave = 64;
 constexpr size_t kSweepXMax = 48 * 1024;  // LDS bytes for a sweep tile slab
-constexpr size_t kSvLdsMax = 60 * 1024;   // dynamic LDS of the footprint sweep
 
 thread_local std::string g_err;
 
@@ -56,6 +55,19 @@ int fail(const std::string& msg) {
     if (e_ != hipSuccess)                                                   \
       return fail(std::string(#call) + ": " + hipGetErrorString(e_));       \
   } while (0)
+
+// Runs CALL with the compile-time window width kW = W (2 mult_xy + 1) of
+// the sweep kernels.
+#define MBIR_DISPATCH_W(W, CALL)                                                  \
+  switch (W) {                                                                    \
+    case 3: { constexpr int kW = 3; CALL; } break;                                \
+    case 5: { constexpr int kW = 5; CALL; } break;                                \
+    case 7: { constexpr int kW = 7; CALL; } break;                                \
+    case 9: { constexpr int kW = 9; CALL; } break;                                \
+    case 17: { constexpr int kW = 17; CALL; } break;                              \
+    case 33: { constexpr int kW = 33; CALL; } break;                              \
+    default: return fail("mult_xy must be 1, 2, 3, 4, 8 or 16 for the sweep");    \
+  }
 
 // Parameters passed to kernels by value.
 struct DevGeom {
@@ -346,6 +358,15 @@ __device__ __forceinline__ void e_add(float* p, float v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Wave-scope ordering: one wavefront per workgroup, and a wavefront's LDS
+// operations execute in issue order, so a compiler fence at wavefront scope
+// is enough between LDS phases (no s_waitcnt on in-flight global loads).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 struct ColumnArgs {
   const float* aval;
   const int* afirst;
@@ -356,38 +377,49 @@ struct ColumnArgs {
 
 constexpr int kMaxViews = 512;  // views per frame held in LDS
 constexpr int kMaxAV = 1024;    // A values per (column, frame) held in LDS
-constexpr int kECh = 6;         // sinogram entries per lane per load batch
 constexpr int kPreA = kMaxAV / kWave;
 constexpr int kPreI = kMaxViews / kWave;
+constexpr int kTnPre = 2 * kNZMax / kWave;  // t-1/t+1 column values per lane
 
-// One (column, frame) A window in LDS: first bin per view slot, W values each.
-struct AWin {
-  int* a0;
+// One (column, frame) A window in LDS: per view slot the error-sinogram
+// offset of the window's first bin at row 0 (-1 where the column misses the
+// detector in that view), and the W values of each slot.
+struct Win {
+  int* base;
   float* av;
 };
 
-// Dynamic LDS of the sweep: the tile slab, two A windows, the view list.
+// Dynamic LDS of the sweep: the tile slab, two A windows, the frame's view
+// list and the tile's column list (sizes in 4-byte words).
+__host__ __device__ inline size_t sweep_lds_words(size_t xfloats, int nvmax, int W, int ncols) {
+  return xfloats + 2 * ((size_t)nvmax + (size_t)nvmax * W) + (size_t)nvmax + (size_t)ncols;
+}
+
 struct SweepLds {
   float* x;
-  AWin win[2];
+  Win win[2];
   int* views;
+  int* cols;
   __device__ SweepLds(float* base, int xfloats, int nvmax, int W) {
     x = base;
     float* p = base + xfloats;
     for (int b = 0; b < 2; ++b) {
-      win[b].a0 = reinterpret_cast<int*>(p);
+      win[b].base = reinterpret_cast<int*>(p);
       p += nvmax;
       win[b].av = p;
       p += nvmax * W;
     }
     views = reinterpret_cast<int*>(p);
+    p += nvmax;
+    cols = reinterpret_cast<int*>(p);
   }
 };
 
-// Register staging of the next column's A window (prefetch).
+// Register staging of the next column's A window and t+-1 values (prefetch).
 struct APre {
   int a0[kPreI];
   float av[kPreA];
+  float tn[kTnPre];
 };
 
 __device__ __forceinline__ void awin_fetch(const DevGeom& G, const ColumnArgs& A, int c,
@@ -407,110 +439,275 @@ __device__ __forceinline__ void awin_fetch(const DevGeom& G, const ColumnArgs& A
   }
 }
 
-__device__ __forceinline__ void awin_store(const APre& pre, int nv, int nav, const AWin& win) {
+// The t-1 and t+1 columns of voxel column c (not updated in a sweep launch).
+__device__ __forceinline__ void tn_fetch(const DevGeom& G, const float* x, int f, int c,
+                                         size_t vol, APre& pre) {
+#pragma unroll
+  for (int i = 0; i < kTnPre; ++i) {
+    const int j = threadIdx.x + i * kWave;
+    const int ff = j < G.N_z ? f - 1 : f + 1;
+    const bool ok = j < 2 * G.N_z && ff >= 0 && ff < G.N_time;
+    pre.tn[i] = ok ? x[(size_t)ff * vol + (size_t)c * G.N_z + j % G.N_z] : 0.f;
+  }
+}
+
+// Stores a prefetched window (first bins turned into sinogram offsets
+// through the frame's view list) and t+-1 values.
+__device__ __forceinline__ void awin_store(const DevGeom& G, const APre& pre, int nv,
+                                           const int* views, const Win& win, float* tn) {
   const int lane = threadIdx.x;
 #pragma unroll
   for (int i = 0; i < kPreI; ++i) {
     const int j = lane + i * kWave;
-    if (j < nv) win.a0[j] = pre.a0[i];
+    if (j < nv) win.base[j] = pre.a0[i] >= 0 ? (views[j] * G.N_r + pre.a0[i]) * G.N_t : -1;
   }
+  const int nav = nv * G.W;
 #pragma unroll
   for (int i = 0; i < kPreA; ++i) {
     const int j = lane + i * kWave;
     if (j < nav) win.av[j] = pre.av[i];
   }
+#pragma unroll
+  for (int i = 0; i < kTnPre; ++i) {
+    const int j = lane + i * kWave;
+    if (j < 2 * G.N_z) tn[j] = pre.tn[i];
+  }
 }
 
-// Lane layout over a column's sinogram footprint: lane = (entry group g,
-// detector row rr); each entry is one (view slot, bin) pair of the window,
-// whose rows are contiguous in e and w.
+// Lane layout over a column's sinogram footprint: lane = (slot group g,
+// detector row rr); a lane takes view slots g, g + groups, ... in row rr (and
+// rr + rpl, ... when N_t > 64).  A slot's W bins lie N_t apart in e and w.
+// With a power-of-two rpl the slot groups of a row are lanes rr + k rpl,
+// reduced with lane shuffles.
 struct Lanes {
   int rpl, groups, g, rr;
+  bool pow2;
   __device__ Lanes(int N_t) {
     rpl = N_t < kWave ? N_t : kWave;
     groups = kWave / rpl;
     g = threadIdx.x / rpl;
     rr = threadIdx.x % rpl;
+    pow2 = (rpl & (rpl - 1)) == 0;
   }
 };
 
 // theta1/theta2 per detector row of one voxel column (XT_VoxUpdate.c:56-85,
-// generalized-Huber weight included): each lane issues its batch of e/w
-// loads back to back, then accumulates; per-row sums land in row1/row2.
-__device__ void window_thetas(const DevGeom& G, const ColumnArgs& A, const AWin& win,
-                              const int* views, int nv, float* row1, float* row2,
-                              float* part1, float* part2) {
+// generalized-Huber weight included).  Each lane issues the e/w loads of a
+// batch of its slots back to back, then accumulates; row sums land in
+// row1/row2 (part1/part2: LDS partials for the non-power-of-two case).
+template <int kW>
+__device__ __forceinline__ void col_thetas(const DevGeom& G, const ColumnArgs& A, const Win& win, int nv,
+                           float* row1, float* row2, float* part1, float* part2) {
+  constexpr int SB = kW <= 5 ? 2 : 1;  // slots per load batch
   const int lane = threadIdx.x;
   const Lanes L(G.N_t);
-  const int nent = nv * G.W;
-  const float T = G.huber_T, hd = G.huber_delta;
-  for (int t0 = 0; t0 < G.N_t; t0 += kWave) {
+  const float T2 = G.huber_T * G.huber_T, Thd = G.huber_T * G.huber_delta;
+  for (int t0 = 0; t0 < G.N_t; t0 += L.rpl) {
     const int row = t0 + L.rr;
     const bool lane_ok = L.g < L.groups && row < G.N_t;
     float a1 = 0.f, a2 = 0.f;
-    for (int base = 0; base < nent; base += L.groups * kECh) {
-      float ev[kECh], wv[kECh], av[kECh];
+    for (int sb = 0; sb < nv; sb += SB * L.groups) {
+      float ev[SB][kW], wv[SB][kW], av[SB][kW];
 #pragma unroll
-      for (int i = 0; i < kECh; ++i) {
-        const int sk = base + L.g + i * L.groups;
-        bool ok = lane_ok && sk < nent;
-        const int s = ok ? sk / G.W : 0;
-        const int r0 = ok ? win.a0[s] : -1;
-        const float a = ok ? win.av[sk] : 0.f;
-        const int r = r0 + (sk - s * G.W);
-        ok = ok && r0 >= 0 && a != 0.f && r < G.N_r;
-        const size_t o = ok ? ((size_t)views[s] * G.N_r + r) * G.N_t + row : 0;
-        ev[i] = e_load(A.e + o);
-        wv[i] = ok ? A.w[o] : 0.f;
-        av[i] = ok ? a * G.delta_t : 0.f;
+      for (int b = 0; b < SB; ++b) {
+        const int sl = sb + L.g + b * L.groups;
+        const int base = (lane_ok && sl < nv) ? win.base[sl] : -1;
+#pragma unroll
+        for (int k = 0; k < kW; ++k) {
+          const float a = base >= 0 ? win.av[sl * kW + k] : 0.f;
+          const bool ok = a != 0.f;
+          const int o = ok ? base + k * G.N_t + row : 0;
+          const float e = e_load(A.e + o);
+          ev[b][k] = ok ? e : 0.f;
+          wv[b][k] = ok ? A.w[o] : 0.f;
+          av[b][k] = a * G.delta_t;
+        }
       }
 #pragma unroll
-      for (int i = 0; i < kECh; ++i) {
-        // |e sqrt(w)| < T tested as e^2 w < T^2; the generalized-Huber weight
-        // T delta sqrt(w)/|e| only where an entry is an outlier
-        float hw = wv[i];
-        if (ev[i] * ev[i] * wv[i] >= T * T)
-          hw = T * hd * sqrtf(wv[i]) * __builtin_amdgcn_rcpf(fabsf(ev[i]));
-        a2 += av[i] * av[i] * hw;
-        a1 -= ev[i] * av[i] * hw;
+      for (int b = 0; b < SB; ++b)
+#pragma unroll
+        for (int k = 0; k < kW; ++k) {
+          // |e sqrt(w)| < T tested as e^2 w < T^2; the generalized-Huber
+          // weight T delta sqrt(w)/|e| only where an entry is an outlier
+          const float e = ev[b][k], w = wv[b][k], a = av[b][k];
+          float hw = w;
+          if (e * e * w >= T2) hw = Thd * sqrtf(w) * __builtin_amdgcn_rcpf(fabsf(e));
+          a2 += a * a * hw;
+          a1 -= e * a * hw;
+        }
+    }
+    if (L.pow2) {
+      for (int off = L.rpl; off < kWave; off <<= 1) {
+        a1 += __shfl_xor(a1, off);
+        a2 += __shfl_xor(a2, off);
+      }
+      if (L.g == 0 && row < G.N_t) {
+        row1[row] = a1;
+        row2[row] = a2;
+      }
+    } else {
+      part1[lane] = a1;
+      part2[lane] = a2;
+      wave_sync();
+      if (lane < L.rpl && row < G.N_t) {
+        float s1 = 0.f, s2 = 0.f;
+        for (int gg = 0; gg < L.groups; ++gg) {
+          s1 += part1[gg * L.rpl + lane];
+          s2 += part2[gg * L.rpl + lane];
+        }
+        row1[row] = s1;
+        row2[row] = s2;
       }
     }
-    part1[lane] = a1;
-    part2[lane] = a2;
-    __syncthreads();
-    if (lane < L.rpl && row < G.N_t) {
-      float s1 = 0.f, s2 = 0.f;
-      for (int gg = 0; gg < L.groups; ++gg) {
-        s1 += part1[gg * L.rpl + lane];
-        s2 += part2[gg * L.rpl + lane];
-      }
-      row1[row] = s1;
-      row2[row] = s2;
-    }
-    __syncthreads();
+    wave_sync();
   }
 }
 
 // e -= A delta_t dx along one voxel column (XT_VoxUpdate.c:96-116), with
 // atomics because other wavefronts' columns share detector bins.
-__device__ void window_update_error(const DevGeom& G, const ColumnArgs& A, const AWin& win,
-                                    const int* views, int nv, const float* dxz) {
+template <int kW>
+__device__ __forceinline__ void col_update(const DevGeom& G, const ColumnArgs& A, const Win& win, int nv,
+                           const float* dxz) {
   const Lanes L(G.N_t);
-  const int nent = nv * G.W;
-  for (int t0 = 0; t0 < G.N_t; t0 += kWave) {
+  for (int t0 = 0; t0 < G.N_t; t0 += L.rpl) {
     const int row = t0 + L.rr;
     if (L.g >= L.groups || row >= G.N_t) continue;
     const float dz = dxz[row / G.zo];
     if (dz == 0.f) continue;
-    for (int sk = L.g; sk < nent; sk += L.groups) {
-      const int s = sk / G.W;
-      const int r0 = win.a0[s];
-      const float a = win.av[sk];
-      const int r = r0 + (sk - s * G.W);
-      if (r0 < 0 || a == 0.f || r >= G.N_r) continue;
-      e_add(A.e + ((size_t)views[s] * G.N_r + r) * G.N_t + row, -(a * G.delta_t) * dz);
+    for (int sl = L.g; sl < nv; sl += L.groups) {
+      const int base = win.base[sl];
+      if (base < 0) continue;
+#pragma unroll
+      for (int k = 0; k < kW; ++k) {
+        const float a = win.av[sl * kW + k];
+        if (a != 0.f) e_add(A.e + base + k * G.N_t + row, -(a * G.delta_t) * dz);
+      }
     }
   }
+}
+
+// Lanes per voxel for the fixed part of the surrogate: the largest power of
+// two with lanes-per-voxel x min(N_z, 64) <= 64.
+__device__ __forceinline__ int fixed_lpv(int Nz) {
+  const int v = Nz < kWave ? Nz : kWave;
+  int lpv = 1;
+  while (2 * lpv * v <= kWave) lpv <<= 1;
+  return lpv;
+}
+
+// Neighbour m (0..25) outside the voxel's own column: the 24 spatial
+// neighbours ([dz][dy][dx] index j, skipping 4, 13 and 22, the column's own
+// z-1, centre and z+1) and t-1 / t+1 (j = 27, 28).
+__device__ __forceinline__ int fixed_nb(int m) {
+  return m < 24 ? m + (m >= 4) + (m >= 12) + (m >= 20) : m + 3;
+}
+
+// The part of each voxel's surrogate (XT_ICD_update.c:133-182) that the
+// column's own update leaves unchanged: the terms of its neighbours outside
+// the column, evaluated once per column at the voxel's current value V.
+// lpv lanes per voxel, reduced with lane shuffles; writes fnum (sum of
+// weight x neighbour), fden (sum of weights) and fpos (some neighbour > 0).
+__device__ __forceinline__ void col_fixed(const DevGeom& G, const float* filt, const float* sx, int BW, int ly,
+                          int lx, int cy, int cx, int f, const float* tn, float* fnum,
+                          float* fden, int* fpos) {
+  const int lane = threadIdx.x, Nz = G.N_z;
+  const int lpv = fixed_lpv(Nz), vpp = kWave / lpv;
+  const int sub = lane % lpv;
+  for (int zb = 0; zb < Nz; zb += vpp) {
+    const int z = zb + lane / lpv;
+    const bool act = z < Nz;
+    const int centre = (ly * BW + lx) * Nz + z;
+    const float V = act ? sx[centre] : 0.f;
+    float num = 0.f, den = 0.f;
+    int pos = 0;
+    if (act) {
+      for (int m = sub; m < 26; m += lpv) {
+        const int j = fixed_nb(m);
+        bool ok;
+        float nbv;
+        if (j < 27) {
+          const int dz = j / 9 - 1, dy = (j / 3) % 3 - 1, dx = j % 3 - 1;
+          ok = z + dz >= 0 && z + dz < Nz && cy + dy >= 0 && cy + dy < G.N_y && cx + dx >= 0 &&
+               cx + dx < G.N_x;
+          nbv = ok ? sx[centre + (dy * BW + dx) * Nz + dz] : 0.f;
+        } else {
+          const int ff = j == 27 ? f - 1 : f + 1;
+          ok = ff >= 0 && ff < G.N_time;
+          nbv = ok ? tn[(j - 27) * Nz + z] : 0.f;
+        }
+        if (nbv > 0.f) pos = 1;
+        if (ok) {
+          const float wt = nb_weight(G, filt, j, V, nbv);
+          num += wt * nbv;
+          den += wt;
+        }
+      }
+    }
+    for (int off = lpv / 2; off > 0; off >>= 1) {
+      num += __shfl_xor(num, off);
+      den += __shfl_xor(den, off);
+      pos |= __shfl_xor(pos, off);
+    }
+    if (act && sub == 0) {
+      fnum[z] = num;
+      fden[z] = den;
+      fpos[z] = pos;
+    }
+  }
+}
+
+struct SolveStats {
+  double dx = 0.0, x = 0.0, skip = 0.0, upd = 0.0;
+};
+
+// Solves the voxels of one z parity of the column, one lane per voxel: the
+// fixed part plus the column's own z-1 / z+1 neighbours at their current
+// values, zero skipping (XT_VoxUpdate.c:238-262), the surrogate minimiser and
+// over-relaxation (XT_ICD_update.c:174-176).  New values go to the slab,
+// changes to dxz.
+__device__ __forceinline__ void col_solve(const DevGeom& G, const float* filt, float* sx, int centre0, int par,
+                          int iter, float alpha, const float* fnum, const float* fden,
+                          const int* fpos, const float* row1, const float* row2, float* dxz,
+                          SolveStats& st) {
+  const int Nz = G.N_z;
+  const int nzp = (Nz - par + 1) / 2;
+  const int vi = threadIdx.x;
+  if (vi >= nzp) return;
+  const int z = par + 2 * vi;
+  const int centre = centre0 + z;
+  const float V = sx[centre];
+  float num = fnum[z], den = fden[z];
+  int pos = fpos[z];
+  if (z > 0) {
+    const float nbv = sx[centre - 1];
+    const float wt = nb_weight(G, filt, 4, V, nbv);
+    num += wt * nbv;
+    den += wt;
+    if (nbv > 0.f) pos = 1;
+  }
+  if (z + 1 < Nz) {
+    const float nbv = sx[centre + 1];
+    const float wt = nb_weight(G, filt, 22, V, nbv);
+    num += wt * nbv;
+    den += wt;
+    if (nbv > 0.f) pos = 1;
+  }
+  const bool skip = V == 0.f && iter > 1 && !pos;
+  float xn = V;
+  if (!skip) {
+    float th1 = 0.f, th2 = 0.f;
+    for (int m = 0; m < G.zo; ++m) {
+      th1 += row1[z * G.zo + m];
+      th2 += row2[z * G.zo + m];
+    }
+    xn = substitute(G, alpha, V, th1, th2, num, den);
+  }
+  sx[centre] = xn;
+  dxz[z] = xn - V;
+  st.skip += skip ? 1.0 : 0.0;
+  st.upd += skip ? 0.0 : 1.0;
+  st.dx += fabs((double)xn - (double)V);
+  st.x += skip ? 0.0 : fabs((double)xn);
 }
 
 struct SweepArgs {
@@ -535,17 +732,21 @@ struct SweepArgs {
 // 2x2-checkerboard colour) for the frames of one parity, one wavefront per
 // (tile, frame).  The tile's object slab plus a one-column halo lives in LDS
 // for the whole pass (neighbouring tiles and adjacent frames are not updated
-// in the same launch, so the halo and the t+-1 values are constant); the
-// next column's A window is prefetched into a second LDS buffer while the
-// current column is solved; the error sinogram is shared with the other
-// wavefronts and updated with atomics.  Within a column, z is solved in two
-// parity halves so each voxel sees its z+-1 neighbours' new values.
+// in the same launch, so the halo and the t+-1 values are constant), with the
+// tile's column list and the frame's view list.  Per column: theta1/theta2
+// from the shared error sinogram, the surrogate's fixed part (neighbours
+// outside the column), the z-parity halves (each voxel sees its z+-1
+// neighbours' new values), and the error update with atomics; the next
+// column's A window and t+-1 values are prefetched into registers meanwhile.
+// kW = W, the A window width.
+template <int kW>
 __global__ __launch_bounds__(kWave) void k_sweep(DevGeom G, SweepArgs A) {
   extern __shared__ __align__(16) float smem[];
-  __shared__ float s_tn[2 * kNZMax];
+  __shared__ float s_tn[2][2 * kNZMax];
   __shared__ float s_row1[kNTMax], s_row2[kNTMax];
   __shared__ float s_part1[kWave], s_part2[kWave];
-  __shared__ float s_dx[kNZMax];
+  __shared__ float s_dx[kNZMax], s_fnum[kNZMax], s_fden[kNZMax];
+  __shared__ int s_fpos[kNZMax];
   __shared__ float s_filt[28];
   const int lane = threadIdx.x;
   const int wid = blockIdx.x;
@@ -559,10 +760,8 @@ __global__ __launch_bounds__(kWave) void k_sweep(DevGeom G, SweepArgs A) {
   const int nv = A.frame_ptr[f + 1] - s0;
   if (n == 0 || nv == 0) return;
   const int Nz = G.N_z, T = A.tile, BW = T + 2;
-  const SweepLds lds(smem, BW * BW * Nz, A.nvmax, G.W);
-  float* s_x = lds.x;
-  const AWin win0 = lds.win[0], win1 = lds.win[1];
-  int* s_views = lds.views;
+  const int nx = BW * BW * Nz;
+  const SweepLds L(smem, nx, A.nvmax, kW);
   const int oy = (tile / A.tiles_x) * T - 1, ox = (tile % A.tiles_x) * T - 1;
   const size_t vol = (size_t)G.N_y * G.N_x * Nz;
   float* xf = A.x + (size_t)f * vol;
@@ -574,440 +773,75 @@ __global__ __launch_bounds__(kWave) void k_sweep(DevGeom G, SweepArgs A) {
     while (gcd_i(pa, n) != 1) pa = pa % (n - 1) + 1;
     pb = (int)(mix32(h + 0x9e3779b9u) % (unsigned)n);
   }
-  // the tile's slab with halo, the frame's view list, the first A window
-  for (int j = lane; j < BW * BW * Nz; j += kWave) {
-    const int z = j % Nz, b = j / Nz;
-    const int yy = oy + b / BW, xx = ox + b % BW;
-    s_x[j] = (yy >= 0 && yy < G.N_y && xx >= 0 && xx < G.N_x)
+  // the tile's slab with halo (8 loads per lane in flight), column and view lists
+  for (int j0 = lane; j0 < nx; j0 += 8 * kWave) {
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int j = j0 + u * kWave;
+      const int z = j % Nz, b = j / Nz;
+      const int yy = oy + b / BW, xx = ox + b % BW;
+      v[u] = (j < nx && yy >= 0 && yy < G.N_y && xx >= 0 && xx < G.N_x)
                  ? xf[((size_t)yy * G.N_x + xx) * Nz + z]
                  : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (j0 + u * kWave < nx) L.x[j0 + u * kWave] = v[u];
   }
-  for (int j = lane; j < nv; j += kWave) s_views[j] = A.col.frame_views[s0 + j];
-  const int nav = nv * G.W;
-  APre pre;
+  for (int j = lane; j < n; j += kWave) L.cols[j] = A.tile_cols[c0 + j];
+  for (int j = lane; j < nv; j += kWave) L.views[j] = A.col.frame_views[s0 + j];
+  wave_sync();
   int pos = pb;  // visiting order: pos_k = (pb + k pa) mod n
-  awin_fetch(G, A.col, A.tile_cols[c0 + pos], s0, nv, pre);
-  awin_store(pre, nv, nav, win0);
-  __syncthreads();
-  double st_dx = 0.0, st_x = 0.0, st_skip = 0.0, st_upd = 0.0;
+  {
+    APre pre;
+    const int cf = L.cols[pos];
+    awin_fetch(G, A.col, cf, s0, nv, pre);
+    tn_fetch(G, A.x, f, cf, vol, pre);
+    awin_store(G, pre, nv, L.views, L.win[0], s_tn[0]);
+  }
+  // nothing stays in flight: keeps the compiler's wait analysis from
+  // carrying these loads into the column loop
+  __builtin_amdgcn_s_waitcnt(0);
+  wave_sync();
+  SolveStats st;
   for (int k = 0; k < n; ++k) {
     const bool odd = (k & 1) != 0;
-    const AWin wcur = odd ? win1 : win0, wnext = odd ? win0 : win1;
-    const int c = A.tile_cols[c0 + pos];
+    const Win wcur = odd ? L.win[1] : L.win[0], wnext = odd ? L.win[0] : L.win[1];
+    const float* tcur = odd ? s_tn[1] : s_tn[0];
+    float* tnext = odd ? s_tn[0] : s_tn[1];
+    const int c = L.cols[pos];
     pos += pa;
     if (pos >= n) pos -= n;
     const int cy = c / G.N_x, cx = c % G.N_x;
     const int ly = cy - oy, lx = cx - ox;
-    // t-1 / t+1 columns (read-only in this launch)
-    for (int j = lane; j < 2 * Nz; j += kWave) {
-      const int ff = j < Nz ? f - 1 : f + 1;
-      s_tn[j] = (ff >= 0 && ff < G.N_time) ? A.x[(size_t)ff * vol + (size_t)c * Nz + j % Nz] : 0.f;
+    APre pre;
+    if (k + 1 < n) {
+      const int cn = L.cols[pos];
+      awin_fetch(G, A.col, cn, s0, nv, pre);
+      tn_fetch(G, A.x, f, cn, vol, pre);
     }
-    for (int z = lane; z < Nz; z += kWave) s_dx[z] = 0.f;
-    if (k + 1 < n) awin_fetch(G, A.col, A.tile_cols[c0 + pos], s0, nv, pre);
-    window_thetas(G, A.col, wcur, s_views, nv, s_row1, s_row2, s_part1, s_part2);
-    // surrogate solve, z parity halves
-    for (int par = 0; par < 2; ++par) {
-      const int nzp = (Nz - par + 1) / 2;
-      int lpv = 32;
-      while (lpv > 1 && lpv * nzp > kWave) lpv >>= 1;
-      const int vpp = kWave / lpv;
-      for (int vb = 0; vb < nzp; vb += vpp) {
-        const int vi = vb + lane / lpv, sub = lane % lpv;
-        const bool act = vi < nzp;
-        const int z = par + 2 * vi;
-        const int centre = ((ly * BW + lx) * Nz) + z;
-        const float V = act ? s_x[centre] : 0.f;
-        float num = 0.f, den = 0.f;
-        int pos = 0;
-        if (act) {
-          for (int j = sub; j < 29; j += lpv) {
-            if (j == 13) continue;
-            int ok;
-            float nbv;
-            if (j < 27) {
-              const int dz = j / 9 - 1, dy = (j / 3) % 3 - 1, dx = j % 3 - 1;
-              const int z2 = z + dz, y2 = cy + dy, x2 = cx + dx;
-              ok = z2 >= 0 && z2 < Nz && y2 >= 0 && y2 < G.N_y && x2 >= 0 && x2 < G.N_x;
-              nbv = ok ? s_x[((ly + dy) * BW + lx + dx) * Nz + z2] : 0.f;
-            } else {
-              const int ff = j == 27 ? f - 1 : f + 1;
-              ok = ff >= 0 && ff < G.N_time;
-              nbv = ok ? s_tn[(j - 27) * Nz + z] : 0.f;
-            }
-            if (nbv > 0.f) pos = 1;
-            if (!ok) continue;
-            const float wt = nb_weight(G, s_filt, j, V, nbv);
-            num += wt * nbv;
-            den += wt;
-          }
-        }
-        for (int off = lpv / 2; off > 0; off >>= 1) {
-          num += __shfl_xor(num, off);
-          den += __shfl_xor(den, off);
-          pos |= __shfl_xor(pos, off);
-        }
-        if (act && sub == 0) {
-          // zero skipping (XT_VoxUpdate.c:238-262)
-          if (V == 0.f && A.iter > 1 && !pos) {
-            st_skip += 1.0;
-          } else {
-            float th1 = 0.f, th2 = 0.f;
-            for (int m = 0; m < G.zo; ++m) {
-              th1 += s_row1[z * G.zo + m];
-              th2 += s_row2[z * G.zo + m];
-            }
-            const float xn = substitute(G, A.alpha, V, th1, th2, num, den);
-            s_x[centre] = xn;
-            s_dx[z] = xn - V;
-            st_dx += fabs((double)xn - (double)V);
-            st_x += fabs((double)xn);
-            st_upd += 1.0;
-          }
-        }
-      }
-      __syncthreads();
-    }
-    window_update_error(G, A.col, wcur, s_views, nv, s_dx);
-    if (k + 1 < n) awin_store(pre, nv, nav, wnext);
-    __syncthreads();
+    col_thetas<kW>(G, A.col, wcur, nv, s_row1, s_row2, s_part1, s_part2);
+    col_fixed(G, s_filt, L.x, BW, ly, lx, cy, cx, f, tcur, s_fnum, s_fden, s_fpos);
+    wave_sync();
+    const int centre0 = (ly * BW + lx) * Nz;
+    col_solve(G, s_filt, L.x, centre0, 0, A.iter, A.alpha, s_fnum, s_fden, s_fpos, s_row1, s_row2,
+              s_dx, st);
+    wave_sync();
+    col_solve(G, s_filt, L.x, centre0, 1, A.iter, A.alpha, s_fnum, s_fden, s_fpos, s_row1, s_row2,
+              s_dx, st);
+    wave_sync();
+    // the prefetch landed with the theta loads; store it, and retire every
+    // load explicitly (a free wait here) so that the compiler's wait analysis
+    // carries none of them into the next column, before the error update's
+    // atomics go out: no wait in this column is on those
+    if (k + 1 < n) awin_store(G, pre, nv, L.views, wnext, tnext);
+    __builtin_amdgcn_s_waitcnt(0);
+    col_update<kW>(G, A.col, wcur, nv, s_dx);
+    wave_sync();
   }
   // write the tile back (interior of the box, inside the object)
-  for (int j = lane; j < BW * BW * Nz; j += kWave) {
-    const int z = j % Nz, b = j / Nz;
-    const int by = b / BW, bx = b % BW;
-    if (by == 0 || bx == 0 || by == BW - 1 || bx == BW - 1) continue;
-    const int yy = oy + by, xx = ox + bx;
-    if (yy < G.N_y && xx < G.N_x) xf[((size_t)yy * G.N_x + xx) * Nz + z] = s_x[j];
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    st_dx += __shfl_xor(st_dx, off);
-    st_x += __shfl_xor(st_x, off);
-    st_skip += __shfl_xor(st_skip, off);
-    st_upd += __shfl_xor(st_upd, off);
-  }
-  if (lane == 0) {
-    atomicAdd(&A.stats[0], st_dx);
-    atomicAdd(&A.stats[1], st_x);
-    atomicAdd(&A.stats[2], st_skip);
-    atomicAdd(&A.stats[3], st_upd);
-  }
-}
-
-// ------------------------------------------------- LDS-footprint sweep
-// Wave-scope ordering: one wavefront per workgroup, and a wavefront's LDS
-// operations execute in issue order, so a compiler fence at wavefront scope
-// is enough between LDS phases (no s_waitcnt on in-flight global loads).
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-struct SvArgs {
-  SweepArgs b;
-  int sync_every;  // columns between refreshes of the LDS error footprint
-  int fp_bins;     // capacity in detector bins per view of a tile footprint
-  int* overflow;   // set if a footprint exceeded fp_bins (host sizing bug)
-};
-
-// Dynamic LDS of the footprint sweep.
-struct SvLds {
-  float* x;
-  AWin win[2];
-  int *views, *fp_lo, *fp_off;
-  float *e, *w;
-  __device__ SvLds(float* base, int xfloats, int nvmax, int W, int fp_cap) {
-    x = base;
-    float* p = base + xfloats;
-    for (int b = 0; b < 2; ++b) {
-      win[b].a0 = reinterpret_cast<int*>(p);
-      p += nvmax;
-      win[b].av = p;
-      p += nvmax * W;
-    }
-    views = reinterpret_cast<int*>(p);
-    p += nvmax;
-    fp_lo = reinterpret_cast<int*>(p);
-    p += nvmax;
-    fp_off = reinterpret_cast<int*>(p);
-    p += nvmax + 1;
-    e = p;
-    p += fp_cap;
-    w = p;
-  }
-};
-
-// Loads the footprint's error (and, if with_w, weight) entries from global
-// memory: slot s covers bins fp_lo[s].. of view views[s], all rows, which is
-// one contiguous run of e.  Loads go out in batches of 8 per lane.
-__device__ void sv_load_footprint(const DevGeom& G, const ColumnArgs& A, const SvLds& L,
-                                  int nv, bool with_w) {
-  const int lane = threadIdx.x;
-  const int F = L.fp_off[nv];
-  int s = 0;
-  for (int i0 = lane; i0 < F; i0 += 8 * kWave) {
-    size_t o[8];
-    float ev[8], wv[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int i = i0 + u * kWave;
-      o[u] = 0;
-      if (i < F) {
-        while (i >= L.fp_off[s + 1]) ++s;
-        const int j = i - L.fp_off[s];
-        o[u] = ((size_t)L.views[s] * G.N_r + L.fp_lo[s]) * G.N_t + j;
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      ev[u] = e_load(A.e + o[u]);
-      wv[u] = with_w ? A.w[o[u]] : 0.f;
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int i = i0 + u * kWave;
-      if (i < F) {
-        L.e[i] = ev[u];
-        if (with_w) L.w[i] = wv[u];
-      }
-    }
-  }
-}
-
-// The LDS-footprint variant of k_sweep.  The tile's error and weight
-// footprint (every detector bin any of its columns touches, in the frame's
-// views, all rows) is held in LDS: each column's theta1/theta2 and its error
-// update run from LDS, the update is also sent to the shared error sinogram
-// with atomics (fire and forget), and every sync_every columns the LDS copy
-// is reloaded from global memory after this wavefront's atomics drained, so
-// the other tiles' updates are seen at most sync_every columns late.
-__global__ __launch_bounds__(kWave) void k_sweep_sv(DevGeom G, SvArgs S) {
-  extern __shared__ __align__(16) float smem[];
-  __shared__ float s_tn[2 * kNZMax];
-  __shared__ float s_row1[kNTMax], s_row2[kNTMax];
-  __shared__ float s_part1[kWave], s_part2[kWave];
-  __shared__ float s_dx[kNZMax];
-  __shared__ float s_filt[28];
-  const SweepArgs& A = S.b;
-  const int lane = threadIdx.x;
-  load_filter(G, s_filt);
-  const int wid = blockIdx.x;
-  const int tile = A.launch_tiles[wid % A.n_launch_tiles];
-  const int f = A.fparity + 2 * (wid / A.n_launch_tiles);
-  if (f >= G.N_time) return;
-  const int c0 = A.tile_ptr[tile];
-  const int n = A.tile_ptr[tile + 1] - c0;
-  const int s0 = A.frame_ptr[f];
-  const int nv = A.frame_ptr[f + 1] - s0;
-  if (n == 0 || nv == 0) return;
-  const int Nz = G.N_z, T = A.tile, BW = T + 2;
-  const SvLds L(smem, BW * BW * Nz, A.nvmax, G.W, A.nvmax * S.fp_bins * G.N_t);
-  const int oy = (tile / A.tiles_x) * T - 1, ox = (tile % A.tiles_x) * T - 1;
-  const size_t vol = (size_t)G.N_y * G.N_x * Nz;
-  float* xf = A.x + (size_t)f * vol;
-  const unsigned h = mix32(A.seed ^ mix32((unsigned)tile * 131071u + (unsigned)f * 524287u));
-  int pa = 1, pb = 0;
-  if (n > 1) {
-    pa = 1 + (int)(h % (unsigned)(n - 1));
-    while (gcd_i(pa, n) != 1) pa = pa % (n - 1) + 1;
-    pb = (int)(mix32(h + 0x9e3779b9u) % (unsigned)n);
-  }
-  // slab with halo, view list, first A window, footprint bounds
-  for (int j = lane; j < BW * BW * Nz; j += kWave) {
-    const int z = j % Nz, b = j / Nz;
-    const int yy = oy + b / BW, xx = ox + b % BW;
-    L.x[j] = (yy >= 0 && yy < G.N_y && xx >= 0 && xx < G.N_x)
-                 ? xf[((size_t)yy * G.N_x + xx) * Nz + z]
-                 : 0.f;
-  }
-  for (int j = lane; j < nv; j += kWave) {
-    L.views[j] = A.col.frame_views[s0 + j];
-    L.fp_lo[j] = 0x7fffffff;
-    L.fp_off[j] = -1;  // holds the highest bin until the prefix sum
-  }
-  APre pre;
-  int pos = pb;  // visiting order: pos_k = (pb + k pa) mod n
-  awin_fetch(G, A.col, A.tile_cols[c0 + pos], s0, nv, pre);
-  wave_sync();
-  for (int j = lane; j < n * nv; j += kWave) {
-    const int i = j / nv, sl = j - i * nv;
-    const int c = A.tile_cols[c0 + i];
-    const int a0 = A.col.afirst[(size_t)c * G.nslots + s0 + sl];
-    if (a0 >= 0) {
-      atomicMin(&L.fp_lo[sl], a0);
-      atomicMax(&L.fp_off[sl], min(a0 + G.W - 1, G.N_r - 1));
-    }
-  }
-  awin_store(pre, nv, nv * G.W, L.win[0]);
-  wave_sync();
-  __shared__ int s_over;
-  if (lane == 0) {
-    int acc = 0, over = 0;
-    for (int sl = 0; sl < nv; ++sl) {
-      const int hi = L.fp_off[sl];
-      int nb = 0;
-      if (hi >= 0) {
-        nb = hi - L.fp_lo[sl] + 1;
-      } else {
-        L.fp_lo[sl] = 0;
-      }
-      if (nb > S.fp_bins) {
-        over = 1;
-        nb = S.fp_bins;
-      }
-      L.fp_off[sl] = acc;
-      acc += nb * G.N_t;
-    }
-    L.fp_off[nv] = acc;
-    s_over = over;
-    if (over) atomicOr(S.overflow, 1);
-  }
-  wave_sync();
-  if (s_over) return;  // the host sized fp_bins too small: nothing is touched
-  sv_load_footprint(G, A.col, L, nv, true);
-  wave_sync();
-  const int nent = nv * G.W;
-  const Lanes LN(G.N_t);
-  const float Th = G.huber_T, hd = G.huber_delta;
-  double st_dx = 0.0, st_x = 0.0, st_skip = 0.0, st_upd = 0.0;
-  for (int k = 0; k < n; ++k) {
-    const bool odd = (k & 1) != 0;
-    const AWin wcur = odd ? L.win[1] : L.win[0], wnext = odd ? L.win[0] : L.win[1];
-    const int c = A.tile_cols[c0 + pos];
-    pos += pa;
-    if (pos >= n) pos -= n;
-    const int cy = c / G.N_x, cx = c % G.N_x;
-    const int ly = cy - oy, lx = cx - ox;
-    for (int j = lane; j < 2 * Nz; j += kWave) {
-      const int ff = j < Nz ? f - 1 : f + 1;
-      s_tn[j] = (ff >= 0 && ff < G.N_time) ? A.x[(size_t)ff * vol + (size_t)c * Nz + j % Nz] : 0.f;
-    }
-    for (int z = lane; z < Nz; z += kWave) s_dx[z] = 0.f;
-    if (k + 1 < n) awin_fetch(G, A.col, A.tile_cols[c0 + pos], s0, nv, pre);
-    // theta1/theta2 per row from the LDS footprint
-    for (int t0 = 0; t0 < G.N_t; t0 += kWave) {
-      const int row = t0 + LN.rr;
-      float a1 = 0.f, a2 = 0.f;
-      if (LN.g < LN.groups && row < G.N_t) {
-        for (int sk = LN.g; sk < nent; sk += LN.groups) {
-          const int sl = sk / G.W;
-          const int r0 = wcur.a0[sl];
-          const float a0v = wcur.av[sk];
-          if (r0 < 0 || a0v == 0.f) continue;
-          const int r = r0 + (sk - sl * G.W);
-          if (r >= G.N_r) continue;
-          const int loc = L.fp_off[sl] + (r - L.fp_lo[sl]) * G.N_t + row;
-          const float ev = L.e[loc], wv = L.w[loc];
-          const float a = a0v * G.delta_t;
-          float hw = wv;
-          if (ev * ev * wv >= Th * Th) hw = Th * hd * sqrtf(wv) * __builtin_amdgcn_rcpf(fabsf(ev));
-          a2 += a * a * hw;
-          a1 -= ev * a * hw;
-        }
-      }
-      s_part1[lane] = a1;
-      s_part2[lane] = a2;
-      wave_sync();
-      if (lane < LN.rpl && row < G.N_t) {
-        float q1 = 0.f, q2 = 0.f;
-        for (int gg = 0; gg < LN.groups; ++gg) {
-          q1 += s_part1[gg * LN.rpl + lane];
-          q2 += s_part2[gg * LN.rpl + lane];
-        }
-        s_row1[row] = q1;
-        s_row2[row] = q2;
-      }
-      wave_sync();
-    }
-    // surrogate solve, z parity halves
-    for (int par = 0; par < 2; ++par) {
-      const int nzp = (Nz - par + 1) / 2;
-      int lpv = 32;
-      while (lpv > 1 && lpv * nzp > kWave) lpv >>= 1;
-      const int vpp = kWave / lpv;
-      for (int vb = 0; vb < nzp; vb += vpp) {
-        const int vi = vb + lane / lpv, sub = lane % lpv;
-        const bool act = vi < nzp;
-        const int z = par + 2 * vi;
-        const int centre = ((ly * BW + lx) * Nz) + z;
-        const float V = act ? L.x[centre] : 0.f;
-        float num = 0.f, den = 0.f;
-        int pos = 0;
-        if (act) {
-          for (int j = sub; j < 29; j += lpv) {
-            if (j == 13) continue;
-            int ok;
-            float nbv;
-            if (j < 27) {
-              const int dz = j / 9 - 1, dy = (j / 3) % 3 - 1, dx = j % 3 - 1;
-              const int z2 = z + dz, y2 = cy + dy, x2 = cx + dx;
-              ok = z2 >= 0 && z2 < Nz && y2 >= 0 && y2 < G.N_y && x2 >= 0 && x2 < G.N_x;
-              nbv = ok ? L.x[((ly + dy) * BW + lx + dx) * Nz + z2] : 0.f;
-            } else {
-              const int ff = j == 27 ? f - 1 : f + 1;
-              ok = ff >= 0 && ff < G.N_time;
-              nbv = ok ? s_tn[(j - 27) * Nz + z] : 0.f;
-            }
-            if (nbv > 0.f) pos = 1;
-            if (!ok) continue;
-            const float wt = nb_weight(G, s_filt, j, V, nbv);
-            num += wt * nbv;
-            den += wt;
-          }
-        }
-        for (int off = lpv / 2; off > 0; off >>= 1) {
-          num += __shfl_xor(num, off);
-          den += __shfl_xor(den, off);
-          pos |= __shfl_xor(pos, off);
-        }
-        if (act && sub == 0) {
-          if (V == 0.f && A.iter > 1 && !pos) {
-            st_skip += 1.0;
-          } else {
-            float th1 = 0.f, th2 = 0.f;
-            for (int m = 0; m < G.zo; ++m) {
-              th1 += s_row1[z * G.zo + m];
-              th2 += s_row2[z * G.zo + m];
-            }
-            const float xn = substitute(G, A.alpha, V, th1, th2, num, den);
-            L.x[centre] = xn;
-            s_dx[z] = xn - V;
-            st_dx += fabs((double)xn - (double)V);
-            st_x += fabs((double)xn);
-            st_upd += 1.0;
-          }
-        }
-      }
-      wave_sync();
-    }
-    // error update: LDS copy and the shared sinogram
-    for (int t0 = 0; t0 < G.N_t; t0 += kWave) {
-      const int row = t0 + LN.rr;
-      if (LN.g >= LN.groups || row >= G.N_t) continue;
-      const float dz = s_dx[row / G.zo];
-      if (dz == 0.f) continue;
-      for (int sk = LN.g; sk < nent; sk += LN.groups) {
-        const int sl = sk / G.W;
-        const int r0 = wcur.a0[sl];
-        const float a0v = wcur.av[sk];
-        const int r = r0 + (sk - sl * G.W);
-        if (r0 < 0 || a0v == 0.f || r >= G.N_r) continue;
-        const float d = -(a0v * G.delta_t) * dz;
-        L.e[L.fp_off[sl] + (r - L.fp_lo[sl]) * G.N_t + row] += d;
-        e_add(A.col.e + ((size_t)L.views[sl] * G.N_r + r) * G.N_t + row, d);
-      }
-    }
-    if (k + 1 < n) awin_store(pre, nv, nv * G.W, wnext);
-    wave_sync();
-    if (S.sync_every > 0 && (k + 1) % S.sync_every == 0 && k + 1 < n) {
-      // this wavefront's atomics are complete before the reload
-      __builtin_amdgcn_s_waitcnt(0);
-      sv_load_footprint(G, A.col, L, nv, false);
-      wave_sync();
-    }
-  }
-  for (int j = lane; j < BW * BW * Nz; j += kWave) {
+  for (int j = lane; j < nx; j += kWave) {
     const int z = j % Nz, b = j / Nz;
     const int by = b / BW, bx = b % BW;
     if (by == 0 || bx == 0 || by == BW - 1 || bx == BW - 1) continue;
@@ -1015,41 +849,44 @@ __global__ __launch_bounds__(kWave) void k_sweep_sv(DevGeom G, SvArgs S) {
     if (yy < G.N_y && xx < G.N_x) xf[((size_t)yy * G.N_x + xx) * Nz + z] = L.x[j];
   }
   for (int off = 32; off > 0; off >>= 1) {
-    st_dx += __shfl_xor(st_dx, off);
-    st_x += __shfl_xor(st_x, off);
-    st_skip += __shfl_xor(st_skip, off);
-    st_upd += __shfl_xor(st_upd, off);
+    st.dx += __shfl_xor(st.dx, off);
+    st.x += __shfl_xor(st.x, off);
+    st.skip += __shfl_xor(st.skip, off);
+    st.upd += __shfl_xor(st.upd, off);
   }
   if (lane == 0) {
-    atomicAdd(&A.stats[0], st_dx);
-    atomicAdd(&A.stats[1], st_x);
-    atomicAdd(&A.stats[2], st_skip);
-    atomicAdd(&A.stats[3], st_upd);
+    atomicAdd(&A.stats[0], st.dx);
+    atomicAdd(&A.stats[1], st.x);
+    atomicAdd(&A.stats[2], st.skip);
+    atomicAdd(&A.stats[3], st.upd);
   }
 }
 
-// theta1/theta2 of individual voxels through the sweep's window routine.
+// theta1/theta2 of individual voxels through the sweep's column routine.
+template <int kW>
 __global__ __launch_bounds__(kWave) void k_thetas(DevGeom G, ColumnArgs A,
                                                   const int* __restrict__ frame_ptr,
-                                                  const int* __restrict__ tzyx,
+                                                  const int* __restrict__ tzyx, int nvmax,
                                                   float* __restrict__ th1,
                                                   float* __restrict__ th2) {
   extern __shared__ __align__(16) float smem[];
   __shared__ float s_row1[kNTMax], s_row2[kNTMax];
   __shared__ float s_part1[kWave], s_part2[kWave];
-  const SweepLds lds(smem, 0, kMaxViews, G.W);
-  const AWin s_win = lds.win[0];
-  int* s_views = lds.views;
+  __shared__ float s_tn[2 * kNZMax];
+  const SweepLds L(smem, 0, nvmax, kW);
   const int i = blockIdx.x;
   const int f = tzyx[4 * i], z = tzyx[4 * i + 1];
   const int c = tzyx[4 * i + 2] * G.N_x + tzyx[4 * i + 3];
   const int s0 = frame_ptr[f], nv = frame_ptr[f + 1] - s0;
-  for (int j = threadIdx.x; j < nv; j += kWave) s_views[j] = A.frame_views[s0 + j];
+  for (int j = threadIdx.x; j < nv; j += kWave) L.views[j] = A.frame_views[s0 + j];
+  wave_sync();
   APre pre;
   awin_fetch(G, A, c, s0, nv, pre);
-  awin_store(pre, nv, nv * G.W, s_win);
-  __syncthreads();
-  window_thetas(G, A, s_win, s_views, nv, s_row1, s_row2, s_part1, s_part2);
+  for (int q = 0; q < kTnPre; ++q) pre.tn[q] = 0.f;
+  awin_store(G, pre, nv, L.views, L.win[0], s_tn);
+  __builtin_amdgcn_s_waitcnt(0);
+  wave_sync();
+  col_thetas<kW>(G, A, L.win[0], nv, s_row1, s_row2, s_part1, s_part2);
   if (threadIdx.x == 0) {
     float a = 0.f, b = 0.f;
     for (int m = 0; m < G.zo; ++m) {
@@ -1237,9 +1074,8 @@ struct mbir_hip_ctx {
   int *d_tile_ptr = nullptr, *d_tile_cols = nullptr, *d_launch_tiles = nullptr;
   int launch_off[5] = {0, 0, 0, 0, 0};
   int tile = 8, tiles_for = -1;
-  int sync_every = 8, use_sv = 0;
   double alpha = -1.0;
-  int* d_overflow = nullptr;
+  size_t lds_max = 64 * 1024;  // LDS bytes one workgroup may allocate
   double* d_stats = nullptr;
   double* d_partial = nullptr;
   size_t partial_cap = 0;
@@ -1381,6 +1217,8 @@ int check_geom(const mbir_hip_geom* g) {
   if (g->N_z > kNZMax) return fail("N_z exceeds the sweep kernel's LDS column (128)");
   if (g->N_t > kNTMax) return fail("N_t exceeds the sweep kernel's LDS rows (256)");
   if (2 * g->mult_xy + 1 > kWMax) return fail("mult_xy exceeds 16");
+  if ((long long)g->N_p * g->N_r * g->N_t >= (1LL << 31))
+    return fail("sinogram too large for 32-bit offsets");
   if ((long long)g->N_z * g->mult_z > g->N_t) return fail("N_z*mult_z exceeds N_t");
   return 0;
 }
@@ -1429,6 +1267,11 @@ int mbir_hip_create(mbir_hip_ctx** out, const mbir_hip_geom* g, const float* H,
       view_frame[frame_views[s]] = f;
     }
   int rc = 0;
+  int dev = 0, lds = 0;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+      lds > 0)
+    c->lds_max = (size_t)lds;
   if (hipStreamCreate(&c->stream) != hipSuccess) rc = fail("hipStreamCreate failed");
   rc = rc || dev_alloc(&c->d_H, (size_t)g->N_p * kNB) || dev_alloc(&c->d_cos, g->N_p) ||
        dev_alloc(&c->d_sin, g->N_p) || dev_alloc(&c->d_frame_ptr, g->N_time + 1) ||
@@ -1474,8 +1317,7 @@ void mbir_hip_destroy(mbir_hip_ctx* c) {
   void* bufs[] = {c->d_H, c->d_cos, c->d_sin, c->d_frame_ptr, c->d_frame_views,
                   c->d_view_frame, c->d_y, c->d_w, c->d_d, c->d_e, c->d_x,
                   c->d_stage, c->d_stage2, c->d_afirst, c->d_aval, c->d_tile_ptr,
-                  c->d_tile_cols, c->d_launch_tiles, c->d_stats, c->d_partial,
-                  c->d_overflow};
+                  c->d_tile_cols, c->d_launch_tiles, c->d_stats, c->d_partial};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (int k = 0; k < MBIR_K_COUNT; ++k)
@@ -1495,11 +1337,6 @@ int mbir_hip_set_option(mbir_hip_ctx* c, const char* key, double value) {
     c->tile = (int)value;
   } else if (k == "alpha") {
     c->alpha = value;
-  } else if (k == "sync") {
-    if (value < 0) return fail("sync must be >= 0");
-    c->sync_every = (int)value;
-  } else if (k == "sv") {
-    c->use_sv = value != 0;
   } else {
     return fail("unknown option " + k);
   }
@@ -1637,20 +1474,10 @@ int mbir_hip_icd_equits(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t
   if (xfl * sizeof(float) > kSweepXMax)
     return fail("tile too large for N_z: the tile slab must fit 48 KiB of LDS");
   const size_t xbytes =
-      sizeof(float) * (xfl + 2 * ((size_t)c->nvmax + (size_t)c->nvmax * c->G.W) + c->nvmax);
-  // footprint variant: bins a T x T tile can touch in one view
-  SvArgs S;
-  S.sync_every = c->sync_every;
-  S.fp_bins = (int)(((c->tile - 1) * 1.41422 + 2.0) * c->g.mult_xy) + 3;
-  const size_t fp_cap = (size_t)c->nvmax * S.fp_bins * c->g.N_t;
-  const size_t svbytes = sizeof(float) * (xfl + 2 * ((size_t)c->nvmax + (size_t)c->nvmax * c->G.W) +
-                                          3 * (size_t)c->nvmax + 1 + 2 * fp_cap);
-  const bool sv = c->use_sv && svbytes <= kSvLdsMax;
-  if (sv && !c->d_overflow) {
-    if (dev_alloc(&c->d_overflow, 1)) return -1;
-    HIP_OK(hipMemset(c->d_overflow, 0, sizeof(int)));
-  }
-  S.overflow = c->d_overflow;
+      sizeof(float) * sweep_lds_words(xfl, c->nvmax, c->G.W, c->tile * c->tile);
+  // static LDS of k_sweep (t+-1 buffers, row sums, per-voxel arrays) < 8 KiB
+  if (xbytes + 8 * 1024 > c->lds_max)
+    return fail("tile slab and A windows exceed the LDS of one workgroup");
   const int nf = c->g.N_time;
   for (int it = 0; it < n; ++it) {
     A.iter = first_iter + it;
@@ -1659,20 +1486,15 @@ int mbir_hip_icd_equits(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t
       if (ntl == 0) continue;
       A.launch_tiles = c->d_launch_tiles + c->launch_off[colour];
       A.n_launch_tiles = ntl;
-      for (int fp = 0; fp < 2; ++fp) {
-        const int frames = (nf - fp + 1) / 2;
+      for (int fpar = 0; fpar < 2; ++fpar) {
+        const int frames = (nf - fpar + 1) / 2;
         if (frames <= 0) continue;
-        A.fparity = fp;
-        A.seed = seed * 2654435761u + (unsigned)(it * 8 + colour * 2 + fp) * 40503u;
+        A.fparity = fpar;
+        A.seed = seed * 2654435761u + (unsigned)(it * 8 + colour * 2 + fpar) * 40503u;
         if (ev_begin(c, MBIR_K_SWEEP)) return -1;
-        if (sv) {
-          S.b = A;
-          hipLaunchKernelGGL(k_sweep_sv, dim3((unsigned)(ntl * frames)), dim3(kWave), svbytes,
-                             c->stream, c->G, S);
-        } else {
-          hipLaunchKernelGGL(k_sweep, dim3((unsigned)(ntl * frames)), dim3(kWave), xbytes,
-                             c->stream, c->G, A);
-        }
+        MBIR_DISPATCH_W(c->G.W,
+                        hipLaunchKernelGGL(k_sweep<kW>, dim3((unsigned)(ntl * frames)),
+                                           dim3(kWave), xbytes, c->stream, c->G, A));
         if (ev_end(c, MBIR_K_SWEEP)) return -1;
       }
     }
@@ -1681,12 +1503,6 @@ int mbir_hip_icd_equits(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t
     HIP_OK(hipMemcpyAsync(stats, c->d_stats, 4 * sizeof(double), hipMemcpyDeviceToHost,
                           c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
-  }
-  if (sv) {
-    int over = 0;
-    HIP_OK(hipMemcpyAsync(&over, c->d_overflow, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIP_OK(hipStreamSynchronize(c->stream));
-    if (over) return fail("tile footprint exceeded its LDS capacity");
   }
   return 0;
 }
@@ -1776,9 +1592,9 @@ int mbir_hip_voxel_thetas(mbir_hip_ctx* c, int32_t n, const int32_t* tzyx, float
   A.frame_views = c->d_frame_views;
   A.w = c->d_w;
   A.e = c->d_e;
-  const size_t shm = sizeof(float) * (size_t)(2 * (kMaxViews + kMaxViews * c->G.W) + kMaxViews);
-  hipLaunchKernelGGL(k_thetas, dim3(n), dim3(kWave), shm, c->stream, c->G, A, c->d_frame_ptr, dq,
-                     d1, d2);
+  const size_t shm = sizeof(float) * sweep_lds_words(0, c->nvmax, c->G.W, 0);
+  MBIR_DISPATCH_W(c->G.W, hipLaunchKernelGGL(k_thetas<kW>, dim3(n), dim3(kWave), shm, c->stream,
+                                             c->G, A, c->d_frame_ptr, dq, c->nvmax, d1, d2));
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(theta1, d1, n * 4, hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipMemcpyAsync(theta2, d2, n * 4, hipMemcpyDeviceToHost, c->stream));

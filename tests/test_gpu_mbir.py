@@ -146,8 +146,9 @@ def test_cost_matches_oracle(pair):
     assert abs(kept[0] - oc) <= 1e-6 * abs(oc)
 
 
-@pytest.mark.parametrize("name,n_gpu", [("tiny3d", 120), ("ch4d_small", 120)])
-def test_icd_converges_to_oracle(name, n_gpu, request):
+@pytest.mark.parametrize("name,n_gpu,tile", [
+    ("tiny3d", 120, 8), ("ch4d_small", 120, 8), ("ch4d_small", 120, 4)])
+def test_icd_converges_to_oracle(name, n_gpu, tile, request):
     from oracle.oracle import Oracle
     from phasecon_amd.mbir import Stage
     scan = request.getfixturevalue("tiny_scan" if name == "tiny3d" else "small4d_scan")
@@ -155,7 +156,7 @@ def test_icd_converges_to_oracle(name, n_gpu, request):
     ox, oe, ocost = o.run_icd(300)
     cstar = ocost[-1]
     assert abs(ocost[-1] - ocost[-2]) < 1e-9 * cstar  # oracle converged
-    st = Stage(scan, tile=8)
+    st = Stage(scan, tile=tile)
     st.build_amatrix()
     st.init_error()
     costs = [st.cost()[0]]
@@ -170,6 +171,25 @@ def test_icd_converges_to_oracle(name, n_gpu, request):
     assert (final - cstar) / cstar < 1e-5
     rmse = np.sqrt(np.mean((x - ox) ** 2)) / np.sqrt(np.mean(ox ** 2))
     assert rmse < 5e-3
+    st.close()
+
+
+@pytest.mark.parametrize("tile", [4, 8, 16])
+def test_sweep_keeps_error_sinogram(tile, small4d_scan):
+    """After sweeps the maintained e must still be y - A x: every voxel
+    change reaches the shared error sinogram exactly once."""
+    from phasecon_amd.mbir import Stage
+    st = Stage(small4d_scan, tile=tile)
+    st.build_amatrix()
+    st.init_error()
+    st.icd(6, 1, seed=5)
+    kept = st.get_error()
+    st.init_error()
+    exact = st.get_error()
+    scale = np.abs(small4d_scan["projections"]).max()
+    err = np.abs(kept - exact).max()
+    print(f"tile={tile}: max |e_kept - e| = {err:.3g} (scale {scale:.3g})", flush=True)
+    assert err <= 2e-5 * scale
     st.close()
 
 

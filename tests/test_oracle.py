@@ -1,6 +1,6 @@
 """Pins the FP64 oracle (oracle/mbir_oracle.c) before it is trusted.
 
-Three kinds of check, per SURVEY-listed function:
+Three kinds of check, for each restated function of the hot path:
   * golden vectors written by tests/golden/make_golden.py from the reference
     build (bit-exact where the reference is exact, float-rounding otherwise);
   * the live reference build in oracle/_ref when present (skipped otherwise);
