@@ -3,8 +3,10 @@
     python bench.py [--steps K] [--warmup W] [--workload ch4d] [--tile T]
 
 A step is one full reconstruction of the workload from the plain (zero)
-initialiser: A build, error sinogram, parallel ICD equits and the FP64 cost
-check after each equit, until the MAP cost is within 1e-5 (relative) of C*,
+initialiser: A build, error sinogram, parallel ICD equits and a cost check
+after each equit (maintained error sinogram, FP32 prior potentials summed in
+FP64), until the MAP cost is within 1e-5 (relative) of C* as confirmed by a
+fresh FP64 cost evaluation,
 the FP64 oracle's converged cost recorded in tests/golden/cstar_<workload>.json
 (generated from the same seeded scan; its SHA-256 is checked).  Inputs are
 resident in HBM before the timed region.  Prints one JSON line.
@@ -79,14 +81,14 @@ def run_once(st, target, max_equits, seed, refresh):
     t0 = time.perf_counter()
     st.build_amatrix()
     st.init_error()
-    cost = st.cost()[0]
+    cost = st.cost(estimate=True)[0]
     it = 0
     while it < max_equits:
         it += 1
         st.icd(1, it, seed)
         if refresh and it % refresh == 0:
             st.init_error()
-        cost = st.cost()[0]
+        cost = st.cost(estimate=True)[0]
         if cost <= target:
             fresh = st.cost(fresh=True)[0]   # FP64 check from a recomputed e
             if fresh <= target:

@@ -109,7 +109,9 @@ int mbir_hip_icd_equits(mbir_hip_ctx* ctx, int32_t n, int32_t first_iter,
 
 /* FP64 MAP cost (computeCost, XT_ICD_update.c:185-318).  fresh = 1
  * recomputes e = y - A x - d in FP64 on the fly; fresh = 0 uses the error
- * sinogram the sweeps maintain.  out = [total, data term incl.
+ * sinogram the sweeps maintain; fresh = 2 is an estimate for stopping tests:
+ * the maintained e and each prior pair's potential in FP32 (within about
+ * 1e-6 relative), summed in FP64.  out = [total, data term incl.
  * N log(var)/2, prior term]. */
 int mbir_hip_cost(mbir_hip_ctx* ctx, int32_t fresh, double* out);
 

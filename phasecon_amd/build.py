@@ -1,10 +1,11 @@
 """Builds the in-tree native libraries for gfx950.
 
-  phasecon_amd/libmbir4d.so  -- HIP kernels (csrc/*.hip, hipcc) + the C host
-                                engine (csrc/*.c, gcc), exporting reconstruct()
-                                and the C-ABI of include/mbir_hip.h
-  phasecon_amd/XT_Main       -- the command-line driver with the reference's
-                                flags (reconstruct/basic/XT_Main.c)
+  phasecon_amd/libmbir4d.so    -- MBIR: HIP kernels (csrc/mbir_hip.hip) + the C
+                                  host engine (csrc/mbir_engine.c), exporting
+                                  reconstruct() and the C-ABI of
+                                  include/mbir_hip.h
+  phasecon_amd/libpcmbir4d.so  -- PCMBIR phase retrieval: csrc/pcmbir_*.hip /
+                                  pcmbir_*.c, the C-ABI of include/pcmbir_hip.h
 
 Usage: python -m phasecon_amd.build   (always recompiles)
 """
@@ -37,35 +38,36 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
-def hip_sources():
-    return sorted(f for f in os.listdir(CSRC) if f.endswith(".hip"))
+# library -> source files in csrc/ (prefix match)
+LIBS = {
+    "libmbir4d.so": ("mbir_",),
+    "libpcmbir4d.so": ("pcmbir_",),
+}
 
 
-def c_sources():
-    return sorted(f for f in os.listdir(CSRC) if f.endswith(".c") and f != "xt_main.c")
+def sources(prefixes):
+    return sorted(f for f in os.listdir(CSRC)
+                  if f.endswith((".hip", ".c")) and f.startswith(prefixes))
 
 
 def build(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
-    objs = []
-    for src in hip_sources():
-        obj = os.path.join(BUILD, src.replace(".hip", ".o"))
-        _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
-        objs.append(obj)
-    for src in c_sources():
-        obj = os.path.join(BUILD, src.replace(".c", ".o"))
-        _run(["gcc", *C_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
-        objs.append(obj)
-    lib = os.path.join(PKG, "libmbir4d.so")
-    tmp = lib + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-lm"])
-    os.replace(tmp, lib)
-    main_src = os.path.join(CSRC, "xt_main.c")
-    if os.path.exists(main_src):
-        exe = os.path.join(PKG, "XT_Main")
-        _run(["gcc", *C_FLAGS, "-o", exe, main_src, f"-L{PKG}", "-lmbir4d",
-              f"-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + PKG, "-lm"])
-    return lib
+    out = []
+    for lib_name, prefixes in LIBS.items():
+        objs = []
+        for src in sources(prefixes):
+            obj = os.path.join(BUILD, src.rsplit(".", 1)[0] + ".o")
+            if src.endswith(".hip"):
+                _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+            else:
+                _run(["gcc", *C_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+            objs.append(obj)
+        lib = os.path.join(PKG, lib_name)
+        tmp = lib + ".tmp"
+        _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-lm"])
+        os.replace(tmp, lib)
+        out.append(lib)
+    return out
 
 
 if __name__ == "__main__":

@@ -272,8 +272,21 @@ __device__ __forceinline__ double qg_value64(double delta, double q, double p,
   return (aq / sq) / (c + exp2((q - p) * lg) / sqp);
 }
 
-// Prior term: each neighbour pair once (13 forward spatial + t+1), FP64
-// (XT_ICD_update.c:205-298).
+// qGGMRF potential with q = 2 evaluated in FP32 (v_log_f32 / v_exp_f32 /
+// v_rcp_f32, a few ulp per term) for the cost estimate of kFast.
+__device__ __forceinline__ float qg_value32(float delta, float qmp, float c, float inv_sq,
+                                            float inv_sqp) {
+  const float ad = fabsf(delta);
+  const float r = __builtin_amdgcn_exp2f(qmp * __builtin_amdgcn_logf(ad)) * inv_sqp;
+  const float v = ad * ad * inv_sq * __builtin_amdgcn_rcpf(c + r);
+  return ad == 0.f ? 0.f : v;
+}
+
+// Prior term: each neighbour pair once (13 forward spatial + t+1)
+// (XT_ICD_update.c:205-298), accumulated in FP64.  Each pair's potential is
+// FP64 (kFast = false) or FP32 (kFast = true, an estimate within about 1e-6
+// relative, for stopping tests that an FP64 evaluation then confirms).
+template <bool kFast>
 __global__ __launch_bounds__(256) void k_prior_cost(DevGeom G,
                                                     const float* __restrict__ x,
                                                     double* __restrict__ partial) {
@@ -297,11 +310,20 @@ __global__ __launch_bounds__(256) void k_prior_cost(DevGeom G,
       if (z2 >= G.N_z || y2 < 0 || y2 >= G.N_y || x2 < 0 || x2 >= G.N_x) continue;
       const double nv = x[(size_t)f * vol + ((size_t)y2 * G.N_x + x2) * G.N_z + z2];
       const int fi = ((fwd[k][0] + 1) * 3 + (fwd[k][1] + 1)) * 3 + fwd[k][2] + 1;
-      local += G.dfilt[fi] * qg_value64(v - nv, G.dq, G.dp_s, G.dc_s, G.dsq_s, G.dsqp_s);
+      if (kFast)
+        local += (double)(G.filt[fi] * qg_value32((float)(v - nv), G.q - G.p_s, G.c_s,
+                                                  G.inv_sq_s, G.inv_sqp_s));
+      else
+        local += G.dfilt[fi] * qg_value64(v - nv, G.dq, G.dp_s, G.dc_s, G.dsq_s, G.dsqp_s);
     }
-    if (f + 1 < G.N_time)
-      local += G.dfilt_t * qg_value64(v - (double)x[i + vol], G.dq, G.dp_t, G.dc_t,
-                                      G.dsq_t, G.dsqp_t);
+    if (f + 1 < G.N_time) {
+      const double d = v - (double)x[i + vol];
+      if (kFast)
+        local += (double)(G.filt_t *
+                          qg_value32((float)d, G.q - G.p_t, G.c_t, G.inv_sq_t, G.inv_sqp_t));
+      else
+        local += G.dfilt_t * qg_value64(d, G.dq, G.dp_t, G.dc_t, G.dsq_t, G.dsqp_t);
+    }
   }
   red[threadIdx.x] = local;
   __syncthreads();
@@ -313,39 +335,56 @@ __global__ __launch_bounds__(256) void k_prior_cost(DevGeom G,
 }
 
 // --------------------------------------------------------- ICD building blocks
-// qGGMRF b(delta) = rho'(delta)/delta, FP32 (XT_ICD_update.c:92-128): with
-// r = |d|^(q-p)/sigma^(q-p), b = |d|^(q-2) (q - (q-p) r/(c+r)) / ((c+r) sigma^q),
-// and b(0) = q/(sigma^q c).  |d|^(q-p) is one v_log_f32 / v_exp_f32 pair and
-// the divisions are v_rcp_f32 (each within about 1 ulp).
-__device__ __forceinline__ float qg_b(float d, float q, float p, float c,
-                                      float inv_sq, float inv_sqp, float b0) {
+// qGGMRF b(delta) = rho'(delta)/delta, FP32, with q = 2 (MRF_Q,
+// XT_Constants.h; XT_ICD_update.c:92-128): with r = |d|^(2-p)/sigma^(2-p),
+// b = (2 - (2-p) r/(c+r)) / ((c+r) sigma^2), and b(0) = 2/(sigma^2 c).
+// |d|^(2-p) is one v_log_f32 / v_exp_f32 pair and the division a v_rcp_f32
+// (each within about 1 ulp).
+__device__ __forceinline__ float qg_b(float d, float qmp, float c, float inv_sq, float inv_sqp,
+                                      float b0) {
   const float ad = fabsf(d);
-  if (ad == 0.f) return b0;
   const float lg = __builtin_amdgcn_logf(ad);
-  const float ratio = __builtin_amdgcn_exp2f((q - p) * lg) * inv_sqp;
+  const float ratio = __builtin_amdgcn_exp2f(qmp * lg) * inv_sqp;
   const float rden = __builtin_amdgcn_rcpf(c + ratio);
-  const float shape = q - (q - p) * ratio * rden;
-  const float mag = (q == 2.f) ? 1.f : __builtin_amdgcn_exp2f((q - 2.f) * lg);
-  return mag * shape * inv_sq * rden;
+  const float b = (2.f - qmp * ratio * rden) * inv_sq * rden;
+  return ad == 0.f ? b0 : b;
+}
+
+// Prior constants of the ICD kernels: per prior (0 spatial, 1 temporal)
+// 2 - p, c, 1/sigma^2, 1/sigma^(2-p) and b(0), then the 27 spatial
+// neighbour weights ([dz][dy][dx]) and the temporal one.  Kernels copy it
+// into LDS once; values read from there live in vector registers, which
+// keeps them out of the scalar register file.
+struct PriorTab {
+  float qmp[2], c[2], inv_sq[2], inv_sqp[2], b0[2];
+  float filt[28];
+};
+
+// Geometry of the ICD kernels (a subset of the stage geometry).
+struct SwGeom {
+  int N_r, N_t, N_x, N_y, N_z, N_time, zo, nslots;
+  float delta_t, huber_T, huber_delta;
+  PriorTab prior;
+};
+
+__device__ __forceinline__ void load_prior(const SwGeom& G, PriorTab* dst) {
+  const float* src = reinterpret_cast<const float*>(&G.prior);
+  float* d = reinterpret_cast<float*>(dst);
+  for (int j = threadIdx.x; j < (int)(sizeof(PriorTab) / sizeof(float)); j += blockDim.x)
+    d[j] = src[j];
 }
 
 // Weight of neighbour j of a voxel with value V: j < 27 spatial ([dz][dy][dx],
-// 13 is the centre and never used), 27/28 = t-1/t+1; filt[0..27] holds the
-// 27 spatial weights and the temporal one (in LDS).
-__device__ __forceinline__ float nb_weight(const DevGeom& G, const float* filt, int j,
-                                           float V, float nbv) {
-  if (j < 27)
-    return filt[j] * qg_b(V - nbv, G.q, G.p_s, G.c_s, G.inv_sq_s, G.inv_sqp_s, G.b0_s);
-  return filt[27] * qg_b(V - nbv, G.q, G.p_t, G.c_t, G.inv_sq_t, G.inv_sqp_t, G.b0_t);
-}
-
-__device__ __forceinline__ void load_filter(const DevGeom& G, float* filt) {
-  for (int j = threadIdx.x; j < 28; j += blockDim.x) filt[j] = j < 27 ? G.filt[j] : G.filt_t;
+// 13 is the centre and never used), 27/28 = t-1/t+1.
+__device__ __forceinline__ float nb_weight(const PriorTab& P, int j, float V, float nbv) {
+  const int t = j >= 27 ? 1 : 0;
+  return P.filt[j < 27 ? j : 27] *
+         qg_b(V - nbv, P.qmp[t], P.c[t], P.inv_sq[t], P.inv_sqp[t], P.b0[t]);
 }
 
 // Surrogate solve and over-relaxation (XT_ICD_update.c:174-176).
-__device__ __forceinline__ float substitute(const DevGeom& G, float alpha, float V,
-                                            float th1, float th2, float num, float den) {
+__device__ __forceinline__ float substitute(float alpha, float V, float th1, float th2,
+                                            float num, float den) {
   const float u = (num + th2 * V - th1) * __builtin_amdgcn_rcpf(den + th2);
   return V + alpha * (u - V);
 }
@@ -365,6 +404,47 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Sum over aligned groups of n consecutive lanes (n a power of two <= 64),
+// the result in every lane of the group: DPP quad permutes and row mirrors
+// within 16-lane rows (VALU operand modifiers, no LDS traffic), lane
+// shuffles across rows.
+__device__ __forceinline__ float dpp_xor1(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                               0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float dpp_xor2(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                               0x4E, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float dpp_half_mirror(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                               0x141, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float dpp_mirror(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                               0x140, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float group_sum(float v, int n) {
+  if (n >= 2) v += dpp_xor1(v);
+  if (n >= 4) v += dpp_xor2(v);
+  if (n >= 8) v += dpp_half_mirror(v);
+  if (n >= 16) v += dpp_mirror(v);
+  if (n >= 32) v += __shfl_xor(v, 16);
+  if (n >= 64) v += __shfl_xor(v, 32);
+  return v;
+}
+__device__ __forceinline__ int group_or(int v, int n) {
+  const auto f = [](int a) { return __builtin_bit_cast(float, a); };
+  const auto i = [](float a) { return __builtin_bit_cast(int, a); };
+  if (n >= 2) v |= i(dpp_xor1(f(v)));
+  if (n >= 4) v |= i(dpp_xor2(f(v)));
+  if (n >= 8) v |= i(dpp_half_mirror(f(v)));
+  if (n >= 16) v |= i(dpp_mirror(f(v)));
+  if (n >= 32) v |= __shfl_xor(v, 16);
+  if (n >= 64) v |= __shfl_xor(v, 32);
+  return v;
 }
 
 struct ColumnArgs {
@@ -422,29 +502,32 @@ struct APre {
   float tn[kTnPre];
 };
 
-__device__ __forceinline__ void awin_fetch(const DevGeom& G, const ColumnArgs& A, int c,
-                                           int s0, int nv, APre& pre) {
-  const int lane = threadIdx.x;
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+
+template <int kW>
+__device__ __forceinline__ void awin_fetch(const SwGeom& G, const ColumnArgs& A, int c, int s0,
+                                           int nv, APre& pre) {
+  const int lane = lane_id();
   const size_t base = (size_t)c * G.nslots + s0;
 #pragma unroll
   for (int i = 0; i < kPreI; ++i) {
     const int j = lane + i * kWave;
     pre.a0[i] = j < nv ? A.afirst[base + j] : -1;
   }
-  const int nav = nv * G.W;
+  const int nav = nv * kW;
 #pragma unroll
   for (int i = 0; i < kPreA; ++i) {
     const int j = lane + i * kWave;
-    pre.av[i] = j < nav ? A.aval[base * G.W + j] : 0.f;
+    pre.av[i] = j < nav ? A.aval[base * kW + j] : 0.f;
   }
 }
 
 // The t-1 and t+1 columns of voxel column c (not updated in a sweep launch).
-__device__ __forceinline__ void tn_fetch(const DevGeom& G, const float* x, int f, int c,
+__device__ __forceinline__ void tn_fetch(const SwGeom& G, const float* x, int f, int c,
                                          size_t vol, APre& pre) {
 #pragma unroll
   for (int i = 0; i < kTnPre; ++i) {
-    const int j = threadIdx.x + i * kWave;
+    const int j = lane_id() + i * kWave;
     const int ff = j < G.N_z ? f - 1 : f + 1;
     const bool ok = j < 2 * G.N_z && ff >= 0 && ff < G.N_time;
     pre.tn[i] = ok ? x[(size_t)ff * vol + (size_t)c * G.N_z + j % G.N_z] : 0.f;
@@ -453,15 +536,16 @@ __device__ __forceinline__ void tn_fetch(const DevGeom& G, const float* x, int f
 
 // Stores a prefetched window (first bins turned into sinogram offsets
 // through the frame's view list) and t+-1 values.
-__device__ __forceinline__ void awin_store(const DevGeom& G, const APre& pre, int nv,
+template <int kW>
+__device__ __forceinline__ void awin_store(const SwGeom& G, const APre& pre, int nv,
                                            const int* views, const Win& win, float* tn) {
-  const int lane = threadIdx.x;
+  const int lane = lane_id();
 #pragma unroll
   for (int i = 0; i < kPreI; ++i) {
     const int j = lane + i * kWave;
     if (j < nv) win.base[j] = pre.a0[i] >= 0 ? (views[j] * G.N_r + pre.a0[i]) * G.N_t : -1;
   }
-  const int nav = nv * G.W;
+  const int nav = nv * kW;
 #pragma unroll
   for (int i = 0; i < kPreA; ++i) {
     const int j = lane + i * kWave;
@@ -474,19 +558,20 @@ __device__ __forceinline__ void awin_store(const DevGeom& G, const APre& pre, in
   }
 }
 
-// Lane layout over a column's sinogram footprint: lane = (slot group g,
-// detector row rr); a lane takes view slots g, g + groups, ... in row rr (and
-// rr + rpl, ... when N_t > 64).  A slot's W bins lie N_t apart in e and w.
-// With a power-of-two rpl the slot groups of a row are lanes rr + k rpl,
-// reduced with lane shuffles.
+// Lane layout over a column's sinogram footprint: lane = (detector row rr,
+// slot group g), g fastest; a lane takes view slots g, g + groups, ... in row
+// rr (and rr + rpl, ... when N_t > 64).  A slot's W bins lie N_t apart in e
+// and w.  With a power-of-two rpl the slot groups of a row are `groups`
+// consecutive lanes, summed with DPP.
 struct Lanes {
   int rpl, groups, g, rr;
   bool pow2;
   __device__ Lanes(int N_t) {
+    const int lane = lane_id();
     rpl = N_t < kWave ? N_t : kWave;
     groups = kWave / rpl;
-    g = threadIdx.x / rpl;
-    rr = threadIdx.x % rpl;
+    rr = lane / groups;
+    g = lane % groups;
     pow2 = (rpl & (rpl - 1)) == 0;
   }
 };
@@ -496,15 +581,16 @@ struct Lanes {
 // batch of its slots back to back, then accumulates; row sums land in
 // row1/row2 (part1/part2: LDS partials for the non-power-of-two case).
 template <int kW>
-__device__ __forceinline__ void col_thetas(const DevGeom& G, const ColumnArgs& A, const Win& win, int nv,
-                           float* row1, float* row2, float* part1, float* part2) {
+__device__ __forceinline__ void col_thetas(const SwGeom& G, const ColumnArgs& A, const Win& win,
+                                           int nv, float* row1, float* row2, float* part1,
+                                           float* part2) {
   constexpr int SB = kW <= 5 ? 2 : 1;  // slots per load batch
-  const int lane = threadIdx.x;
+  const int lane = lane_id();
   const Lanes L(G.N_t);
   const float T2 = G.huber_T * G.huber_T, Thd = G.huber_T * G.huber_delta;
   for (int t0 = 0; t0 < G.N_t; t0 += L.rpl) {
     const int row = t0 + L.rr;
-    const bool lane_ok = L.g < L.groups && row < G.N_t;
+    const bool lane_ok = L.rr < L.rpl && row < G.N_t;
     float a1 = 0.f, a2 = 0.f;
     for (int sb = 0; sb < nv; sb += SB * L.groups) {
       float ev[SB][kW], wv[SB][kW], av[SB][kW];
@@ -537,10 +623,8 @@ __device__ __forceinline__ void col_thetas(const DevGeom& G, const ColumnArgs& A
         }
     }
     if (L.pow2) {
-      for (int off = L.rpl; off < kWave; off <<= 1) {
-        a1 += __shfl_xor(a1, off);
-        a2 += __shfl_xor(a2, off);
-      }
+      a1 = group_sum(a1, L.groups);
+      a2 = group_sum(a2, L.groups);
       if (L.g == 0 && row < G.N_t) {
         row1[row] = a1;
         row2[row] = a2;
@@ -549,14 +633,14 @@ __device__ __forceinline__ void col_thetas(const DevGeom& G, const ColumnArgs& A
       part1[lane] = a1;
       part2[lane] = a2;
       wave_sync();
-      if (lane < L.rpl && row < G.N_t) {
+      if (lane < L.rpl && t0 + lane < G.N_t) {
         float s1 = 0.f, s2 = 0.f;
         for (int gg = 0; gg < L.groups; ++gg) {
-          s1 += part1[gg * L.rpl + lane];
-          s2 += part2[gg * L.rpl + lane];
+          s1 += part1[lane * L.groups + gg];
+          s2 += part2[lane * L.groups + gg];
         }
-        row1[row] = s1;
-        row2[row] = s2;
+        row1[t0 + lane] = s1;
+        row2[t0 + lane] = s2;
       }
     }
     wave_sync();
@@ -566,12 +650,12 @@ __device__ __forceinline__ void col_thetas(const DevGeom& G, const ColumnArgs& A
 // e -= A delta_t dx along one voxel column (XT_VoxUpdate.c:96-116), with
 // atomics because other wavefronts' columns share detector bins.
 template <int kW>
-__device__ __forceinline__ void col_update(const DevGeom& G, const ColumnArgs& A, const Win& win, int nv,
-                           const float* dxz) {
+__device__ __forceinline__ void col_update(const SwGeom& G, const ColumnArgs& A, const Win& win,
+                                           int nv, const float* dxz) {
   const Lanes L(G.N_t);
   for (int t0 = 0; t0 < G.N_t; t0 += L.rpl) {
     const int row = t0 + L.rr;
-    if (L.g >= L.groups || row >= G.N_t) continue;
+    if (L.rr >= L.rpl || row >= G.N_t) continue;
     const float dz = dxz[row / G.zo];
     if (dz == 0.f) continue;
     for (int sl = L.g; sl < nv; sl += L.groups) {
@@ -605,12 +689,13 @@ __device__ __forceinline__ int fixed_nb(int m) {
 // The part of each voxel's surrogate (XT_ICD_update.c:133-182) that the
 // column's own update leaves unchanged: the terms of its neighbours outside
 // the column, evaluated once per column at the voxel's current value V.
-// lpv lanes per voxel, reduced with lane shuffles; writes fnum (sum of
+// lpv consecutive lanes per voxel, summed with DPP; writes fnum (sum of
 // weight x neighbour), fden (sum of weights) and fpos (some neighbour > 0).
-__device__ __forceinline__ void col_fixed(const DevGeom& G, const float* filt, const float* sx, int BW, int ly,
-                          int lx, int cy, int cx, int f, const float* tn, float* fnum,
-                          float* fden, int* fpos) {
-  const int lane = threadIdx.x, Nz = G.N_z;
+__device__ __forceinline__ void col_fixed(const SwGeom& G, const PriorTab& P, const float* sx,
+                                          int BW, int ly, int lx, int cy, int cx, int f,
+                                          const float* tn, float* fnum, float* fden,
+                                          int* fpos) {
+  const int lane = lane_id(), Nz = G.N_z;
   const int lpv = fixed_lpv(Nz), vpp = kWave / lpv;
   const int sub = lane % lpv;
   for (int zb = 0; zb < Nz; zb += vpp) {
@@ -637,17 +722,15 @@ __device__ __forceinline__ void col_fixed(const DevGeom& G, const float* filt, c
         }
         if (nbv > 0.f) pos = 1;
         if (ok) {
-          const float wt = nb_weight(G, filt, j, V, nbv);
+          const float wt = nb_weight(P, j, V, nbv);
           num += wt * nbv;
           den += wt;
         }
       }
     }
-    for (int off = lpv / 2; off > 0; off >>= 1) {
-      num += __shfl_xor(num, off);
-      den += __shfl_xor(den, off);
-      pos |= __shfl_xor(pos, off);
-    }
+    num = group_sum(num, lpv);
+    den = group_sum(den, lpv);
+    pos = group_or(pos, lpv);
     if (act && sub == 0) {
       fnum[z] = num;
       fden[z] = den;
@@ -665,13 +748,14 @@ struct SolveStats {
 // values, zero skipping (XT_VoxUpdate.c:238-262), the surrogate minimiser and
 // over-relaxation (XT_ICD_update.c:174-176).  New values go to the slab,
 // changes to dxz.
-__device__ __forceinline__ void col_solve(const DevGeom& G, const float* filt, float* sx, int centre0, int par,
-                          int iter, float alpha, const float* fnum, const float* fden,
-                          const int* fpos, const float* row1, const float* row2, float* dxz,
-                          SolveStats& st) {
+__device__ __forceinline__ void col_solve(const SwGeom& G, const PriorTab& P, float* sx,
+                                          int centre0, int par, int iter, float alpha,
+                                          const float* fnum, const float* fden,
+                                          const int* fpos, const float* row1,
+                                          const float* row2, float* dxz, SolveStats& st) {
   const int Nz = G.N_z;
   const int nzp = (Nz - par + 1) / 2;
-  const int vi = threadIdx.x;
+  const int vi = lane_id();
   if (vi >= nzp) return;
   const int z = par + 2 * vi;
   const int centre = centre0 + z;
@@ -680,14 +764,14 @@ __device__ __forceinline__ void col_solve(const DevGeom& G, const float* filt, f
   int pos = fpos[z];
   if (z > 0) {
     const float nbv = sx[centre - 1];
-    const float wt = nb_weight(G, filt, 4, V, nbv);
+    const float wt = nb_weight(P, 4, V, nbv);
     num += wt * nbv;
     den += wt;
     if (nbv > 0.f) pos = 1;
   }
   if (z + 1 < Nz) {
     const float nbv = sx[centre + 1];
-    const float wt = nb_weight(G, filt, 22, V, nbv);
+    const float wt = nb_weight(P, 22, V, nbv);
     num += wt * nbv;
     den += wt;
     if (nbv > 0.f) pos = 1;
@@ -700,7 +784,7 @@ __device__ __forceinline__ void col_solve(const DevGeom& G, const float* filt, f
       th1 += row1[z * G.zo + m];
       th2 += row2[z * G.zo + m];
     }
-    xn = substitute(G, alpha, V, th1, th2, num, den);
+    xn = substitute(alpha, V, th1, th2, num, den);
   }
   sx[centre] = xn;
   dxz[z] = xn - V;
@@ -728,29 +812,37 @@ struct SweepArgs {
   double* stats;
 };
 
+constexpr int kSweepThreads = 2 * kWave;
+
 // One ICD pass over the voxel columns of the tiles in this launch (one
-// 2x2-checkerboard colour) for the frames of one parity, one wavefront per
-// (tile, frame).  The tile's object slab plus a one-column halo lives in LDS
-// for the whole pass (neighbouring tiles and adjacent frames are not updated
-// in the same launch, so the halo and the t+-1 values are constant), with the
-// tile's column list and the frame's view list.  Per column: theta1/theta2
-// from the shared error sinogram, the surrogate's fixed part (neighbours
-// outside the column), the z-parity halves (each voxel sees its z+-1
-// neighbours' new values), and the error update with atomics; the next
-// column's A window and t+-1 values are prefetched into registers meanwhile.
+// 2x2-checkerboard colour) for the frames of one parity, one workgroup of two
+// wavefronts per (tile, frame).  The tile's object slab plus a one-column
+// halo lives in LDS for the whole pass (neighbouring tiles and adjacent frames
+// are not updated in the same launch, so the halo and the t+-1 values are
+// constant), with the tile's column list and the frame's view list.  Per
+// column, in two barrier-separated phases:
+//   1. wavefront 0 computes theta1/theta2 from the shared error sinogram and
+//      prefetches the next column's A window and t+-1 values into registers,
+//      while wavefront 1 evaluates the surrogate's terms of the neighbours
+//      outside the column;
+//   2. wavefront 1 solves the z-parity halves (each voxel sees its z+-1
+//      neighbours' new values) while wavefront 0 stores the prefetch;
+// then wavefront 0 sends the column's error update with atomics and moves on
+// to the next column's theta while wavefront 1 starts its terms.
 // kW = W, the A window width.
 template <int kW>
-__global__ __launch_bounds__(kWave) void k_sweep(DevGeom G, SweepArgs A) {
+__global__ __launch_bounds__(kSweepThreads) void k_sweep(SwGeom G, SweepArgs A) {
   extern __shared__ __align__(16) float smem[];
   __shared__ float s_tn[2][2 * kNZMax];
   __shared__ float s_row1[kNTMax], s_row2[kNTMax];
   __shared__ float s_part1[kWave], s_part2[kWave];
   __shared__ float s_dx[kNZMax], s_fnum[kNZMax], s_fden[kNZMax];
   __shared__ int s_fpos[kNZMax];
-  __shared__ float s_filt[28];
-  const int lane = threadIdx.x;
+  __shared__ PriorTab s_prior;
+  const int tid = threadIdx.x;
+  const bool w0 = tid < kWave;  // wavefront 0: sinogram side; 1: prior side
   const int wid = blockIdx.x;
-  load_filter(G, s_filt);
+  load_prior(G, &s_prior);
   const int tile = A.launch_tiles[wid % A.n_launch_tiles];
   const int f = A.fparity + 2 * (wid / A.n_launch_tiles);
   if (f >= G.N_time) return;
@@ -773,12 +865,12 @@ __global__ __launch_bounds__(kWave) void k_sweep(DevGeom G, SweepArgs A) {
     while (gcd_i(pa, n) != 1) pa = pa % (n - 1) + 1;
     pb = (int)(mix32(h + 0x9e3779b9u) % (unsigned)n);
   }
-  // the tile's slab with halo (8 loads per lane in flight), column and view lists
-  for (int j0 = lane; j0 < nx; j0 += 8 * kWave) {
+  // the tile's slab with halo (8 loads per thread in flight), column and view lists
+  for (int j0 = tid; j0 < nx; j0 += 8 * kSweepThreads) {
     float v[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      const int j = j0 + u * kWave;
+      const int j = j0 + u * kSweepThreads;
       const int z = j % Nz, b = j / Nz;
       const int yy = oy + b / BW, xx = ox + b % BW;
       v[u] = (j < nx && yy >= 0 && yy < G.N_y && xx >= 0 && xx < G.N_x)
@@ -787,23 +879,23 @@ __global__ __launch_bounds__(kWave) void k_sweep(DevGeom G, SweepArgs A) {
     }
 #pragma unroll
     for (int u = 0; u < 8; ++u)
-      if (j0 + u * kWave < nx) L.x[j0 + u * kWave] = v[u];
+      if (j0 + u * kSweepThreads < nx) L.x[j0 + u * kSweepThreads] = v[u];
   }
-  for (int j = lane; j < n; j += kWave) L.cols[j] = A.tile_cols[c0 + j];
-  for (int j = lane; j < nv; j += kWave) L.views[j] = A.col.frame_views[s0 + j];
-  wave_sync();
+  for (int j = tid; j < n; j += kSweepThreads) L.cols[j] = A.tile_cols[c0 + j];
+  for (int j = tid; j < nv; j += kSweepThreads) L.views[j] = A.col.frame_views[s0 + j];
+  __syncthreads();
   int pos = pb;  // visiting order: pos_k = (pb + k pa) mod n
-  {
+  if (w0) {
     APre pre;
     const int cf = L.cols[pos];
-    awin_fetch(G, A.col, cf, s0, nv, pre);
+    awin_fetch<kW>(G, A.col, cf, s0, nv, pre);
     tn_fetch(G, A.x, f, cf, vol, pre);
-    awin_store(G, pre, nv, L.views, L.win[0], s_tn[0]);
+    awin_store<kW>(G, pre, nv, L.views, L.win[0], s_tn[0]);
   }
   // nothing stays in flight: keeps the compiler's wait analysis from
   // carrying these loads into the column loop
   __builtin_amdgcn_s_waitcnt(0);
-  wave_sync();
+  __syncthreads();
   SolveStats st;
   for (int k = 0; k < n; ++k) {
     const bool odd = (k & 1) != 0;
@@ -815,56 +907,65 @@ __global__ __launch_bounds__(kWave) void k_sweep(DevGeom G, SweepArgs A) {
     if (pos >= n) pos -= n;
     const int cy = c / G.N_x, cx = c % G.N_x;
     const int ly = cy - oy, lx = cx - ox;
-    APre pre;
-    if (k + 1 < n) {
-      const int cn = L.cols[pos];
-      awin_fetch(G, A.col, cn, s0, nv, pre);
-      tn_fetch(G, A.x, f, cn, vol, pre);
-    }
-    col_thetas<kW>(G, A.col, wcur, nv, s_row1, s_row2, s_part1, s_part2);
-    col_fixed(G, s_filt, L.x, BW, ly, lx, cy, cx, f, tcur, s_fnum, s_fden, s_fpos);
-    wave_sync();
     const int centre0 = (ly * BW + lx) * Nz;
-    col_solve(G, s_filt, L.x, centre0, 0, A.iter, A.alpha, s_fnum, s_fden, s_fpos, s_row1, s_row2,
-              s_dx, st);
-    wave_sync();
-    col_solve(G, s_filt, L.x, centre0, 1, A.iter, A.alpha, s_fnum, s_fden, s_fpos, s_row1, s_row2,
-              s_dx, st);
-    wave_sync();
-    // the prefetch landed with the theta loads; store it, and retire every
-    // load explicitly (a free wait here) so that the compiler's wait analysis
-    // carries none of them into the next column, before the error update's
-    // atomics go out: no wait in this column is on those
-    if (k + 1 < n) awin_store(G, pre, nv, L.views, wnext, tnext);
-    __builtin_amdgcn_s_waitcnt(0);
-    col_update<kW>(G, A.col, wcur, nv, s_dx);
-    wave_sync();
+    APre pre;
+    // phase 1
+    if (w0) {
+      if (k + 1 < n) {
+        const int cn = L.cols[pos];
+        awin_fetch<kW>(G, A.col, cn, s0, nv, pre);
+        tn_fetch(G, A.x, f, cn, vol, pre);
+      }
+      col_thetas<kW>(G, A.col, wcur, nv, s_row1, s_row2, s_part1, s_part2);
+    } else {
+      col_fixed(G, s_prior, L.x, BW, ly, lx, cy, cx, f, tcur, s_fnum, s_fden, s_fpos);
+    }
+    __syncthreads();
+    // phase 2
+    if (w0) {
+      // the prefetch landed with the theta loads; retire every load
+      // explicitly (a free wait) so that the compiler's wait analysis carries
+      // none into the next column, before the error update's atomics go out
+      if (k + 1 < n) awin_store<kW>(G, pre, nv, L.views, wnext, tnext);
+      __builtin_amdgcn_s_waitcnt(0);
+    } else {
+      col_solve(G, s_prior, L.x, centre0, 0, A.iter, A.alpha, s_fnum, s_fden, s_fpos, s_row1,
+                s_row2, s_dx, st);
+      wave_sync();
+      col_solve(G, s_prior, L.x, centre0, 1, A.iter, A.alpha, s_fnum, s_fden, s_fpos, s_row1,
+                s_row2, s_dx, st);
+    }
+    __syncthreads();
+    if (w0) col_update<kW>(G, A.col, wcur, nv, s_dx);
   }
+  __syncthreads();
   // write the tile back (interior of the box, inside the object)
-  for (int j = lane; j < nx; j += kWave) {
+  for (int j = tid; j < nx; j += kSweepThreads) {
     const int z = j % Nz, b = j / Nz;
     const int by = b / BW, bx = b % BW;
     if (by == 0 || bx == 0 || by == BW - 1 || bx == BW - 1) continue;
     const int yy = oy + by, xx = ox + bx;
     if (yy < G.N_y && xx < G.N_x) xf[((size_t)yy * G.N_x + xx) * Nz + z] = L.x[j];
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    st.dx += __shfl_xor(st.dx, off);
-    st.x += __shfl_xor(st.x, off);
-    st.skip += __shfl_xor(st.skip, off);
-    st.upd += __shfl_xor(st.upd, off);
-  }
-  if (lane == 0) {
-    atomicAdd(&A.stats[0], st.dx);
-    atomicAdd(&A.stats[1], st.x);
-    atomicAdd(&A.stats[2], st.skip);
-    atomicAdd(&A.stats[3], st.upd);
+  if (!w0) {
+    for (int off = 32; off > 0; off >>= 1) {
+      st.dx += __shfl_xor(st.dx, off);
+      st.x += __shfl_xor(st.x, off);
+      st.skip += __shfl_xor(st.skip, off);
+      st.upd += __shfl_xor(st.upd, off);
+    }
+    if (tid == kWave) {
+      atomicAdd(&A.stats[0], st.dx);
+      atomicAdd(&A.stats[1], st.x);
+      atomicAdd(&A.stats[2], st.skip);
+      atomicAdd(&A.stats[3], st.upd);
+    }
   }
 }
 
 // theta1/theta2 of individual voxels through the sweep's column routine.
 template <int kW>
-__global__ __launch_bounds__(kWave) void k_thetas(DevGeom G, ColumnArgs A,
+__global__ __launch_bounds__(kWave) void k_thetas(SwGeom G, ColumnArgs A,
                                                   const int* __restrict__ frame_ptr,
                                                   const int* __restrict__ tzyx, int nvmax,
                                                   float* __restrict__ th1,
@@ -881,9 +982,9 @@ __global__ __launch_bounds__(kWave) void k_thetas(DevGeom G, ColumnArgs A,
   for (int j = threadIdx.x; j < nv; j += kWave) L.views[j] = A.frame_views[s0 + j];
   wave_sync();
   APre pre;
-  awin_fetch(G, A, c, s0, nv, pre);
+  awin_fetch<kW>(G, A, c, s0, nv, pre);
   for (int q = 0; q < kTnPre; ++q) pre.tn[q] = 0.f;
-  awin_store(G, pre, nv, L.views, L.win[0], s_tn);
+  awin_store<kW>(G, pre, nv, L.views, L.win[0], s_tn);
   __builtin_amdgcn_s_waitcnt(0);
   wave_sync();
   col_thetas<kW>(G, A, L.win[0], nv, s_row1, s_row2, s_part1, s_part2);
@@ -900,7 +1001,7 @@ __global__ __launch_bounds__(kWave) void k_thetas(DevGeom G, ColumnArgs A,
 
 // Functional substitution on independent voxels with the sweep's lane split
 // (16 lanes per voxel, 4 voxels per wavefront).
-__global__ __launch_bounds__(kWave) void k_subst(DevGeom G, int n,
+__global__ __launch_bounds__(kWave) void k_subst(SwGeom G, float alpha, int n,
                                                  const float* __restrict__ V,
                                                  const float* __restrict__ th1,
                                                  const float* __restrict__ th2,
@@ -910,8 +1011,8 @@ __global__ __launch_bounds__(kWave) void k_subst(DevGeom G, int n,
                                                  const unsigned char* __restrict__ tf,
                                                  float* __restrict__ out) {
   constexpr int lpv = 16;
-  __shared__ float s_filt[28];
-  load_filter(G, s_filt);
+  __shared__ PriorTab s_prior;
+  load_prior(G, &s_prior);
   __syncthreads();
   const int lane = threadIdx.x;
   const int i = blockIdx.x * (kWave / lpv) + lane / lpv;
@@ -925,7 +1026,7 @@ __global__ __launch_bounds__(kWave) void k_subst(DevGeom G, int n,
       const bool ok = j < 27 ? nbf[(size_t)i * 27 + j] != 0 : tf[(size_t)i * 2 + j - 27] != 0;
       if (!ok) continue;
       const float nbv = j < 27 ? nb[(size_t)i * 27 + j] : tn[(size_t)i * 2 + j - 27];
-      const float wt = nb_weight(G, s_filt, j, v, nbv);
+      const float wt = nb_weight(s_prior, j, v, nbv);
       num += wt * nbv;
       den += wt;
     }
@@ -934,7 +1035,7 @@ __global__ __launch_bounds__(kWave) void k_subst(DevGeom G, int n,
     num += __shfl_xor(num, off);
     den += __shfl_xor(den, off);
   }
-  if (act && sub == 0) out[i] = substitute(G, G.alpha, v, th1[i], th2[i], num, den);
+  if (act && sub == 0) out[i] = substitute(alpha, v, th1[i], th2[i], num, den);
 }
 
 // A^T v: one thread per (column, frame), z_overlap rows folded into z.
@@ -1047,6 +1148,34 @@ DevGeom make_dev_geom(const mbir_hip_geom& g, int nslots) {
   return G;
 }
 
+SwGeom make_sw_geom(const mbir_hip_geom& g, int nslots) {
+  SwGeom S;
+  memset(&S, 0, sizeof(S));
+  S.N_r = g.N_r;
+  S.N_t = g.N_t;
+  S.N_x = g.N_x;
+  S.N_y = g.N_y;
+  S.N_z = g.N_z;
+  S.N_time = g.N_time;
+  S.zo = g.mult_z;
+  S.nslots = nslots;
+  S.delta_t = (float)g.delta_t;
+  S.huber_T = (float)g.huber_T;
+  S.huber_delta = (float)g.huber_delta;
+  const double p[2] = {g.p_s, g.p_t}, c[2] = {g.c_s, g.c_t};
+  const double sq[2] = {g.sig_s_q, g.sig_t_q}, sqp[2] = {g.sig_s_qp, g.sig_t_qp};
+  for (int t = 0; t < 2; ++t) {
+    S.prior.qmp[t] = (float)(g.q - p[t]);
+    S.prior.c[t] = (float)c[t];
+    S.prior.inv_sq[t] = (float)(1.0 / sq[t]);
+    S.prior.inv_sqp[t] = (float)(1.0 / sqp[t]);
+    S.prior.b0[t] = (float)(g.q / (sq[t] * c[t]));
+  }
+  for (int k = 0; k < 27; ++k) S.prior.filt[k] = (float)g.filt[k];
+  S.prior.filt[27] = (float)g.filt_t;
+  return S;
+}
+
 template <typename T>
 int dev_alloc(T** p, size_t n) {
   *p = nullptr;
@@ -1060,6 +1189,7 @@ int dev_alloc(T** p, size_t n) {
 struct mbir_hip_ctx {
   mbir_hip_geom g;
   DevGeom G;
+  SwGeom SG;
   hipStream_t stream = nullptr;
   int nslots = 0, ncol = 0, nvmax = 0;
   size_t nsino = 0, nvox = 0;
@@ -1217,6 +1347,7 @@ int check_geom(const mbir_hip_geom* g) {
   if (g->N_z > kNZMax) return fail("N_z exceeds the sweep kernel's LDS column (128)");
   if (g->N_t > kNTMax) return fail("N_t exceeds the sweep kernel's LDS rows (256)");
   if (2 * g->mult_xy + 1 > kWMax) return fail("mult_xy exceeds 16");
+  if (g->q != 2.0) return fail("q must be 2 (the qGGMRF of the sweep kernels)");
   if ((long long)g->N_p * g->N_r * g->N_t >= (1LL << 31))
     return fail("sinogram too large for 32-bit offsets");
   if ((long long)g->N_z * g->mult_z > g->N_t) return fail("N_z*mult_z exceeds N_t");
@@ -1245,6 +1376,7 @@ int mbir_hip_create(mbir_hip_ctx** out, const mbir_hip_geom* g, const float* H,
   c->nsino = (size_t)g->N_p * g->N_r * g->N_t;
   c->nvox = (size_t)g->N_time * g->N_z * c->ncol;
   c->G = make_dev_geom(*g, c->nslots);
+  c->SG = make_sw_geom(*g, c->nslots);
   c->mask.assign(mask, mask + c->ncol);
   for (int f = 0; f < g->N_time; ++f)
     if (frame_ptr[f + 1] < frame_ptr[f]) {
@@ -1494,7 +1626,7 @@ int mbir_hip_icd_equits(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t
         if (ev_begin(c, MBIR_K_SWEEP)) return -1;
         MBIR_DISPATCH_W(c->G.W,
                         hipLaunchKernelGGL(k_sweep<kW>, dim3((unsigned)(ntl * frames)),
-                                           dim3(kWave), xbytes, c->stream, c->G, A));
+                                           dim3(kSweepThreads), xbytes, c->stream, c->SG, A));
         if (ev_end(c, MBIR_K_SWEEP)) return -1;
       }
     }
@@ -1511,7 +1643,8 @@ int mbir_hip_cost(mbir_hip_ctx* c, int32_t fresh, double* out) {
   if (!c || !out) return fail("null argument");
   const mbir_hip_geom& g = c->g;
   double data = 0.0, prior = 0.0;
-  if (fresh) {
+  if (fresh < 0 || fresh > 2) return fail("fresh must be 0, 1 or 2");
+  if (fresh == 1) {
     if (run_fwdproj(c, 1, c->d_x, nullptr)) return -1;
     const size_t nch = (g.N_t + fwd_tchunk(g, 8) - 1) / fwd_tchunk(g, 8);
     if (sum_partials(c, (size_t)g.N_p * nch, &data)) return -1;
@@ -1530,8 +1663,12 @@ int mbir_hip_cost(mbir_hip_ctx* c, int32_t fresh, double* out) {
     const unsigned blocks = 1024;
     if (ensure_partial(c, blocks)) return -1;
     if (ev_begin(c, MBIR_K_COST)) return -1;
-    hipLaunchKernelGGL(k_prior_cost, dim3(blocks), dim3(256), 0, c->stream, c->G, c->d_x,
-                       c->d_partial);
+    if (fresh == 2)
+      hipLaunchKernelGGL(k_prior_cost<true>, dim3(blocks), dim3(256), 0, c->stream, c->G,
+                         c->d_x, c->d_partial);
+    else
+      hipLaunchKernelGGL(k_prior_cost<false>, dim3(blocks), dim3(256), 0, c->stream, c->G,
+                         c->d_x, c->d_partial);
     if (ev_end(c, MBIR_K_COST)) return -1;
     if (sum_partials(c, blocks, &prior)) return -1;
   }
@@ -1548,7 +1685,8 @@ int mbir_hip_substitution_batch(const mbir_hip_geom* g, int32_t n, const float* 
   if (!g || n < 0 || (n > 0 && (!V || !theta1 || !theta2 || !nb || !nbf || !tn || !tf || !out)))
     return fail("bad argument");
   if (n == 0) return 0;
-  DevGeom G = make_dev_geom(*g, 0);
+  if (g->q != 2.0) return fail("q must be 2 (the qGGMRF of the sweep kernels)");
+  const SwGeom G = make_sw_geom(*g, 0);
   float *dV, *d1, *d2, *dnb, *dtn, *dout;
   unsigned char *dnbf, *dtf;
   if (dev_alloc(&dV, n) || dev_alloc(&d1, n) || dev_alloc(&d2, n) ||
@@ -1562,7 +1700,7 @@ int mbir_hip_substitution_batch(const mbir_hip_geom* g, int32_t n, const float* 
   HIP_OK(hipMemcpy(dtn, tn, (size_t)n * 2 * 4, hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(dnbf, nbf, (size_t)n * 27, hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(dtf, tf, (size_t)n * 2, hipMemcpyHostToDevice));
-  hipLaunchKernelGGL(k_subst, dim3((unsigned)((n + 3) / 4)), dim3(kWave), 0, 0, G, n, dV, d1, d2,
+  hipLaunchKernelGGL(k_subst, dim3((unsigned)((n + 3) / 4)), dim3(kWave), 0, 0, G, (float)g->alpha, n, dV, d1, d2,
                      dnb, dnbf, dtn, dtf, dout);
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpy(out, dout, n * 4, hipMemcpyDeviceToHost));
@@ -1594,7 +1732,7 @@ int mbir_hip_voxel_thetas(mbir_hip_ctx* c, int32_t n, const int32_t* tzyx, float
   A.e = c->d_e;
   const size_t shm = sizeof(float) * sweep_lds_words(0, c->nvmax, c->G.W, 0);
   MBIR_DISPATCH_W(c->G.W, hipLaunchKernelGGL(k_thetas<kW>, dim3(n), dim3(kWave), shm, c->stream,
-                                             c->G, A, c->d_frame_ptr, dq, c->nvmax, d1, d2));
+                                             c->SG, A, c->d_frame_ptr, dq, c->nvmax, d1, d2));
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(theta1, d1, n * 4, hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipMemcpyAsync(theta2, d2, n * 4, hipMemcpyDeviceToHost, c->stream));

@@ -163,9 +163,12 @@ class Stage:
         _lib.check(self.L.mbir_hip_icd_equits(self.h, n, first_iter, seed, _p(st)), "icd_equits")
         return st
 
-    def cost(self, fresh=False):
+    def cost(self, fresh=False, estimate=False):
+        """[total, data, prior] FP64 MAP cost; fresh: e recomputed from x;
+        estimate: maintained e with FP32 prior potentials (stopping tests)."""
         out = np.zeros(3, np.float64)
-        _lib.check(self.L.mbir_hip_cost(self.h, int(bool(fresh)), _p(out)), "cost")
+        mode = 2 if estimate else int(bool(fresh))
+        _lib.check(self.L.mbir_hip_cost(self.h, mode, _p(out)), "cost")
         return out
 
     def thetas(self, tzyx):

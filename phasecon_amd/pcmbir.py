@@ -1,0 +1,166 @@
+"""ctypes binding of phasecon_amd/libpcmbir4d.so (include/pcmbir_hip.h): the
+Fresnel transform and the batched steepest-descent phase retrieval on the GPU.
+There is no fallback: a missing library raises."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libpcmbir4d.so")
+
+_SIGS = {
+    "pcmbir_hip_last_error": (C.c_char_p, []),
+    "pcmbir_hip_create": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_double,
+                                    C.c_double, C.c_double, C.c_double, C.c_void_p]),
+    "pcmbir_hip_destroy": (None, [C.c_void_p]),
+    "pcmbir_hip_fresnel": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    "pcmbir_hip_set_phase_problem": (C.c_int, [C.c_void_p] + [C.c_void_p] * 6 + [C.c_double]),
+    "pcmbir_hip_sd_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_void_p, C.c_void_p]),
+    "pcmbir_hip_get_phase_state": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pcmbir_hip_phase_cost": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pcmbir_hip_fft_time": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pcmbir_hip_set_nms_prior": (C.c_int, [C.c_void_p] + [C.c_double] * 5 + [C.c_int32] +
+                                 [C.c_double] * 2),
+    "pcmbir_hip_nms_pass": (C.c_int, [C.c_void_p] * 5),
+    "pcmbir_hip_estimate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double,
+                                      C.c_int32, C.c_double] + [C.c_void_p] * 5),
+}
+
+_LIB = None
+
+
+def exported_names():
+    return list(_SIGS)
+
+
+def load():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run `python -m phasecon_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed: {load().pcmbir_hip_last_error().decode()}")
+
+
+def _c64(a, shape):
+    """complex array -> contiguous interleaved float32 pairs"""
+    a = np.asarray(a).reshape(shape)
+    out = np.empty(shape + (2,), np.float32)
+    out[..., 0] = a.real
+    out[..., 1] = a.imag
+    return out
+
+
+def _cplx(f):
+    return f[..., 0].astype(np.float64) + 1j * f[..., 1].astype(np.float64)
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class PhaseRetrieval:
+    """GPU context for n_views views of rows x cols (powers of two)."""
+
+    def __init__(self, n_views, rows, cols, d_rows, d_cols, lam, dist, window=None):
+        self.L = load()
+        self.shape = (n_views, rows, cols)
+        self.h = C.c_void_p()
+        win = None if window is None else np.ascontiguousarray(window, np.float32)
+        self._win = win
+        _check(self.L.pcmbir_hip_create(C.byref(self.h), n_views, rows, cols, d_rows, d_cols,
+                                        lam, dist, None if win is None else _p(win)),
+               "pcmbir_hip_create")
+
+    def close(self):
+        if self.h:
+            self.L.pcmbir_hip_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def fresnel(self, x, herm=False):
+        xi = _c64(x, self.shape)
+        y = np.empty_like(xi)
+        _check(self.L.pcmbir_hip_fresnel(self.h, int(herm), _p(xi), _p(y)), "pcmbir_hip_fresnel")
+        return _cplx(y)
+
+    def set_problem(self, y, D, lam, v, w, omega, nu):
+        self._keep = [_c64(a, self.shape) for a in (y, D, v, w, omega)]
+        lm = np.ascontiguousarray(np.broadcast_to(lam, self.shape), np.float32)
+        self._keep.append(lm)
+        yy, dd, vv, ww, oo = self._keep[:5]
+        _check(self.L.pcmbir_hip_set_phase_problem(self.h, _p(yy), _p(dd), _p(lm), _p(vv),
+                                                   _p(ww), _p(oo), float(nu)),
+               "pcmbir_hip_set_phase_problem")
+
+    def sd_run(self, max_iters, thresh):
+        it = np.zeros(self.shape[0], np.int32)
+        upd = np.zeros(self.shape[0], np.float64)
+        _check(self.L.pcmbir_hip_sd_run(self.h, int(max_iters), float(thresh), _p(it), _p(upd)),
+               "pcmbir_hip_sd_run")
+        return it, upd
+
+    def state(self):
+        w = np.empty(self.shape + (2,), np.float32)
+        om = np.empty_like(w)
+        _check(self.L.pcmbir_hip_get_phase_state(self.h, _p(w), _p(om)), "get_phase_state")
+        return _cplx(w), _cplx(om)
+
+    def cost(self):
+        out = np.zeros(self.shape[0], np.float64)
+        _check(self.L.pcmbir_hip_phase_cost(self.h, _p(out)), "pcmbir_hip_phase_cost")
+        return out
+
+    def fft_time(self):
+        ms = C.c_double()
+        n = C.c_int64()
+        _check(self.L.pcmbir_hip_fft_time(self.h, C.byref(ms), C.byref(n)), "pcmbir_hip_fft_time")
+        return ms.value, n.value
+
+    def set_nms_prior(self, phase_sig, phase_c, mag_sig, mag_c, delta_over_beta, decorrelate,
+                      p=1.2, nu=0.5):
+        _check(self.L.pcmbir_hip_set_nms_prior(self.h, phase_sig, phase_c, mag_sig, mag_c,
+                                               delta_over_beta, int(decorrelate), p, nu),
+               "pcmbir_hip_set_nms_prior")
+
+    def nms_pass(self, zr, zi, b):
+        zr = np.ascontiguousarray(zr, np.float64).reshape(self.shape).copy()
+        zi = np.ascontiguousarray(zi, np.float64).reshape(self.shape).copy()
+        bb = _c64(b, self.shape)
+        tot = C.c_int64()
+        _check(self.L.pcmbir_hip_nms_pass(self.h, _p(zr), _p(zi), _p(bb), C.byref(tot)),
+               "pcmbir_hip_nms_pass")
+        return zr, zi, tot.value
+
+    def estimate(self, zr, zi, pret_iters, pret_thresh, sd_iters, sd_thresh):
+        n = self.shape[0]
+        zr = np.ascontiguousarray(zr, np.float64).reshape(self.shape).copy()
+        zi = np.ascontiguousarray(zi, np.float64).reshape(self.shape).copy()
+        it = np.zeros(n, np.int32)
+        c0 = np.zeros(n)
+        c1 = np.zeros(n)
+        pr = np.zeros((n, max(pret_iters, 1)))
+        du = np.zeros((n, max(pret_iters, 1)))
+        _check(self.L.pcmbir_hip_estimate(self.h, _p(zr), _p(zi), int(pret_iters),
+                                          float(pret_thresh), int(sd_iters), float(sd_thresh),
+                                          _p(it), _p(c0), _p(c1), _p(pr), _p(du)),
+               "pcmbir_hip_estimate")
+        return dict(zr=zr, zi=zi, iters=it, cost_init=c0, cost_final=c1, primal=pr, dual=du)

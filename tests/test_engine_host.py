@@ -20,9 +20,9 @@ LIB = os.path.join(ROOT, "phasecon_amd", "libmbir4d.so")
 needs_lib = pytest.mark.skipif(not os.path.exists(LIB), reason="libmbir4d.so not built")
 
 
-def _header_functions():
+def _header_functions(headers=("mbir4d.h", "mbir_engine.h", "mbir_hip.h")):
     names = set()
-    for h in ("mbir4d.h", "mbir_engine.h", "mbir_hip.h"):
+    for h in headers:
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         for m in re.finditer(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text):
@@ -42,6 +42,20 @@ def test_library_exports_every_declared_function():
     for name in declared:
         assert hasattr(L, name), name
     assert set(lib.exported_names()) <= set(declared)
+
+
+PCLIB = os.path.join(ROOT, "phasecon_amd", "libpcmbir4d.so")
+
+
+@pytest.mark.skipif(not os.path.exists(PCLIB), reason="libpcmbir4d.so not built")
+def test_pcmbir_library_exports_every_declared_function():
+    from phasecon_amd import pcmbir
+    L = C.CDLL(PCLIB)
+    declared = _header_functions(("pcmbir_hip.h",))
+    assert "pcmbir_hip_sd_run" in declared
+    for name in declared:
+        assert hasattr(L, name), name
+    assert set(pcmbir.exported_names()) == set(declared)
 
 
 @needs_lib

@@ -139,11 +139,15 @@ def test_cost_matches_oracle(pair):
     st.init_error()
     fresh = st.cost(fresh=True)
     kept = st.cost(fresh=False)
+    est = st.cost(estimate=True)
     ox = x.astype(np.float64)
     oc, parts = o.cost(o.init_error(ox), ox)
     assert abs(fresh[0] - oc) <= 1e-9 * abs(oc)
     assert abs(fresh[2] - parts[1]) <= 1e-10 * abs(parts[1])
     assert abs(kept[0] - oc) <= 1e-6 * abs(oc)
+    # the stopping estimate: FP32 prior potentials within 1e-6 of the FP64 prior
+    assert abs(est[2] - parts[1]) <= 1e-6 * abs(parts[1])
+    assert est[1] == kept[1]
 
 
 @pytest.mark.parametrize("name,n_gpu,tile", [

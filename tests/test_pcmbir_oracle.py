@@ -1,0 +1,233 @@
+"""Pins the numpy PCMBIR oracle (oracle/pcmbir_oracle.py) by closed forms.
+
+FFTW is absent, so no reference build pins the Fresnel transform; the checks
+below fix what the mathematics fixes, each chosen so that a plausible mistake
+(a sign of the phase, a wrong frequency index, a missing 1/N, the window on
+the adjoint, a dropped term of the gradient) fails one of them.  CPU only.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+from oracle import pcmbir_oracle as P  # noqa: E402
+
+GEO = dict(d_rows=1.0, d_cols=1.0, lam=P.wavelength(17.0), dist=1e5)
+
+
+def rnd(shape, seed):
+    r = np.random.default_rng(seed)
+    return r.standard_normal(shape) + 1j * r.standard_normal(shape)
+
+
+def test_wavelength_uses_reference_constant():
+    # Q10: the reference's Planck constant is 10x below CODATA
+    assert P.wavelength(17.0) == pytest.approx(4.135668e-18 * 299792458e6 / 17.0 / 10, rel=1e-15)
+
+
+def test_freqs_match_fft_grid():
+    u, v = P.freqs(8, 16, 0.5, 2.0)
+    assert np.allclose(u[:, 0], np.fft.fftfreq(8, 0.5))
+    assert np.allclose(v[0], np.fft.fftfreq(16, 2.0))
+
+
+def test_unit_window_is_unitary_and_adjoint_inverts():
+    x = rnd((16, 32), 1)
+    y = P.fresnel(x, **GEO)
+    assert np.linalg.norm(y) == pytest.approx(np.linalg.norm(x), rel=1e-12)
+    back = P.fresnel(y, **GEO, herm=True)
+    assert np.allclose(back, x, atol=1e-12)
+
+
+def test_adjoint_identity_without_window():
+    x, z = rnd((16, 16), 2), rnd((16, 16), 3)
+    lhs = np.vdot(P.fresnel(x, **GEO), z)
+    rhs = np.vdot(x, P.fresnel(z, **GEO, herm=True))
+    assert lhs == pytest.approx(rhs, rel=1e-12)
+
+
+def test_window_breaks_adjointness_as_the_reference_does():
+    # Q11: H applies the window, H^H does not
+    win = P.window(16, 16, 1.0, 1.0, 0.2)
+    x, z = rnd((16, 16), 4), rnd((16, 16), 5)
+    lhs = np.vdot(P.fresnel(x, **GEO, win=win), z)
+    rhs = np.vdot(x, P.fresnel(z, **GEO, herm=True))
+    assert abs(lhs - rhs) > 1e-3 * abs(lhs)
+
+
+def test_zero_distance_is_identity():
+    x = rnd((8, 8), 6)
+    assert np.allclose(P.fresnel(x, 1.0, 1.0, GEO["lam"], 0.0), x, atol=1e-13)
+
+
+@pytest.mark.parametrize("k0,l0", [(1, 0), (3, 5), (-2, 7)])
+def test_plane_wave_eigenvalue(k0, l0):
+    rows, cols, dr, dc = 16, 32, 0.7, 1.3
+    r = np.arange(rows)[:, None]
+    t = np.arange(cols)[None, :]
+    x = np.exp(2j * np.pi * (k0 * r / rows + l0 * t / cols))
+    u0, v0 = k0 / (rows * dr), l0 / (cols * dc)
+    sigma = 0.3
+    win = P.window(rows, cols, dr, dc, sigma)
+    y = P.fresnel(x, dr, dc, GEO["lam"], GEO["dist"], win=win)
+    phase = -np.pi * GEO["lam"] * GEO["dist"] * (u0 * u0 + v0 * v0)
+    wv = np.exp(-(u0 * u0 + v0 * v0) / (2 * sigma * sigma))
+    assert np.allclose(y, x * np.exp(1j * phase) * wv, atol=1e-12)
+
+
+def problem(seed, rows=16, cols=16, windowed=False):
+    """A view's problem; with the reference's window (windowed) H^H is not
+    the adjoint of H (Q11), so gradient and step are exact only without it."""
+    r = np.random.default_rng(seed)
+    win = (P.window(rows, cols, 1.0, 1.0, 2.0 / np.sqrt(GEO["lam"] * GEO["dist"]))
+           if windowed else None)
+    y = np.abs(rnd((rows, cols), seed)) + 0.5
+    D = np.exp(-0.1 * r.random((rows, cols)) - 1j * 0.3 * r.random((rows, cols)))
+    Lam = 0.5 + r.random((rows, cols))
+    v = rnd((rows, cols), seed + 1) * 0.1 + 1.0
+    return P.PhaseProblem(y, D, Lam, v, 0.7, 1.0, 1.0, GEO["lam"], GEO["dist"], win)
+
+
+def test_gradient_is_the_cost_derivative():
+    pp = problem(10)
+    w = rnd((16, 16), 11) * 0.2 + 1.0
+    om = pp.phase_projection(w)
+    g = pp.gradient(om, w)
+    h = 1e-6
+    for (i, j) in [(0, 0), (3, 7), (15, 2)]:
+        for part, unit in ((g.real, 1.0), (g.imag, 1j)):
+            wp, wm = w.copy(), w.copy()
+            wp[i, j] += h * unit
+            wm[i, j] -= h * unit
+            fd = (pp.cost(om, wp) - pp.cost(om, wm)) / (2 * h)
+            assert fd == pytest.approx(part[i, j], rel=1e-5, abs=1e-7)
+
+
+def test_windowed_gradient_differs_from_the_cost_derivative():
+    # Q11 consequence: with the window the reference's gradient is inexact
+    pp = problem(10, windowed=True)
+    w = rnd((16, 16), 11) * 0.2 + 1.0
+    om = pp.phase_projection(w)
+    g = pp.gradient(om, w)
+    h = 1e-6
+    wp, wm = w.copy(), w.copy()
+    wp[3, 7] += h
+    wm[3, 7] -= h
+    fd = (pp.cost(om, wp) - pp.cost(om, wm)) / (2 * h)
+    assert abs(fd - g.real[3, 7]) > 1e-4 * abs(fd)
+
+
+def test_stepsize_is_the_exact_line_minimiser():
+    pp = problem(12)
+    w = rnd((16, 16), 13) * 0.2 + 1.0
+    om = pp.phase_projection(w)
+    g = pp.gradient(om, w)
+    a = pp.stepsize(om, g)
+    c0 = pp.cost(om, w - a * g)
+    for s in (0.9, 1.1):
+        assert pp.cost(om, w - s * a * g) > c0
+    # the cost is quadratic along g: three points fix the parabola's vertex
+    c = [pp.cost(om, w - t * a * g) for t in (0.0, 1.0, 2.0)]
+    vertex = 1.0 - (c[2] - c[0]) / (2 * (c[2] - 2 * c[1] + c[0]))
+    assert vertex == pytest.approx(1.0, abs=1e-9)
+
+
+def test_phase_projection_is_unit_modulus_and_optimal():
+    pp = problem(14)
+    w = rnd((16, 16), 15) * 0.2 + 1.0
+    om = pp.phase_projection(w)
+    assert np.allclose(np.abs(om), 1.0, atol=1e-12)
+    # Omega maximises Re(conj(y) Omega H D w) pointwise: any other phase costs more
+    c0 = pp.cost(om, w)
+    assert pp.cost(om * np.exp(0.05j), w) > c0
+
+
+def test_sd_decreases_the_cost():
+    pp = problem(16)  # unit window: exact gradient and step
+    w = rnd((16, 16), 17) * 0.2 + 1.0
+    om = pp.phase_projection(w)
+    c_prev = pp.cost(om, w)
+    for _ in range(5):
+        w, om, upd = pp.sd_iter(om, w)
+        c = pp.cost(om, w)
+        assert c <= c_prev * (1 + 1e-12)
+        c_prev = c
+
+
+# ------------------------------------------------------------- NMS (a13)
+PRIOR = dict(phase_sig=2e-2, phase_c=1e-2, mag_sig=2e-2, mag_c=1e-2, delta_over_beta=50.0)
+
+
+def test_prior_filter_is_the_normalised_inverse_distance_slice():
+    f = P.prior_filter_2d()
+    s = 6 + 12 / np.sqrt(2) + 8 / np.sqrt(3)
+    assert f[1, 1] == 0
+    assert f[0, 1] == pytest.approx(1 / s) and f[0, 0] == pytest.approx(1 / np.sqrt(2) / s)
+    assert np.allclose(f, f.T) and np.allclose(f, f[::-1, ::-1])
+
+
+def test_qggmrf_value_special_cases():
+    pr = P.NmsPrior(**PRIOR, decorrelate=False)
+    assert pr.value(0.0, 0) == 0.0
+    # p = q: the potential is the quadratic d^2 / (sigma^2 (c + 1))
+    pq = P.NmsPrior(**PRIOR, decorrelate=False, p=2.0)
+    assert pq.value(0.3, 1) == pytest.approx(0.09 / (2e-2 ** 2 * (1e-2 + 1)), rel=1e-14)
+
+
+def test_cost_without_neighbours_is_the_data_term():
+    pr = P.NmsPrior(**PRIOR, decorrelate=True)
+    zr, zi = 0.3, 1.1
+    b = 0.4 - 0.2j
+    fl = np.zeros((3, 3), bool)
+    c = P.cost_nms(zr, zi, b.real, b.imag, 0.5, np.zeros((3, 3)), np.zeros((3, 3)), fl, pr)
+    assert c == pytest.approx(0.5 / 2 * abs(np.exp(-(zr + 1j * zi)) - b) ** 2, rel=1e-14)
+
+
+def test_cost_of_equal_neighbours_has_no_prior_term():
+    pr = P.NmsPrior(**PRIOR, decorrelate=True)
+    fl = np.ones((3, 3), bool)
+    nr, ni = np.full((3, 3), 0.3), np.full((3, 3), 1.1)
+    c = P.cost_nms(0.3, 1.1, 0.0, 0.0, 0.5, nr, ni, fl, pr)
+    assert c == pytest.approx(0.5 / 2 * np.exp(-0.6), rel=1e-14)
+
+
+def test_cost_single_neighbour_decorrelated_terms():
+    pr = P.NmsPrior(**PRIOR, decorrelate=True)
+    fl = np.zeros((3, 3), bool)
+    fl[1, 2] = True
+    nr, ni = np.zeros((3, 3)), np.zeros((3, 3))
+    nr[1, 2], ni[1, 2] = 0.1, 0.5
+    zr, zi = 0.3, 1.1
+    data = P.cost_nms(zr, zi, 0, 0, 0.5, nr, ni, np.zeros((3, 3), bool), pr)
+    c = P.cost_nms(zr, zi, 0, 0, 0.5, nr, ni, fl, pr)
+    db, dd = zr - 0.1, zi - 0.5
+    k = PRIOR["delta_over_beta"]
+    want = pr.filt[1, 2] * (pr.value(dd + k * db, 0) + pr.value(dd - k * db, 1))
+    assert c - data == pytest.approx(want, rel=1e-12)
+
+
+def test_nms_finds_the_known_minimum():
+    pr = P.NmsPrior(**PRIOR, decorrelate=False)
+    zs = 0.35 + 0.8j
+    b = np.exp(-zs)
+    fl = np.zeros((3, 3), bool)
+    zr, zi, it = P.nms_search(0.3, 0.7, b.real, b.imag, 0.5, np.zeros((3, 3)),
+                              np.zeros((3, 3)), fl, pr, thresh=1e-9, max_iter=400)
+    assert abs(zr - zs.real) < 1e-4 and abs(zi - zs.imag) < 1e-4
+
+
+def test_nms_pass_never_raises_a_pixel_cost():
+    # the reference checks this after every search (XT_CmplxProjEst.c:162-163)
+    pr = P.NmsPrior(**PRIOR, decorrelate=True)
+    r = np.random.default_rng(3)
+    zr, zi = r.random((6, 6)) * 0.05 + 0.02, r.random((6, 6)) * 0.5 + 0.2
+    b = P.transmittance(zr * 1.2, zi * 0.8)
+    for (j, k) in [(0, 0), (2, 3), (5, 5)]:
+        nr, ni, fl = P.neighbours(zr, zi, j, k)
+        c0 = P.cost_nms(zr[j, k], zi[j, k], b[j, k].real, b[j, k].imag, 0.5, nr, ni, fl, pr)
+        a, bb, _ = P.nms_search(zr[j, k], zi[j, k], b[j, k].real, b[j, k].imag, 0.5, nr, ni, fl, pr)
+        assert P.cost_nms(a, bb, b[j, k].real, b[j, k].imag, 0.5, nr, ni, fl, pr) <= c0
