@@ -45,6 +45,11 @@ METRIC = "sec to MAP cost ≤1e-5 rel of FP64 oracle; equits/s; HBM GB/s; learne
 PEAK_HBM_GBPS = 8000.0
 REL_TOL = 1e-5
 
+# HBM bytes per k_sweep launch measured with rocprofv3 on this build:
+# FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (KiB x 1024), the factors read off a
+# cold 1 GiB device copy (bench.py --profile-equits 3 --calibrate).
+MEASURED_TRAFFIC = {("ch4d", 8): 152582.8 * 1024 * 2.0 + 120815.0 * 1024 * 1.0}
+
 
 def scan_digest(scan):
     h = hashlib.sha256()
@@ -128,6 +133,83 @@ def cpu_baseline(scan, equits=2):
             "equit_seconds": [float(v) for v in secs]}
 
 
+def calibration_copy(nbytes=1 << 30):
+    """A plain device-to-device copy of nbytes (the HIP runtime's copy
+    kernel) whose source was evicted from L2 and the Infinity Cache first (a
+    second buffer of the same size is filled after it): the reference reading
+    for FETCH_SIZE / WRITE_SIZE counter scaling.  Returns the bytes read
+    (= bytes written)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    bufs = [C.c_void_p() for _ in range(3)]
+    for b in bufs:
+        if hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    src, dst, other = bufs
+    hip.hipMemset(src, 1, C.c_size_t(nbytes))
+    hip.hipMemset(other, 2, C.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    rc = hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    for b in bufs:
+        hip.hipFree(b)
+    if rc != 0:
+        raise RuntimeError("hipMemcpy failed")
+    return nbytes
+
+
+def pcmbir_bench(views, n, sd_steps, nms):
+    """Phase-retrieval throughput on one GPU at the PCMBIR workload's shape
+    (views x n x n transforms, n = 512 the padded 256^2 detector): seconds
+    per steepest-descent iteration over all views (6 Fresnel transforms of
+    3 passes each per view), the FFT passes' HBM rate against 8 TB/s, and one
+    Nelder-Mead pass (pixels per second).  Synthetic data: a weak random phase
+    object propagated by the same operator."""
+    from phasecon_amd.pcmbir import PhaseRetrieval, wavelength, fresnel_window
+    lam, dist = wavelength(17.0), 1e5
+    win = fresnel_window(n, n, 1.0, 1.0, 2.0 / np.sqrt(lam * dist))
+    rng = np.random.default_rng(0)
+    shape = (views, n, n)
+    t0 = time.time()
+    ctx = PhaseRetrieval(views, n, n, 1.0, 1.0, lam, dist, win)
+    obj = np.exp(-0.01 * rng.random(shape) - 0.2j * rng.random(shape))
+    y = np.abs(ctx.fresnel(obj)).astype(np.complex64)
+    D = np.ones(shape, np.complex64)
+    lamw = np.ones(shape, np.float32)
+    w = np.ones(shape, np.complex64)
+    om = np.ones(shape, np.complex64)
+    ctx.set_problem(y, D, lamw, w, w, om, 0.5)
+    print(f"[bench] pcmbir setup {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    ctx.sd_run(1, 0.0)            # warm-up
+    ctx.fft_time()
+    t0 = time.perf_counter()
+    ctx.sd_run(sd_steps, 0.0)
+    dt = (time.perf_counter() - t0) / sd_steps
+    fft_ms, launches = ctx.fft_time()
+    per_transform_bytes = 3 * 2 * n * n * 8          # 3 passes, read + write, complex64
+    transforms = 6 * views * sd_steps                # 5 batched calls, one over 2 x views
+    gbps = per_transform_bytes * transforms / (fft_ms * 1e-3) / 1e9
+    out = {"pcmbir": {"views": views, "n": n, "s_per_sd_iter": dt,
+                      "fft_ms_per_sd_iter": fft_ms / sd_steps,
+                      "fft_hbm_gbps": gbps, "fft_frac": gbps / PEAK_HBM_GBPS}}
+    ctx.close()
+    if nms:
+        nv = min(views, 8)  # the per-pixel FP64 search is timed on a few views
+        sub = (nv, n, n)
+        cn = PhaseRetrieval(nv, n, n, 1.0, 1.0, lam, dist, win)
+        cn.set_nms_prior(2e-2, 1e-2, 2e-2, 1e-2, 50.0, True, 1.2, 0.5)
+        zr = rng.random(sub) * 0.05 + 0.02
+        zi = rng.random(sub) * 0.5 + 0.2
+        t0 = time.perf_counter()
+        _, _, its = cn.nms_pass(zr, zi, obj[:nv])
+        dt_nms = time.perf_counter() - t0
+        out["pcmbir"].update({"nms_views": nv, "nms_pass_s": dt_nms,
+                              "nms_pixels_per_s": nv * n * n / dt_nms,
+                              "nms_avg_iters": its / (nv * n * n)})
+        cn.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,7 +225,16 @@ def main():
     ap.add_argument("--longrun", type=int, default=0)
     ap.add_argument("--dump-x", default=None)
     ap.add_argument("--profile-equits", type=int, default=0)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="with --profile-equits: also run the 1 GiB calibration copy")
+    ap.add_argument("--pcmbir", action="store_true",
+                    help="phase-retrieval throughput line instead of the MBIR metric")
+    ap.add_argument("--pcmbir-views", type=int, default=360)
+    ap.add_argument("--pcmbir-n", type=int, default=512)
     args = ap.parse_args()
+    if args.pcmbir and args.gpus == 1:
+        pcmbir_bench(args.pcmbir_views, args.pcmbir_n, max(args.steps, 1), nms=True)
+        return
     if args.gpus != 1:
         print("bench.py: the MBIR sweep runs as one process on one GPU; --gpus must be 1 "
               "(multi-GPU runs are not supported).", file=sys.stderr)
@@ -168,9 +259,11 @@ def main():
             st.icd(1, it, args.seed)
         ms, n = st.kernel_time(K_SWEEP)
         bytes_equit = algorithmic_bytes_per_equit(st)
-        print(json.dumps({"profile_equits": args.profile_equits, "sweep_launches": n,
-                          "sweep_avg_ms": ms / n, "bytes_per_launch": bytes_equit * args.profile_equits / n,
-                          "calibration_copy_bytes": 4 * int(np.prod(st.shape_x))}))
+        rec = {"profile_equits": args.profile_equits, "sweep_launches": n,
+               "sweep_avg_ms": ms / n, "bytes_per_launch": bytes_equit * args.profile_equits / n}
+        if args.calibrate:
+            rec["calibration_copy_bytes"] = calibration_copy()
+        print(json.dumps(rec))
         return
 
     if args.longrun:
@@ -264,7 +357,7 @@ def main():
             "peak": PEAK_HBM_GBPS,
             "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBPS,
-            "traffic": None,
+            "traffic": MEASURED_TRAFFIC.get((args.workload, args.tile)),
             "kernel": "k_sweep",
             "bytes_per_launch": bytes_launch,
             "avg_launch_ms": avg_launch_ms,

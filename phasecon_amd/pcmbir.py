@@ -31,6 +31,25 @@ _SIGS = {
 
 _LIB = None
 
+PLANCKS_CONSTANT = 4.135668e-19  # keV s, the reference's value (PCMBIR_3D/XT_Constants.h:84)
+LIGHT_SPEED = 299792458e6        # um/s (PCMBIR_3D/XT_Constants.h:85)
+
+
+def wavelength(light_energy_kev):
+    """X-ray wavelength in um for the photon energy in keV as the reference
+    computes it (PCMBIR_3D/XT_Init.c:258), including its Planck constant."""
+    return PLANCKS_CONSTANT * LIGHT_SPEED / light_energy_kev
+
+
+def fresnel_window(rows, cols, d_rows, d_cols, sigma):
+    """Gaussian frequency window exp(-(u^2+v^2)/(2 sigma^2)) on the FFT grid
+    (create_FresnelTranWindow, PCMBIR_3D/XT_Init.c:210-231)."""
+    i = np.arange(rows, dtype=np.float64)
+    j = np.arange(cols, dtype=np.float64)
+    u = np.where(i >= rows // 2, -(rows - i), i) / (rows * d_rows)
+    v = np.where(j >= cols // 2, -(cols - j), j) / (cols * d_cols)
+    return np.exp(-(u[:, None] ** 2 + v[None, :] ** 2) / (2 * sigma * sigma))
+
 
 def exported_names():
     return list(_SIGS)

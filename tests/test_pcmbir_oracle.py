@@ -231,3 +231,10 @@ def test_nms_pass_never_raises_a_pixel_cost():
         c0 = P.cost_nms(zr[j, k], zi[j, k], b[j, k].real, b[j, k].imag, 0.5, nr, ni, fl, pr)
         a, bb, _ = P.nms_search(zr[j, k], zi[j, k], b[j, k].real, b[j, k].imag, 0.5, nr, ni, fl, pr)
         assert P.cost_nms(a, bb, b[j, k].real, b[j, k].imag, 0.5, nr, ni, fl, pr) <= c0
+
+
+def test_product_geometry_helpers_match_the_oracle():
+    from phasecon_amd.pcmbir import wavelength, fresnel_window
+    assert wavelength(17.0) == P.wavelength(17.0)
+    assert np.allclose(fresnel_window(16, 32, 0.7, 1.3, 0.2), P.window(16, 32, 0.7, 1.3, 0.2),
+                       rtol=1e-15, atol=0)
