@@ -739,6 +739,111 @@ __device__ __forceinline__ void col_fixed(const SwGeom& G, const PriorTab& P, co
   }
 }
 
+// Prior constants in registers (the prior wavefront loads them once).
+struct PriorReg {
+  float qmp[2], c[2], inv_sq[2], inv_sqp[2], b0[2];
+  float f4, f22;  // weights of the column's own z-1 / z+1 neighbours
+  __device__ void load(const PriorTab& P) {
+    for (int t = 0; t < 2; ++t) {
+      qmp[t] = P.qmp[t];
+      c[t] = P.c[t];
+      inv_sq[t] = P.inv_sq[t];
+      inv_sqp[t] = P.inv_sqp[t];
+      b0[t] = P.b0[t];
+    }
+    f4 = P.filt[4];
+    f22 = P.filt[22];
+  }
+  // b(d) of prior t (0 spatial, 1 temporal)
+  __device__ __forceinline__ float b(bool temporal, float d) const {
+    return temporal ? qg_b(d, qmp[1], c[1], inv_sq[1], inv_sqp[1], b0[1])
+                    : qg_b(d, qmp[0], c[0], inv_sq[0], inv_sqp[0], b0[0]);
+  }
+};
+
+// A lane's share of the out-of-column neighbours of its voxel, fixed for
+// the whole sweep when lpv >= 8 (N_z <= 8: at most 4 of the 26 per lane):
+// LDS offset (from the voxel centre in the slab, or from the voxel's z in
+// the t+-1 buffer), weight, and validity: a z / temporal condition fixed for
+// the lane, and the tile-edge conditions (bits: needs row above / below,
+// column left / right) tested per column.
+constexpr int kNbLane = 4;
+struct NbTab {
+  int off[kNbLane];
+  float wt[kNbLane];
+  unsigned needs[kNbLane];  // bits 0-3 edges; bit 4 temporal; bit 5 usable
+};
+
+__device__ __forceinline__ void nbtab_init(const SwGeom& G, const PriorTab& P, int BW, int f,
+                                           NbTab& T) {
+  const int lane = lane_id(), Nz = G.N_z;
+  const int lpv = fixed_lpv(Nz), sub = lane % lpv, z = lane / lpv;
+#pragma unroll
+  for (int i = 0; i < kNbLane; ++i) {
+    const int m = sub + i * lpv;
+    T.off[i] = 0;
+    T.wt[i] = 0.f;
+    T.needs[i] = 0;
+    if (m >= 26 || z >= Nz) continue;
+    const int j = fixed_nb(m);
+    if (j < 27) {
+      const int dz = j / 9 - 1, dy = (j / 3) % 3 - 1, dx = j % 3 - 1;
+      if (z + dz < 0 || z + dz >= Nz) continue;
+      T.off[i] = (dy * BW + dx) * Nz + dz;
+      T.wt[i] = P.filt[j];
+      T.needs[i] = (dy < 0 ? 1u : 0u) | (dy > 0 ? 2u : 0u) | (dx < 0 ? 4u : 0u) |
+                   (dx > 0 ? 8u : 0u) | 32u;
+    } else {
+      const int ff = j == 27 ? f - 1 : f + 1;
+      if (ff < 0 || ff >= G.N_time) continue;
+      T.off[i] = (j - 27) * Nz;
+      T.wt[i] = P.filt[27];
+      T.needs[i] = 16u | 32u;
+    }
+  }
+}
+
+// col_fixed for lpv >= 8 from the lane's table: per neighbour one edge test,
+// one LDS read and one qGGMRF weight.
+__device__ __forceinline__ void col_fixed_tab(const SwGeom& G, const PriorReg& R, const NbTab& T,
+                                              const float* sx, int BW, int ly, int lx, int cy,
+                                              int cx, const float* tn, float* fnum, float* fden,
+                                              int* fpos) {
+  const int lane = lane_id(), Nz = G.N_z;
+  const int lpv = fixed_lpv(Nz), sub = lane % lpv, z = lane / lpv;
+  const bool act = z < Nz;
+  const int centre = (ly * BW + lx) * Nz + z;
+  const unsigned edge = (cy == 0 ? 1u : 0u) | (cy == G.N_y - 1 ? 2u : 0u) |
+                        (cx == 0 ? 4u : 0u) | (cx == G.N_x - 1 ? 8u : 0u);
+  const float V = act ? sx[centre] : 0.f;
+  float nbv[kNbLane];
+  bool ok[kNbLane];
+#pragma unroll
+  for (int i = 0; i < kNbLane; ++i) {
+    const unsigned nd = T.needs[i];
+    ok[i] = act && (nd & 32u) && !(nd & edge & 15u);
+    const float* src = (nd & 16u) ? tn + z : sx + centre;
+    nbv[i] = ok[i] ? src[T.off[i]] : 0.f;
+  }
+  float num = 0.f, den = 0.f;
+  int pos = 0;
+#pragma unroll
+  for (int i = 0; i < kNbLane; ++i) {
+    if (nbv[i] > 0.f) pos = 1;
+    const float wt = ok[i] ? T.wt[i] * R.b((T.needs[i] & 16u) != 0, V - nbv[i]) : 0.f;
+    num += wt * nbv[i];
+    den += wt;
+  }
+  num = group_sum(num, lpv);
+  den = group_sum(den, lpv);
+  pos = group_or(pos, lpv);
+  if (act && sub == 0) {
+    fnum[z] = num;
+    fden[z] = den;
+    fpos[z] = pos;
+  }
+}
+
 struct SolveStats {
   double dx = 0.0, x = 0.0, skip = 0.0, upd = 0.0;
 };
@@ -748,7 +853,7 @@ struct SolveStats {
 // values, zero skipping (XT_VoxUpdate.c:238-262), the surrogate minimiser and
 // over-relaxation (XT_ICD_update.c:174-176).  New values go to the slab,
 // changes to dxz.
-__device__ __forceinline__ void col_solve(const SwGeom& G, const PriorTab& P, float* sx,
+__device__ __forceinline__ void col_solve(const SwGeom& G, const PriorReg& R, float* sx,
                                           int centre0, int par, int iter, float alpha,
                                           const float* fnum, const float* fden,
                                           const int* fpos, const float* row1,
@@ -764,14 +869,14 @@ __device__ __forceinline__ void col_solve(const SwGeom& G, const PriorTab& P, fl
   int pos = fpos[z];
   if (z > 0) {
     const float nbv = sx[centre - 1];
-    const float wt = nb_weight(P, 4, V, nbv);
+    const float wt = R.f4 * R.b(false, V - nbv);
     num += wt * nbv;
     den += wt;
     if (nbv > 0.f) pos = 1;
   }
   if (z + 1 < Nz) {
     const float nbv = sx[centre + 1];
-    const float wt = nb_weight(P, 22, V, nbv);
+    const float wt = R.f22 * R.b(false, V - nbv);
     num += wt * nbv;
     den += wt;
     if (nbv > 0.f) pos = 1;
@@ -840,7 +945,10 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SwGeom G, SweepArgs A) 
   __shared__ int s_fpos[kNZMax];
   __shared__ PriorTab s_prior;
   const int tid = threadIdx.x;
-  const bool w0 = tid < kWave;  // wavefront 0: sinogram side; 1: prior side
+  // wavefront 0: sinogram side; 1: prior side.  Read from lane 0 into a
+  // scalar register, so the role branch is a uniform (scalar) branch and each
+  // wavefront executes only its own loop's barriers.
+  const bool w0 = __builtin_amdgcn_readfirstlane(tid) < kWave;
   const int wid = blockIdx.x;
   load_prior(G, &s_prior);
   const int tile = A.launch_tiles[wid % A.n_launch_tiles];
@@ -897,46 +1005,62 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SwGeom G, SweepArgs A) 
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   SolveStats st;
-  for (int k = 0; k < n; ++k) {
-    const bool odd = (k & 1) != 0;
-    const Win wcur = odd ? L.win[1] : L.win[0], wnext = odd ? L.win[0] : L.win[1];
-    const float* tcur = odd ? s_tn[1] : s_tn[0];
-    float* tnext = odd ? s_tn[0] : s_tn[1];
-    const int c = L.cols[pos];
-    pos += pa;
-    if (pos >= n) pos -= n;
-    const int cy = c / G.N_x, cx = c % G.N_x;
-    const int ly = cy - oy, lx = cx - ox;
-    const int centre0 = (ly * BW + lx) * Nz;
-    APre pre;
-    // phase 1
-    if (w0) {
+  // Each wavefront runs its own loop over the columns; both loops pass the
+  // same two workgroup barriers per column, so the phases stay aligned.
+  if (w0) {
+    for (int k = 0; k < n; ++k) {
+      const bool odd = (k & 1) != 0;
+      const Win wcur = odd ? L.win[1] : L.win[0], wnext = odd ? L.win[0] : L.win[1];
+      float* tnext = odd ? s_tn[0] : s_tn[1];
+      pos += pa;
+      if (pos >= n) pos -= n;
+      APre pre;
+      // phase 1: prefetch the next column, theta of this one
       if (k + 1 < n) {
         const int cn = L.cols[pos];
         awin_fetch<kW>(G, A.col, cn, s0, nv, pre);
         tn_fetch(G, A.x, f, cn, vol, pre);
       }
       col_thetas<kW>(G, A.col, wcur, nv, s_row1, s_row2, s_part1, s_part2);
-    } else {
-      col_fixed(G, s_prior, L.x, BW, ly, lx, cy, cx, f, tcur, s_fnum, s_fden, s_fpos);
-    }
-    __syncthreads();
-    // phase 2
-    if (w0) {
-      // the prefetch landed with the theta loads; retire every load
-      // explicitly (a free wait) so that the compiler's wait analysis carries
-      // none into the next column, before the error update's atomics go out
+      __syncthreads();
+      // phase 2: store the prefetch (it landed with the theta loads) and
+      // retire every load explicitly (a free wait) so that the compiler's
+      // wait analysis carries none into the next column, before the error
+      // update's atomics go out
       if (k + 1 < n) awin_store<kW>(G, pre, nv, L.views, wnext, tnext);
       __builtin_amdgcn_s_waitcnt(0);
-    } else {
-      col_solve(G, s_prior, L.x, centre0, 0, A.iter, A.alpha, s_fnum, s_fden, s_fpos, s_row1,
-                s_row2, s_dx, st);
-      wave_sync();
-      col_solve(G, s_prior, L.x, centre0, 1, A.iter, A.alpha, s_fnum, s_fden, s_fpos, s_row1,
-                s_row2, s_dx, st);
+      __syncthreads();
+      col_update<kW>(G, A.col, wcur, nv, s_dx);
     }
-    __syncthreads();
-    if (w0) col_update<kW>(G, A.col, wcur, nv, s_dx);
+  } else {
+    PriorReg R;
+    R.load(s_prior);
+    NbTab NT;
+    const bool tab = fixed_lpv(Nz) >= 8;
+    if (tab) nbtab_init(G, s_prior, BW, f, NT);
+    for (int k = 0; k < n; ++k) {
+      const bool odd = (k & 1) != 0;
+      const float* tcur = odd ? s_tn[1] : s_tn[0];
+      const int c = L.cols[pos];
+      pos += pa;
+      if (pos >= n) pos -= n;
+      const int cy = c / G.N_x, cx = c % G.N_x;
+      const int ly = cy - oy, lx = cx - ox;
+      const int centre0 = (ly * BW + lx) * Nz;
+      // phase 1: the surrogate's terms of the neighbours outside the column
+      if (tab)
+        col_fixed_tab(G, R, NT, L.x, BW, ly, lx, cy, cx, tcur, s_fnum, s_fden, s_fpos);
+      else
+        col_fixed(G, s_prior, L.x, BW, ly, lx, cy, cx, f, tcur, s_fnum, s_fden, s_fpos);
+      __syncthreads();
+      // phase 2: the z-parity halves
+      col_solve(G, R, L.x, centre0, 0, A.iter, A.alpha, s_fnum, s_fden, s_fpos, s_row1, s_row2,
+                s_dx, st);
+      wave_sync();
+      col_solve(G, R, L.x, centre0, 1, A.iter, A.alpha, s_fnum, s_fden, s_fpos, s_row1, s_row2,
+                s_dx, st);
+      __syncthreads();
+    }
   }
   __syncthreads();
   // write the tile back (interior of the box, inside the object)
