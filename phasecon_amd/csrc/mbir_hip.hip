@@ -469,21 +469,26 @@ struct Win {
   float* av;
 };
 
-// Dynamic LDS of the sweep: the tile slab, two A windows, the frame's view
-// list and the tile's column list (sizes in 4-byte words).
+// Dynamic LDS of the sweep: the tile slab, three A windows (the previous,
+// current and next column), the frame's view list and the tile's column list
+// (sizes in 4-byte words).
+constexpr int kWinBufs = 3;
 __host__ __device__ inline size_t sweep_lds_words(size_t xfloats, int nvmax, int W, int ncols) {
-  return xfloats + 2 * ((size_t)nvmax + (size_t)nvmax * W) + (size_t)nvmax + (size_t)ncols;
+  return xfloats + kWinBufs * ((size_t)nvmax + (size_t)nvmax * W) + (size_t)nvmax +
+         (size_t)ncols;
 }
 
 struct SweepLds {
   float* x;
-  Win win[2];
+  Win win[kWinBufs];
   int* views;
   int* cols;
+  int wstride;  // words between consecutive window buffers
   __device__ SweepLds(float* base, int xfloats, int nvmax, int W) {
     x = base;
+    wstride = nvmax * (1 + W);
     float* p = base + xfloats;
-    for (int b = 0; b < 2; ++b) {
+    for (int b = 0; b < kWinBufs; ++b) {
       win[b].base = reinterpret_cast<int*>(p);
       p += nvmax;
       win[b].av = p;
@@ -495,36 +500,52 @@ struct SweepLds {
   }
 };
 
-// Register staging of the next column's A window and t+-1 values (prefetch).
+// Register staging of a column's A window and t+-1 values (prefetch): PI
+// first bins and PA values per lane (kSmall: frames of at most 64 views and
+// 256 window values).
+template <bool kSmall>
 struct APre {
-  int a0[kPreI];
-  float av[kPreA];
+  static constexpr int PI = kSmall ? 1 : kPreI;
+  static constexpr int PA = kSmall ? 4 : kPreA;
+  int a0[PI];
+  float av[PA];
   float tn[kTnPre];
 };
 
+// Window buffer b of the sweep's LDS by address arithmetic (the buffers are
+// contiguous), so that no array of pointers is indexed at run time.
+template <typename L_>
+__device__ __forceinline__ Win pick_win(const L_& L, int b) {
+  Win w;
+  w.base = L.win[0].base + b * L.wstride;
+  w.av = L.win[0].av + b * L.wstride;
+  return w;
+}
+
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 
-template <int kW>
+template <int kW, bool kSmall>
 __device__ __forceinline__ void awin_fetch(const SwGeom& G, const ColumnArgs& A, int c, int s0,
-                                           int nv, APre& pre) {
+                                           int nv, APre<kSmall>& pre) {
   const int lane = lane_id();
   const size_t base = (size_t)c * G.nslots + s0;
 #pragma unroll
-  for (int i = 0; i < kPreI; ++i) {
+  for (int i = 0; i < APre<kSmall>::PI; ++i) {
     const int j = lane + i * kWave;
     pre.a0[i] = j < nv ? A.afirst[base + j] : -1;
   }
   const int nav = nv * kW;
 #pragma unroll
-  for (int i = 0; i < kPreA; ++i) {
+  for (int i = 0; i < APre<kSmall>::PA; ++i) {
     const int j = lane + i * kWave;
     pre.av[i] = j < nav ? A.aval[base * kW + j] : 0.f;
   }
 }
 
 // The t-1 and t+1 columns of voxel column c (not updated in a sweep launch).
+template <bool kSmall>
 __device__ __forceinline__ void tn_fetch(const SwGeom& G, const float* x, int f, int c,
-                                         size_t vol, APre& pre) {
+                                         size_t vol, APre<kSmall>& pre) {
 #pragma unroll
   for (int i = 0; i < kTnPre; ++i) {
     const int j = lane_id() + i * kWave;
@@ -536,18 +557,18 @@ __device__ __forceinline__ void tn_fetch(const SwGeom& G, const float* x, int f,
 
 // Stores a prefetched window (first bins turned into sinogram offsets
 // through the frame's view list) and t+-1 values.
-template <int kW>
-__device__ __forceinline__ void awin_store(const SwGeom& G, const APre& pre, int nv,
+template <int kW, bool kSmall>
+__device__ __forceinline__ void awin_store(const SwGeom& G, const APre<kSmall>& pre, int nv,
                                            const int* views, const Win& win, float* tn) {
   const int lane = lane_id();
 #pragma unroll
-  for (int i = 0; i < kPreI; ++i) {
+  for (int i = 0; i < APre<kSmall>::PI; ++i) {
     const int j = lane + i * kWave;
     if (j < nv) win.base[j] = pre.a0[i] >= 0 ? (views[j] * G.N_r + pre.a0[i]) * G.N_t : -1;
   }
   const int nav = nv * kW;
 #pragma unroll
-  for (int i = 0; i < kPreA; ++i) {
+  for (int i = 0; i < APre<kSmall>::PA; ++i) {
     const int j = lane + i * kWave;
     if (j < nav) win.av[j] = pre.av[i];
   }
@@ -644,6 +665,87 @@ __device__ __forceinline__ void col_thetas(const SwGeom& G, const ColumnArgs& A,
       }
     }
     wave_sync();
+  }
+}
+
+// Pipelined theta (the column's window fits one load batch: every slot of
+// a lane's row in SB slots, N_t <= 64 and a power of two): the e/w loads of
+// the next column go out while the current column is solved, and are
+// consumed after it, corrected for the current column's own error update,
+// which reaches the shared sinogram only after them.
+template <int kW>
+struct ThetaLoads {
+  static constexpr int SB = kW <= 5 ? 2 : 1;
+  float ev[SB][kW], wv[SB][kW];
+};
+
+template <int kW>
+__device__ __forceinline__ void theta_issue(const SwGeom& G, const ColumnArgs& A, const Win& win,
+                                            int nv, ThetaLoads<kW>& tl) {
+  constexpr int SB = ThetaLoads<kW>::SB;
+  const Lanes L(G.N_t);
+  const int row = L.rr;
+  const bool lane_ok = L.rr < L.rpl && row < G.N_t;
+#pragma unroll
+  for (int b = 0; b < SB; ++b) {
+    const int sl = L.g + b * L.groups;
+    const int base = (lane_ok && sl < nv) ? win.base[sl] : -1;
+#pragma unroll
+    for (int k = 0; k < kW; ++k) {
+      const float a = base >= 0 ? win.av[sl * kW + k] : 0.f;
+      const bool ok = a != 0.f;
+      const int o = ok ? base + k * G.N_t + row : 0;
+      // unconditional loads (a fixed count per lane keeps the compiler's
+      // wait counts exact); entries with a = 0 are not used
+      tl.ev[b][k] = e_load(A.e + o);
+      tl.wv[b][k] = A.w[o];
+    }
+  }
+}
+
+// theta1/theta2 per row from loads issued by theta_issue on window `win`,
+// first subtracting the previous column's own update (window `wprev`, voxel
+// changes dxp; has_prev = 0 for the first column), as the sinogram held no
+// part of it when the loads went out.
+template <int kW>
+__device__ __forceinline__ void theta_finish(const SwGeom& G, const Win& win, const Win& wprev,
+                                             bool has_prev, const float* dxp, int lognt,
+                                             int nv, const ThetaLoads<kW>& tl, float* row1,
+                                             float* row2) {
+  constexpr int SB = ThetaLoads<kW>::SB;
+  const Lanes L(G.N_t);
+  const int row = L.rr;
+  const bool lane_ok = L.rr < L.rpl && row < G.N_t;
+  const float T2 = G.huber_T * G.huber_T, Thd = G.huber_T * G.huber_delta;
+  const float dzp = (has_prev && lane_ok) ? dxp[row / G.zo] : 0.f;
+  float a1 = 0.f, a2 = 0.f;
+#pragma unroll
+  for (int b = 0; b < SB; ++b) {
+    const int sl = L.g + b * L.groups;
+    const bool sok = lane_ok && sl < nv;
+    const int base = sok ? win.base[sl] : -1;
+    const int bprev = (sok && dzp != 0.f) ? wprev.base[sl] : -1;
+    // bin offset of this window's first bin in the previous window
+    const int shift = (base >= 0 && bprev >= 0) ? (base - bprev) >> lognt : kW;
+#pragma unroll
+    for (int k = 0; k < kW; ++k) {
+      const float a = base >= 0 ? win.av[sl * kW + k] : 0.f;
+      const bool ok = a != 0.f;
+      float e = ok ? tl.ev[b][k] : 0.f;
+      const int d = shift + k;
+      if (ok && d >= 0 && d < kW) e -= (wprev.av[sl * kW + d] * G.delta_t) * dzp;
+      const float w = ok ? tl.wv[b][k] : 0.f, av = a * G.delta_t;
+      float hw = w;
+      if (e * e * w >= T2) hw = Thd * sqrtf(w) * __builtin_amdgcn_rcpf(fabsf(e));
+      a2 += av * av * hw;
+      a1 -= e * av * hw;
+    }
+  }
+  a1 = group_sum(a1, L.groups);
+  a2 = group_sum(a2, L.groups);
+  if (L.g == 0 && lane_ok) {
+    row1[row] = a1;
+    row2[row] = a2;
   }
 }
 
@@ -935,10 +1037,10 @@ constexpr int kSweepThreads = 2 * kWave;
 // then wavefront 0 sends the column's error update with atomics and moves on
 // to the next column's theta while wavefront 1 starts its terms.
 // kW = W, the A window width.
-template <int kW>
+template <int kW, bool kSmall>
 __global__ __launch_bounds__(kSweepThreads) void k_sweep(SwGeom G, SweepArgs A) {
   extern __shared__ __align__(16) float smem[];
-  __shared__ float s_tn[2][2 * kNZMax];
+  __shared__ float s_tn[kWinBufs][2 * kNZMax];
   __shared__ float s_row1[kNTMax], s_row2[kNTMax];
   __shared__ float s_part1[kWave], s_part2[kWave];
   __shared__ float s_dx[kNZMax], s_fnum[kNZMax], s_fden[kNZMax];
@@ -993,12 +1095,18 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SwGeom G, SweepArgs A) 
   for (int j = tid; j < nv; j += kSweepThreads) L.views[j] = A.col.frame_views[s0 + j];
   __syncthreads();
   int pos = pb;  // visiting order: pos_k = (pb + k pa) mod n
+  // the theta pipeline applies when a column's window is one load batch per
+  // lane (every slot of a lane's row in SB slots, N_t <= 64, a power of two)
+  constexpr int SB = ThetaLoads<kW>::SB;
+  const int rpl = G.N_t < kWave ? G.N_t : kWave;
+  const bool pipe = G.N_t <= kWave && (G.N_t & (G.N_t - 1)) == 0 && nv <= SB * (kWave / rpl);
   if (w0) {
-    APre pre;
+    // window and t+-1 values of the first column
+    APre<kSmall> pre;
     const int cf = L.cols[pos];
-    awin_fetch<kW>(G, A.col, cf, s0, nv, pre);
-    tn_fetch(G, A.x, f, cf, vol, pre);
-    awin_store<kW>(G, pre, nv, L.views, L.win[0], s_tn[0]);
+    awin_fetch<kW, kSmall>(G, A.col, cf, s0, nv, pre);
+    tn_fetch<kSmall>(G, A.x, f, cf, vol, pre);
+    awin_store<kW, kSmall>(G, pre, nv, L.views, L.win[0], s_tn[0]);
   }
   // nothing stays in flight: keeps the compiler's wait analysis from
   // carrying these loads into the column loop
@@ -1008,30 +1116,68 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SwGeom G, SweepArgs A) 
   // Each wavefront runs its own loop over the columns; both loops pass the
   // same two workgroup barriers per column, so the phases stay aligned.
   if (w0) {
-    for (int k = 0; k < n; ++k) {
-      const bool odd = (k & 1) != 0;
-      const Win wcur = odd ? L.win[1] : L.win[0], wnext = odd ? L.win[0] : L.win[1];
-      float* tnext = odd ? s_tn[0] : s_tn[1];
-      pos += pa;
-      if (pos >= n) pos -= n;
-      APre pre;
-      // phase 1: prefetch the next column, theta of this one
-      if (k + 1 < n) {
-        const int cn = L.cols[pos];
-        awin_fetch<kW>(G, A.col, cn, s0, nv, pre);
-        tn_fetch(G, A.x, f, cn, vol, pre);
+    const int lognt = __builtin_ctz((unsigned)G.N_t);
+    ThetaLoads<kW> tl;
+    // pipeline: the prefetch of column k+1 is in flight during column k's
+    // solve, and stored at the start of column k's step (before the theta
+    // loads of column k+1 go out)
+    APre<kSmall> pre;
+    if (pipe) {
+      if (n > 1) {
+        int p1 = pos + pa;
+        if (p1 >= n) p1 -= n;
+        const int cn = L.cols[p1];
+        awin_fetch<kW, kSmall>(G, A.col, cn, s0, nv, pre);
+        tn_fetch<kSmall>(G, A.x, f, cn, vol, pre);
       }
-      col_thetas<kW>(G, A.col, wcur, nv, s_row1, s_row2, s_part1, s_part2);
-      __syncthreads();
-      // phase 2: store the prefetch (it landed with the theta loads) and
-      // retire every load explicitly (a free wait) so that the compiler's
-      // wait analysis carries none into the next column, before the error
-      // update's atomics go out
-      if (k + 1 < n) awin_store<kW>(G, pre, nv, L.views, wnext, tnext);
-      __builtin_amdgcn_s_waitcnt(0);
-      __syncthreads();
-      col_update<kW>(G, A.col, wcur, nv, s_dx);
+      theta_issue<kW>(G, A.col, L.win[0], nv, tl);
     }
+    for (int k = 0; k < n; ++k) {
+      const int bc = k % kWinBufs, bn = (k + 1) % kWinBufs, bp = (k + 2) % kWinBufs;
+      const Win wcur = pick_win(L, bc), wnext = pick_win(L, bn);
+      const Win wprev = pick_win(L, bp);  // column k-1's window
+      int p1 = pos + pa;
+      if (p1 >= n) p1 -= n;
+      int p2 = p1 + pa;
+      if (p2 >= n) p2 -= n;
+      pos = p1;
+      if (pipe) {
+        // phase 1: theta of column k from the loads issued during the last
+        // solve, corrected for column k-1's own update; that update's
+        // atomics; column k+1's prefetch (it landed with the theta loads)
+        // stored over column k-2's window; the prefetch of column k+2
+        theta_finish<kW>(G, wcur, wprev, k > 0, s_dx, lognt, nv, tl, s_row1, s_row2);
+        if (k > 0) col_update<kW>(G, A.col, wprev, nv, s_dx);
+        if (k + 1 < n) awin_store<kW, kSmall>(G, pre, nv, L.views, wnext, s_tn[bn]);
+        if (k + 2 < n) {
+          const int cn = L.cols[p2];
+          awin_fetch<kW, kSmall>(G, A.col, cn, s0, nv, pre);
+          tn_fetch<kSmall>(G, A.x, f, cn, vol, pre);
+        }
+        __syncthreads();
+        // phase 2 (during the solve of column k): column k+1's theta loads
+        if (k + 1 < n) theta_issue<kW>(G, A.col, wnext, nv, tl);
+        __syncthreads();
+      } else {
+        // phase 1: prefetch of column k+1, theta of column k
+        if (k + 1 < n) {
+          const int cn = L.cols[p1];
+          awin_fetch<kW, kSmall>(G, A.col, cn, s0, nv, pre);
+          tn_fetch<kSmall>(G, A.x, f, cn, vol, pre);
+        }
+        col_thetas<kW>(G, A.col, wcur, nv, s_row1, s_row2, s_part1, s_part2);
+        __syncthreads();
+        // phase 2: store the prefetch (it landed with the theta loads) and
+        // retire every load explicitly (a free wait here) so that the
+        // compiler's wait analysis carries none into the next column, before
+        // the error update's atomics go out
+        if (k + 1 < n) awin_store<kW, kSmall>(G, pre, nv, L.views, wnext, s_tn[bn]);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        col_update<kW>(G, A.col, wcur, nv, s_dx);
+      }
+    }
+    if (pipe) col_update<kW>(G, A.col, pick_win(L, (n - 1) % kWinBufs), nv, s_dx);
   } else {
     PriorReg R;
     R.load(s_prior);
@@ -1039,8 +1185,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SwGeom G, SweepArgs A) 
     const bool tab = fixed_lpv(Nz) >= 8;
     if (tab) nbtab_init(G, s_prior, BW, f, NT);
     for (int k = 0; k < n; ++k) {
-      const bool odd = (k & 1) != 0;
-      const float* tcur = odd ? s_tn[1] : s_tn[0];
+      const float* tcur = s_tn[k % kWinBufs];
       const int c = L.cols[pos];
       pos += pa;
       if (pos >= n) pos -= n;
@@ -1105,10 +1250,10 @@ __global__ __launch_bounds__(kWave) void k_thetas(SwGeom G, ColumnArgs A,
   const int s0 = frame_ptr[f], nv = frame_ptr[f + 1] - s0;
   for (int j = threadIdx.x; j < nv; j += kWave) L.views[j] = A.frame_views[s0 + j];
   wave_sync();
-  APre pre;
-  awin_fetch<kW>(G, A, c, s0, nv, pre);
+  APre<false> pre;
+  awin_fetch<kW, false>(G, A, c, s0, nv, pre);
   for (int q = 0; q < kTnPre; ++q) pre.tn[q] = 0.f;
-  awin_store<kW>(G, pre, nv, L.views, L.win[0], s_tn);
+  awin_store<kW, false>(G, pre, nv, L.views, L.win[0], s_tn);
   __builtin_amdgcn_s_waitcnt(0);
   wave_sync();
   col_thetas<kW>(G, A, L.win[0], nv, s_row1, s_row2, s_part1, s_part2);
@@ -1735,6 +1880,8 @@ int mbir_hip_icd_equits(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t
   if (xbytes + 8 * 1024 > c->lds_max)
     return fail("tile slab and A windows exceed the LDS of one workgroup");
   const int nf = c->g.N_time;
+  // register-light prefetch for frames of at most 64 views and 256 A values
+  const bool small = c->nvmax <= kWave && c->nvmax * c->G.W <= 4 * kWave;
   for (int it = 0; it < n; ++it) {
     A.iter = first_iter + it;
     for (int colour = 0; colour < 4; ++colour) {
@@ -1748,9 +1895,15 @@ int mbir_hip_icd_equits(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t
         A.fparity = fpar;
         A.seed = seed * 2654435761u + (unsigned)(it * 8 + colour * 2 + fpar) * 40503u;
         if (ev_begin(c, MBIR_K_SWEEP)) return -1;
-        MBIR_DISPATCH_W(c->G.W,
-                        hipLaunchKernelGGL(k_sweep<kW>, dim3((unsigned)(ntl * frames)),
-                                           dim3(kSweepThreads), xbytes, c->stream, c->SG, A));
+        if (small) {
+          MBIR_DISPATCH_W(c->G.W,
+                          hipLaunchKernelGGL((k_sweep<kW, true>), dim3((unsigned)(ntl * frames)),
+                                             dim3(kSweepThreads), xbytes, c->stream, c->SG, A));
+        } else {
+          MBIR_DISPATCH_W(c->G.W,
+                          hipLaunchKernelGGL((k_sweep<kW, false>), dim3((unsigned)(ntl * frames)),
+                                             dim3(kSweepThreads), xbytes, c->stream, c->SG, A));
+        }
         if (ev_end(c, MBIR_K_SWEEP)) return -1;
       }
     }
