@@ -11,7 +11,16 @@ themselves are not pinned by any reference fixture; these functions are pinned
 instead by closed forms in tests/test_pcmbir_oracle.py (unitarity and
 H^H H = I with a unit window, the identity at zero distance, the plane-wave
 eigenvalue, the gradient by finite differences of the cost, the step size as
-the exact line minimiser).
+the exact line minimiser, the cost's hand-derived minimiser at zero distance,
+the NMS cost's special cases and a known minimum, pretcost by brute force at
+zero distance, the transmittance against numpy's complex exponential).
+
+Parity unpinned: the order of the steps inside estimate_complex_projection's
+loop, its dual update and its residual normalisations follow
+XT_CmplxProjEst.c:105-229 as read; each step is pinned above, the loop as a
+whole only by its own definition (no FFTW build and no reference fixture for
+it exist here).  nms_pass(order="colour") is the GPU's four-colour visiting
+order and has no reference counterpart; order="raster" is the reference's.
 
 Arrays are per view [rows][cols] = [N_r][N_t], complex128 (real/imag pairs
 of the reference's *_real / *_imag arrays).

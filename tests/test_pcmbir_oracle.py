@@ -238,3 +238,60 @@ def test_product_geometry_helpers_match_the_oracle():
     assert wavelength(17.0) == P.wavelength(17.0)
     assert np.allclose(fresnel_window(16, 32, 0.7, 1.3, 0.2), P.window(16, 32, 0.7, 1.3, 0.2),
                        rtol=1e-15, atol=0)
+
+
+# ------------------------------------------------- composed quantities (a13)
+def test_transmittance_is_the_complex_exponential():
+    r = np.random.default_rng(5)
+    zr, zi = r.random((4, 5)), r.standard_normal((4, 5)) * 3
+    assert np.allclose(P.transmittance(zr, zi), np.exp(-(zr + 1j * zi)), rtol=1e-14, atol=0)
+
+
+def test_pretcost_at_zero_distance_by_brute_force():
+    """At zero distance H is the identity, so the data term is the per-pixel
+    sum of Lam (|y| - |D| exp(-zr))^2 / 2; the prior over the four forward
+    pairs equals half the sum over every ordered pair of 8-neighbours (the
+    filter is symmetric and the potential even), enumerated independently."""
+    rows, cols = 5, 6
+    r = np.random.default_rng(6)
+    zr, zi = r.random((rows, cols)) * 0.1, r.random((rows, cols))
+    meas = (r.random((rows, cols)) + 0.5).astype(complex)
+    D = np.exp(-0.2 * r.random((rows, cols)) - 1j * r.random((rows, cols)))
+    Lam = 0.5 + r.random((rows, cols))
+    pr = P.NmsPrior(**PRIOR, decorrelate=True)
+    got = P.pret_cost(meas, D, Lam, zr, zi, pr, (1.0, 1.0, P.wavelength(17.0), 0.0, None))
+    data = np.sum(Lam * (meas.real - np.abs(D) * np.exp(-zr)) ** 2) / 2
+    T = pr.T
+    prior = 0.0
+    for j in range(rows):
+        for k in range(cols):
+            for dj in (-1, 0, 1):
+                for dk in (-1, 0, 1):
+                    a, b = j + dj, k + dk
+                    if (dj, dk) == (0, 0) or not (0 <= a < rows and 0 <= b < cols):
+                        continue
+                    db, dd = zr[j, k] - zr[a, b], zi[j, k] - zi[a, b]
+                    prior += pr.filt[1 + dj, 1 + dk] * (
+                        pr.value(T[0, 0] * dd + T[0, 1] * db, 0)
+                        + pr.value(T[1, 0] * dd + T[1, 1] * db, 1)) / 2
+    assert got == pytest.approx(data + prior, rel=1e-12)
+
+
+def test_phase_cost_minimiser_at_zero_distance():
+    """With H the identity and |Omega| = 1 the phase-retrieval cost is
+    separable and minimised by w* = (Lam conj(Omega D) y + nu v) /
+    (Lam |D|^2 + nu), derived by hand: the gradient vanishes there and every
+    perturbation raises the cost."""
+    rows, cols, nu = 4, 8, 0.7
+    r = np.random.default_rng(7)
+    y = rnd((rows, cols), 8)
+    D = np.exp(-0.1 * r.random((rows, cols)) - 0.3j * r.random((rows, cols)))
+    Lam = 0.5 + r.random((rows, cols))
+    v = rnd((rows, cols), 9)
+    om = np.exp(6j * r.random((rows, cols)))
+    pp = P.PhaseProblem(y, D, Lam, v, nu, 1.0, 1.0, P.wavelength(17.0), 0.0, None)
+    ws = (Lam * np.conj(om * D) * y + nu * v) / (Lam * np.abs(D) ** 2 + nu)
+    assert np.max(np.abs(pp.gradient(om, ws))) < 1e-12 * np.max(np.abs(y))
+    c0 = pp.cost(om, ws)
+    for s in range(5):
+        assert pp.cost(om, ws + 1e-3 * rnd((rows, cols), 100 + s)) > c0
