@@ -1,0 +1,63 @@
+/*
+ * pcmbir4d.h -- drop-in replacement for the reference's PCMBIR library
+ * header (Source_Code/include/pcmbir4d.h:35-36).
+ *
+ * phcontomo_reconstruct() keeps the reference's argument list, argument
+ * meaning, return codes (0 / -1) and CWD output files with their byte
+ * layouts (PCMBIR_3D/XT_Engine.c:82-196, PCMBIR_3D/XT_ICD_update.c:956-1212).
+ * As in the shipped reference, whose tomographic reconstruction is compiled
+ * out (ENABLE_TOMO_RECONS is never defined), it retrieves projections and
+ * returns no object: *magobject and *phaseobject are set to NULL.
+ *   recon_type 1: single-material Paganin retrieval of every view
+ *                 (pag_mag_ret.bin, pag_phase_ret.bin);
+ *   recon_type 2: the per-view ADMM phase retrieval
+ *                 (estimate_complex_projection) started from the Paganin
+ *                 estimate read back from those files (mag_tomo_aux.bin,
+ *                 phase_tomo_aux.bin and the auxiliary files).
+ * The caller owns every input.
+ *
+ * Differences from the reference, by design of this build:
+ *   - single process, one GPU: no MPI; proj_rows is not split across ranks;
+ *   - proj_rows and proj_cols must be powers of two up to 1024 (the GPU
+ *     transforms are radix-2; the reference's FFTW takes any size);
+ *   - all views run together and the simplex search visits pixels in four
+ *     colours instead of raster order;
+ *   - recon_type 2 without readable pag_*_ret.bin files returns -1;
+ *   - no TIFF files are written.
+ *
+ * phcontomo_forward_project() is declared with the reference's prototype;
+ * this build returns -1 from it with a message (the simulator is not built).
+ */
+#ifndef PCMBIR4D_H_
+#define PCMBIR4D_H_
+#include <stdint.h>
+#include <stdio.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+int phcontomo_reconstruct(float** magobject, float** phaseobject,
+                          float* measurements, float* brights,
+                          float* proj_angles, float* proj_times,
+                          float* recon_times, int32_t proj_rows,
+                          int32_t proj_cols, int32_t proj_num,
+                          int32_t recon_num, float vox_wid, float rot_center,
+                          float mag_sig_s, float mag_sig_t, float mag_c_s,
+                          float mag_c_t, float phase_sig_s, float phase_sig_t,
+                          float phase_c_s, float phase_c_t, float convg_thresh,
+                          float obj2det_dist, float light_energy,
+                          float pag_regparam, uint8_t restart,
+                          uint8_t recon_type, FILE* debug_msg_ptr);
+
+int phcontomo_forward_project(float** measurements, float** brights,
+                              float* proj_angles, int32_t proj_rows,
+                              int32_t proj_cols, int32_t proj_num,
+                              float vox_wid, float rot_center,
+                              float obj2det_dist, float light_energy,
+                              float pag_regparam, FILE* debug_msg_ptr);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,7 +1,9 @@
 /*
  * pcmbir_hip.h -- C-ABI of the gfx950 kernels of the PCMBIR phase-retrieval
- * path: the Fresnel transform and its Hermitian (rows a11), and the batched
- * steepest-descent phase retrieval of w (a12), over all views at once.
+ * path, over all views at once: the Fresnel transform and its Hermitian, the
+ * steepest-descent phase retrieval of w, the Nelder-Mead pass and the ADMM
+ * projection estimate (estimate_complex_projection), and the single-material
+ * Paganin retrieval.  phcontomo_reconstruct (include/pcmbir4d.h) drives it.
  *
  * Paths of reference files are relative to the reference's
  * Source_Code/PCMBIR_3D/ directory.  Complex arrays are interleaved float
@@ -41,7 +43,10 @@ int pcmbir_hip_fresnel(pcmbir_hip_ctx* ctx, int32_t herm, const float* x,
 
 /* Phase-retrieval problem of every view (XT_PhaseRet.c): measurements y, the
  * factor D, weights lambda [n_views][rows][cols] (real), proximal target v,
- * initial estimate w and phase Omega, penalty nu. */
+ * initial estimate w and phase Omega, penalty nu.  omega = NULL sets Omega to
+ * the phase projection of w (compute_phase_projection, XT_PhaseRet.c:10-33),
+ * as initErrorSinogam does before the phase retrieval
+ * (PCMBIR_3D/XT_ICD_update.c:788-789). */
 int pcmbir_hip_set_phase_problem(pcmbir_hip_ctx* ctx, const float* y,
                                  const float* D, const float* lambda,
                                  const float* v, const float* w,
@@ -91,13 +96,30 @@ int pcmbir_hip_nms_pass(pcmbir_hip_ctx* ctx, double* zr, double* zi,
  * (sd_iters, sd_thresh); v += exp(-z) - w; residuals; pretcost }, a view
  * stopping after an iteration > 1 whose dual residual is below pret_thresh.
  * zr, zi in and out; per view iters, cost_init / cost_final (pretcost,
- * XT_CmplxProjEst.c:15-103) and the residual histories primal / dual
- * [n_views][pret_iters] (any output may be NULL). */
+ * XT_CmplxProjEst.c:15-103) and the histories of the residuals primal / dual
+ * and of the pretcost after each iteration, costs, all [n_views][pret_iters]
+ * with the first iters[v] entries of view v written (any output may be
+ * NULL). */
 int pcmbir_hip_estimate(pcmbir_hip_ctx* ctx, double* zr, double* zi,
                         int32_t pret_iters, double pret_thresh,
                         int32_t sd_iters, double sd_thresh, int32_t* iters,
                         double* cost_init, double* cost_final, double* primal,
-                        double* dual);
+                        double* dual, double* costs);
+
+/* The ADMM dual v of every view after pcmbir_hip_estimate (the reference's
+ * MagPRetDual / PhasePRetDual), complex [n_views][rows][cols]; zeros before
+ * the first estimate. */
+int pcmbir_hip_get_dual(pcmbir_hip_ctx* ctx, float* v);
+
+/* Single-material Paganin phase retrieval of every view
+ * (paganins_1mat_phase_retrieval, PCMBIR_3D/XT_Paganin.c:77-134): intensity
+ * |Re y|^2 / |D|^2, filtered by 1 / (1 + z (delta/beta) / (2 k) 4 pi^2
+ * (u^2 + v^2)) with the context's distance z and spacings, thickness
+ * -log|Re|; zr = thickness / 2, zi = thickness (delta/beta) / 2, FP64
+ * [n_views][rows][cols].  y, D complex; k = the wavenumber 2 pi / lambda. */
+int pcmbir_hip_paganin(pcmbir_hip_ctx* ctx, const float* y, const float* D,
+                       double delta_over_beta, double wavenumber, double* zr,
+                       double* zi);
 
 /* Device time (ms) and launch count of the Fresnel FFT passes since the
  * last call, from HIP events on the context's stream. */

@@ -13,7 +13,9 @@ H^H H = I with a unit window, the identity at zero distance, the plane-wave
 eigenvalue, the gradient by finite differences of the cost, the step size as
 the exact line minimiser, the cost's hand-derived minimiser at zero distance,
 the NMS cost's special cases and a known minimum, pretcost by brute force at
-zero distance, the transmittance against numpy's complex exponential).
+zero distance, the transmittance against numpy's complex exponential, the
+Paganin filter against its published form 1 / (1 + pi lambda z (delta/beta)
+|u|^2) and a uniform slab's thickness).
 
 Parity unpinned: the order of the steps inside estimate_complex_projection's
 loop, its dual update and its residual normalisations follow
@@ -73,6 +75,24 @@ def fresnel(x, d_rows, d_cols, lam, dist, win=None, herm=False):
     rows, cols = x.shape[-2:]
     f = fresnel_factor(rows, cols, d_rows, d_cols, lam, dist, win, herm)
     return np.fft.ifft2(np.fft.fft2(x) * f)
+
+
+def paganin(meas, D, d_rows, d_cols, lam, dist, delta_over_beta):
+    """Single-material Paganin retrieval of one view
+    (paganins_1mat_phase_retrieval, XT_Paganin.c:77-134): intensity
+    meas^2 / |D|^2 (real part of the measurement amplitude), FFT, times
+    1 / ((z (delta/beta) / (2 k) (u^2 + v^2) 4 pi^2 + 1) N), inverse FFT,
+    thickness -log|Re|, and (z_real, z_imag) = (thick / 2,
+    thick (delta/beta) / 2); k = 2 pi / lam."""
+    rows, cols = meas.shape
+    k = 2 * np.pi / lam
+    reg = delta_over_beta / (2 * k)
+    inten = meas.real ** 2 / (D.real ** 2 + D.imag ** 2)
+    u, v = freqs(rows, cols, d_rows, d_cols)
+    tran = dist * reg * (u * u + v * v) * 4 * np.pi * np.pi + 1
+    f = np.fft.ifft2(np.fft.fft2(inten) / tran)
+    thick = -np.log(np.abs(f.real))
+    return thick / 2, thick * delta_over_beta / 2
 
 
 class PhaseProblem:

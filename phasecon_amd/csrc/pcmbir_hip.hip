@@ -567,6 +567,31 @@ __global__ __launch_bounds__(kThreads) void k_dual(const double* __restrict__ zr
   }
 }
 
+// Paganin input: the intensity ratio |y|^2 / |D|^2 as a real-valued complex
+// array (paganins_1mat_phase_retrieval, XT_Paganin.c:84-93; the reference
+// squares the real part of the measurement amplitude).
+__global__ void k_pag_pre(const float2* __restrict__ y, const float2* __restrict__ D,
+                          long long nall, float2* __restrict__ out) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nall;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float2 d = D[i];
+    const double i0 = (double)d.x * d.x + (double)d.y * d.y;
+    out[i] = make_float2((float)((double)y[i].x * y[i].x / i0), 0.f);
+  }
+}
+
+// Paganin output: thickness -log|Re| of the filtered intensity, z_real =
+// thick / 2, z_imag = thick delta/beta / 2 (XT_Paganin.c:117-131).
+__global__ void k_pag_post(const float2* __restrict__ f, long long nall, double dob,
+                           double* __restrict__ zr, double* __restrict__ zi) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nall;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double thick = -log(fabs((double)f[i].x));
+    zr[i] = thick / 2;
+    zi[i] = thick * dob / 2;
+  }
+}
+
 __global__ void k_fill_int(int* p, int n, int val) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = val;
@@ -593,6 +618,9 @@ struct pcmbir_hip_ctx {
   long long npix = 0, nall = 0;
   hipStream_t stream = nullptr;
   float2 *tw_r = nullptr, *tw_c = nullptr, *fac_h = nullptr, *fac_hh = nullptr;
+  float2* fac_pag = nullptr;  // Paganin filter (built on first use)
+  double pag_dob = -1.0, pag_k = 0.0;
+  double delta_rows = 0.0, delta_cols = 0.0, wavelength = 0.0, dist = 0.0;
   float2 *y = nullptr, *D = nullptr, *v = nullptr, *w = nullptr, *om = nullptr, *g = nullptr;
   float2* b = nullptr;  // b1 | b2, contiguous: one batched transform of 2 n_views
   float* lam = nullptr;
@@ -623,8 +651,9 @@ std::vector<float2> twiddles(int n) {
   return t;
 }
 
-// Fresnel transform (herm = 0: H, 1: H^H) of nbatch views stored at buf.
-int fresnel_dev(pcmbir_hip_ctx* c, float2* buf, int nbatch, int herm) {
+// FFT, multiplication by the spectral factor fac [rows][cols], inverse FFT
+// (unnormalised; fac carries 1/N) of nbatch views stored at buf.
+int filter_dev(pcmbir_hip_ctx* c, float2* buf, int nbatch, const float2* fac) {
   if (c->ev_used == c->ev.size()) {
     hipEvent_t a, b;
     HIP_OK(hipEventCreate(&a));
@@ -638,7 +667,7 @@ int fresnel_dev(pcmbir_hip_ctx* c, float2* buf, int nbatch, int herm) {
                      c->tw_c, 0);
   const int tiles = c->cols / col_tile(c->logr, c->logc);
   hipLaunchKernelGGL(k_fresnel_cols, dim3((unsigned)(nbatch * tiles)), dim3(kThreads), 0,
-                     c->stream, buf, c->logr, c->logc, c->tw_r, herm ? c->fac_hh : c->fac_h);
+                     c->stream, buf, c->logr, c->logc, c->tw_r, fac);
   hipLaunchKernelGGL(k_fft_rows, dim3(rb), dim3(kThreads), 0, c->stream, buf, nrows, c->logc,
                      c->tw_c, 1);
   HIP_OK(hipGetLastError());
@@ -646,6 +675,11 @@ int fresnel_dev(pcmbir_hip_ctx* c, float2* buf, int nbatch, int herm) {
   c->ev_used++;
   c->fft_launches += 3;
   return 0;
+}
+
+// Fresnel transform (herm = 0: H, 1: H^H) of nbatch views stored at buf.
+int fresnel_dev(pcmbir_hip_ctx* c, float2* buf, int nbatch, int herm) {
+  return filter_dev(c, buf, nbatch, herm ? c->fac_hh : c->fac_h);
 }
 
 int collect_events(pcmbir_hip_ctx* c) {
@@ -681,6 +715,19 @@ PrArgs pr_args(pcmbir_hip_ctx* c) {
   P.npix = c->npix;
   P.nall = c->nall;
   return P;
+}
+
+// Omega = phase projection of the current w (compute_phase_projection,
+// XT_PhaseRet.c:10-33), every view.
+int omega_from_w(pcmbir_hip_ctx* c) {
+  const unsigned eb = ew_blocks(c->nall), vb = (unsigned)((c->nv + 255) / 256);
+  PrArgs P = pr_args(c);
+  hipLaunchKernelGGL(k_fill_int, dim3(vb), dim3(256), 0, c->stream, c->active, c->nv, 1);
+  hipLaunchKernelGGL(k_pr_dmul, dim3(eb), dim3(kThreads), 0, c->stream, P, c->w, P.b1);
+  if (fresnel_dev(c, P.b1, c->nv, 0)) return -1;
+  hipLaunchKernelGGL(k_pr_omega, dim3(eb), dim3(kThreads), 0, c->stream, P);
+  HIP_OK(hipGetLastError());
+  return 0;
 }
 
 // The steepest-descent loop over every view whose init_active flag is set
@@ -787,6 +834,10 @@ int pcmbir_hip_create(pcmbir_hip_ctx** out, int32_t n_views, int32_t rows, int32
   c->logc = lc;
   c->npix = (long long)rows * cols;
   c->nall = c->npix * n_views;
+  c->delta_rows = delta_rows;
+  c->delta_cols = delta_cols;
+  c->wavelength = lam;
+  c->dist = dist;
   // spectral factors in FP64 (XT_FresnelTran.c:33-50 and :104-118), with 1/N
   const double N = (double)rows * cols;
   std::vector<float2> fh(c->npix), fhh(c->npix);
@@ -825,7 +876,7 @@ int pcmbir_hip_create(pcmbir_hip_ctx** out, int32_t n_views, int32_t rows, int32
 void pcmbir_hip_destroy(pcmbir_hip_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  void* bufs[] = {c->tw_r, c->tw_c, c->fac_h, c->fac_hh, c->y, c->D, c->v, c->w, c->om,
+  void* bufs[] = {c->tw_r, c->tw_c, c->fac_h, c->fac_hh, c->fac_pag, c->y, c->D, c->v, c->w, c->om,
                   c->g, c->b, c->lam, c->active, c->iters, c->scal, c->cost,
                   c->zr, c->zi, c->res, c->vd, c->wold, c->pix_iters, c->admm};
   for (void* b : bufs)
@@ -850,7 +901,7 @@ int pcmbir_hip_fresnel(pcmbir_hip_ctx* c, int32_t herm, const float* x, float* y
 int pcmbir_hip_set_phase_problem(pcmbir_hip_ctx* c, const float* y, const float* D,
                                  const float* lambda, const float* v, const float* w,
                                  const float* omega, double nu) {
-  if (!c || !y || !D || !lambda || !v || !w || !omega) return fail("null argument");
+  if (!c || !y || !D || !lambda || !v || !w) return fail("null argument");
   if (!c->y) {
     if (dev_alloc(&c->y, c->nall) || dev_alloc(&c->D, c->nall) || dev_alloc(&c->v, c->nall) ||
         dev_alloc(&c->w, c->nall) || dev_alloc(&c->om, c->nall) || dev_alloc(&c->g, c->nall) ||
@@ -864,13 +915,16 @@ int pcmbir_hip_set_phase_problem(pcmbir_hip_ctx* c, const float* y, const float*
   HIP_OK(hipMemcpyAsync(c->D, D, cb, hipMemcpyHostToDevice, c->stream));
   HIP_OK(hipMemcpyAsync(c->v, v, cb, hipMemcpyHostToDevice, c->stream));
   HIP_OK(hipMemcpyAsync(c->w, w, cb, hipMemcpyHostToDevice, c->stream));
-  HIP_OK(hipMemcpyAsync(c->om, omega, cb, hipMemcpyHostToDevice, c->stream));
   HIP_OK(hipMemcpyAsync(c->lam, lambda, (size_t)c->nall * sizeof(float), hipMemcpyHostToDevice,
                         c->stream));
-  HIP_OK(hipStreamSynchronize(c->stream));
+  if (omega) {
+    HIP_OK(hipMemcpyAsync(c->om, omega, cb, hipMemcpyHostToDevice, c->stream));
+  } else if (omega_from_w(c)) {
+    return -1;
+  }
   c->nu = (float)nu;
   c->problem = true;
-  return 0;
+  return collect_events(c);
 }
 
 int pcmbir_hip_sd_run(pcmbir_hip_ctx* c, int32_t max_iters, double thresh, int32_t* iters,
@@ -985,7 +1039,8 @@ int pcmbir_hip_nms_pass(pcmbir_hip_ctx* c, double* zr, double* zi, const float* 
 
 int pcmbir_hip_estimate(pcmbir_hip_ctx* c, double* zr, double* zi, int32_t pret_iters,
                         double pret_thresh, int32_t sd_iters, double sd_thresh, int32_t* iters,
-                        double* cost_init, double* cost_final, double* primal, double* dual) {
+                        double* cost_init, double* cost_final, double* primal, double* dual,
+                        double* costs) {
   if (!c || !zr || !zi) return fail("null argument");
   if (!c->problem) return fail("no phase problem set");
   if (!c->nms_set) return fail("NMS prior not set");
@@ -1024,6 +1079,7 @@ int pcmbir_hip_estimate(pcmbir_hip_ctx* c, double* zr, double* zi, int32_t pret_
       if (!act[v]) continue;
       if (primal) primal[(size_t)v * pret_iters + it] = res[3 * (size_t)v];
       if (dual) dual[(size_t)v * pret_iters + it] = res[3 * (size_t)v + 1];
+      if (costs) costs[(size_t)v * pret_iters + it] = pc[v];
       if (cost_final) cost_final[v] = pc[v];
       if (res[3 * (size_t)v + 1] < pret_thresh && it > 1) {
         act[v] = 0;
@@ -1041,6 +1097,58 @@ int pcmbir_hip_estimate(pcmbir_hip_ctx* c, double* zr, double* zi, int32_t pret_
   HIP_OK(hipMemcpyAsync(zi, c->zi, db, hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
   return 0;
+}
+
+int pcmbir_hip_get_dual(pcmbir_hip_ctx* c, float* v) {
+  if (!c || !v) return fail("null argument");
+  const size_t cb = (size_t)c->nall * sizeof(float2);
+  if (!c->vd) {
+    memset(v, 0, cb);
+    return 0;
+  }
+  HIP_OK(hipMemcpyAsync(v, c->vd, cb, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int pcmbir_hip_paganin(pcmbir_hip_ctx* c, const float* y, const float* D,
+                       double delta_over_beta, double wavenumber, double* zr, double* zi) {
+  if (!c || !y || !D || !zr || !zi) return fail("null argument");
+  if (!(wavenumber > 0)) return fail("wavenumber must be positive");
+  if (!c->fac_pag || c->pag_dob != delta_over_beta || c->pag_k != wavenumber) {
+    // 1 / ((z (delta/beta) / (2 k) 4 pi^2 (u^2 + v^2) + 1) N), FFT-shifted
+    // frequencies (XT_Paganin.c:97-113), in FP64
+    const double N = (double)c->rows * c->cols, reg = delta_over_beta / (2.0 * wavenumber);
+    std::vector<float2> f(c->npix);
+    for (int i = 0; i < c->rows; ++i)
+      for (int j = 0; j < c->cols; ++j) {
+        const double u = (i >= c->rows / 2 ? -(double)(c->rows - i) : (double)i) /
+                         (c->rows * c->delta_rows);
+        const double v = (j >= c->cols / 2 ? -(double)(c->cols - j) : (double)j) /
+                         (c->cols * c->delta_cols);
+        const double tran = c->dist * reg * (u * u + v * v) * 4 * M_PI * M_PI + 1;
+        f[(size_t)i * c->cols + j] = make_float2((float)(1.0 / (tran * N)), 0.f);
+      }
+    if (!c->fac_pag && dev_alloc(&c->fac_pag, c->npix)) return -1;
+    HIP_OK(hipMemcpy(c->fac_pag, f.data(), f.size() * sizeof(float2), hipMemcpyHostToDevice));
+    c->pag_dob = delta_over_beta;
+    c->pag_k = wavenumber;
+  }
+  if (alloc_nms_state(c)) return -1;
+  const size_t cb = (size_t)c->nall * sizeof(float2), db = (size_t)c->nall * sizeof(double);
+  float2* yb = c->b + c->nall;  // second half of the transform buffer
+  HIP_OK(hipMemcpyAsync(yb, y, cb, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipMemcpyAsync(c->wold, D, cb, hipMemcpyHostToDevice, c->stream));
+  const unsigned eb = ew_blocks(c->nall);
+  hipLaunchKernelGGL(k_pag_pre, dim3(eb), dim3(kThreads), 0, c->stream, yb, c->wold, c->nall,
+                     c->b);
+  if (filter_dev(c, c->b, c->nv, c->fac_pag)) return -1;
+  hipLaunchKernelGGL(k_pag_post, dim3(eb), dim3(kThreads), 0, c->stream, c->b, c->nall,
+                     delta_over_beta, c->zr, c->zi);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(zr, c->zr, db, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipMemcpyAsync(zi, c->zi, db, hipMemcpyDeviceToHost, c->stream));
+  return collect_events(c);
 }
 
 int pcmbir_hip_fft_time(pcmbir_hip_ctx* c, double* ms, int64_t* launches) {

@@ -26,7 +26,10 @@ _SIGS = {
                                  [C.c_double] * 2),
     "pcmbir_hip_nms_pass": (C.c_int, [C.c_void_p] * 5),
     "pcmbir_hip_estimate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double,
-                                      C.c_int32, C.c_double] + [C.c_void_p] * 5),
+                                      C.c_int32, C.c_double] + [C.c_void_p] * 6),
+    "pcmbir_hip_get_dual": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pcmbir_hip_paganin": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                                     C.c_void_p, C.c_void_p]),
 }
 
 _LIB = None
@@ -122,12 +125,15 @@ class PhaseRetrieval:
         return _cplx(y)
 
     def set_problem(self, y, D, lam, v, w, omega, nu):
-        self._keep = [_c64(a, self.shape) for a in (y, D, v, w, omega)]
+        """omega=None: Omega becomes the phase projection of w."""
+        self._keep = [_c64(a, self.shape) for a in (y, D, v, w)]
+        oo = None if omega is None else _c64(omega, self.shape)
         lm = np.ascontiguousarray(np.broadcast_to(lam, self.shape), np.float32)
-        self._keep.append(lm)
-        yy, dd, vv, ww, oo = self._keep[:5]
+        self._keep += [lm, oo]
+        yy, dd, vv, ww = self._keep[:4]
         _check(self.L.pcmbir_hip_set_phase_problem(self.h, _p(yy), _p(dd), _p(lm), _p(vv),
-                                                   _p(ww), _p(oo), float(nu)),
+                                                   _p(ww), None if oo is None else _p(oo),
+                                                   float(nu)),
                "pcmbir_hip_set_phase_problem")
 
     def sd_run(self, max_iters, thresh):
@@ -178,8 +184,78 @@ class PhaseRetrieval:
         c1 = np.zeros(n)
         pr = np.zeros((n, max(pret_iters, 1)))
         du = np.zeros((n, max(pret_iters, 1)))
+        cs = np.zeros((n, max(pret_iters, 1)))
         _check(self.L.pcmbir_hip_estimate(self.h, _p(zr), _p(zi), int(pret_iters),
                                           float(pret_thresh), int(sd_iters), float(sd_thresh),
-                                          _p(it), _p(c0), _p(c1), _p(pr), _p(du)),
+                                          _p(it), _p(c0), _p(c1), _p(pr), _p(du), _p(cs)),
                "pcmbir_hip_estimate")
-        return dict(zr=zr, zi=zi, iters=it, cost_init=c0, cost_final=c1, primal=pr, dual=du)
+        return dict(zr=zr, zi=zi, iters=it, cost_init=c0, cost_final=c1, primal=pr, dual=du,
+                    costs=cs)
+
+    def dual(self):
+        v = np.empty(self.shape + (2,), np.float32)
+        _check(self.L.pcmbir_hip_get_dual(self.h, _p(v)), "pcmbir_hip_get_dual")
+        return _cplx(v)
+
+    def paganin(self, y, D, delta_over_beta, wavenumber):
+        """Single-material Paganin retrieval: returns (z_real, z_imag), FP64."""
+        yy, dd = _c64(y, self.shape), _c64(D, self.shape)
+        zr = np.empty(self.shape, np.float64)
+        zi = np.empty(self.shape, np.float64)
+        _check(self.L.pcmbir_hip_paganin(self.h, _p(yy), _p(dd), float(delta_over_beta),
+                                         float(wavenumber), _p(zr), _p(zi)),
+               "pcmbir_hip_paganin")
+        return zr, zi
+
+
+_DROPIN = {
+    "phcontomo_reconstruct": (C.c_int, [C.c_void_p, C.c_void_p] + [C.c_void_p] * 5 +
+                              [C.c_int32] * 4 + [C.c_float] * 14 + [C.c_uint8, C.c_uint8,
+                                                                    C.c_void_p]),
+    "phcontomo_forward_project": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p] +
+                                  [C.c_int32] * 3 + [C.c_float] * 5 + [C.c_void_p]),
+}
+
+
+def dropin_names():
+    return list(_DROPIN)
+
+
+def phcontomo_reconstruct(measurements, brights, angles, proj_times, recon_times, vox_wid,
+                          rot_center, mag_sig_s, mag_sig_t, mag_c_s, mag_c_t, phase_sig_s,
+                          phase_sig_t, phase_c_s, phase_c_t, convg_thresh, obj2det_dist,
+                          light_energy, pag_regparam, restart=0, recon_type=1, log_path=None):
+    """Calls the drop-in phcontomo_reconstruct (include/pcmbir4d.h) with the
+    reference's arguments; measurements [proj_num][proj_rows][proj_cols],
+    brights [proj_rows][proj_cols].  Results are the CWD files; returns the
+    return code."""
+    L = load()
+    for name, (res, args) in _DROPIN.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    meas = np.ascontiguousarray(measurements, np.float32)
+    br = np.ascontiguousarray(brights, np.float32)
+    keep = [meas, br] + [np.ascontiguousarray(a, np.float32)
+                         for a in (angles, proj_times, recon_times)]
+    proj_num, proj_rows, proj_cols = meas.shape
+    mag = C.c_void_p()
+    ph = C.c_void_p()
+    libc = C.CDLL(None)
+    libc.fopen.restype = C.c_void_p
+    libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+    libc.fclose.argtypes = [C.c_void_p]
+    fp = libc.fopen((log_path or "/dev/null").encode(), b"w")
+    f = C.c_float
+    try:
+        rc = L.phcontomo_reconstruct(
+            C.byref(mag), C.byref(ph), *[_p(a) for a in keep], proj_rows, proj_cols, proj_num,
+            len(keep[4]) - 1, f(vox_wid), f(rot_center), f(mag_sig_s), f(mag_sig_t), f(mag_c_s),
+            f(mag_c_t), f(phase_sig_s), f(phase_sig_t), f(phase_c_s), f(phase_c_t),
+            f(convg_thresh), f(obj2det_dist), f(light_energy), f(pag_regparam),
+            C.c_uint8(restart), C.c_uint8(recon_type), C.c_void_p(fp))
+    finally:
+        libc.fclose(C.c_void_p(fp))
+    if rc == 0 and (mag.value or ph.value):
+        raise RuntimeError("phcontomo_reconstruct returned object pointers; expected NULL")
+    return rc

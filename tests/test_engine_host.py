@@ -51,11 +51,53 @@ PCLIB = os.path.join(ROOT, "phasecon_amd", "libpcmbir4d.so")
 def test_pcmbir_library_exports_every_declared_function():
     from phasecon_amd import pcmbir
     L = C.CDLL(PCLIB)
-    declared = _header_functions(("pcmbir_hip.h",))
-    assert "pcmbir_hip_sd_run" in declared
+    declared = _header_functions(("pcmbir_hip.h", "pcmbir4d.h"))
+    assert "pcmbir_hip_sd_run" in declared and "phcontomo_reconstruct" in declared
     for name in declared:
         assert hasattr(L, name), name
-    assert set(pcmbir.exported_names()) == set(declared)
+    assert set(pcmbir.exported_names()) | set(pcmbir.dropin_names()) == set(declared)
+
+
+PC_ARGS = dict(vox_wid=1.0, rot_center=8.0, mag_sig_s=2e-2, mag_sig_t=1.0, mag_c_s=1e-2,
+               mag_c_t=1e-2, phase_sig_s=2e-2, phase_sig_t=1.0, phase_c_s=1e-2, phase_c_t=1e-2,
+               convg_thresh=1.0, obj2det_dist=1e5, light_energy=17.0, pag_regparam=50.0)
+
+
+@pytest.mark.skipif(not os.path.exists(PCLIB), reason="libpcmbir4d.so not built")
+@pytest.mark.parametrize("rows,cols,why", [(2, 16, "minimum number of projection rows"),
+                                           (4, 15, "columns should be even"),
+                                           (6, 16, "powers of two")])
+def test_pcmbir_dropin_rejects_bad_sizes_before_any_gpu_call(rows, cols, why, tmp_path,
+                                                             monkeypatch):
+    """The reference's size checks (PCMBIR_3D/XT_Engine.c:100-103) and this
+    build's power-of-two rule return -1 with the message in the log, on the
+    host, before a device is touched."""
+    from phasecon_amd import pcmbir
+    monkeypatch.chdir(tmp_path)
+    log = tmp_path / "debug.log"
+    rc = pcmbir.phcontomo_reconstruct(np.ones((3, rows, cols)), np.ones((rows, cols)),
+                                      np.zeros(3), np.arange(3.0), [0.0, 3.0], **PC_ARGS,
+                                      recon_type=1, log_path=str(log))
+    assert rc == -1
+    assert why in log.read_text()
+    assert not list(tmp_path.glob("*.bin"))
+
+
+@pytest.mark.skipif(not os.path.exists(PCLIB), reason="libpcmbir4d.so not built")
+def test_pcmbir_forward_project_reports_that_it_is_absent(tmp_path):
+    L = C.CDLL(PCLIB)
+    libc = C.CDLL(None)
+    libc.fopen.restype = C.c_void_p
+    libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+    libc.fclose.argtypes = [C.c_void_p]
+    fp = libc.fopen(str(tmp_path / "log").encode(), b"w")
+    m, b = C.c_void_p(), C.c_void_p()
+    L.phcontomo_forward_project.argtypes = [C.c_void_p] * 3 + [C.c_int32] * 3 + [C.c_float] * 5 + [C.c_void_p]
+    rc = L.phcontomo_forward_project(C.byref(m), C.byref(b), None, 4, 16, 3, 1.0, 8.0, 1e5,
+                                     17.0, 50.0, C.c_void_p(fp))
+    libc.fclose(C.c_void_p(fp))
+    assert rc == -1
+    assert "not available" in (tmp_path / "log").read_text()
 
 
 @needs_lib

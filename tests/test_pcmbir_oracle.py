@@ -295,3 +295,40 @@ def test_phase_cost_minimiser_at_zero_distance():
     c0 = pp.cost(om, ws)
     for s in range(5):
         assert pp.cost(om, ws + 1e-3 * rnd((rows, cols), 100 + s)) > c0
+
+
+# ----------------------------------------------------- Paganin (next row)
+def test_paganin_uniform_slab_gives_its_projection():
+    """A uniform slab of thickness T transmits exp(-2 k beta T) at every
+    pixel; the filter passes the mean unchanged, so the retrieved projection
+    is (k beta T, k delta T) -- the slab's own, by Beer-Lambert."""
+    lam, dob = P.wavelength(17.0), 50.0
+    k = 2 * np.pi / lam
+    beta, T = 1e-9, 2000.0
+    inten = np.exp(-2 * k * beta * T)
+    D = np.full((8, 16), 0.9 + 0.1j)
+    meas = np.sqrt(inten) * np.abs(D) * np.ones((8, 16))
+    zr, zi = P.paganin(meas.astype(complex), D, 1.0, 1.0, lam, 1e5, dob)
+    assert np.allclose(zr, k * beta * T, rtol=1e-12)
+    assert np.allclose(zi, k * dob * beta * T, rtol=1e-12)
+
+
+@pytest.mark.parametrize("k0,l0", [(1, 0), (2, 3), (0, 5)])
+def test_paganin_filter_is_the_published_form(k0, l0):
+    """For an intensity 1 + eps cos(2 pi (u0 r + v0 t)) the filtered
+    intensity is 1 + eps f cos(...) with Paganin's filter
+    f = 1 / (1 + pi lambda z (delta/beta) |u0|^2) (Paganin et al., J.
+    Microsc. 206 (2002) 33-40), so the thickness is -log of that exactly."""
+    rows, cols, dr, dc = 16, 32, 0.7, 1.3
+    lam, dist, dob, eps = P.wavelength(17.0), 1e5, 30.0, 0.2
+    r = np.arange(rows)[:, None] * dr
+    t = np.arange(cols)[None, :] * dc
+    u0, v0 = k0 / (rows * dr), l0 / (cols * dc)
+    ph = 2 * np.pi * (u0 * r + v0 * t)
+    inten = 1 + eps * np.cos(ph)
+    D = np.ones((rows, cols), complex)
+    zr, zi = P.paganin(np.sqrt(inten).astype(complex), D, dr, dc, lam, dist, dob)
+    f = 1 / (1 + np.pi * lam * dist * dob * (u0 ** 2 + v0 ** 2))
+    thick = -np.log(1 + eps * f * np.cos(ph))
+    assert np.allclose(zr, thick / 2, rtol=0, atol=1e-13)
+    assert np.allclose(zi, thick * dob / 2, rtol=0, atol=1e-11)
