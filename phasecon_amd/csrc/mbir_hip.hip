@@ -389,8 +389,16 @@ __device__ __forceinline__ float substitute(float alpha, float V, float th1, flo
   return V + alpha * (u - V);
 }
 
+// theta's reads of the shared error sinogram: workgroup-scope loads, which
+// skip the CU's L1 and are served by the XCD's L2.  An L2 copy of a line may
+// predate atomics performed since (by other XCDs' tiles, or at the memory
+// side), so theta can miss recent updates of concurrent tiles -- the
+// asynchronous staleness the parallel ICD already has, bounded by one launch
+// (a launch starts from coherent caches).  The sinogram itself stays exact:
+// every update is an agent-scope atomic.  Agent-scope loads (which refetch
+// past the L2) cost 10 % more time per launch for the same convergence.
 __device__ __forceinline__ float e_load(const float* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 __device__ __forceinline__ void e_add(float* p, float v) {
