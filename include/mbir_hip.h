@@ -77,6 +77,17 @@ int mbir_hip_set_option(mbir_hip_ctx* ctx, const char* key, double value);
 int mbir_hip_set_object(mbir_hip_ctx* ctx, const float* x);
 int mbir_hip_get_object(mbir_hip_ctx* ctx, float* x);
 
+/* The object of the coarse stage up-sampled on the device into the fine
+ * stage (the next multi-resolution stage: twice N_x and N_y, the same or
+ * twice N_z, the same frames), by the reference's rule for the next stage's
+ * initial object (XT_ICD_update.c:358-390 in x-y, :448-496 in x-y-z: ends
+ * copied, interior samples 3/4 and 1/4 of the two nearest).  masked = 1 also
+ * zeroes the voxels outside the fine stage's update mask, which the
+ * reference leaves at their interpolated values and never updates; with it
+ * the fine stage's ICD solves the same problem as a start from zero. */
+int mbir_hip_upsample_object(mbir_hip_ctx* fine, mbir_hip_ctx* coarse,
+                             int32_t masked);
+
 /* Builds the stored system matrix for every (column, view) pair;
  * replaces the per-visit calcAMatrixColumnforAngle (XT_AMatrix.c:50-110).
  * get_amatrix returns it in the window layout above (values dequantised

@@ -1346,6 +1346,70 @@ __global__ __launch_bounds__(256) void k_backproj(DevGeom G,
 }
 
 // [t][z][y][x] <-> [t][y][x][z]
+// Fine-grid sample i of a 2x up-sampling along an axis of n coarse samples
+// (XT_ICD_update.c:358-390, 448-496): the ends copy, interior samples weigh
+// the two nearest coarse samples 3/4 and 1/4.
+__device__ __forceinline__ void up_axis(int i, int n, int& a0, double& w0, int& a1, double& w1) {
+  const int k = i >> 1;
+  if (i == 0 || i == 2 * n - 1) {
+    a0 = a1 = i == 0 ? 0 : n - 1;
+    w0 = 1.0;
+    w1 = 0.0;
+  } else if (i & 1) {
+    a0 = k;
+    w0 = 0.75;
+    a1 = k + 1;
+    w1 = 0.25;
+  } else {
+    a0 = k - 1;
+    w0 = 0.25;
+    a1 = k;
+    w1 = 0.75;
+  }
+}
+
+// One stage's object up-sampled into the next stage's (column-major
+// [N_time][N_y * N_x][N_z] both): 2x in x and y, and in z when zf = 2; FP64
+// weights, one rounding.  mask (may be null): zero outside the fine stage's
+// update mask.
+__global__ __launch_bounds__(256) void k_upsample(int Nt, int Nyc, int Nxc, int Nzc, int zf,
+                                                  const float* __restrict__ xc,
+                                                  const uint8_t* __restrict__ mask,
+                                                  float* __restrict__ xf) {
+  const int Nyf = 2 * Nyc, Nxf = 2 * Nxc, Nzf = zf * Nzc;
+  const long long ncf = (long long)Nyf * Nxf, ncc = (long long)Nyc * Nxc;
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (long long)Nt * ncf * Nzf) return;
+  const int z = (int)(i % Nzf);
+  const long long c = (i / Nzf) % ncf;
+  const int f = (int)(i / (ncf * Nzf));
+  if (mask && !mask[c]) {
+    xf[i] = 0.f;
+    return;
+  }
+  const int y = (int)(c / Nxf), x = (int)(c % Nxf);
+  int ya[2], xa[2], za[2];
+  double yw[2], xw[2], zw[2];
+  up_axis(y, Nyc, ya[0], yw[0], ya[1], yw[1]);
+  up_axis(x, Nxc, xa[0], xw[0], xa[1], xw[1]);
+  if (zf == 2) {
+    up_axis(z, Nzc, za[0], zw[0], za[1], zw[1]);
+  } else {
+    za[0] = za[1] = z;
+    zw[0] = 1.0;
+    zw[1] = 0.0;
+  }
+  const float* src = xc + (size_t)f * ncc * Nzc;
+  double v = 0.0;
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int d = 0; d < 2; ++d) {
+        const double w = yw[a] * xw[b] * zw[d];
+        if (w != 0.0) v += w * src[((size_t)ya[a] * Nxc + xa[b]) * Nzc + za[d]];
+      }
+  xf[i] = (float)v;
+}
+
 __global__ __launch_bounds__(256) void k_to_colmajor(int Nt, int Nz, int ncol,
                                                      const float* __restrict__ in,
                                                      float* __restrict__ out) {
@@ -1475,6 +1539,7 @@ struct mbir_hip_ctx {
   int *d_frame_ptr = nullptr, *d_frame_views = nullptr, *d_view_frame = nullptr;
   float *d_y = nullptr, *d_w = nullptr, *d_d = nullptr, *d_e = nullptr;
   float *d_x = nullptr, *d_stage = nullptr, *d_stage2 = nullptr;
+  uint8_t* d_mask = nullptr;  // update mask [N_y * N_x]
   int* d_afirst = nullptr;
   float* d_aval = nullptr;
   bool a_built = false;
@@ -1689,7 +1754,7 @@ int mbir_hip_create(mbir_hip_ctx** out, const mbir_hip_geom* g, const float* H,
        dev_alloc(&c->d_e, c->nsino) || dev_alloc(&c->d_x, c->nvox) ||
        dev_alloc(&c->d_stage, std::max(c->nvox, c->nsino)) ||
        dev_alloc(&c->d_stage2, std::max(c->nvox, c->nsino)) ||
-       dev_alloc(&c->d_stats, 4);
+       dev_alloc(&c->d_stats, 4) || dev_alloc(&c->d_mask, (size_t)c->ncol);
   if (!rc && d) rc = dev_alloc(&c->d_d, (size_t)g->N_r * g->N_t);
   if (rc) {
     std::string keep = g_err;
@@ -1709,6 +1774,7 @@ int mbir_hip_create(mbir_hip_ctx** out, const mbir_hip_geom* g, const float* H,
   UP(c->d_view_frame, view_frame.data(), g->N_p);
   UP(c->d_y, y, c->nsino);
   UP(c->d_w, w, c->nsino);
+  UP(c->d_mask, mask, (size_t)c->ncol);
   if (d) UP(c->d_d, d, (size_t)g->N_r * g->N_t);
 #undef UP
   if (hipMemset(c->d_x, 0, c->nvox * sizeof(float)) != hipSuccess ||
@@ -1726,7 +1792,7 @@ void mbir_hip_destroy(mbir_hip_ctx* c) {
   void* bufs[] = {c->d_H, c->d_cos, c->d_sin, c->d_frame_ptr, c->d_frame_views,
                   c->d_view_frame, c->d_y, c->d_w, c->d_d, c->d_e, c->d_x,
                   c->d_stage, c->d_stage2, c->d_afirst, c->d_aval, c->d_tile_ptr,
-                  c->d_tile_cols, c->d_launch_tiles, c->d_stats, c->d_partial};
+                  c->d_tile_cols, c->d_launch_tiles, c->d_stats, c->d_partial, c->d_mask};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (int k = 0; k < MBIR_K_COUNT; ++k)
@@ -1761,6 +1827,23 @@ int mbir_hip_set_object(mbir_hip_ctx* c, const float* x) {
                      c->g.N_z, c->ncol, c->d_stage, c->d_x);
   HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mbir_hip_upsample_object(mbir_hip_ctx* fine, mbir_hip_ctx* coarse, int32_t masked) {
+  if (!fine || !coarse) return fail("null context");
+  const mbir_hip_geom& F = fine->g;
+  const mbir_hip_geom& Cg = coarse->g;
+  const int zf = F.N_z == Cg.N_z ? 1 : (F.N_z == 2 * Cg.N_z ? 2 : 0);
+  if (F.N_time != Cg.N_time || F.N_y != 2 * Cg.N_y || F.N_x != 2 * Cg.N_x || zf == 0)
+    return fail("the fine stage must double x and y (and keep or double z) of the coarse stage");
+  HIP_OK(hipStreamSynchronize(coarse->stream));
+  const long long n = (long long)fine->nvox;
+  hipLaunchKernelGGL(k_upsample, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, fine->stream,
+                     F.N_time, Cg.N_y, Cg.N_x, Cg.N_z, zf, coarse->d_x,
+                     masked ? fine->d_mask : nullptr, fine->d_x);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(fine->stream));
   return 0;
 }
 

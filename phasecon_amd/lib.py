@@ -55,6 +55,7 @@ _SIGS = {
     "mbir_hip_set_option": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),
     "mbir_hip_set_object": (C.c_int, [C.c_void_p, C.c_void_p]),
     "mbir_hip_get_object": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "mbir_hip_upsample_object": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
     "mbir_hip_build_amatrix": (C.c_int, [C.c_void_p]),
     "mbir_hip_get_amatrix": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "mbir_hip_init_error": (C.c_int, [C.c_void_p]),

@@ -122,6 +122,13 @@ class Stage:
         _lib.check(self.L.mbir_hip_get_object(self.h, _p(x)), "get_object")
         return x
 
+    def upsample_from(self, coarse, masked=True):
+        """This stage's object := the coarse stage's, up-sampled on the
+        device (mbir_hip_upsample_object); masked zeroes voxels outside this
+        stage's update mask."""
+        _lib.check(self.L.mbir_hip_upsample_object(self.h, coarse.h, int(bool(masked))),
+                   "mbir_hip_upsample_object")
+
     def build_amatrix(self):
         _lib.check(self.L.mbir_hip_build_amatrix(self.h), "build_amatrix")
 

@@ -215,3 +215,58 @@ def test_reconstruct_dropin_files(tiny_scan, tmp_path):
     assert np.fromfile(tmp_path / "status.bin", np.int32).tolist() == [0]
     assert np.fromfile(tmp_path / "variance_estimate.bin", np.float64).tolist() == [1.0]
     assert equits >= 2
+
+
+def _up_axis(a, axis):
+    """2x along axis by the reference's rule for the next stage's initial
+    object (XT_ICD_update.c:358-390): ends copied, interior samples 3/4 and
+    1/4 of the two nearest coarse samples."""
+    a = np.moveaxis(a.astype(np.float64), axis, -1)
+    n = a.shape[-1]
+    out = np.empty(a.shape[:-1] + (2 * n,))
+    out[..., 0], out[..., -1] = a[..., 0], a[..., -1]
+    out[..., 1] = (3 * a[..., 0] + a[..., 1]) / 4
+    out[..., -2] = (a[..., -2] + 3 * a[..., -1]) / 4
+    out[..., 2:-2:2] = (a[..., :-2] + 3 * a[..., 1:-1]) / 4
+    out[..., 3:-2:2] = (3 * a[..., 1:-1] + a[..., 2:]) / 4
+    return np.moveaxis(out, -1, axis)
+
+
+@pytest.mark.parametrize("mz", [2, 1])
+def test_upsample_object(mz, small4d_scan):
+    """mbir_hip_upsample_object: a constant stays constant; a linear ramp is
+    reproduced exactly at interior samples (the 3/4, 1/4 weights are linear
+    interpolation at the fine sample centres); random data equals the rule
+    above within FP32 rounding; masked zeroes exactly the voxels outside the
+    fine stage's update mask and leaves the others."""
+    from phasecon_amd.mbir import Stage
+    coarse = Stage(small4d_scan, mult_xy=2, mult_z=mz, tile=4)
+    fine = Stage(small4d_scan, tile=4)
+    T, Z, Y, X = coarse.shape_x
+    print(f"coarse {coarse.shape_x} -> fine {fine.shape_x}", flush=True)
+    coarse.set_object(np.full(coarse.shape_x, 0.37, np.float32))
+    fine.upsample_from(coarse, masked=False)
+    assert np.allclose(fine.get_object(), 0.37, rtol=1e-7)
+    ramp = np.broadcast_to(np.arange(X, dtype=np.float32) * 0.25, coarse.shape_x).copy()
+    coarse.set_object(ramp)
+    fine.upsample_from(coarse, masked=False)
+    got = fine.get_object()
+    xf = (np.arange(2 * X) + 0.5) / 2 - 0.5  # fine sample centres in coarse units
+    assert np.allclose(got[..., 1:-1], (xf * 0.25)[1:-1], atol=1e-6)
+    r = np.random.default_rng(11)
+    xc = r.random(coarse.shape_x).astype(np.float32)
+    coarse.set_object(xc)
+    want = _up_axis(_up_axis(xc, 3), 2)
+    if mz == 2:
+        want = _up_axis(want, 1)
+    fine.upsample_from(coarse, masked=False)
+    got = fine.get_object()
+    assert got.shape == want.shape
+    assert np.max(np.abs(got - want)) < 1e-6
+    fine.upsample_from(coarse, masked=True)
+    gm = fine.get_object()
+    m = fine.mask.astype(bool)
+    assert np.all(gm[..., ~m] == 0)
+    assert np.array_equal(gm[..., m], got[..., m])
+    coarse.close()
+    fine.close()
