@@ -69,8 +69,8 @@ int mbir_hip_create(mbir_hip_ctx** out, const mbir_hip_geom* g,
                     const uint8_t* mask);
 void mbir_hip_destroy(mbir_hip_ctx* ctx);
 
-/* Tuning knobs: "tile" (ICD tile edge, default 8) and "alpha" (override of
- * g->alpha for the sweep). */
+/* Tuning knobs: "tile" (ICD tile edge in [4, 1024], default 8) and "alpha"
+ * (override of g->alpha for the sweep). */
 int mbir_hip_set_option(mbir_hip_ctx* ctx, const char* key, double value);
 
 /* Object transfer (replaces the Object array of XT_Structures.h:215). */
