@@ -165,6 +165,11 @@ WORKLOADS = {
     # Tiny 3D case for unit parity tests.
     "tiny3d": dict(rows=4, cols=32, views=24, frames=1, I0=1e4, K=None,
                    sig_s=0.005, sig_t=1.0, c_s=0.01, c_t=0.01, p_s=1.2, p_t=1.2),
+    # Tall 3D case: 16 detector rows and 24 views per frame, so a column's
+    # sinogram window is not one load batch per lane (the sweep's
+    # non-pipelined theta path).
+    "tall3d": dict(rows=16, cols=32, views=24, frames=1, I0=1e4, K=None,
+                   sig_s=0.005, sig_t=1.0, c_s=0.01, c_t=0.01, p_s=1.2, p_t=1.2),
 }
 
 

@@ -151,11 +151,16 @@ def test_cost_matches_oracle(pair):
 
 
 @pytest.mark.parametrize("name,n_gpu,tile", [
-    ("tiny3d", 120, 8), ("ch4d_small", 120, 8), ("ch4d_small", 120, 4)])
+    ("tiny3d", 120, 8), ("ch4d_small", 120, 8), ("ch4d_small", 120, 4), ("tall3d", 150, 8)])
 def test_icd_converges_to_oracle(name, n_gpu, tile, request):
+    """tiny3d and ch4d_small run the sweep's pipelined theta path, tall3d (16
+    detector rows, 24 views) the non-pipelined one."""
     from oracle.oracle import Oracle
     from phasecon_amd.mbir import Stage
-    scan = request.getfixturevalue("tiny_scan" if name == "tiny3d" else "small4d_scan")
+    from phasecon_amd.synth import make_scan
+    scan = (request.getfixturevalue("tiny_scan") if name == "tiny3d" else
+            request.getfixturevalue("small4d_scan") if name == "ch4d_small" else
+            make_scan(name, seed=0))
     o = Oracle(scan)
     ox, oe, ocost = o.run_icd(300)
     cstar = ocost[-1]
@@ -178,19 +183,23 @@ def test_icd_converges_to_oracle(name, n_gpu, tile, request):
     st.close()
 
 
-@pytest.mark.parametrize("tile", [4, 8, 16])
-def test_sweep_keeps_error_sinogram(tile, small4d_scan):
+@pytest.mark.parametrize("tile,name", [(4, "ch4d_small"), (8, "ch4d_small"),
+                                       (16, "ch4d_small"), (8, "tall3d")])
+def test_sweep_keeps_error_sinogram(tile, name, small4d_scan):
     """After sweeps the maintained e must still be y - A x: every voxel
-    change reaches the shared error sinogram exactly once."""
+    change reaches the shared error sinogram exactly once (pipelined theta
+    path on ch4d_small, non-pipelined on tall3d)."""
     from phasecon_amd.mbir import Stage
-    st = Stage(small4d_scan, tile=tile)
+    from phasecon_amd.synth import make_scan
+    scan = small4d_scan if name == "ch4d_small" else make_scan(name, seed=0)
+    st = Stage(scan, tile=tile)
     st.build_amatrix()
     st.init_error()
     st.icd(6, 1, seed=5)
     kept = st.get_error()
     st.init_error()
     exact = st.get_error()
-    scale = np.abs(small4d_scan["projections"]).max()
+    scale = np.abs(scan["projections"]).max()
     err = np.abs(kept - exact).max()
     print(f"tile={tile}: max |e_kept - e| = {err:.3g} (scale {scale:.3g})", flush=True)
     assert err <= 2e-5 * scale
