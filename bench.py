@@ -56,7 +56,7 @@ REL_TOL = 1e-5
 # HBM bytes per k_sweep launch measured with rocprofv3 on this build:
 # FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (KiB x 1024), the factors read off a
 # cold 1 GiB device copy (bench.py --profile-equits 3 --calibrate).
-MEASURED_TRAFFIC = {("ch4d", 8): 145060.6 * 1024 * 2.0 + 120815.1 * 1024 * 1.0}
+MEASURED_TRAFFIC = {("ch4d", 8, False): 145060.6 * 1024 * 2.0 + 120815.1 * 1024 * 1.0}
 
 
 def scan_digest(scan):
@@ -68,7 +68,8 @@ def scan_digest(scan):
 
 def algorithmic_bytes_per_equit(st):
     """Bytes one equit must move for the updated voxels: per (column, frame)
-    visit the A window (first index + W values per view); per voxel the
+    visit the A window (first index + W values per view, the values in FP16
+    plus one FP32 column scale when A is stored in FP16); per voxel the
     error sinogram read+write (8 B) and weight read (4 B) of each nonzero
     (view, bin, row) entry, plus the voxel's read and write (8 B)."""
     first, val = st.get_amatrix()
@@ -80,7 +81,10 @@ def algorithmic_bytes_per_equit(st):
         s0, s1 = int(st.frame_ptr[f]), int(st.frame_ptr[f + 1])
         per_col = nnz[:, s0:s1].sum(axis=1)[mask]       # nonzero entries per column
         nv = s1 - s0
-        total += mask.sum() * nv * (4 * st.W + 4)       # A window reads
+        if st.afp16:
+            total += mask.sum() * (nv * (2 * st.W + 4) + 4)
+        else:
+            total += mask.sum() * nv * (4 * st.W + 4)   # A window reads
         total += g.N_z * np.sum(12.0 * per_col * g.mult_z + 8.0)
     return total
 
@@ -249,6 +253,8 @@ def main():
     ap.add_argument("--max-equits", type=int, default=3000)
     ap.add_argument("--seed", type=int, default=761521)
     ap.add_argument("--refresh", type=int, default=10)
+    ap.add_argument("--afp16", action="store_true",
+                    help="store A in FP16 with a per-column FP32 scale")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-equits", type=int, default=2)
     ap.add_argument("--longrun", type=int, default=0)
@@ -277,7 +283,7 @@ def main():
     scan = make_scan(args.workload, seed=0)
     print(f"[bench] scan {args.workload} generated in {time.time() - t0:.1f}s", file=sys.stderr,
           flush=True)
-    st = Stage(scan, tile=args.tile)
+    st = Stage(scan, tile=args.tile, afp16=args.afp16)
     coarse = []
     if args.schedule == "multires" and not (args.profile_equits or args.longrun):
         from phasecon_amd.mbir import multires_schedule
@@ -287,7 +293,8 @@ def main():
         if len(counts) != len(stages):
             sys.exit(f"bench.py: --coarse-equits needs 1 or {len(stages)} counts")
         for (mxy, mz), n in zip(stages, counts):
-            coarse.append((Stage(scan, mult_xy=mxy, mult_z=mz, tile=args.coarse_tile), n))
+            coarse.append((Stage(scan, mult_xy=mxy, mult_z=mz, tile=args.coarse_tile,
+                                 afp16=args.afp16), n))
 
     if args.profile_equits:
         st.set_object(np.zeros(st.shape_x, np.float32))
@@ -380,6 +387,7 @@ def main():
                          f"frames, {w['views']} views" + (f" interlaced K={w['K']}" if w['K'] else "")
                          + f", I0={w['I0']:g}, qGGMRF p_s={w['p_s']} p_t={w['p_t']} q=2, from zero"),
             "tile": args.tile,
+            "a_storage": "fp16 + fp32 column scale" if args.afp16 else "fp32",
             "schedule": (("multires: " + " -> ".join(
                 f"{cs.g.N_x}x{cs.g.N_y}x{cs.g.N_z} ({n} equits, tile {args.coarse_tile})"
                 for cs, n in coarse) + f" -> {st.g.N_x}x{st.g.N_y}x{st.g.N_z} to target, "
@@ -402,7 +410,7 @@ def main():
             "peak": PEAK_HBM_GBPS,
             "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBPS,
-            "traffic": MEASURED_TRAFFIC.get((args.workload, args.tile)),
+            "traffic": MEASURED_TRAFFIC.get((args.workload, args.tile, args.afp16)),
             "kernel": "k_sweep",
             "bytes_per_launch": bytes_launch,
             "avg_launch_ms": avg_launch_ms,

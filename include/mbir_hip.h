@@ -69,8 +69,14 @@ int mbir_hip_create(mbir_hip_ctx** out, const mbir_hip_geom* g,
                     const uint8_t* mask);
 void mbir_hip_destroy(mbir_hip_ctx* ctx);
 
-/* Tuning knobs: "tile" (ICD tile edge in [4, 1024], default 8) and "alpha"
- * (override of g->alpha for the sweep). */
+/* Tuning knobs: "tile" (ICD tile edge in [4, 1024], default 8), "alpha"
+ * (override of g->alpha for the sweep) and "afp16" (0 or 1, default 0; takes
+ * effect at the next mbir_hip_build_amatrix): A is stored in FP16 with one
+ * FP32 scale per voxel column (its largest |a|), and the sweep streams the
+ * FP16 values; the FP32 copy of A (mbir_hip_get_amatrix, the back-projector)
+ * then holds the same rounded values.  The forward projector and the cost
+ * compute A on the fly and stay exact.  The reference stores A in FP32
+ * (XT_AMatrix.c:50-110); this knob has no counterpart there. */
 int mbir_hip_set_option(mbir_hip_ctx* ctx, const char* key, double value);
 
 /* Object transfer (replaces the Object array of XT_Structures.h:215). */

@@ -27,6 +27,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <algorithm>
 #include <vector>
 
@@ -160,6 +161,29 @@ __global__ __launch_bounds__(256) void k_amatrix(DevGeom G,
                                 });
   // a column with no kept bin is empty (count 0 in XT_AMatrix.c:109)
   afirst[i] = kept ? jlo : -1;
+}
+
+// FP16 copy of A with one FP32 scale per voxel column, the largest |a| of the
+// column over all view slots: q = fp16(a / s), rounded to nearest even, and
+// the FP32 copy rewritten to q s, so that every kernel that reads stored A
+// uses the values the FP16 sweep converts back.  One wavefront per column.
+__global__ __launch_bounds__(256) void k_aquant(int ncol, int per_col, float* __restrict__ aval,
+                                                unsigned short* __restrict__ aval16,
+                                                float* __restrict__ scale) {
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (c >= ncol) return;
+  const int lane = threadIdx.x & 63;
+  float* a = aval + (size_t)c * per_col;
+  unsigned short* h = aval16 + (size_t)c * per_col;
+  float m = 0.f;
+  for (int j = lane; j < per_col; j += 64) m = fmaxf(m, fabsf(a[j]));
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+  if (lane == 0) scale[c] = m;
+  for (int j = lane; j < per_col; j += 64) {
+    const _Float16 q = m > 0.f ? (_Float16)(a[j] / m) : (_Float16)0.f;
+    h[j] = __builtin_bit_cast(unsigned short, q);
+    a[j] = (float)q * m;
+  }
 }
 
 // ----------------------------------------------------------- projection
@@ -457,6 +481,8 @@ __device__ __forceinline__ int group_or(int v, int n) {
 
 struct ColumnArgs {
   const float* aval;
+  const unsigned short* aval16;  // FP16 A (bits) and its per-column scale,
+  const float* ascale;           // read by the kHalf sweep only
   const int* afirst;
   const int* frame_views;
   const float* w;
@@ -510,13 +536,16 @@ struct SweepLds {
 
 // Register staging of a column's A window and t+-1 values (prefetch): PI
 // first bins and PA values per lane (kSmall: frames of at most 64 views and
-// 256 window values).
-template <bool kSmall>
+// 256 window values).  kHalf: the values stay FP16 bits until they are
+// stored, so that no conversion waits on the loads in flight.
+template <bool kSmall, bool kHalf = false>
 struct APre {
   static constexpr int PI = kSmall ? 1 : kPreI;
   static constexpr int PA = kSmall ? 4 : kPreA;
+  using AV = typename std::conditional<kHalf, unsigned short, float>::type;
   int a0[PI];
-  float av[PA];
+  AV av[PA];
+  float scale;
   float tn[kTnPre];
 };
 
@@ -532,28 +561,32 @@ __device__ __forceinline__ Win pick_win(const L_& L, int b) {
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 
-template <int kW, bool kSmall>
+template <int kW, bool kSmall, bool kHalf = false>
 __device__ __forceinline__ void awin_fetch(const SwGeom& G, const ColumnArgs& A, int c, int s0,
-                                           int nv, APre<kSmall>& pre) {
+                                           int nv, APre<kSmall, kHalf>& pre) {
   const int lane = lane_id();
   const size_t base = (size_t)c * G.nslots + s0;
 #pragma unroll
-  for (int i = 0; i < APre<kSmall>::PI; ++i) {
+  for (int i = 0; i < APre<kSmall, kHalf>::PI; ++i) {
     const int j = lane + i * kWave;
     pre.a0[i] = j < nv ? A.afirst[base + j] : -1;
   }
   const int nav = nv * kW;
 #pragma unroll
-  for (int i = 0; i < APre<kSmall>::PA; ++i) {
+  for (int i = 0; i < APre<kSmall, kHalf>::PA; ++i) {
     const int j = lane + i * kWave;
-    pre.av[i] = j < nav ? A.aval[base * kW + j] : 0.f;
+    if constexpr (kHalf)
+      pre.av[i] = j < nav ? A.aval16[base * kW + j] : (unsigned short)0;
+    else
+      pre.av[i] = j < nav ? A.aval[base * kW + j] : 0.f;
   }
+  if constexpr (kHalf) pre.scale = A.ascale[c];
 }
 
 // The t-1 and t+1 columns of voxel column c (not updated in a sweep launch).
-template <bool kSmall>
+template <typename Pre>
 __device__ __forceinline__ void tn_fetch(const SwGeom& G, const float* x, int f, int c,
-                                         size_t vol, APre<kSmall>& pre) {
+                                         size_t vol, Pre& pre) {
 #pragma unroll
   for (int i = 0; i < kTnPre; ++i) {
     const int j = lane_id() + i * kWave;
@@ -565,20 +598,25 @@ __device__ __forceinline__ void tn_fetch(const SwGeom& G, const float* x, int f,
 
 // Stores a prefetched window (first bins turned into sinogram offsets
 // through the frame's view list) and t+-1 values.
-template <int kW, bool kSmall>
-__device__ __forceinline__ void awin_store(const SwGeom& G, const APre<kSmall>& pre, int nv,
-                                           const int* views, const Win& win, float* tn) {
+template <int kW, bool kSmall, bool kHalf = false>
+__device__ __forceinline__ void awin_store(const SwGeom& G, const APre<kSmall, kHalf>& pre,
+                                           int nv, const int* views, const Win& win, float* tn) {
   const int lane = lane_id();
 #pragma unroll
-  for (int i = 0; i < APre<kSmall>::PI; ++i) {
+  for (int i = 0; i < APre<kSmall, kHalf>::PI; ++i) {
     const int j = lane + i * kWave;
     if (j < nv) win.base[j] = pre.a0[i] >= 0 ? (views[j] * G.N_r + pre.a0[i]) * G.N_t : -1;
   }
   const int nav = nv * kW;
 #pragma unroll
-  for (int i = 0; i < APre<kSmall>::PA; ++i) {
+  for (int i = 0; i < APre<kSmall, kHalf>::PA; ++i) {
     const int j = lane + i * kWave;
-    if (j < nav) win.av[j] = pre.av[i];
+    if constexpr (kHalf) {
+      // the same product k_aquant wrote into the FP32 copy of A
+      if (j < nav) win.av[j] = (float)__builtin_bit_cast(_Float16, pre.av[i]) * pre.scale;
+    } else {
+      if (j < nav) win.av[j] = pre.av[i];
+    }
   }
 #pragma unroll
   for (int i = 0; i < kTnPre; ++i) {
@@ -1045,7 +1083,7 @@ constexpr int kSweepThreads = 2 * kWave;
 // then wavefront 0 sends the column's error update with atomics and moves on
 // to the next column's theta while wavefront 1 starts its terms.
 // kW = W, the A window width.
-template <int kW, bool kSmall>
+template <int kW, bool kSmall, bool kHalf>
 __global__ __launch_bounds__(kSweepThreads) void k_sweep(SwGeom G, SweepArgs A) {
   extern __shared__ __align__(16) float smem[];
   __shared__ float s_tn[kWinBufs][2 * kNZMax];
@@ -1110,11 +1148,11 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SwGeom G, SweepArgs A) 
   const bool pipe = G.N_t <= kWave && (G.N_t & (G.N_t - 1)) == 0 && nv <= SB * (kWave / rpl);
   if (w0) {
     // window and t+-1 values of the first column
-    APre<kSmall> pre;
+    APre<kSmall, kHalf> pre;
     const int cf = L.cols[pos];
-    awin_fetch<kW, kSmall>(G, A.col, cf, s0, nv, pre);
-    tn_fetch<kSmall>(G, A.x, f, cf, vol, pre);
-    awin_store<kW, kSmall>(G, pre, nv, L.views, L.win[0], s_tn[0]);
+    awin_fetch<kW, kSmall, kHalf>(G, A.col, cf, s0, nv, pre);
+    tn_fetch(G, A.x, f, cf, vol, pre);
+    awin_store<kW, kSmall, kHalf>(G, pre, nv, L.views, L.win[0], s_tn[0]);
   }
   // nothing stays in flight: keeps the compiler's wait analysis from
   // carrying these loads into the column loop
@@ -1129,14 +1167,14 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SwGeom G, SweepArgs A) 
     // pipeline: the prefetch of column k+1 is in flight during column k's
     // solve, and stored at the start of column k's step (before the theta
     // loads of column k+1 go out)
-    APre<kSmall> pre;
+    APre<kSmall, kHalf> pre;
     if (pipe) {
       if (n > 1) {
         int p1 = pos + pa;
         if (p1 >= n) p1 -= n;
         const int cn = L.cols[p1];
-        awin_fetch<kW, kSmall>(G, A.col, cn, s0, nv, pre);
-        tn_fetch<kSmall>(G, A.x, f, cn, vol, pre);
+        awin_fetch<kW, kSmall, kHalf>(G, A.col, cn, s0, nv, pre);
+        tn_fetch(G, A.x, f, cn, vol, pre);
       }
       theta_issue<kW>(G, A.col, L.win[0], nv, tl);
     }
@@ -1156,11 +1194,11 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SwGeom G, SweepArgs A) 
         // stored over column k-2's window; the prefetch of column k+2
         theta_finish<kW>(G, wcur, wprev, k > 0, s_dx, lognt, nv, tl, s_row1, s_row2);
         if (k > 0) col_update<kW>(G, A.col, wprev, nv, s_dx);
-        if (k + 1 < n) awin_store<kW, kSmall>(G, pre, nv, L.views, wnext, s_tn[bn]);
+        if (k + 1 < n) awin_store<kW, kSmall, kHalf>(G, pre, nv, L.views, wnext, s_tn[bn]);
         if (k + 2 < n) {
           const int cn = L.cols[p2];
-          awin_fetch<kW, kSmall>(G, A.col, cn, s0, nv, pre);
-          tn_fetch<kSmall>(G, A.x, f, cn, vol, pre);
+          awin_fetch<kW, kSmall, kHalf>(G, A.col, cn, s0, nv, pre);
+          tn_fetch(G, A.x, f, cn, vol, pre);
         }
         __syncthreads();
         // phase 2 (during the solve of column k): column k+1's theta loads
@@ -1170,8 +1208,8 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SwGeom G, SweepArgs A) 
         // phase 1: prefetch of column k+1, theta of column k
         if (k + 1 < n) {
           const int cn = L.cols[p1];
-          awin_fetch<kW, kSmall>(G, A.col, cn, s0, nv, pre);
-          tn_fetch<kSmall>(G, A.x, f, cn, vol, pre);
+          awin_fetch<kW, kSmall, kHalf>(G, A.col, cn, s0, nv, pre);
+          tn_fetch(G, A.x, f, cn, vol, pre);
         }
         col_thetas<kW>(G, A.col, wcur, nv, s_row1, s_row2, s_part1, s_part2);
         __syncthreads();
@@ -1179,7 +1217,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SwGeom G, SweepArgs A) 
         // retire every load explicitly (a free wait here) so that the
         // compiler's wait analysis carries none into the next column, before
         // the error update's atomics go out
-        if (k + 1 < n) awin_store<kW, kSmall>(G, pre, nv, L.views, wnext, s_tn[bn]);
+        if (k + 1 < n) awin_store<kW, kSmall, kHalf>(G, pre, nv, L.views, wnext, s_tn[bn]);
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         col_update<kW>(G, A.col, wcur, nv, s_dx);
@@ -1542,6 +1580,9 @@ struct mbir_hip_ctx {
   uint8_t* d_mask = nullptr;  // update mask [N_y * N_x]
   int* d_afirst = nullptr;
   float* d_aval = nullptr;
+  bool afp16 = false;                 // the sweep streams FP16 A + column scales
+  unsigned short* d_aval16 = nullptr;  // [ncol][nslots][W] FP16 bits
+  float* d_ascale = nullptr;           // [ncol]
   bool a_built = false;
   int *d_tile_ptr = nullptr, *d_tile_cols = nullptr, *d_launch_tiles = nullptr;
   int launch_off[5] = {0, 0, 0, 0, 0};
@@ -1792,7 +1833,8 @@ void mbir_hip_destroy(mbir_hip_ctx* c) {
   void* bufs[] = {c->d_H, c->d_cos, c->d_sin, c->d_frame_ptr, c->d_frame_views,
                   c->d_view_frame, c->d_y, c->d_w, c->d_d, c->d_e, c->d_x,
                   c->d_stage, c->d_stage2, c->d_afirst, c->d_aval, c->d_tile_ptr,
-                  c->d_tile_cols, c->d_launch_tiles, c->d_stats, c->d_partial, c->d_mask};
+                  c->d_tile_cols, c->d_launch_tiles, c->d_stats, c->d_partial, c->d_mask,
+                  c->d_aval16, c->d_ascale};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (int k = 0; k < MBIR_K_COUNT; ++k)
@@ -1816,6 +1858,10 @@ int mbir_hip_set_option(mbir_hip_ctx* c, const char* key, double value) {
     c->tile = (int)value;
   } else if (k == "alpha") {
     c->alpha = value;
+  } else if (k == "afp16") {
+    if (value != 0 && value != 1) return fail("afp16 must be 0 or 1");
+    if (c->afp16 != (value == 1)) c->a_built = false;  // A must be built again
+    c->afp16 = value == 1;
   } else {
     return fail("unknown option " + k);
   }
@@ -1874,6 +1920,13 @@ int mbir_hip_build_amatrix(mbir_hip_ctx* c) {
   hipLaunchKernelGGL(k_amatrix, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
                      c->G, c->d_H, c->d_cos, c->d_sin, c->d_frame_views, c->d_afirst,
                      c->d_aval);
+  if (c->afp16) {
+    if (!c->d_aval16 &&
+        (dev_alloc(&c->d_aval16, n * c->G.W) || dev_alloc(&c->d_ascale, (size_t)c->ncol)))
+      return -1;
+    hipLaunchKernelGGL(k_aquant, dim3((unsigned)((c->ncol + 3) / 4)), dim3(256), 0, c->stream,
+                       c->ncol, c->nslots * c->G.W, c->d_aval, c->d_aval16, c->d_ascale);
+  }
   if (ev_end(c, MBIR_K_AMATRIX)) return -1;
   c->a_built = true;
   return 0;
@@ -1953,6 +2006,8 @@ int mbir_hip_icd_equits(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t
   HIP_OK(hipMemsetAsync(c->d_stats, 0, 4 * sizeof(double), c->stream));
   SweepArgs A;
   A.col.aval = c->d_aval;
+  A.col.aval16 = c->afp16 ? c->d_aval16 : nullptr;
+  A.col.ascale = c->afp16 ? c->d_ascale : nullptr;
   A.col.afirst = c->d_afirst;
   A.col.frame_views = c->d_frame_views;
   A.col.w = c->d_w;
@@ -1990,14 +2045,19 @@ int mbir_hip_icd_equits(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t
         A.fparity = fpar;
         A.seed = seed * 2654435761u + (unsigned)(it * 8 + colour * 2 + fpar) * 40503u;
         if (ev_begin(c, MBIR_K_SWEEP)) return -1;
-        if (small) {
-          MBIR_DISPATCH_W(c->G.W,
-                          hipLaunchKernelGGL((k_sweep<kW, true>), dim3((unsigned)(ntl * frames)),
-                                             dim3(kSweepThreads), xbytes, c->stream, c->SG, A));
+        const dim3 grid((unsigned)(ntl * frames)), block(kSweepThreads);
+        if (small && c->afp16) {
+          MBIR_DISPATCH_W(c->G.W, hipLaunchKernelGGL((k_sweep<kW, true, true>), grid, block,
+                                                     xbytes, c->stream, c->SG, A));
+        } else if (small) {
+          MBIR_DISPATCH_W(c->G.W, hipLaunchKernelGGL((k_sweep<kW, true, false>), grid, block,
+                                                     xbytes, c->stream, c->SG, A));
+        } else if (c->afp16) {
+          MBIR_DISPATCH_W(c->G.W, hipLaunchKernelGGL((k_sweep<kW, false, true>), grid, block,
+                                                     xbytes, c->stream, c->SG, A));
         } else {
-          MBIR_DISPATCH_W(c->G.W,
-                          hipLaunchKernelGGL((k_sweep<kW, false>), dim3((unsigned)(ntl * frames)),
-                                             dim3(kSweepThreads), xbytes, c->stream, c->SG, A));
+          MBIR_DISPATCH_W(c->G.W, hipLaunchKernelGGL((k_sweep<kW, false, false>), grid, block,
+                                                     xbytes, c->stream, c->SG, A));
         }
         if (ev_end(c, MBIR_K_SWEEP)) return -1;
       }
@@ -2097,7 +2157,9 @@ int mbir_hip_voxel_thetas(mbir_hip_ctx* c, int32_t n, const int32_t* tzyx, float
   if (dev_alloc(&dq, (size_t)n * 4) || dev_alloc(&d1, n) || dev_alloc(&d2, n)) return -1;
   HIP_OK(hipMemcpyAsync(dq, tzyx, (size_t)n * 16, hipMemcpyHostToDevice, c->stream));
   ColumnArgs A;
-  A.aval = c->d_aval;
+  A.aval = c->d_aval;  // the FP32 copy (FP16-rounded values when afp16 is on)
+  A.aval16 = nullptr;
+  A.ascale = nullptr;
   A.afirst = c->d_afirst;
   A.frame_views = c->d_frame_views;
   A.w = c->d_w;

@@ -48,7 +48,8 @@ class Stage:
     """One multi-resolution stage of one scan, resident on the GPU."""
 
     def __init__(self, scan, mult_xy=1, mult_z=1, quad_convex=1, huber_delta=0.5,
-                 huber_T=4.0, p_s=None, p_t=None, tile=16, alpha=None, var_est=1.0):
+                 huber_T=4.0, p_s=None, p_t=None, tile=16, alpha=None, var_est=1.0,
+                 afp16=False):
         L = self.L = _lib.load()
         s = self.scan = scan
         N_p, N_t, N_r = s["proj_num"], s["proj_rows"], s["proj_cols"]
@@ -85,6 +86,9 @@ class Stage:
         self.option("tile", tile)
         if alpha is not None:
             self.option("alpha", alpha)
+        self.afp16 = bool(afp16)
+        if afp16:
+            self.option("afp16", 1)
 
     def _internal(self, a):
         s = self.scan
