@@ -150,11 +150,13 @@ def test_cost_matches_oracle(pair):
     assert est[1] == kept[1]
 
 
-@pytest.mark.parametrize("name,n_gpu,tile", [
-    ("tiny3d", 120, 8), ("ch4d_small", 120, 8), ("ch4d_small", 120, 4), ("tall3d", 150, 8)])
-def test_icd_converges_to_oracle(name, n_gpu, tile, request):
+@pytest.mark.parametrize("name,n_gpu,tile,alpha", [
+    ("tiny3d", 120, 8, None), ("ch4d_small", 120, 8, None), ("ch4d_small", 120, 4, None),
+    ("tall3d", 150, 8, None), ("ch4d_small", 120, 8, 1.9), ("tall3d", 150, 8, 1.9)])
+def test_icd_converges_to_oracle(name, n_gpu, tile, alpha, request):
     """tiny3d and ch4d_small run the sweep's pipelined theta path, tall3d (16
-    detector rows, 24 views) the non-pipelined one."""
+    detector rows, 24 views) the non-pipelined one; alpha None is the
+    reference's over-relaxation factor 1.5, 1.9 the one the benchmark uses."""
     from oracle.oracle import Oracle
     from phasecon_amd.mbir import Stage
     from phasecon_amd.synth import make_scan
@@ -165,7 +167,7 @@ def test_icd_converges_to_oracle(name, n_gpu, tile, request):
     ox, oe, ocost = o.run_icd(300)
     cstar = ocost[-1]
     assert abs(ocost[-1] - ocost[-2]) < 1e-9 * cstar  # oracle converged
-    st = Stage(scan, tile=tile)
+    st = Stage(scan, tile=tile, alpha=alpha)
     st.build_amatrix()
     st.init_error()
     costs = [st.cost()[0]]
