@@ -115,6 +115,14 @@ int mbir_hip_set_error(mbir_hip_ctx* ctx, const float* e);
 int mbir_hip_forward_project(mbir_hip_ctx* ctx, const float* x, float* ax);
 int mbir_hip_backproject(mbir_hip_ctx* ctx, const float* v, float* bt);
 
+/* Normalised back-projection, the input of the learned warm start
+ * (include/unet_hip.h; no counterpart in the reference):
+ * out[t][z][y][x] = A^T(w (y - d)) / A^T(w A m) at the voxel, m the update
+ * mask's indicator, over its frame's views (0 where the denominator is 0):
+ * exact for an object that is uniform inside the mask.  Needs A
+ * (mbir_hip_build_amatrix); the projection A m is computed on the fly. */
+int mbir_hip_bp_normalized(mbir_hip_ctx* ctx, float* out);
+
 /* n equits of the parallel ICD (updateVoxelsTimeSlices,
  * XT_ICD_update.c:932-1092, with the voxel update of XT_VoxUpdate.c:45-283
  * and the substitution of XT_ICD_update.c:133-182).  Iteration numbers run

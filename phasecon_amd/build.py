@@ -3,7 +3,8 @@
   phasecon_amd/libmbir4d.so    -- MBIR: HIP kernels (csrc/mbir_hip.hip) + the C
                                   host engine (csrc/mbir_engine.c), exporting
                                   reconstruct() and the C-ABI of
-                                  include/mbir_hip.h
+                                  include/mbir_hip.h; the learned warm start
+                                  (csrc/unet_hip.hip, include/unet_hip.h)
   phasecon_amd/libpcmbir4d.so  -- PCMBIR phase retrieval: csrc/pcmbir_*.hip /
                                   pcmbir_*.c, the C-ABI of include/pcmbir_hip.h
 
@@ -40,7 +41,7 @@ def _run(cmd):
 
 # library -> source files in csrc/ (prefix match)
 LIBS = {
-    "libmbir4d.so": ("mbir_",),
+    "libmbir4d.so": ("mbir_", "unet_"),
     "libpcmbir4d.so": ("pcmbir_",),
 }
 

@@ -1383,6 +1383,59 @@ __global__ __launch_bounds__(256) void k_backproj(DevGeom G,
   }
 }
 
+// Normalised back-projection: per voxel sum(w a (y - d)) / sum(w a (A m))
+// over its sinogram entries, m the update mask's indicator (A m its
+// projection, `aone`): exact for a uniform object inside the mask.  The
+// input of the learned warm start.  One thread per (frame, column), voxels
+// column-major [frame][column][z].
+__global__ __launch_bounds__(256) void k_bpnorm(DevGeom G, const float* __restrict__ aval,
+                                                const int* __restrict__ afirst,
+                                                const int* __restrict__ frame_ptr,
+                                                const int* __restrict__ frame_views,
+                                                const float* __restrict__ y,
+                                                const float* __restrict__ w,
+                                                const float* __restrict__ d,
+                                                const float* __restrict__ aone,
+                                                float* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long ncol = (long long)G.N_x * G.N_y;
+  if (i >= ncol * G.N_time) return;
+  const int f = (int)(i / ncol);
+  const int c = (int)(i % ncol);
+  for (int z = 0; z < G.N_z; ++z) {
+    double num = 0.0, den = 0.0;
+    for (int s = frame_ptr[f]; s < frame_ptr[f + 1]; ++s) {
+      const size_t slot = (size_t)c * G.nslots + s;
+      const int r0 = afirst[slot];
+      if (r0 < 0) continue;
+      const int p = frame_views[s];
+      for (int k = 0; k < G.W; ++k) {
+        const float a = aval[slot * G.W + k];
+        const int r = r0 + k;
+        if (a == 0.f || r >= G.N_r) continue;
+        const size_t row = ((size_t)p * G.N_r + r) * G.N_t;
+        double sn = 0.0, sd = 0.0;
+        for (int t = z * G.zo; t < (z + 1) * G.zo; ++t) {
+          const double wt = w[row + t];
+          sn += wt * ((double)y[row + t] - (d ? (double)d[(size_t)r * G.N_t + t] : 0.0));
+          sd += wt * aone[row + t];
+        }
+        num += (double)a * sn;
+        den += (double)a * sd;
+      }
+    }
+    out[(size_t)i * G.N_z + z] = den > 0.0 ? (float)(num / den) : 0.f;
+  }
+}
+
+// the update mask's indicator, column-major [frame][column][z]
+__global__ void k_mask_ones(int Nt, int ncol, int Nz, const uint8_t* __restrict__ mask,
+                            float* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)Nt * ncol * Nz) return;
+  out[i] = mask[(i / Nz) % ncol] ? 1.f : 0.f;
+}
+
 // [t][z][y][x] <-> [t][y][x][z]
 // Fine-grid sample i of a 2x up-sampling along an axis of n coarse samples
 // (XT_ICD_update.c:358-390, 448-496): the ends copy, interior samples weigh
@@ -1993,6 +2046,27 @@ int mbir_hip_backproject(mbir_hip_ctx* c, const float* v, float* bt) {
                      c->g.N_z, c->ncol, c->d_stage2, c->d_stage);
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(bt, c->d_stage, c->nvox * sizeof(float), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mbir_hip_bp_normalized(mbir_hip_ctx* c, float* out) {
+  if (!c || !out) return fail("null argument");
+  if (!c->a_built) return fail("A not built");
+  const long long n = (long long)c->ncol * c->g.N_time;
+  const unsigned blocks = (unsigned)((c->nvox + 255) / 256);
+  // A m into d_stage, then the ratio into d_stage2
+  hipLaunchKernelGGL(k_mask_ones, dim3(blocks), dim3(256), 0, c->stream, c->g.N_time, c->ncol,
+                     c->g.N_z, c->d_mask, c->d_stage2);
+  if (run_fwdproj(c, 2, c->d_stage2, c->d_stage)) return -1;
+  hipLaunchKernelGGL(k_bpnorm, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->G,
+                     c->d_aval, c->d_afirst, c->d_frame_ptr, c->d_frame_views, c->d_y, c->d_w,
+                     c->d_d, c->d_stage, c->d_stage2);
+  hipLaunchKernelGGL(k_from_colmajor, dim3(blocks), dim3(256), 0, c->stream, c->g.N_time,
+                     c->g.N_z, c->ncol, c->d_stage2, c->d_stage);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(out, c->d_stage, c->nvox * sizeof(float), hipMemcpyDeviceToHost,
                         c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
   return 0;

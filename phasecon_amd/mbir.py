@@ -145,6 +145,13 @@ class Stage:
         _lib.check(self.L.mbir_hip_get_amatrix(self.h, _p(first), _p(val)), "get_amatrix")
         return first[:, :ns], val[:, :ns]
 
+    def bp_normalized(self):
+        """Per voxel sum(w a (y - d)) / sum(w a^2) (mbir_hip_bp_normalized),
+        [N_time][N_z][N_y][N_x]; needs build_amatrix()."""
+        out = np.zeros(self.shape_x, np.float32)
+        _lib.check(self.L.mbir_hip_bp_normalized(self.h, _p(out)), "bp_normalized")
+        return out
+
     def init_error(self):
         _lib.check(self.L.mbir_hip_init_error(self.h), "init_error")
 

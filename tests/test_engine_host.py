@@ -20,7 +20,7 @@ LIB = os.path.join(ROOT, "phasecon_amd", "libmbir4d.so")
 needs_lib = pytest.mark.skipif(not os.path.exists(LIB), reason="libmbir4d.so not built")
 
 
-def _header_functions(headers=("mbir4d.h", "mbir_engine.h", "mbir_hip.h")):
+def _header_functions(headers=("mbir4d.h", "mbir_engine.h", "mbir_hip.h", "unet_hip.h")):
     names = set()
     for h in headers:
         text = open(os.path.join(ROOT, "include", h)).read()
