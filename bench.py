@@ -22,6 +22,18 @@ full-resolution equit (maintained error sinogram, FP32 prior potentials
 summed in FP64).  Inputs are resident in HBM before the timed region.
 Prints one JSON line.
 
+Learned warm start (skipped with --no-learned): training scans of the same
+workload with seeds 1..--learned-scans are generated on the host before the
+GPU is used; the GPU MBIR converges each (the schedule above plus 200
+full-resolution equits) and a 3-level 3D U-Net (BF16 MFMA kernels) is trained
+on (filtered back-projection of the frame and its mean over the frames,
+converged image) patches.  Then the benchmark scan is reconstructed from the
+network's estimate (A build, filtered back-projection, inference and, for
+each --learned-coarse-equits count above 0, a two-grid correction of the
+estimate's low frequencies on the 2x coarser grid, all timed) to the same
+target, and once from zero at full resolution only (the plain initialiser).
+Reported under "learned"; training and pair generation are timed apart.
+
 Also reported: equits/s and the sweep kernel's algorithmic HBM bytes per
 second with its roofline fraction (HIP events recorded by the library on its
 own stream around every launch), and a CPU baseline: the reference's own ICD
@@ -92,6 +104,44 @@ def algorithmic_bytes_per_equit(st):
     return total
 
 
+def to_target(st, target, max_equits, seed, refresh):
+    """Full-resolution equits from the stage's current object until the
+    FP64 MAP cost is <= target; returns (equits, cost)."""
+    st.build_amatrix()
+    st.init_error()
+    it = 0
+    cost = st.cost(estimate=True)[0]
+    while it < max_equits:
+        it += 1
+        st.icd(1, it, seed)
+        if refresh and it % refresh == 0:
+            st.init_error()
+        cost = st.cost(estimate=True)[0]
+        if cost <= target:
+            fresh = st.cost(fresh=True)[0]   # FP64 check from a recomputed e
+            if fresh <= target:
+                cost = fresh
+                break
+    return it, cost
+
+
+def run_learned(st, net, target, max_equits, seed, refresh, source="fbp2", coarse=None,
+                coarse_equits=0):
+    """One timed reconstruction from the learned warm start: A, the
+    network's input and estimate (with the two-grid correction when
+    coarse_equits > 0), then full resolution to the target; returns
+    (seconds, equits, cost, seconds to the start)."""
+    from phasecon_amd.warmstart import learned_start
+    st.synchronize()
+    t0 = time.perf_counter()
+    st.build_amatrix()
+    learned_start(st, net, source, coarse, coarse_equits, seed)
+    t_start = time.perf_counter() - t0
+    it, cost = to_target(st, target, max_equits, seed, refresh)
+    st.synchronize()
+    return time.perf_counter() - t0, it, cost, t_start
+
+
 def run_once(st, target, max_equits, seed, refresh, coarse=()):
     """One timed reconstruction from zero; returns (seconds, equits, cost).
     coarse: (stage, equits) pairs of the multi-resolution stages that run
@@ -117,23 +167,109 @@ def run_once(st, target, max_equits, seed, refresh, coarse=()):
                 cs.init_error()
     if prev is not None:
         st.upsample_from(prev, masked=True)
-    st.build_amatrix()
-    st.init_error()
-    cost = st.cost(estimate=True)[0]
-    it = 0
-    while it < max_equits:
-        it += 1
-        st.icd(1, it, seed)
-        if refresh and it % refresh == 0:
-            st.init_error()
-        cost = st.cost(estimate=True)[0]
-        if cost <= target:
-            fresh = st.cost(fresh=True)[0]   # FP64 check from a recomputed e
-            if fresh <= target:
-                cost = fresh
-                break
+    it, cost = to_target(st, target, max_equits, seed, refresh)
     st.synchronize()
     return time.perf_counter() - t0, it, cost
+
+
+def _make_train_scan(args):
+    from phasecon_amd.synth import make_scan
+    workload, seed = args
+    return make_scan(workload, seed=seed)
+
+
+def learned_leg(args, st, train_scans, target):
+    """Config of the learned warm start: training pairs from the GPU MBIR on
+    the training scans (other seeds of the workload), the U-Net trained on
+    them, then timed reconstructions of the benchmark scan from its
+    estimate (training and pair generation are reported, not timed)."""
+    from phasecon_amd.warmstart import IN_CHANNELS, UNet, converged_pair, train
+    log = lambda m: print("[bench] learned: " + m, file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    src = args.learned_input
+    pairs = [converged_pair(sc, alpha=args.alpha, source=src) for sc in train_scans]
+    t_pairs = time.perf_counter() - t0
+    log(f"{len(pairs)} training pairs in {t_pairs:.1f}s")
+    net = UNet(channels=args.learned_channels, seed=0, in_channels=IN_CHANNELS[src])
+    t0 = time.perf_counter()
+    hist = train(net, pairs, steps=args.learned_steps, batch=8, size=64, lr=1e-3, seed=0, log=log)
+    t_train = time.perf_counter() - t0
+    from phasecon_amd.mbir import Stage, multires_schedule
+    mxy, mz = multires_schedule(st.scan["proj_rows"], st.scan["proj_cols"])[-2]
+    coarse = Stage(st.scan, mult_xy=mxy, mult_z=mz, tile=args.coarse_tile, alpha=args.coarse_alpha)
+    variants = []
+    for nc in [int(v) for v in str(args.learned_coarse_equits).split(",")]:
+        run_learned(st, net, target, args.max_equits, args.seed, args.refresh, src, coarse, nc)
+        times, equits, costs, starts = [], [], [], []
+        for k in range(args.steps):
+            dt, n, c, ts = run_learned(st, net, target, args.max_equits, args.seed + k,
+                                       args.refresh, src, coarse, nc)
+            times.append(dt)
+            equits.append(n)
+            costs.append(c)
+            starts.append(ts)
+            log(f"coarse {nc}: step {k}: {dt:.3f}s ({ts * 1e3:.1f} ms to the start), {n} "
+                f"equits, cost {c:.10g}")
+        variants.append({
+            "coarse_equits": nc,
+            "value": float(np.mean(times)) if all(c <= target for c in costs) else None,
+            "equits_to_target": equits, "ms_to_start": float(np.mean(starts)) * 1e3})
+    coarse.close()
+    net.close()
+    # the plain initialiser the network replaces: zero, full resolution only
+    st.set_object(np.zeros(st.shape_x, np.float32))
+    st.synchronize()
+    t0 = time.perf_counter()
+    n0, c0 = to_target(st, target, args.max_equits, args.seed, args.refresh)
+    st.synchronize()
+    zero = {"value": time.perf_counter() - t0 if c0 <= target else None, "equits_to_target": n0}
+    log(f"zero start: {zero['value']}s, {n0} equits")
+    best = min((v for v in variants if v["value"]), key=lambda v: v["value"], default=variants[0])
+    return {
+        "value": best["value"],
+        "unit": "s",
+        "equits_to_target": best["equits_to_target"],
+        "ms_to_start": best["ms_to_start"],
+        "coarse_equits": best["coarse_equits"],
+        "variants": variants,
+        "zero_start": zero,
+        "network": f"3-level 3D U-Net, channels {args.learned_channels}/"
+                   f"{2 * args.learned_channels}/{4 * args.learned_channels}, BF16 MFMA",
+        "training": {"scans": len(train_scans), "seeds": list(range(1, len(train_scans) + 1)),
+                     "pairs_s": t_pairs, "steps": args.learned_steps, "batch": 8,
+                     "patch": "full depth x 64 x 64", "train_s": t_train,
+                     "loss_first": float(hist[:20].mean()), "loss_last": float(hist[-20:].mean())},
+        "input": {"fbp3": "filtered back-projection of the frame, its mean over the frame and "
+                          "its neighbours, and its mean over the frames",
+                  "fbp2": "filtered back-projection of the frame and its mean over the frames",
+                  "fbp": "filtered back-projection of the frame",
+                  "bp": "normalised back-projection A^T(w y) / A^T(w A m)"}[src],
+        "start": ("U-Net estimate; with coarse_equits > 0 its low frequencies corrected by "
+                  "that many equits on the 2x coarser grid from its block mean (two-grid), "
+                  "then full resolution to the target"),
+    }
+
+
+def profile_unet(n, c=16, in_ch=2, N=16, D=8, H=256, W=256):
+    """n forward passes of the network on a random volume of the benchmark's
+    shape (frames as the batch); prints the mean time and TFLOP/s."""
+    from phasecon_amd.warmstart import UNet
+    net = UNet(channels=c, seed=0, in_channels=in_ch)
+    x = np.random.default_rng(0).random((N, D, H, W, in_ch)).astype(np.float32)
+    net.forward(x)  # allocates
+    t0 = time.perf_counter()
+    for _ in range(n):
+        net.forward(x)
+    dt = (time.perf_counter() - t0) / n
+    net.close()
+    shapes = [(in_ch, c), (c, c), (c, 2 * c), (2 * c, 2 * c), (2 * c, 4 * c), (4 * c, 4 * c),
+              (6 * c, 2 * c), (2 * c, 2 * c), (3 * c, c), (c, c), (c, 1)]
+    lev = [0, 0, 1, 1, 2, 2, 1, 1, 0, 0, 0]
+    V = N * D * H * W
+    macs = sum(27 * ci * co * V // 8 ** l for (ci, co), l in zip(shapes, lev))
+    print(json.dumps({"profile_unet": n, "shape": [N, D, H, W, in_ch], "channels": c,
+                      "ms_per_forward": dt * 1e3, "gmac": macs / 1e9,
+                      "tflops_incl_transfers": 2 * macs / dt / 1e12}))
 
 
 def cpu_baseline(scan, equits=2):
@@ -262,6 +398,16 @@ def main():
                     help="over-relaxation factor of the full-resolution sweeps "
                          "(the reference's is 1.5)")
     ap.add_argument("--coarse-alpha", type=float, default=1.9)
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned warm start leg")
+    ap.add_argument("--learned-scans", type=int, default=8)
+    ap.add_argument("--learned-steps", type=int, default=3000)
+    ap.add_argument("--learned-channels", type=int, default=16)
+    ap.add_argument("--profile-unet", type=int, default=0,
+                    help="only time N forward passes of the warm-start network")
+    ap.add_argument("--learned-coarse-equits", default="0,60",
+                    help="comma list: equits of the two-grid correction of the learned start")
+    ap.add_argument("--learned-input", choices=("fbp3", "fbp2", "fbp", "bp"), default="fbp2")
     ap.add_argument("--afp16", action="store_true",
                     help="store A in FP16 with a per-column FP32 scale")
     ap.add_argument("--no-cpu", action="store_true")
@@ -276,6 +422,9 @@ def main():
     ap.add_argument("--pcmbir-views", type=int, default=360)
     ap.add_argument("--pcmbir-n", type=int, default=512)
     args = ap.parse_args()
+    if args.profile_unet and args.gpus == 1:
+        profile_unet(args.profile_unet)
+        return
     if args.pcmbir and args.gpus == 1:
         pcmbir_bench(args.pcmbir_views, args.pcmbir_n, max(args.steps, 1), nms=True)
         return
@@ -288,6 +437,17 @@ def main():
     from phasecon_amd.mbir import Stage
     from phasecon_amd.lib import K_SWEEP, K_COST, K_AMATRIX, K_FWDPROJ
 
+    learned = not (args.no_learned or args.profile_equits or args.longrun)
+    train_scans = []
+    if learned:
+        # training scans (other seeds) on the host, before the GPU is used
+        import multiprocessing as mp
+        t0 = time.time()
+        with mp.get_context("fork").Pool(min(args.learned_scans, 8)) as pool:
+            train_scans = pool.map(_make_train_scan,
+                                   [(args.workload, s) for s in range(1, args.learned_scans + 1)])
+        print(f"[bench] {len(train_scans)} training scans generated in {time.time() - t0:.1f}s",
+              file=sys.stderr, flush=True)
     t0 = time.time()
     scan = make_scan(args.workload, seed=0)
     print(f"[bench] scan {args.workload} generated in {time.time() - t0:.1f}s", file=sys.stderr,
@@ -430,6 +590,17 @@ def main():
             "avg_launch_ms": avg_launch_ms,
         },
     }
+    if learned:
+        try:
+            out["learned"] = learned_leg(args, st, train_scans, target)
+            lv = out["learned"]["value"]
+            if lv and value:
+                out["learned"]["vs_multires"] = value / lv
+            z = out["learned"]["zero_start"]["value"]
+            if lv and z:
+                out["learned"]["vs_zero_start"] = z / lv
+        except Exception as exc:  # the headline line must still be printed
+            out["learned"] = {"error": f"{type(exc).__name__}: {exc}"}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scan, args.cpu_equits)
         cb = out["cpu_baseline"]

@@ -123,6 +123,14 @@ int mbir_hip_backproject(mbir_hip_ctx* ctx, const float* v, float* bt);
  * (mbir_hip_build_amatrix); the projection A m is computed on the fly. */
 int mbir_hip_bp_normalized(mbir_hip_ctx* ctx, float* out);
 
+/* Filtered back-projection, the learned warm start's default input (no
+ * counterpart in the reference): out[t][z][y][x] = A^T(h * (y - d)) / K
+ * over each frame's views, h the Ram-Lak filter along the detector channels
+ * (h(0) = 1/4, h(n) = -1/(pi n)^2 for odd n, 0 for even n != 0) and K the
+ * mean over the update mask of A^T(h * A m), so that an object uniform
+ * inside the mask reconstructs to its value.  Needs A. */
+int mbir_hip_fbp(mbir_hip_ctx* ctx, float* out);
+
 /* n equits of the parallel ICD (updateVoxelsTimeSlices,
  * XT_ICD_update.c:932-1092, with the voxel update of XT_VoxUpdate.c:45-283
  * and the substitution of XT_ICD_update.c:133-182).  Iteration numbers run

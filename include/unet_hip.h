@@ -3,18 +3,21 @@
  *
  * No counterpart in the reference: the reference starts every
  * multi-resolution stage from zero or from the up-sampled coarser stage
- * (XT_ICD_update.c:1127-1180).  This network maps the normalised
- * back-projection of a scan (mbir_hip_bp_normalized) to an estimate of the
- * converged MBIR image; the solver then starts from it and still runs to
- * convergence, so the estimate changes the work, not the result.
+ * (XT_ICD_update.c:1127-1180).  This network maps images computed directly
+ * from a scan (the filtered back-projection of each frame and its mean over
+ * the frames, mbir_hip_fbp; or the normalised back-projection,
+ * mbir_hip_bp_normalized) to an estimate of the converged MBIR image; the
+ * solver then starts from it and still runs to convergence, so the estimate
+ * changes the work, not the result.
  *
  * Network: 3-level 3D U-Net, channels c / 2c / 4c, two 3x3x3 convolutions
  * (zero padding, ReLU) per level, 2x2x2 average pooling, nearest 2x
  * up-sampling, skips concatenated ([up-sampled | skip] channels), a 3x3x3
- * output convolution and a residual connection: y = x + net(x).  BF16
- * weights and activations, FP32 accumulation (MFMA), FP32 master weights,
- * Adam.  Volumes are [N][D][H][W] FP32 in host memory, D and H divisible
- * by 4, W by 64.
+ * output convolution and a residual connection: y = x_0 + net(x), x_0 the
+ * first input channel.  BF16 weights and activations, FP32 accumulation
+ * (MFMA), FP32 master weights, Adam.  Inputs are [N][D][H][W][in_channels]
+ * FP32 in host memory (channels last), outputs and targets [N][D][H][W];
+ * D and H divisible by 4, W by 64.
  *
  * Parameter order (unet_get_params / unet_set_params / unet_gradients): per
  * layer (enc1a, enc1b, enc2a, enc2b, bott_a, bott_b, dec2a, dec2b, dec1a,
@@ -36,18 +39,19 @@ typedef struct unet_ctx unet_ctx;
 
 const char* unet_last_error(void);
 
-/* channels c in {8, 16, 24, 32}; He-normal initial weights from seed. */
-int unet_create(unet_ctx** out, int32_t channels, uint32_t seed);
+/* channels c in {8, 16, 24, 32}, in_channels in [1, 8]; He-normal initial
+ * weights from seed. */
+int unet_create(unet_ctx** out, int32_t channels, int32_t in_channels, uint32_t seed);
 void unet_destroy(unet_ctx* ctx);
 int64_t unet_param_count(const unet_ctx* ctx);
 int unet_get_params(unet_ctx* ctx, float* params);
 int unet_set_params(unet_ctx* ctx, const float* params);
 
-/* y = x + net(x) for a batch [N][D][H][W]. */
+/* y = x_0 + net(x) for a batch of N volumes D x H x W. */
 int unet_forward(unet_ctx* ctx, int32_t N, int32_t D, int32_t H, int32_t W, const float* x,
                  float* y);
 
-/* Loss mean((x + net(x) - t)^2) and its gradient with respect to every
+/* Loss mean((x_0 + net(x) - t)^2) and its gradient with respect to every
  * parameter (in the parameter order above), without an update. */
 int unet_gradients(unet_ctx* ctx, int32_t N, int32_t D, int32_t H, int32_t W, const float* x,
                    const float* t, float* grads, double* loss);

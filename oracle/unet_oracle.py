@@ -9,8 +9,8 @@ start); the oracle restates its definition in numpy float64:
     with zero padding, weights [cout][27][cin], tap = (dz+1)*9 + (dy+1)*3 +
     (dx+1); activations [N][D][H][W][C];
   * avgpool2 / up2 and their adjoints;
-  * unet_forward_backward: the 11-layer network, loss mean((x + net(x) -
-    t)^2) and the gradient of every parameter.  With bf16=True every value
+  * unet_forward_backward: the 11-layer network, loss mean((x_0 + net(x) -
+    t)^2) (x_0 the first input channel) and the gradient of every parameter.  With bf16=True every value
     the GPU stores in BF16 (the input, the packed weights, every activation
     and activation gradient) is rounded to BF16 (round to nearest even) at
     the same point, so the GPU's FP32 accumulation is the only difference.
@@ -29,16 +29,16 @@ LAYERS = ("enc1a", "enc1b", "enc2a", "enc2b", "bott_a", "bott_b", "dec2a", "dec2
           "dec1a", "dec1b", "out")
 
 
-def layer_shapes(c):
+def layer_shapes(c, in_channels=1):
     """(cin, cout, relu) of the 11 layers for base channels c."""
     c1, c2, c3 = c, 2 * c, 4 * c
-    return [(1, c1, 1), (c1, c1, 1), (c1, c2, 1), (c2, c2, 1), (c2, c3, 1), (c3, c3, 1),
+    return [(in_channels, c1, 1), (c1, c1, 1), (c1, c2, 1), (c2, c2, 1), (c2, c3, 1), (c3, c3, 1),
             (c3 + c2, c2, 1), (c2, c2, 1), (c2 + c1, c1, 1), (c1, c1, 1), (c1, 1, 0)]
 
 
-def unflatten(flat, c):
+def unflatten(flat, c, in_channels=1):
     out, i = [], 0
-    for cin, cout, _ in layer_shapes(c):
+    for cin, cout, _ in layer_shapes(c, in_channels):
         nw = cout * 27 * cin
         w = np.asarray(flat[i:i + nw], np.float64).reshape(cout, 27, cin)
         i += nw
@@ -120,14 +120,16 @@ def up2_bwd(g):
 
 
 def unet_forward_backward(params, x, t, c, bf16_storage=True, grads=True):
-    """x, t: [N][D][H][W].  Returns (loss, grads as a list of (dw, db), y)
-    with y = x + net(x)."""
+    """x: [N][D][H][W] or [N][D][H][W][in_channels], t: [N][D][H][W].
+    Returns (loss, grads as a list of (dw, db), y) with y = x_0 + net(x),
+    x_0 the first input channel."""
     r = bf16 if bf16_storage else (lambda a: np.asarray(a, np.float64))
     relu = lambda a: np.maximum(a, 0.0)
     W_ = [(r(w), b) for w, b in params]
     x = np.asarray(x, np.float64)
-    X0 = r(x)[..., None]
-    acts = {}
+    if x.ndim == 4:
+        x = x[..., None]
+    X0 = r(x)
     A1 = r(relu(conv3d(X0, *W_[0])))
     S1 = r(relu(conv3d(A1, *W_[1])))
     P1 = r(avgpool2(S1))
@@ -143,7 +145,7 @@ def unet_forward_backward(params, x, t, c, bf16_storage=True, grads=True):
     A5 = r(relu(conv3d(C1, *W_[8])))
     B5 = r(relu(conv3d(A5, *W_[9])))
     net = conv3d(B5, *W_[10])[..., 0]
-    y = net + x
+    y = net + x[..., 0]
     d = y - np.asarray(t, np.float64)
     loss = float(np.mean(d * d))
     if not grads:
@@ -175,5 +177,4 @@ def unet_forward_backward(params, x, t, c, bf16_storage=True, grads=True):
     GS1 = r(GC1[..., c2:] + avgpool2_bwd(GP1))
     GA1 = layer(1, GS1, S1, A1)
     layer(0, GA1, A1, X0, need_in=False)
-    del acts
     return loss, G, y

@@ -1428,6 +1428,58 @@ __global__ __launch_bounds__(256) void k_bpnorm(DevGeom G, const float* __restri
   }
 }
 
+// Ram-Lak filter along the detector channels of every (view, row):
+// out[p][r][t] = sum_r' h(r - r') (v[p][r'][t] - d[r'][t]), h(0) = 1/4,
+// h(n) = -1/(pi n)^2 for odd n, 0 for even n != 0 (the band-limited ramp
+// sampled at the channel spacing).  One thread per output.
+__global__ __launch_bounds__(256) void k_ramp(DevGeom G, const float* __restrict__ v,
+                                              const float* __restrict__ d,
+                                              float* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n = (long long)G.N_p * G.N_r * G.N_t;
+  if (i >= n) return;
+  const int t = (int)(i % G.N_t);
+  const int r = (int)((i / G.N_t) % G.N_r);
+  const long long p = i / ((long long)G.N_t * G.N_r);
+  const float* vp = v + (size_t)p * G.N_r * G.N_t;
+  double s = 0.0;
+  for (int q = 0; q < G.N_r; ++q) {
+    const int k = r - q;
+    double h;
+    if (k == 0)
+      h = 0.25;
+    else if (k & 1)
+      h = -1.0 / (M_PI * M_PI * (double)k * (double)k);
+    else
+      continue;
+    s += h * ((double)vp[(size_t)q * G.N_t + t] - (d ? (double)d[(size_t)q * G.N_t + t] : 0.0));
+  }
+  out[i] = (float)s;
+}
+
+// per-block sums of x over masked voxels (column-major object), FP64
+__global__ void k_masked_sum(int Nt, int ncol, int Nz, const uint8_t* __restrict__ mask,
+                             const float* __restrict__ x, double* __restrict__ partial) {
+  __shared__ double red[256];
+  const long long n = (long long)Nt * ncol * Nz;
+  double s = 0.0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (long long)gridDim.x * 256)
+    if (mask[(i / Nz) % ncol]) s += x[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void k_scale(float* x, long long n, float s) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] *= s;
+}
+
 // the update mask's indicator, column-major [frame][column][z]
 __global__ void k_mask_ones(int Nt, int ncol, int Nz, const uint8_t* __restrict__ mask,
                             float* __restrict__ out) {
@@ -1631,6 +1683,7 @@ struct mbir_hip_ctx {
   float *d_y = nullptr, *d_w = nullptr, *d_d = nullptr, *d_e = nullptr;
   float *d_x = nullptr, *d_stage = nullptr, *d_stage2 = nullptr;
   uint8_t* d_mask = nullptr;  // update mask [N_y * N_x]
+  float* d_ws = nullptr;      // warm-start sinogram scratch (allocated on first use)
   int* d_afirst = nullptr;
   float* d_aval = nullptr;
   bool afp16 = false;                 // the sweep streams FP16 A + column scales
@@ -1887,7 +1940,7 @@ void mbir_hip_destroy(mbir_hip_ctx* c) {
                   c->d_view_frame, c->d_y, c->d_w, c->d_d, c->d_e, c->d_x,
                   c->d_stage, c->d_stage2, c->d_afirst, c->d_aval, c->d_tile_ptr,
                   c->d_tile_cols, c->d_launch_tiles, c->d_stats, c->d_partial, c->d_mask,
-                  c->d_aval16, c->d_ascale};
+                  c->d_aval16, c->d_ascale, c->d_ws};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (int k = 0; k < MBIR_K_COUNT; ++k)
@@ -2065,6 +2118,47 @@ int mbir_hip_bp_normalized(mbir_hip_ctx* c, float* out) {
                      c->d_d, c->d_stage, c->d_stage2);
   hipLaunchKernelGGL(k_from_colmajor, dim3(blocks), dim3(256), 0, c->stream, c->g.N_time,
                      c->g.N_z, c->ncol, c->d_stage2, c->d_stage);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(out, c->d_stage, c->nvox * sizeof(float), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mbir_hip_fbp(mbir_hip_ctx* c, float* out) {
+  if (!c || !out) return fail("null argument");
+  if (!c->a_built) return fail("A not built");
+  if (!c->d_ws && dev_alloc(&c->d_ws, c->nsino)) return -1;
+  const long long ncf = (long long)c->ncol * c->g.N_time;
+  const unsigned bo = (unsigned)((c->nvox + 255) / 256);
+  const unsigned bs = (unsigned)((c->nsino + 255) / 256);
+  const unsigned bc = (unsigned)((ncf + 255) / 256);
+  // K = mean over the mask of A^T ramp(A m): what a uniform object of value 1
+  // inside the mask reconstructs to
+  hipLaunchKernelGGL(k_mask_ones, dim3(bo), dim3(256), 0, c->stream, c->g.N_time, c->ncol,
+                     c->g.N_z, c->d_mask, c->d_stage2);
+  if (run_fwdproj(c, 2, c->d_stage2, c->d_stage)) return -1;
+  hipLaunchKernelGGL(k_ramp, dim3(bs), dim3(256), 0, c->stream, c->G, c->d_stage, nullptr, c->d_ws);
+  hipLaunchKernelGGL(k_backproj, dim3(bc), dim3(256), 0, c->stream, c->G, c->d_aval, c->d_afirst,
+                     c->d_frame_ptr, c->d_frame_views, c->d_ws, c->d_stage2);
+  const unsigned nb = 1024;
+  if (ensure_partial(c, nb)) return -1;
+  hipLaunchKernelGGL(k_masked_sum, dim3(nb), dim3(256), 0, c->stream, c->g.N_time, c->ncol,
+                     c->g.N_z, c->d_mask, c->d_stage2, c->d_partial);
+  double sum = 0.0;
+  if (sum_partials(c, nb, &sum)) return -1;
+  size_t nmask = 0;
+  for (uint8_t m : c->mask) nmask += m ? 1 : 0;
+  const double K = sum / ((double)nmask * c->g.N_z * c->g.N_time);
+  if (!(K > 0.0)) return fail("filtered back-projection of the mask is not positive");
+  // A^T ramp(y - d) / K
+  hipLaunchKernelGGL(k_ramp, dim3(bs), dim3(256), 0, c->stream, c->G, c->d_y, c->d_d, c->d_ws);
+  hipLaunchKernelGGL(k_backproj, dim3(bc), dim3(256), 0, c->stream, c->G, c->d_aval, c->d_afirst,
+                     c->d_frame_ptr, c->d_frame_views, c->d_ws, c->d_stage2);
+  hipLaunchKernelGGL(k_scale, dim3(bo), dim3(256), 0, c->stream, c->d_stage2, (long long)c->nvox,
+                     (float)(1.0 / K));
+  hipLaunchKernelGGL(k_from_colmajor, dim3(bo), dim3(256), 0, c->stream, c->g.N_time, c->g.N_z,
+                     c->ncol, c->d_stage2, c->d_stage);
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(out, c->d_stage, c->nvox * sizeof(float), hipMemcpyDeviceToHost,
                         c->stream));

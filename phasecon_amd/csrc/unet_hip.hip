@@ -327,14 +327,14 @@ __global__ void k_bias_grad(const u16* g, int cs, int co, int C, long long V, fl
   if (threadIdx.x == 0) atomicAdd(&db[c], red[0]);
 }
 
-// pred = net + x; grad channel 0 := 2 (pred - t) / V (others 0); partial sums
-// of (pred - t)^2 per block
-__global__ void k_mse(const float* net, const float* x, const float* t, long long V, u16* g,
-                      int cs_g, double* partial) {
+// pred = net + x (input channel 0 of cin); grad channel 0 := 2 (pred - t) / V
+// (others 0); partial sums of (pred - t)^2 per block
+__global__ void k_mse(const float* net, const float* x, int cin, const float* t, long long V,
+                      u16* g, int cs_g, double* partial) {
   __shared__ double red[256];
   double s = 0.0;
   for (long long v = (long long)blockIdx.x * 256 + threadIdx.x; v < V; v += (long long)gridDim.x * 256) {
-    const float d = net[v] + x[v] - t[v];
+    const float d = net[v] + x[v * cin] - t[v];
     s += (double)d * d;
     for (int c = 0; c < cs_g; ++c) g[v * cs_g + c] = c == 0 ? f2bf(2.f * d / (float)V) : (u16)0;
   }
@@ -347,17 +347,19 @@ __global__ void k_mse(const float* net, const float* x, const float* t, long lon
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-__global__ void k_add(float* y, const float* x, long long n) {
+// y[v] += x[v][0] (the residual: input channel 0 of cin)
+__global__ void k_add(float* y, const float* x, int cin, long long n) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[i] += x[i];
+  if (i < n) y[i] += x[i * cin];
 }
 
-// x (FP32, [v]) into channel 0 of a BF16 buffer of stride cs (others 0)
-__global__ void k_in_pack(const float* x, long long V, u16* out, int cs) {
+// x (FP32, [v][cin]) into channels 0 .. cin-1 of a BF16 buffer of stride cs
+// (the others 0)
+__global__ void k_in_pack(const float* x, int cin, long long V, u16* out, int cs) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= V * cs) return;
   const int c = (int)(i % cs);
-  out[i] = c == 0 ? f2bf(x[i / cs]) : (u16)0;
+  out[i] = c < cin ? f2bf(x[(i / cs) * cin + c]) : (u16)0;
 }
 
 __global__ void k_adam(float* p, const float* g, float* m, float* v, long long n, float lr,
@@ -525,7 +527,7 @@ int launch_dw(hipStream_t s, const Chan& g, const Chan& in, float* dw, int cin_p
 // ------------------------------------------------------------------ network
 struct unet_ctx {
   hipStream_t s = nullptr;
-  int c = 16;
+  int c = 16, cin = 1;  // base channels, input channels
   Layer L[11];
   // shape the activation buffers hold
   int N = 0, D = 0, H = 0, W = 0;
@@ -590,7 +592,7 @@ int set_shape(unet_ctx* c, int N, int D, int H, int W) {
   if (alloc_act(c, &c->P2, V3 * c2) || alloc_act(c, &c->GP2, V3 * c2)) return -1;
   float** f32s[] = {&c->x32, &c->t32, &c->o32};
   for (float** b : f32s) {
-    if (dalloc(b, (size_t)V1)) return -1;
+    if (dalloc(b, (size_t)V1 * (b == &c->x32 ? c->cin : 1))) return -1;
     c->acts.push_back(*b);
   }
   return 0;
@@ -610,8 +612,8 @@ int forward(unet_ctx* c) {
       C1{c->C1, c2 + c1, 0}, P1{c->P1, c1, 0}, A2{c->A2, c2, 0}, S2{c->C2, c3 + c2, c3},
       U2{c->C2, c3 + c2, 0}, C2{c->C2, c3 + c2, 0}, P2{c->P2, c2, 0}, A3{c->A3, c3, 0},
       B3{c->B3, c3, 0}, A4{c->A4, c2, 0}, B4{c->B4, c2, 0}, A5{c->A5, c1, 0}, B5{c->B5, c1, 0};
-  hipLaunchKernelGGL(k_in_pack, dim3(blocks_for(c->V1 * 8)), dim3(256), 0, c->s, c->x32, c->V1,
-                     c->X0, 8);
+  hipLaunchKernelGGL(k_in_pack, dim3(blocks_for(c->V1 * 8)), dim3(256), 0, c->s, c->x32, c->cin,
+                     c->V1, c->X0, 8);
   if (conv_layer(c, 0, X0, &A1, nullptr, 0) || conv_layer(c, 1, A1, &S1, nullptr, 0)) return -1;
   hipLaunchKernelGGL(k_pool, dim3(blocks_for(c->V2 * c1)), dim3(256), 0, c->s, S1.p, S1.cs, S1.co,
                      P1.p, P1.cs, P1.co, c1, N, D, H, W);
@@ -667,8 +669,8 @@ int backward(unet_ctx* c, double* loss) {
     HIP_OK(hipMemsetAsync(l.gw, 0, (size_t)l.cout * 27 * l.cin_pad * sizeof(float), c->s));
     HIP_OK(hipMemsetAsync(l.gb, 0, (size_t)l.cout * sizeof(float), c->s));
   }
-  hipLaunchKernelGGL(k_mse, dim3(kLossBlocks), dim3(256), 0, c->s, c->o32, c->x32, c->t32, c->V1,
-                     c->GO, 8, c->partial);
+  hipLaunchKernelGGL(k_mse, dim3(kLossBlocks), dim3(256), 0, c->s, c->o32, c->x32, c->cin,
+                     c->t32, c->V1, c->GO, 8, c->partial);
   if (layer_bwd(c, 10, GO, GO, B5, &GB5, 0) || layer_bwd(c, 9, GB5, B5, A5, &GA5, 0) ||
       layer_bwd(c, 8, GA5, A5, C1, &GC1, 0))
     return -1;
@@ -702,7 +704,7 @@ int backward(unet_ctx* c, double* loss) {
 }
 
 int upload_pair(unet_ctx* c, const float* x, const float* t) {
-  HIP_OK(hipMemcpyAsync(c->x32, x, c->V1 * sizeof(float), hipMemcpyHostToDevice, c->s));
+  HIP_OK(hipMemcpyAsync(c->x32, x, c->V1 * c->cin * sizeof(float), hipMemcpyHostToDevice, c->s));
   if (t) HIP_OK(hipMemcpyAsync(c->t32, t, c->V1 * sizeof(float), hipMemcpyHostToDevice, c->s));
   return 0;
 }
@@ -713,14 +715,16 @@ extern "C" {
 
 const char* unet_last_error(void) { return g_err.c_str(); }
 
-int unet_create(unet_ctx** out, int32_t channels, uint32_t seed) {
+int unet_create(unet_ctx** out, int32_t channels, int32_t in_channels, uint32_t seed) {
   if (!out) return fail("null argument");
   *out = nullptr;
   if (channels < 8 || channels > 32 || channels % 8) return fail("channels must be 8, 16, 24 or 32");
+  if (in_channels < 1 || in_channels > 8) return fail("in_channels must be in [1, 8]");
   unet_ctx* c = new unet_ctx();
   c->c = channels;
+  c->cin = in_channels;
   const int c1 = channels, c2 = 2 * channels, c3 = 4 * channels;
-  const int io[11][3] = {{1, c1, 1},  {c1, c1, 1},      {c1, c2, 1}, {c2, c2, 1},
+  const int io[11][3] = {{in_channels, c1, 1},  {c1, c1, 1},      {c1, c2, 1}, {c2, c2, 1},
                          {c2, c3, 1}, {c3, c3, 1},      {c3 + c2, c2, 1}, {c2, c2, 1},
                          {c2 + c1, c1, 1}, {c1, c1, 1}, {c1, 1, 0}};
   if (hipStreamCreate(&c->s) != hipSuccess) {
@@ -800,7 +804,8 @@ int unet_forward(unet_ctx* c, int32_t N, int32_t D, int32_t H, int32_t W, const 
                  float* y) {
   if (!c || !x || !y) return fail("null argument");
   if (set_shape(c, N, D, H, W) || upload_pair(c, x, nullptr) || forward(c)) return -1;
-  hipLaunchKernelGGL(k_add, dim3(blocks_for(c->V1)), dim3(256), 0, c->s, c->o32, c->x32, c->V1);
+  hipLaunchKernelGGL(k_add, dim3(blocks_for(c->V1)), dim3(256), 0, c->s, c->o32, c->x32, c->cin,
+                     c->V1);
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(y, c->o32, c->V1 * sizeof(float), hipMemcpyDeviceToHost, c->s));
   HIP_OK(hipStreamSynchronize(c->s));

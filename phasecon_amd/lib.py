@@ -71,9 +71,10 @@ _SIGS = {
     "mbir_hip_reset_kernel_time": (C.c_int, [C.c_void_p]),
     "mbir_hip_synchronize": (C.c_int, [C.c_void_p]),
     "mbir_hip_bp_normalized": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "mbir_hip_fbp": (C.c_int, [C.c_void_p, C.c_void_p]),
     # learned warm start (include/unet_hip.h)
     "unet_last_error": (C.c_char_p, []),
-    "unet_create": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32]),
+    "unet_create": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_uint32]),
     "unet_destroy": (None, [C.c_void_p]),
     "unet_param_count": (C.c_int64, [C.c_void_p]),
     "unet_get_params": (C.c_int, [C.c_void_p, C.c_void_p]),

@@ -152,6 +152,14 @@ class Stage:
         _lib.check(self.L.mbir_hip_bp_normalized(self.h, _p(out)), "bp_normalized")
         return out
 
+    def fbp(self):
+        """Filtered back-projection (mbir_hip_fbp), [N_time][N_z][N_y][N_x],
+        scaled so an object uniform inside the mask keeps its value; needs
+        build_amatrix()."""
+        out = np.zeros(self.shape_x, np.float32)
+        _lib.check(self.L.mbir_hip_fbp(self.h, _p(out)), "fbp")
+        return out
+
     def init_error(self):
         _lib.check(self.L.mbir_hip_init_error(self.h), "init_error")
 

@@ -68,21 +68,22 @@ def test_conv_layer_matches_oracle(cin, cout):
     assert np.max(np.abs(dw - ref)) <= 2e-5 * np.max(np.abs(ref))
 
 
-def test_unet_gradients_match_oracle():
+@pytest.mark.parametrize("in_ch", [1, 2])
+def test_unet_gradients_match_oracle(in_ch):
     from phasecon_amd.warmstart import UNet
     c = 8
-    net = UNet(channels=c, seed=3)
+    net = UNet(channels=c, seed=3, in_channels=in_ch)
     rng = np.random.default_rng(1)
     params = []
-    for cin, cout, _ in U.layer_shapes(c):
+    for cin, cout, _ in U.layer_shapes(c, in_ch):
         params.append((rng.standard_normal((cout, 27, cin)) * np.sqrt(2.0 / (27 * cin)),
                        rng.standard_normal(cout) * 0.05))
     flat = U.flatten(params).astype(np.float32)
     net.set_params(flat)
     assert np.array_equal(net.params(), flat)
-    params32 = U.unflatten(flat.astype(np.float64), c)
-    x = rng.random((2, 4, 8, 64)).astype(np.float32)
-    t = (x + 0.2 * rng.standard_normal(x.shape)).astype(np.float32)
+    params32 = U.unflatten(flat.astype(np.float64), c, in_ch)
+    x = rng.random((2, 4, 8, 64, in_ch)).astype(np.float32)
+    t = (x[..., 0] + 0.2 * rng.standard_normal(x.shape[:4])).astype(np.float32)
     loss, g = net.gradients(x, t)
     oloss, G, oy = U.unet_forward_backward(params32, x, t, c, bf16_storage=True)
     y = net.forward(x)

@@ -40,12 +40,64 @@ def test_bp_normalized_formula(small4d_scan):
     assert np.max(np.abs(bp - ref)[inside]) <= 1e-5 * np.max(np.abs(ref[inside]))
 
 
+def _ramlak(n):
+    k = np.arange(-(n - 1), n)
+    h = np.zeros(k.size)
+    h[k == 0] = 0.25
+    odd = (k % 2) != 0
+    h[odd] = -1.0 / (np.pi * k[odd]) ** 2
+    return h
+
+
+def _ramp(v):
+    """Ram-Lak filter along the channel axis r of a [p][r][t] sinogram."""
+    h = _ramlak(v.shape[1])
+    out = np.zeros(v.shape)
+    for p in range(v.shape[0]):
+        for t in range(v.shape[2]):
+            full = np.convolve(v[p, :, t].astype(np.float64), h)
+            out[p, :, t] = full[v.shape[1] - 1:2 * v.shape[1] - 1]
+    return out
+
+
+def test_fbp_formula_and_uniform_object(small4d_scan):
+    """fbp = A^T(h * y) / K with h the Ram-Lak filter (numpy convolution)
+    and K the mask mean of A^T(h * A m); an exact scan of an object uniform
+    inside the mask reconstructs to a mask mean equal to its value."""
+    from phasecon_amd.mbir import Stage
+    st = Stage(small4d_scan)
+    st.build_amatrix()
+    fbp = st.fbp().astype(np.float64)
+    m = np.broadcast_to(st.mask[None, None].astype(np.float32), st.shape_x).copy()
+    am = st.forward_project(m)
+    K = st.backproject(_ramp(am).astype(np.float32))[m > 0].astype(np.float64).mean()
+    ref = st.backproject(_ramp(st.y).astype(np.float32)).astype(np.float64) / K
+    inside = m > 0
+    err = np.max(np.abs(fbp - ref)[inside]) / np.max(np.abs(ref[inside]))
+    print(f"fbp vs numpy ramp: {err:.3g}; range {fbp[inside].min():.4g} .. "
+          f"{fbp[inside].max():.4g}", flush=True)
+    assert err <= 1e-4
+    # exact projections of 0.013 m: the mask mean of the reconstruction is 0.013
+    scan = dict(small4d_scan)
+    c = 0.013
+    N_p, N_t, N_r = scan["proj_num"], scan["proj_rows"], scan["proj_cols"]
+    scan["projections"] = np.ascontiguousarray(
+        (c * am).reshape(N_p, N_r, N_t).transpose(0, 2, 1)).astype(np.float32)
+    su = Stage(scan)
+    su.build_amatrix()
+    fu = su.fbp().astype(np.float64)
+    su.close()
+    st.close()
+    print(f"uniform {c}: mask mean {fu[inside].mean():.6g}", flush=True)
+    assert abs(fu[inside].mean() - c) <= 1e-5 * c
+
+
 def test_learned_start_saves_equits(small4d_scan):
     from phasecon_amd.mbir import Stage
     from phasecon_amd.synth import make_scan
     from phasecon_amd.warmstart import UNet, learned_start, train, training_pair
     pairs = [training_pair(make_scan("ch4d_small", seed=s), equits=300) for s in (10, 11, 12, 13)]
-    net = UNet(channels=16, seed=0)
+    net = UNet(channels=16, seed=0, in_channels=2)
     hist = train(net, pairs, steps=300, batch=8, size=64, lr=1e-3, seed=0)
     print(f"training loss {hist[:10].mean():.4g} -> {hist[-10:].mean():.4g}", flush=True)
     assert hist[-10:].mean() < 0.5 * hist[:10].mean()
