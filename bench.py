@@ -391,6 +391,10 @@ def main():
                     help="equits of the coarse stages of the multi-resolution schedule, "
                          "coarsest first: one count for every stage or a comma list")
     ap.add_argument("--coarse-tile", type=int, default=4)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep --tile / --coarse-tile instead of timing candidates per stage")
+    ap.add_argument("--stages", default="",
+                    help="explicit coarse stages, coarsest first: mult_xy:mult_z:equits,...")
     ap.add_argument("--max-equits", type=int, default=3000)
     ap.add_argument("--seed", type=int, default=761521)
     ap.add_argument("--refresh", type=int, default=10)
@@ -458,6 +462,10 @@ def main():
         from phasecon_amd.mbir import multires_schedule
         stages = multires_schedule(scan["proj_rows"], scan["proj_cols"])[:-1]
         counts = [int(v) for v in str(args.coarse_equits).split(",")]
+        if args.stages:  # explicit coarse stages "mxy:mz:equits,..."
+            spec = [tuple(int(v) for v in item.split(":")) for item in args.stages.split(",")]
+            stages = [(a, b) for a, b, _ in spec]
+            counts = [n for _, _, n in spec]
         counts = (counts * len(stages))[:len(stages)] if len(counts) == 1 else counts
         if len(counts) != len(stages):
             sys.exit(f"bench.py: --coarse-equits needs 1 or {len(stages)} counts")
@@ -467,6 +475,23 @@ def main():
             if n > 0 or coarse:
                 coarse.append((Stage(scan, mult_xy=mxy, mult_z=mz, tile=args.coarse_tile,
                                      afp16=args.afp16, alpha=args.coarse_alpha), n))
+
+    tuned = {}
+
+    def tile_of(s_):
+        key = f"{s_.g.N_x}x{s_.g.N_y}x{s_.g.N_z}"
+        return tuned[key]["tile"] if key in tuned else (args.tile if s_ is st else args.coarse_tile)
+
+    if not (args.no_autotune or args.profile_equits or args.longrun):
+        # tile of every stage by timing a few equits at each candidate (untimed)
+        from phasecon_amd.autotune import tune_tile
+        for s_ in [cs for cs, n in coarse if n > 0] + [st]:
+            s_.set_object(np.zeros(s_.shape_x, np.float32))
+            s_.build_amatrix()
+            best, tms = tune_tile(s_)
+            tuned[f"{s_.g.N_x}x{s_.g.N_y}x{s_.g.N_z}"] = {
+                "tile": best, "ms_per_launch": {str(k): round(v, 4) for k, v in tms.items()}}
+        print(f"[bench] autotuned tiles: {tuned}", file=sys.stderr, flush=True)
 
     if args.profile_equits:
         st.set_object(np.zeros(st.shape_x, np.float32))
@@ -558,12 +583,13 @@ def main():
             "workload": (f"{args.workload}: {w['cols']}x{w['cols']}x{w['rows']} voxels x {w['frames']} "
                          f"frames, {w['views']} views" + (f" interlaced K={w['K']}" if w['K'] else "")
                          + f", I0={w['I0']:g}, qGGMRF p_s={w['p_s']} p_t={w['p_t']} q=2, from zero"),
-            "tile": args.tile,
+            "tile": tile_of(st),
+            "autotuned_tiles": tuned or None,
             "alpha": args.alpha,
             "coarse_alpha": args.coarse_alpha,
             "a_storage": "fp16 + fp32 column scale" if args.afp16 else "fp32",
             "schedule": (("multires: " + " -> ".join(
-                f"{cs.g.N_x}x{cs.g.N_y}x{cs.g.N_z} ({n} equits, tile {args.coarse_tile})"
+                f"{cs.g.N_x}x{cs.g.N_y}x{cs.g.N_z} ({n} equits, tile {tile_of(cs)})"
                 for cs, n in coarse) + f" -> {st.g.N_x}x{st.g.N_y}x{st.g.N_z} to target, "
                 "up-sampled on the device with out-of-mask voxels zeroed")
                 if coarse else "single full-resolution stage from zero"),
@@ -584,7 +610,7 @@ def main():
             "peak": PEAK_HBM_GBPS,
             "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBPS,
-            "traffic": MEASURED_TRAFFIC.get((args.workload, args.tile, args.afp16)),
+            "traffic": MEASURED_TRAFFIC.get((args.workload, tile_of(st), args.afp16)),
             "kernel": "k_sweep",
             "bytes_per_launch": bytes_launch,
             "avg_launch_ms": avg_launch_ms,
