@@ -19,7 +19,8 @@ in XT_Constants.h): any factor in (0, 2) keeps the same fixed point, and 1.9
 with 60 + 100 coarse equits was the fastest measured setting.  Every stage's A build and error sinogram are inside the
 timed region, as are the parallel ICD equits and a cost check after each
 full-resolution equit (maintained error sinogram, FP32 prior potentials
-summed in FP64).  Inputs are resident in HBM before the timed region.
+summed in FP64).  Each equit's 8 sweep launches are one replay of a captured
+HIP graph (mbir_hip_icd_graph; --no-graph launches them directly).  Inputs are resident in HBM before the timed region.
 Prints one JSON line.
 
 Learned warm start (skipped with --no-learned): training scans of the same
@@ -104,6 +105,16 @@ def algorithmic_bytes_per_equit(st):
     return total
 
 
+GRAPH = True  # equits as replays of a captured HIP graph (--no-graph: direct launches)
+
+
+def _equit(st, it, seed):
+    if GRAPH:
+        st.icd_graph(1, it, seed)
+    else:
+        st.icd(1, it, seed)
+
+
 def to_target(st, target, max_equits, seed, refresh):
     """Full-resolution equits from the stage's current object until the
     FP64 MAP cost is <= target; returns (equits, cost)."""
@@ -113,7 +124,7 @@ def to_target(st, target, max_equits, seed, refresh):
     cost = st.cost(estimate=True)[0]
     while it < max_equits:
         it += 1
-        st.icd(1, it, seed)
+        _equit(st, it, seed)
         if refresh and it % refresh == 0:
             st.init_error()
         cost = st.cost(estimate=True)[0]
@@ -162,7 +173,7 @@ def run_once(st, target, max_equits, seed, refresh, coarse=()):
         cs.build_amatrix()
         cs.init_error()
         for it in range(1, n + 1):
-            cs.icd(1, it, seed)
+            _equit(cs, it, seed)
             if refresh and it % refresh == 0:
                 cs.init_error()
     if prev is not None:
@@ -391,6 +402,9 @@ def main():
                     help="equits of the coarse stages of the multi-resolution schedule, "
                          "coarsest first: one count for every stage or a comma list")
     ap.add_argument("--coarse-tile", type=int, default=4)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every equit's sweeps directly instead of replaying a "
+                         "captured HIP graph")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep --tile / --coarse-tile instead of timing candidates per stage")
     ap.add_argument("--stages", default="",
@@ -426,6 +440,8 @@ def main():
     ap.add_argument("--pcmbir-views", type=int, default=360)
     ap.add_argument("--pcmbir-n", type=int, default=512)
     args = ap.parse_args()
+    global GRAPH
+    GRAPH = not args.no_graph
     if args.profile_unet and args.gpus == 1:
         profile_unet(args.profile_unet)
         return
@@ -594,6 +610,7 @@ def main():
                 "up-sampled on the device with out-of-mask voxels zeroed")
                 if coarse else "single full-resolution stage from zero"),
             "refresh_every": args.refresh,
+            "equit_launch": "captured HIP graph" if GRAPH else "direct launches",
             "cstar": cstar,
             "cstar_source": csrc,
             "target_rel": REL_TOL,

@@ -140,6 +140,16 @@ int mbir_hip_fbp(mbir_hip_ctx* ctx, float* out);
 int mbir_hip_icd_equits(mbir_hip_ctx* ctx, int32_t n, int32_t first_iter,
                         uint32_t seed, double* stats);
 
+/* The same n equits captured once into a HIP graph (one persistent launch
+ * sequence: the 8 sweep launches of every equit, graph-safe clears, no memset
+ * nodes) and replayed on each call; the iteration number comes from device
+ * memory, so one graph serves every first_iter.  The graph is captured
+ * again when n, seed, the tile, the A storage or alpha change.  Replays are
+ * asynchronous (mbir_hip_synchronize waits).  No counterpart in the
+ * reference, which runs the same schedule as host loops
+ * (XT_ICD_update.c:932-1092). */
+int mbir_hip_icd_graph(mbir_hip_ctx* ctx, int32_t n, int32_t first_iter, uint32_t seed);
+
 /* FP64 MAP cost (computeCost, XT_ICD_update.c:185-318).  fresh = 1
  * recomputes e = y - A x - d in FP64 on the fly; fresh = 0 uses the error
  * sinogram the sweeps maintain; fresh = 2 is an estimate for stopping tests:

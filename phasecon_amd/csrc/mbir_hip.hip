@@ -1058,7 +1058,8 @@ struct SweepArgs {
   int tile;     // tile edge T
   int nvmax;    // most views in one frame (sizes the LDS windows)
   int fparity;
-  int iter;
+  int iter;            // iteration number (offset from *iter_dev when set)
+  const int* iter_dev;  // graph replays: the first iteration, in device memory
   unsigned seed;
   float alpha;
   float* x;
@@ -1098,6 +1099,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SwGeom G, SweepArgs A) 
   // wavefront executes only its own loop's barriers.
   const bool w0 = __builtin_amdgcn_readfirstlane(tid) < kWave;
   const int wid = blockIdx.x;
+  const int iter = A.iter_dev ? *A.iter_dev + A.iter : A.iter;
   load_prior(G, &s_prior);
   const int tile = A.launch_tiles[wid % A.n_launch_tiles];
   const int f = A.fparity + 2 * (wid / A.n_launch_tiles);
@@ -1245,10 +1247,10 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SwGeom G, SweepArgs A) 
         col_fixed(G, s_prior, L.x, BW, ly, lx, cy, cx, f, tcur, s_fnum, s_fden, s_fpos);
       __syncthreads();
       // phase 2: the z-parity halves
-      col_solve(G, R, L.x, centre0, 0, A.iter, A.alpha, s_fnum, s_fden, s_fpos, s_row1, s_row2,
+      col_solve(G, R, L.x, centre0, 0, iter, A.alpha, s_fnum, s_fden, s_fpos, s_row1, s_row2,
                 s_dx, st);
       wave_sync();
-      col_solve(G, R, L.x, centre0, 1, A.iter, A.alpha, s_fnum, s_fden, s_fpos, s_row1, s_row2,
+      col_solve(G, R, L.x, centre0, 1, iter, A.alpha, s_fnum, s_fden, s_fpos, s_row1, s_row2,
                 s_dx, st);
       __syncthreads();
     }
@@ -1475,6 +1477,16 @@ __global__ void k_masked_sum(int Nt, int ncol, int Nz, const uint8_t* __restrict
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// graph-safe clears and counters (no memset nodes in captured graphs)
+__global__ void k_zero_f64(double* p, int n) {
+  const int i = threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+__global__ void k_set_int(int* p, int v) {
+  if (threadIdx.x == 0) p[0] = v;
+}
+
 __global__ void k_scale(float* x, long long n, float s) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x[i] *= s;
@@ -1684,6 +1696,12 @@ struct mbir_hip_ctx {
   float *d_x = nullptr, *d_stage = nullptr, *d_stage2 = nullptr;
   uint8_t* d_mask = nullptr;  // update mask [N_y * N_x]
   float* d_ws = nullptr;      // warm-start sinogram scratch (allocated on first use)
+  // captured equits (mbir_hip_icd_graph): the graph, its key and launches
+  int* d_iter = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  long long gkey[5] = {-1, -1, -1, -1, -1};
+  int glaunches = 0;
   int* d_afirst = nullptr;
   float* d_aval = nullptr;
   bool afp16 = false;                 // the sweep streams FP16 A + column scales
@@ -1943,6 +1961,9 @@ void mbir_hip_destroy(mbir_hip_ctx* c) {
                   c->d_aval16, c->d_ascale, c->d_ws};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  if (c->d_iter) (void)hipFree(c->d_iter);
   for (int k = 0; k < MBIR_K_COUNT; ++k)
     for (auto& pr : c->ev[k]) {
       (void)hipEventDestroy(pr.first);
@@ -2166,12 +2187,14 @@ int mbir_hip_fbp(mbir_hip_ctx* c, float* out) {
   return 0;
 }
 
-int mbir_hip_icd_equits(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t seed,
-                        double* stats) {
-  if (!c) return fail("null context");
-  if (!c->a_built) return fail("A not built");
-  if (build_tiles(c)) return -1;
-  HIP_OK(hipMemsetAsync(c->d_stats, 0, 4 * sizeof(double), c->stream));
+namespace {
+
+// The 8 launches per equit (2x2 tile colours x 2 frame parities) of n equits.
+// iter_dev null: iterations first_iter .. first_iter + n - 1, each launch
+// timed; otherwise the iteration is *iter_dev + it (graph capture: no event
+// records).  Returns the number of launches.
+int sweep_launches(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t seed,
+                   const int* iter_dev, int* launches) {
   SweepArgs A;
   A.col.aval = c->d_aval;
   A.col.aval16 = c->afp16 ? c->d_aval16 : nullptr;
@@ -2189,6 +2212,7 @@ int mbir_hip_icd_equits(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t
   A.tile = c->tile;
   A.tiles_x = (c->g.N_x + c->tile - 1) / c->tile;
   A.nvmax = c->nvmax;
+  A.iter_dev = iter_dev;
   const size_t xfl = (size_t)(c->tile + 2) * (c->tile + 2) * c->g.N_z;
   if (xfl * sizeof(float) > kSweepXMax)
     return fail("tile too large for N_z: the tile slab must fit 48 KiB of LDS");
@@ -2198,10 +2222,12 @@ int mbir_hip_icd_equits(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t
   if (xbytes + 8 * 1024 > c->lds_max)
     return fail("tile slab and A windows exceed the LDS of one workgroup");
   const int nf = c->g.N_time;
+  const bool timed = iter_dev == nullptr;
   // register-light prefetch for frames of at most 64 views and 256 A values
   const bool small = c->nvmax <= kWave && c->nvmax * c->G.W <= 4 * kWave;
+  *launches = 0;
   for (int it = 0; it < n; ++it) {
-    A.iter = first_iter + it;
+    A.iter = (timed ? first_iter : 0) + it;
     for (int colour = 0; colour < 4; ++colour) {
       const int ntl = c->launch_off[colour + 1] - c->launch_off[colour];
       if (ntl == 0) continue;
@@ -2212,7 +2238,7 @@ int mbir_hip_icd_equits(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t
         if (frames <= 0) continue;
         A.fparity = fpar;
         A.seed = seed * 2654435761u + (unsigned)(it * 8 + colour * 2 + fpar) * 40503u;
-        if (ev_begin(c, MBIR_K_SWEEP)) return -1;
+        if (timed && ev_begin(c, MBIR_K_SWEEP)) return -1;
         const dim3 grid((unsigned)(ntl * frames)), block(kSweepThreads);
         if (small && c->afp16) {
           MBIR_DISPATCH_W(c->G.W, hipLaunchKernelGGL((k_sweep<kW, true, true>), grid, block,
@@ -2227,15 +2253,70 @@ int mbir_hip_icd_equits(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t
           MBIR_DISPATCH_W(c->G.W, hipLaunchKernelGGL((k_sweep<kW, false, false>), grid, block,
                                                      xbytes, c->stream, c->SG, A));
         }
-        if (ev_end(c, MBIR_K_SWEEP)) return -1;
+        if (timed) {
+          if (ev_end(c, MBIR_K_SWEEP)) return -1;
+        } else {
+          HIP_OK(hipGetLastError());
+        }
+        ++*launches;
       }
     }
   }
+  return 0;
+}
+
+}  // namespace
+
+int mbir_hip_icd_equits(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t seed,
+                        double* stats) {
+  if (!c) return fail("null context");
+  if (!c->a_built) return fail("A not built");
+  if (build_tiles(c)) return -1;
+  HIP_OK(hipMemsetAsync(c->d_stats, 0, 4 * sizeof(double), c->stream));
+  int launches = 0;
+  if (sweep_launches(c, n, first_iter, seed, nullptr, &launches)) return -1;
   if (stats) {
     HIP_OK(hipMemcpyAsync(stats, c->d_stats, 4 * sizeof(double), hipMemcpyDeviceToHost,
                           c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
   }
+  return 0;
+}
+
+int mbir_hip_icd_graph(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t seed) {
+  if (!c) return fail("null context");
+  if (!c->a_built) return fail("A not built");
+  if (n < 1) return fail("n must be positive");
+  if (build_tiles(c)) return -1;
+  if (!c->d_iter && dev_alloc(&c->d_iter, 1)) return -1;
+  const long long key[5] = {n, (long long)seed, c->tile, c->afp16 ? 1 : 0,
+                            (long long)((c->alpha > 0 ? c->alpha : c->g.alpha) * 1e9)};
+  if (!c->gexec || memcmp(key, c->gkey, sizeof(key)) != 0) {
+    if (c->gexec) HIP_OK(hipGraphExecDestroy(c->gexec));
+    if (c->graph) HIP_OK(hipGraphDestroy(c->graph));
+    c->gexec = nullptr;
+    c->graph = nullptr;
+    HIP_OK(hipStreamSynchronize(c->stream));
+    HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, c->stream, c->d_stats, 4);
+    int launches = 0;
+    const int rc = sweep_launches(c, n, 0, seed, c->d_iter, &launches);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+    if (rc || ec != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc ? -1 : fail(std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+    }
+    c->graph = g;
+    HIP_OK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+    memcpy(c->gkey, key, sizeof(key));
+    c->glaunches = launches;
+  }
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, c->stream, c->d_iter, first_iter);
+  if (ev_begin(c, MBIR_K_SWEEP)) return -1;
+  HIP_OK(hipGraphLaunch(c->gexec, c->stream));
+  if (ev_end(c, MBIR_K_SWEEP)) return -1;
+  c->launches[MBIR_K_SWEEP] += c->glaunches - 1;  // the graph's launches, not one
   return 0;
 }
 

@@ -189,6 +189,12 @@ class Stage:
         _lib.check(self.L.mbir_hip_icd_equits(self.h, n, first_iter, seed, _p(st)), "icd_equits")
         return st
 
+    def icd_graph(self, n, first_iter, seed=761521):
+        """n equits through the captured HIP graph (mbir_hip_icd_graph);
+        asynchronous."""
+        _lib.check(self.L.mbir_hip_icd_graph(self.h, int(n), int(first_iter), int(seed)),
+                   "icd_graph")
+
     def cost(self, fresh=False, estimate=False):
         """[total, data, prior] FP64 MAP cost; fresh: e recomputed from x;
         estimate: maintained e with FP32 prior potentials (stopping tests)."""
