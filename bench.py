@@ -283,6 +283,93 @@ def profile_unet(n, c=16, in_ch=2, N=16, D=8, H=256, W=256):
                       "tflops_incl_transfers": 2 * macs / dt / 1e12}))
 
 
+def config4(args):
+    """The mixed-precision 4D workload (512 x 512 x 32 voxels x 24 frames, 720
+    views interlaced K = 8, 28-32 views per frame): seconds to (1 + 1e-5) C*
+    with FP32 A and with FP16 A + per-column FP32 scale, every stage's equits
+    as graph replays and tiles from the autotuner; C* and the reference image
+    from a long FP32-A run (FP64 projector and cost), and the FP16 run's
+    converged cost and image error against it.  Prints one JSON line."""
+    from phasecon_amd.autotune import tune_tile
+    from phasecon_amd.lib import K_SWEEP
+    from phasecon_amd.mbir import Stage, multires_schedule
+    from phasecon_amd.synth import make_cfg4_scan
+    log = lambda m: print("[bench] config4: " + m, file=sys.stderr, flush=True)
+    t0 = time.time()
+    scan = make_cfg4_scan(0)
+    log(f"scan generated in {time.time() - t0:.1f}s, views per frame "
+        f"{int(scan['views_per_frame'].min())}-{int(scan['views_per_frame'].max())}")
+    sched = multires_schedule(scan["proj_rows"], scan["proj_cols"])
+    counts = [int(v) for v in args.config4_coarse.split(",")]
+    out = {"metric": METRIC, "unit": "s", "n_gpus": 1, "higher_is_better": False,
+           "dtype": "fp32", "data": "synthetic", "steps": args.steps,
+           "config": {"workload": "cfg4: 512x512x32 voxels x 24 frames, 720 views interlaced "
+                                  "K=8, 28-32 views per frame (seeded), I0=5000, drifting "
+                                  "ellipsoids projected by the GPU projector",
+                      "schedule": " -> ".join(f"{mx}:{mz}" for mx, mz in sched),
+                      "coarse_equits": counts, "alpha": args.alpha}}
+    results = {}
+    ref_x = None
+    for afp16 in (False, True):
+        coarse = [(Stage(scan, mult_xy=mx, mult_z=mz, tile=8, alpha=args.coarse_alpha,
+                         afp16=afp16), n) for (mx, mz), n in zip(sched[:-1], counts)]
+        st = Stage(scan, tile=8, alpha=args.alpha, afp16=afp16)
+        tiles = {}
+        for s_ in [cs for cs, _ in coarse] + [st]:
+            s_.set_object(np.zeros(s_.shape_x, np.float32))
+            s_.build_amatrix()
+            # tile 4 is left out: with 30 views per frame and 2-8 detector rows
+            # per voxel, that many concurrent columns make the coarse stages
+            # diverge at alpha >= 1.5 (measured; tiles 8 and up converge)
+            tiles[f"{s_.g.N_x}x{s_.g.N_y}x{s_.g.N_z}"] = tune_tile(s_, candidates=(8, 16))[0]
+        if ref_x is None:
+            # C*: the multi-resolution schedule, then a long full-resolution run
+            run_once(st, -1.0, args.config4_ref_equits, args.seed, args.refresh, coarse)
+            cstar = st.cost(fresh=True)[0]
+            ref_x = st.get_object().astype(np.float64)
+            log(f"C* = {cstar:.12g} after {args.config4_ref_equits} full-resolution equits")
+            if not np.isfinite(cstar) or cstar > st.cost(fresh=True)[0] * 1.000001 or \
+                    not np.all(np.isfinite(ref_x)) or np.abs(ref_x).max() > 1.0:
+                raise SystemExit("bench.py: the config-4 reference run diverged")
+            target = cstar * (1.0 + REL_TOL)
+        st.reset_kernel_time()
+        times, equits = [], []
+        for k in range(args.steps):
+            dt, n, c = run_once(st, target, args.max_equits, args.seed + k, args.refresh, coarse)
+            times.append(dt)
+            equits.append(n)
+            log(f"{'fp16' if afp16 else 'fp32'} A step {k}: {dt:.3f}s, {n} equits, cost {c:.12g}")
+        sweep_ms, sweep_n = st.kernel_time(K_SWEEP)
+        bytes_launch = algorithmic_bytes_per_equit(st) * sum(equits) / max(sweep_n, 1)
+        rec = {"value": float(np.mean(times)), "equits_to_target": equits, "tiles": tiles,
+               "sweep_ms_per_launch": sweep_ms / max(sweep_n, 1),
+               "sweep_gbps": bytes_launch / (sweep_ms / max(sweep_n, 1) * 1e-3) / 1e9}
+        if afp16:
+            # converged FP16 run against the FP32-A reference
+            run_once(st, -1.0, args.config4_ref_equits, args.seed, args.refresh, coarse)
+            c16 = st.cost(fresh=True)[0]
+            x16 = st.get_object().astype(np.float64)
+            rec["error_vs_fp32_a"] = {
+                "cost_rel": (c16 - cstar) / cstar,
+                "image_rmse_rel": float(np.sqrt(np.mean((x16 - ref_x) ** 2) /
+                                                np.mean(ref_x ** 2)))}
+        results["fp16_a" if afp16 else "fp32_a"] = rec
+        for cs, _ in coarse:
+            cs.close()
+        st.close()
+    out.update(results)
+    out["cstar"] = cstar
+    out["cstar_source"] = (f"FP32-A GPU run, {args.config4_ref_equits} full-resolution equits "
+                           "after the coarse stages (FP64 projector and cost)")
+    out["value"] = results["fp16_a"]["value"]
+    out["vs_fp32_a"] = results["fp32_a"]["value"] / results["fp16_a"]["value"]
+    out["roofline"] = {"bound": "hbm", "achieved": results["fp16_a"]["sweep_gbps"],
+                       "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+                       "frac": results["fp16_a"]["sweep_gbps"] / PEAK_HBM_GBPS,
+                       "traffic": None, "kernel": "k_sweep<3, true, true>"}
+    print(json.dumps(out))
+
+
 def cpu_baseline(scan, equits=2):
     """The reference's own ICD (oracle/_ref) on this host, full resolution,
     bounded sample.  The reference has one qGGMRF p (Q6), so it runs p=1.2
@@ -421,6 +508,10 @@ def main():
     ap.add_argument("--learned-scans", type=int, default=8)
     ap.add_argument("--learned-steps", type=int, default=3000)
     ap.add_argument("--learned-channels", type=int, default=16)
+    ap.add_argument("--config4", action="store_true",
+                    help="the mixed-precision 512^2 x 32 x 24 workload, FP32 vs FP16 A")
+    ap.add_argument("--config4-coarse", default="40,60,100")
+    ap.add_argument("--config4-ref-equits", type=int, default=300)
     ap.add_argument("--profile-unet", type=int, default=0,
                     help="only time N forward passes of the warm-start network")
     ap.add_argument("--learned-coarse-equits", default="0,60",
@@ -444,6 +535,9 @@ def main():
     GRAPH = not args.no_graph
     if args.profile_unet and args.gpus == 1:
         profile_unet(args.profile_unet)
+        return
+    if args.config4 and args.gpus == 1:
+        config4(args)
         return
     if args.pcmbir and args.gpus == 1:
         pcmbir_bench(args.pcmbir_views, args.pcmbir_n, max(args.steps, 1), nms=True)

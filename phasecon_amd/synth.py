@@ -170,6 +170,11 @@ WORKLOADS = {
     # non-pipelined theta path).
     "tall3d": dict(rows=16, cols=32, views=24, frames=1, I0=1e4, K=None,
                    sig_s=0.005, sig_t=1.0, c_s=0.01, c_t=0.01, p_s=1.2, p_t=1.2),
+    # The mixed-precision workload's detector height (32 rows) and views per
+    # frame (30) on a small grid, for parity of its coarse stages (window
+    # widths up to 17, up to 8 rows per voxel).
+    "cfg4_small": dict(rows=32, cols=64, views=60, frames=2, I0=5e3, K=None,
+                       sig_s=0.01, sig_t=0.001, c_s=0.01, c_t=0.01, p_s=1.2, p_t=1.2),
 }
 
 
@@ -210,3 +215,78 @@ def make_scan(name, seed=0, **over):
         c_s=w["c_s"], c_t=w["c_t"], p_s=w["p_s"], p_t=w["p_t"],
         phantom=phantom, seed=seed,
     )
+
+
+def drifting_ellipsoids(n, nz, nt, seed, lo=0.005, hi=0.02, count=40):
+    """[nt][nz][n][n] two-phase phantom: value lo inside a disk of radius
+    0.85 (of the half-width), hi inside seeded ellipsoids that drift a little
+    from frame to frame, 0 outside the disk."""
+    rng = np.random.default_rng(seed)
+    c0 = rng.uniform(-0.55, 0.55, (count, 2))
+    rxy = rng.uniform(0.04, 0.14, (count, 2))
+    zc = rng.uniform(0, nz, count)
+    rz = rng.uniform(nz / 6, nz / 2, count)
+    vel = rng.normal(0.0, 0.004, (count, 2))
+    ang = rng.uniform(0, np.pi, count)
+    u = (np.arange(n) + 0.5) / n * 2 - 1
+    yy, xx = np.meshgrid(u, u, indexing="ij")
+    disk = (xx ** 2 + yy ** 2) <= 0.85 ** 2
+    z = np.arange(nz) + 0.5
+    out = np.zeros((nt, nz, n, n), np.float32)
+    for t in range(nt):
+        inside = np.zeros((nz, n, n), bool)
+        for e in range(count):
+            cx, cy = c0[e] + vel[e] * t
+            ca, sa = np.cos(ang[e]), np.sin(ang[e])
+            dx, dy = xx - cx, yy - cy
+            q = ((ca * dx + sa * dy) / rxy[e, 0]) ** 2 + ((-sa * dx + ca * dy) / rxy[e, 1]) ** 2
+            dz2 = ((z - zc[e]) / rz[e]) ** 2
+            inside |= (q[None] + dz2[:, None, None]) <= 1.0
+        out[t] = np.where(disk[None], np.where(inside, hi, lo), 0.0)
+    return out
+
+
+def uneven_frames(n_views, n_frames, lo, hi, seed):
+    """Seeded views per frame in [lo, hi] summing to n_views."""
+    rng = np.random.default_rng(seed)
+    k = rng.integers(lo, hi + 1, n_frames)
+    while k.sum() != n_views:
+        i = rng.integers(n_frames)
+        if k.sum() > n_views and k[i] > lo:
+            k[i] -= 1
+        elif k.sum() < n_views and k[i] < hi:
+            k[i] += 1
+    return k
+
+
+def make_cfg4_scan(seed=0, cols=512, rows=32, views=720, frames=24, K=8, I0=5e3):
+    """The mixed-precision 4D workload: cols x cols x rows voxels x frames,
+    `views` time-interlaced views (K-way), seeded uneven views per frame in
+    [28, 32], a drifting-ellipsoid phantom projected by the GPU forward
+    projector (the CPU projector is too slow at this size), Poisson counts."""
+    from .mbir import Stage
+    angles, times = interlaced_views(K, views, views)
+    per = uneven_frames(views, frames, 28, 32, seed + 77)
+    first = np.concatenate([[0], np.cumsum(per)[:-1]])
+    recon_times = np.concatenate([times[first], [float(views)]]).astype(np.float32)
+    recon_times[0] = 0.0
+    phantom = drifting_ellipsoids(cols, rows, frames, seed)
+    scan = dict(
+        name="cfg4", projections=np.zeros((views, rows, cols), np.float32),
+        weights=np.ones((views, rows, cols), np.float32), angles=angles, proj_times=times,
+        recon_times=recon_times, proj_rows=rows, proj_cols=cols, proj_num=views,
+        recon_num=frames, vox_wid=1.0, rot_center=cols / 2.0, sig_s=0.01, sig_t=0.001,
+        c_s=0.01, c_t=0.01, p_s=1.2, p_t=1.1, seed=seed, views_per_frame=per)
+    st = Stage(scan)
+    try:
+        line = st.forward_project(phantom)            # [N_p][N_r][N_t]
+    finally:
+        st.close()
+    line = np.ascontiguousarray(line.transpose(0, 2, 1)).astype(np.float64)
+    rng = np.random.default_rng(seed + 1000)
+    counts = np.maximum(rng.poisson(I0 * np.exp(-line)).astype(np.float64), 1.0)
+    scan["projections"] = np.log(I0 / counts).astype(np.float32)
+    scan["weights"] = counts.astype(np.float32)
+    scan["phantom"] = phantom
+    return scan
+

@@ -150,6 +150,40 @@ def test_cost_matches_oracle(pair):
     assert est[1] == kept[1]
 
 
+@pytest.mark.parametrize("mult", [(1, 1), (2, 2), (4, 4), (8, 8)])
+def test_stage_amatrix_and_thetas_match_oracle(mult):
+    """A and theta1/theta2 of every multi-resolution stage of a 32-row,
+    30-views-per-frame geometry: window widths 3, 5, 9, 17 and 1, 2, 4, 8
+    detector rows per voxel."""
+    from oracle.oracle import Oracle
+    from phasecon_amd.mbir import Stage
+    from phasecon_amd.synth import make_scan
+    scan = make_scan("cfg4_small", seed=1)
+    st = Stage(scan, mult_xy=mult[0], mult_z=mult[1])
+    o = Oracle(scan, mult_xy=mult[0], mult_z=mult[1])
+    st.build_amatrix()
+    first, val = st.get_amatrix()
+    ofirst, oval = o.amatrix_all()
+    sv = st.frame_views[: st.frame_ptr[-1]]
+    assert np.array_equal(first, ofirst[:, sv])
+    assert rel_err(val, oval[:, sv]) < 1e-6
+    rng = np.random.default_rng(9)
+    e = (rng.standard_normal(st.shape_e) * 0.05).astype(np.float32)
+    st.set_error(e)
+    g = st.g
+    q = np.array([(rng.integers(g.N_time), rng.integers(g.N_z), rng.integers(g.N_y),
+                   rng.integers(g.N_x)) for _ in range(48)], np.int32)
+    t1, t2 = st.thetas(q)
+    st.close()
+    o1 = np.zeros(len(q))
+    o2 = np.zeros(len(q))
+    for i, (t, z, yy, xx) in enumerate(q):
+        o1[i], o2[i] = o.thetas(e.astype(np.float64), int(t), int(z), int(yy), int(xx))
+    print(f"mult {mult}: W {st.W}, theta2 rel {rel_err(t2, o2):.3g}", flush=True)
+    assert rel_err(t2, o2) < 1e-5
+    assert np.max(np.abs(t1 - o1)) <= 1e-5 * (np.max(np.abs(o1)) + 0.05 * np.max(o2))
+
+
 @pytest.mark.parametrize("name,n_gpu,tile,alpha", [
     ("tiny3d", 120, 8, None), ("ch4d_small", 120, 8, None), ("ch4d_small", 120, 4, None),
     ("tall3d", 150, 8, None), ("ch4d_small", 120, 8, 1.9), ("tall3d", 150, 8, 1.9)])
