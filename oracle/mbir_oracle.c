@@ -812,3 +812,180 @@ void orc_sino_from_file(int32_t N_p, int32_t N_t, int32_t N_r,
       for (int32_t t = 0; t < N_t; ++t)
         out[((size_t)p * N_r + r) * N_t + t] = in[((size_t)p * N_t + t) * N_r + r];
 }
+
+
+/* ------------------------------------------------------------------ ring
+ * offset and noise variance */
+
+int32_t orc_offset_windows(int32_t N_r, int32_t N_t, double* win) {
+  /* window sizes and steps of XT_OffsetError.c:57-61 (integer arithmetic) */
+  const int32_t rs = 2 * N_r / (int32_t)(sqrt((double)N_r) + 0.5);
+  const int32_t ts = 2 * N_t / (int32_t)(sqrt((double)N_t) + 0.5);
+  int32_t m = 0;
+  for (int32_t i = 0; i <= N_r - rs / 2; i += rs / 2)
+    for (int32_t j = 0; j <= N_t - ts / 2; j += ts / 2) {
+      if (win) {
+        double* wm = win + (size_t)m * N_r * N_t;
+        /* triangular weights, wrapped around both axes; a wrapped sample
+         * overwrites (XT_OffsetError.c:68-79) */
+        for (int32_t k = i; k < i + rs; ++k)
+          for (int32_t l = j; l < j + ts; ++l) {
+            const double a = (k - i) < rs / 2 ? (k - i + 1) : rs - (k - i);
+            const double b = (l - j) < ts / 2 ? (l - j + 1) : ts - (l - j);
+            wm[(size_t)(k % N_r) * N_t + (l % N_t)] = a * b;
+          }
+      }
+      ++m;
+    }
+  return m;
+}
+
+/* inverse of an n x n matrix by Gauss-Jordan elimination with partial
+ * pivoting (in place, a is n x n row-major); returns -1 if singular */
+static int orc_invert(double* a, int32_t n) {
+  double* inv = (double*)calloc((size_t)n * n, sizeof(double));
+  for (int32_t i = 0; i < n; ++i) inv[(size_t)i * n + i] = 1.0;
+  for (int32_t c = 0; c < n; ++c) {
+    int32_t piv = c;
+    for (int32_t r = c + 1; r < n; ++r)
+      if (fabs(a[(size_t)r * n + c]) > fabs(a[(size_t)piv * n + c])) piv = r;
+    if (a[(size_t)piv * n + c] == 0.0) {
+      free(inv);
+      return -1;
+    }
+    if (piv != c)
+      for (int32_t k = 0; k < n; ++k) {
+        double t = a[(size_t)c * n + k];
+        a[(size_t)c * n + k] = a[(size_t)piv * n + k];
+        a[(size_t)piv * n + k] = t;
+        t = inv[(size_t)c * n + k];
+        inv[(size_t)c * n + k] = inv[(size_t)piv * n + k];
+        inv[(size_t)piv * n + k] = t;
+      }
+    const double p = a[(size_t)c * n + c];
+    for (int32_t k = 0; k < n; ++k) {
+      a[(size_t)c * n + k] /= p;
+      inv[(size_t)c * n + k] /= p;
+    }
+    for (int32_t r = 0; r < n; ++r)
+      if (r != c) {
+        const double f = a[(size_t)r * n + c];
+        if (f == 0.0) continue;
+        for (int32_t k = 0; k < n; ++k) {
+          a[(size_t)r * n + k] -= f * a[(size_t)c * n + k];
+          inv[(size_t)r * n + k] -= f * inv[(size_t)c * n + k];
+        }
+      }
+  }
+  memcpy(a, inv, sizeof(double) * (size_t)n * n);
+  free(inv);
+  return 0;
+}
+
+/* per (r, t): the generalized-Huber weight of every view's entry -- w where
+ * selected, T delta sqrt(w) / |e| otherwise -- summed (lam), and the matching
+ * numerator of the offset step (the mode-3 right-hand side when rhs3) */
+static void orc_offset_sums(int32_t N_p, int32_t N_r, int32_t N_t, double T, double delta,
+                            const double* w, const double* e, const double* d,
+                            const uint8_t* sel, int rhs3, double* num, double* lam) {
+  const size_t nrt = (size_t)N_r * N_t;
+  for (size_t i = 0; i < nrt; ++i) {
+    double a = 0.0, b = 0.0;
+    for (int32_t p = 0; p < N_p; ++p) {
+      const size_t q = (size_t)p * nrt + i;
+      const double tw = T * delta * sqrt(w[q]);
+      if (sel[q]) {
+        b += w[q];
+        a += (rhs3 ? e[q] + d[i] : e[q]) * w[q];
+      } else {
+        const double l = tw / fabs(e[q]);
+        b += l;
+        a += rhs3 ? (e[q] + d[i]) * l : tw * (double)((e[q] > 0) - (e[q] < 0));
+      }
+    }
+    num[i] = a;
+    lam[i] = b;
+  }
+}
+
+static void orc_reselect(size_t n, double T, const double* w, const double* e, uint8_t* sel) {
+  for (size_t q = 0; q < n; ++q) sel[q] = fabs(e[q] * sqrt(w[q])) < T;
+}
+
+void orc_offset_update(int32_t mode, int32_t N_p, int32_t N_r, int32_t N_t,
+                       double T, double delta, const double* w, double* e,
+                       double* d, uint8_t* sel) {
+  const size_t nrt = (size_t)N_r * N_t, n = nrt * N_p;
+  double* num = (double*)malloc(sizeof(double) * nrt);
+  double* lam = (double*)malloc(sizeof(double) * nrt);
+  double* step = (double*)malloc(sizeof(double) * nrt);
+  orc_offset_sums(N_p, N_r, N_t, T, delta, w, e, d, sel, mode == 3, num, lam);
+  if (mode == 1) {
+    for (size_t i = 0; i < nrt; ++i) step[i] = num[i] / lam[i];
+  } else if (mode == 2) {
+    /* the Lagrange multiplier of sum(d) = 0 */
+    double ns = 0.0, ds = 0.0;
+    for (size_t i = 0; i < nrt; ++i) {
+      ns += d[i] + num[i] / lam[i];
+      ds += 1.0 / lam[i];
+    }
+    const double gamma = ns / ds;
+    for (size_t i = 0; i < nrt; ++i) step[i] = (num[i] - gamma) / lam[i];
+  } else {
+    /* minimise sum lam (x - b/lam)^2 subject to <A_m, x> = 0 for every
+     * window m: x = (b - sum_m A_m c_m) / lam, c = D^-1 (A b / lam),
+     * D = A diag(1/lam) A^T */
+    const int32_t M = orc_offset_windows(N_r, N_t, NULL);
+    double* A = (double*)calloc((size_t)M * nrt, sizeof(double));
+    double* D = (double*)calloc((size_t)M * M, sizeof(double));
+    double* v = (double*)calloc(M, sizeof(double));
+    double* c = (double*)calloc(M, sizeof(double));
+    orc_offset_windows(N_r, N_t, A);
+    for (int32_t i = 0; i < M; ++i)
+      for (int32_t j = 0; j < M; ++j) {
+        double s = 0.0;
+        for (size_t k = 0; k < nrt; ++k) s += A[(size_t)i * nrt + k] * A[(size_t)j * nrt + k] / lam[k];
+        D[(size_t)i * M + j] = s;
+      }
+    orc_invert(D, M);
+    for (int32_t i = 0; i < M; ++i) {
+      double s = 0.0;
+      for (size_t k = 0; k < nrt; ++k) s += A[(size_t)i * nrt + k] * num[k] / lam[k];
+      v[i] = s;
+    }
+    for (int32_t i = 0; i < M; ++i) {
+      double s = 0.0;
+      for (int32_t j = 0; j < M; ++j) s += D[(size_t)i * M + j] * v[j];
+      c[i] = s;
+    }
+    for (size_t k = 0; k < nrt; ++k) {
+      double s = 0.0;
+      for (int32_t m = 0; m < M; ++m) s += A[(size_t)m * nrt + k] * c[m];
+      /* new offset x; the step applied to e and d is x - d */
+      step[k] = (num[k] - s) / lam[k] - d[k];
+    }
+    free(A);
+    free(D);
+    free(v);
+    free(c);
+  }
+  for (size_t i = 0; i < nrt; ++i) d[i] += step[i];
+  for (size_t q = 0; q < n; ++q) e[q] -= step[q % nrt];
+  orc_reselect(n, T, w, e, sel);
+  free(num);
+  free(lam);
+  free(step);
+}
+
+void orc_variance_update(int64_t n, double T, double delta, double* w,
+                         const double* e, uint8_t* sel, double* var) {
+  double s = 0.0;
+  for (int64_t q = 0; q < n; ++q) {
+    w[q] *= *var; /* back to the measured weights */
+    s += sel[q] ? e[q] * e[q] * w[q] : fabs(e[q]) * delta * T * sqrt(w[q] * *var);
+  }
+  *var = s / (double)n;
+  for (int64_t q = 0; q < n; ++q) w[q] /= *var;
+  orc_reselect((size_t)n, T, w, e, sel);
+}
+

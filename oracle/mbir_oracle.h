@@ -163,6 +163,29 @@ void orc_icd_equit_tiled_sync(const orc_geom* g, const double* cosv,
 void orc_sino_from_file(int32_t N_p, int32_t N_t, int32_t N_r,
                         const float* in, double* out);
 
+/* Ring-offset windows of the rectangular-patch constraint
+ * (gen_offset_constraint_windows, XT_OffsetError.c:46-96, one node): if win
+ * is not NULL it receives [M][N_r][N_t] weights.  Returns M. */
+int32_t orc_offset_windows(int32_t N_r, int32_t N_t, double* win);
+
+/* One ring-offset update (update_Sinogram_Offset, XT_ICD_update.c:915-923)
+ * of mode 1 (update_d_offset_unconstrained, :866-911), 2
+ * (update_d_offset_zero_mean_constraint, :813-863) or 3
+ * (update_d_offset_rect_patch_constraint, :749-810, with
+ * constrained_quad_opt, XT_OffsetError.c:99-165).  e [N_p][N_r][N_t] and
+ * d [N_r][N_t] are updated in place, sel (the generalized-Huber selection,
+ * 1 where |e sqrt(w)| < T) is used and then recomputed from the new e. */
+void orc_offset_update(int32_t mode, int32_t N_p, int32_t N_r, int32_t N_t,
+                       double T, double delta, const double* w, double* e,
+                       double* d, uint8_t* sel);
+
+/* The noise-variance re-estimate (update_variance_parameter,
+ * XT_ICD_update.c:716-746) over n sinogram entries: w (already divided by
+ * *var) is scaled to w * var_old / var_new, sel used then recomputed, *var
+ * replaced by the new estimate. */
+void orc_variance_update(int64_t n, double T, double delta, double* w,
+                         const double* e, uint8_t* sel, double* var);
+
 #ifdef __cplusplus
 }
 #endif

@@ -197,6 +197,50 @@ class Oracle:
         return x, e, np.array(costs)
 
 
+def _orc():
+    L = _lib(ORACLE_SO)
+    L.orc_offset_windows.restype = C.c_int32
+    return L
+
+
+def offset_windows(N_r, N_t):
+    """orc_offset_windows: [M][N_r][N_t] window weights."""
+    L = _orc()
+    M = L.orc_offset_windows(C.c_int32(N_r), C.c_int32(N_t), None)
+    win = np.zeros((M, N_r, N_t), np.float64)
+    L.orc_offset_windows(C.c_int32(N_r), C.c_int32(N_t), win.ctypes.data_as(C.c_void_p))
+    return win
+
+
+def offset_update(mode, w, e, d, sel, T, delta):
+    """orc_offset_update on float64 copies ([p][r][t], [r][t]); returns
+    (e, d, sel)."""
+    L = _orc()
+    N_p, N_r, N_t = e.shape
+    w = np.ascontiguousarray(w, np.float64)
+    e = np.array(e, np.float64)
+    d = np.array(d, np.float64)
+    sel = np.array(sel, np.uint8)
+    L.orc_offset_update(C.c_int32(mode), C.c_int32(N_p), C.c_int32(N_r), C.c_int32(N_t),
+                        C.c_double(T), C.c_double(delta), w.ctypes.data_as(C.c_void_p),
+                        e.ctypes.data_as(C.c_void_p), d.ctypes.data_as(C.c_void_p),
+                        sel.ctypes.data_as(C.c_void_p))
+    return e, d, sel
+
+
+def variance_update(var, w, e, sel, T, delta):
+    """orc_variance_update on float64 copies; returns (var, w, sel)."""
+    L = _orc()
+    w = np.array(w, np.float64)
+    e = np.ascontiguousarray(e, np.float64)
+    sel = np.array(sel, np.uint8)
+    v = C.c_double(var)
+    L.orc_variance_update(C.c_int64(e.size), C.c_double(T), C.c_double(delta),
+                          w.ctypes.data_as(C.c_void_p), e.ctypes.data_as(C.c_void_p),
+                          sel.ctypes.data_as(C.c_void_p), C.byref(v))
+    return v.value, w, sel
+
+
 def reference_available():
     return os.path.exists(REF_SO)
 
@@ -216,6 +260,8 @@ def _ref_lib():
         L.ref_update_voxel.restype = C.c_long
         L.ref_init_error.restype = C.c_int32
         L.ref_reconstruct.restype = C.c_int32
+        L.ref_variance_update.restype = C.c_double
+        L.ref_variance_update.argtypes = [C.c_void_p, C.c_double] + [C.c_void_p] * 3
         _REF = L
     return _REF
 
@@ -327,6 +373,24 @@ class Reference:
         self.L.ref_icd_run(self.h, C.c_int32(n), C.c_int32(first_iter), self._p(costs),
                            self._p(conv), self._p(secs))
         return costs, conv, secs
+
+    def offset_update(self, mode, w, e, d, sel):
+        """update_Sinogram_Offset on float copies; returns (e, d, sel)."""
+        w = np.ascontiguousarray(w, np.float32)
+        e = np.array(e, np.float32)
+        d = np.array(d, np.float32)
+        sel = np.array(sel, np.uint8)
+        self.L.ref_offset_update(self.h, C.c_int32(mode), self._p(w), self._p(e), self._p(d),
+                                 self._p(sel))
+        return e, d, sel
+
+    def variance_update(self, var, w, e, sel):
+        """update_variance_parameter; returns (var, w, sel)."""
+        w = np.array(w, np.float32)
+        e = np.ascontiguousarray(e, np.float32)
+        sel = np.array(sel, np.uint8)
+        v = self.L.ref_variance_update(self.h, float(var), self._p(w), self._p(e), self._p(sel))
+        return v, w, sel
 
     def close(self):
         if self.h:

@@ -46,6 +46,9 @@ Real_t computeCost(Sinogram*, ScannedObject*, TomoInputs*, Real_arr_t***);
 int updateVoxelsTimeSlices(Sinogram*, ScannedObject*, TomoInputs*,
                            Real_arr_t**, Real_arr_t***, int32_t,
                            Real_arr_t****, uint8_t***);
+void update_Sinogram_Offset(Sinogram*, TomoInputs*, Real_arr_t***);
+void update_variance_parameter(Sinogram*, TomoInputs*, Real_arr_t***);
+void gen_offset_constraint_windows(Sinogram*, TomoInputs*);
 int reconstruct(float**, float*, float*, float*, float*, float*, int32_t,
                 int32_t, int32_t, int32_t, float, float, float, float, float,
                 float, float, int32_t, int32_t, float, float, uint8_t, FILE*);
@@ -61,6 +64,7 @@ typedef struct {
   uint8_t*** mask;
   float *proj, *wgt, *angles, *ptimes, *rtimes;
   FILE* log;
+  int windows;
 } ref_ctx;
 
 static void ref_mpi_up(void) {
@@ -331,6 +335,55 @@ int32_t ref_reconstruct(const char* scratch_dir, float* object,
   free(p); free(w); free(a); free(pt); free(rt);
   fclose(log);
   return flag;
+}
+
+/* Sinogram-sized state in and out of the reference's arrays ([p][r][t]). */
+static void ref_put_sino(ref_ctx* c, const float* w, const float* e, const uint8_t* sel) {
+  const size_t n = (size_t)c->S.N_p * c->S.N_r * c->S.N_t;
+  for (size_t i = 0; i < n; ++i) {
+    if (w) (&c->T.Weight[0][0][0])[i] = w[i];
+    (&c->esino[0][0][0])[i] = e[i];
+    (&c->S.ProjSelect[0][0][0])[i] = sel[i] != 0;
+  }
+}
+
+static void ref_get_sino(ref_ctx* c, float* w, float* e, uint8_t* sel) {
+  const size_t n = (size_t)c->S.N_p * c->S.N_r * c->S.N_t;
+  for (size_t i = 0; i < n; ++i) {
+    if (w) w[i] = (&c->T.Weight[0][0][0])[i];
+    if (e) e[i] = (&c->esino[0][0][0])[i];
+    sel[i] = (&c->S.ProjSelect[0][0][0])[i] ? 1 : 0;
+  }
+}
+
+/* One ring-offset update (update_Sinogram_Offset) of the given mode on the
+ * given weights, error sinogram, offset and selection.  Mode 3's windows
+ * come from gen_offset_constraint_windows, which the reference calls only
+ * when it simulates data (XT_Init.c:528), not on its reconstruction path. */
+void ref_offset_update(void* h, int32_t mode, const float* w, float* e, float* d, uint8_t* sel) {
+  ref_ctx* c = (ref_ctx*)h;
+  ref_put_sino(c, w, e, sel);
+  memcpy(&c->S.ProjOffset[0][0], d, sizeof(float) * c->S.N_r * c->S.N_t);
+  c->T.OffsetConstraintType = mode;
+  if (mode == 3 && !c->windows) {
+    gen_offset_constraint_windows(&c->S, &c->T);
+    c->windows = 1;
+  }
+  update_Sinogram_Offset(&c->S, &c->T, c->esino);
+  ref_get_sino(c, NULL, e, sel);
+  memcpy(d, &c->S.ProjOffset[0][0], sizeof(float) * c->S.N_r * c->S.N_t);
+}
+
+/* The variance re-estimate (update_variance_parameter) from var on the
+ * given weights (already divided by var), error sinogram and selection;
+ * returns the new estimate, w and sel receive the updated arrays. */
+double ref_variance_update(void* h, double var, float* w, const float* e, uint8_t* sel) {
+  ref_ctx* c = (ref_ctx*)h;
+  ref_put_sino(c, w, e, sel);
+  c->T.var_est = var;
+  update_variance_parameter(&c->S, &c->T, c->esino);
+  ref_get_sino(c, w, NULL, sel);
+  return c->T.var_est;
 }
 
 void ref_destroy(void* h) {
