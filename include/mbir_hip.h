@@ -150,6 +150,28 @@ int mbir_hip_icd_equits(mbir_hip_ctx* ctx, int32_t n, int32_t first_iter,
  * (XT_ICD_update.c:932-1092). */
 int mbir_hip_icd_graph(mbir_hip_ctx* ctx, int32_t n, int32_t first_iter, uint32_t seed);
 
+/* One ring-offset update (update_Sinogram_Offset, XT_ICD_update.c:915-923)
+ * of the offset d [N_r][N_t] and the error sinogram together (e + d is
+ * unchanged): mode 1 unconstrained (update_d_offset_unconstrained,
+ * :866-911), 2 with sum d = 0 (:813-863), 3 with <A_m, d> = 0 for the
+ * rectangular-patch windows A_m of XT_OffsetError.c:46-96
+ * (:749-810 with constrained_quad_opt, XT_OffsetError.c:99-165).  Entries
+ * count with the generalized-Huber weight of their current residual
+ * (w where |e sqrt(w)| < huber_T, huber_T huber_delta sqrt(w) / |e|
+ * otherwise); the reference uses the selection stored at its last update.
+ * d starts as the create() argument (zero when NULL) and is also used by the
+ * forward projector from then on. */
+int mbir_hip_offset_update(mbir_hip_ctx* ctx, int32_t mode);
+int mbir_hip_get_offset(mbir_hip_ctx* ctx, float* d);
+
+/* The noise-variance re-estimate (update_variance_parameter,
+ * XT_ICD_update.c:716-746): var_new = mean over the sinogram of e^2 W
+ * (selected entries) or |e| huber_delta huber_T sqrt(W var) (the others),
+ * W = w var the measured weight; w becomes W / var_new and g->var_est
+ * var_new (the cost's log term).  *var_est (may be NULL) receives it. */
+int mbir_hip_variance_update(mbir_hip_ctx* ctx, double* var_est);
+int mbir_hip_get_weight(mbir_hip_ctx* ctx, float* w);
+
 /* FP64 MAP cost (computeCost, XT_ICD_update.c:185-318).  fresh = 1
  * recomputes e = y - A x - d in FP64 on the fly; fresh = 0 uses the error
  * sinogram the sweeps maintain; fresh = 2 is an estimate for stopping tests:

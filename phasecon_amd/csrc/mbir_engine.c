@@ -449,6 +449,7 @@ int reconstruct_ex(float** object, float* projections, float* weights,
   float* prev = NULL; /* previous stage's object [N_time][N_z][N_y][N_x] */
   int32_t prev_nz = 0, prev_ny = 0, prev_nx = 0;
   double var_est = 1.0; /* VAR_PARAM_INIT, XT_Constants.h:95 */
+  float* offset = NULL; /* ring offset [N_r][N_t] carried between stages */
   int rc = -1;
   if (opt_in)
     opt = *opt_in;
@@ -477,8 +478,13 @@ int reconstruct_ex(float** object, float* projections, float* weights,
     say(log, "ERROR", "proj_num and recon_num must be positive.\n");
     return -1;
   }
-  if (remove_rings != 0) {
-    say(log, "ERROR", "remove_rings = %d: ring-offset estimation is not available in this build.\n", remove_rings);
+  if (remove_rings < 0 || remove_rings > 3) {
+    say(log, "ERROR", "remove_rings must be 0, 1, 2 or 3.\n");
+    return -1;
+  }
+  offset = (float*)calloc((size_t)proj_cols * proj_rows, sizeof(float));
+  if (!offset) {
+    say(log, "ERROR", "Out of memory.\n");
     return -1;
   }
   nstage = mbir_multires_schedule(proj_rows, proj_cols, mxy, mz);
@@ -564,8 +570,15 @@ int reconstruct_ex(float** object, float* projections, float* weights,
         }
         free(c);
       }
+      /* ring offset (XT_Init.c:246-254, 345-349): updated from the second
+       * stage on, starting from zero there and from the previous stage's
+       * offset (proj_offset) after it */
+      const int update_offset = remove_rings != 0 && st >= 1;
+      if (remove_rings != 0 && st >= 2 && st == first_stage && restart == 1 && opt.write_files &&
+          read_bin("proj_offset", offset, sizeof(float), (size_t)g->N_r * g->N_t, log))
+        goto stage_fail;
       if (mbir_hip_create(&ctx, g, S.H, S.cosv, S.sinv, S.frame_ptr, S.frame_views, S.y, S.w,
-                          NULL, S.mask) ||
+                          (remove_rings != 0 && st >= 2) ? offset : NULL, S.mask) ||
           mbir_hip_set_option(ctx, "tile", opt.tile) ||
           (opt.alpha > 0 && mbir_hip_set_option(ctx, "alpha", opt.alpha)) ||
           mbir_hip_set_object(ctx, x) || mbir_hip_build_amatrix(ctx) ||
@@ -585,6 +598,12 @@ int reconstruct_ex(float** object, float* projections, float* weights,
         /* the sweeps update e with FP32 atomics; recompute it exactly now
          * and then so rounding drift cannot shift the fixed point */
         if (opt.refresh_every > 0 && it % opt.refresh_every == 0 && mbir_hip_init_error(ctx))
+          goto gpu_fail;
+        /* ring offset and noise variance (XT_ICD_update.c:983-1050): once
+         * per equit here, the reference updates the offset after every
+         * half-sweep of its NHICD subsets */
+        if (update_offset && mbir_hip_offset_update(ctx, remove_rings)) goto gpu_fail;
+        if (st >= 1 && quad_convex == 0 && mbir_hip_variance_update(ctx, &var_est))
           goto gpu_fail;
         /* update magnitude relative to the mean object magnitude of one of
          * the reference's NHICD_Iterations line subsets (XT_ICD_update.c:
@@ -607,6 +626,7 @@ int reconstruct_ex(float** object, float* projections, float* weights,
       if (equits_out) *equits_out = it > opt.max_equits ? opt.max_equits : it;
       if (cost_out) *cost_out = last;
       if (mbir_hip_get_object(ctx, x)) goto gpu_fail;
+      if (update_offset && mbir_hip_get_offset(ctx, offset)) goto gpu_fail;
       mbir_hip_destroy(ctx);
       ctx = NULL;
       /* stage files (XT_IOMisc.c:237-269, XT_ICD_update.c:1196-1199,
@@ -617,9 +637,7 @@ int reconstruct_ex(float** object, float* projections, float* weights,
           snprintf(name, sizeof name, "object_time_%d", f);
           if (write_bin(name, x + (size_t)f * vol, sizeof(float), vol, "wb", log)) goto stage_fail;
         }
-        float* off = (float*)calloc((size_t)g->N_r * g->N_t, sizeof(float));
-        ok = off && !write_bin("proj_offset", off, sizeof(float), (size_t)g->N_r * g->N_t, "wb", log);
-        free(off);
+        ok = !write_bin("proj_offset", offset, sizeof(float), (size_t)g->N_r * g->N_t, "wb", log);
         if (!ok) goto stage_fail;
         if (write_bin("variance_estimate", &var_est, sizeof(double), 1, "wb", log) ||
             write_bin("status", &status_value, sizeof(int32_t), 1, "wb", log))
@@ -662,6 +680,7 @@ int reconstruct_ex(float** object, float* projections, float* weights,
   }
 out:
   free(prev);
+  free(offset);
   return rc;
 }
 

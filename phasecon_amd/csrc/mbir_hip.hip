@@ -1492,6 +1492,72 @@ __global__ void k_scale(float* x, long long n, float s) {
   if (i < n) x[i] *= s;
 }
 
+// Ring offset (one thread per detector element (r, t)): over the views, the
+// generalized-Huber weight lambda of every entry -- w where |e sqrt(w)| < T,
+// T delta sqrt(w) / |e| otherwise -- summed, and the numerator of the offset
+// step: sum lambda e with lambda e = T delta sqrt(w) sign(e) outside the
+// selection, or (rhs3) the right-hand side sum lambda (e + d).  FP64 sums.
+__global__ void k_offset_sums(int N_p, int nrt, double T, double delta,
+                              const float* __restrict__ e, const float* __restrict__ w,
+                              const float* __restrict__ d, int rhs3,
+                              double* __restrict__ num, double* __restrict__ lam) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrt) return;
+  const double di = d ? (double)d[i] : 0.0;
+  double a = 0.0, b = 0.0;
+  for (int p = 0; p < N_p; ++p) {
+    const size_t q = (size_t)p * nrt + i;
+    const double ee = e[q], ww = w[q];
+    const double tw = T * delta * sqrt(ww);
+    if (fabs(ee * sqrt(ww)) < T) {
+      b += ww;
+      a += (rhs3 ? ee + di : ee) * ww;
+    } else {
+      const double l = tw / fabs(ee);
+      b += l;
+      a += rhs3 ? (ee + di) * l : tw * (double)((ee > 0) - (ee < 0));
+    }
+  }
+  num[i] = a;
+  lam[i] = b;
+}
+
+// e[p][r][t] -= step[r][t] (FP64 step, the result rounded once)
+__global__ void k_offset_apply(long long n, int nrt, const double* __restrict__ step,
+                               float* __restrict__ e) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) e[q] = (float)((double)e[q] - step[q % nrt]);
+}
+
+// Noise variance: per-block sums of e^2 W where |e sqrt(w)| < T, else
+// |e| delta T sqrt(W var), with W = w var the measured weight (FP64).
+__global__ void k_var_sums(long long n, double T, double delta, double var,
+                           const float* __restrict__ e, const float* __restrict__ w,
+                           double* __restrict__ partial) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < n;
+       q += (long long)gridDim.x * 256) {
+    const double ee = e[q], wc = w[q];
+    const double W = (float)(wc * var);
+    s += fabs(ee * sqrt(wc)) < T ? ee * ee * W : fabs(ee) * delta * T * sqrt(W * var);
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// w := (w var_old) / var_new, each product rounded to FP32 as the stored
+// weights are
+__global__ void k_var_apply(long long n, double var_old, double var_new, float* __restrict__ w) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) w[q] = (float)((double)(float)((double)w[q] * var_old) / var_new);
+}
+
 // the update mask's indicator, column-major [frame][column][z]
 __global__ void k_mask_ones(int Nt, int ncol, int Nz, const uint8_t* __restrict__ mask,
                             float* __restrict__ out) {
@@ -1696,6 +1762,8 @@ struct mbir_hip_ctx {
   float *d_x = nullptr, *d_stage = nullptr, *d_stage2 = nullptr;
   uint8_t* d_mask = nullptr;  // update mask [N_y * N_x]
   float* d_ws = nullptr;      // warm-start sinogram scratch (allocated on first use)
+  std::vector<float> offset;  // ring offset d [N_r][N_t] (host copy of d_d)
+  double* d_rt = nullptr;     // [3][N_r N_t] FP64 sums and step of the offset update
   // captured equits (mbir_hip_icd_graph): the graph, its key and launches
   int* d_iter = nullptr;
   hipGraph_t graph = nullptr;
@@ -1941,6 +2009,8 @@ int mbir_hip_create(mbir_hip_ctx** out, const mbir_hip_geom* g, const float* H,
   UP(c->d_w, w, c->nsino);
   UP(c->d_mask, mask, (size_t)c->ncol);
   if (d) UP(c->d_d, d, (size_t)g->N_r * g->N_t);
+  c->offset.assign((size_t)g->N_r * g->N_t, 0.f);
+  if (d) c->offset.assign(d, d + (size_t)g->N_r * g->N_t);
 #undef UP
   if (hipMemset(c->d_x, 0, c->nvox * sizeof(float)) != hipSuccess ||
       hipMemset(c->d_e, 0, c->nsino * sizeof(float)) != hipSuccess) {
@@ -1958,7 +2028,7 @@ void mbir_hip_destroy(mbir_hip_ctx* c) {
                   c->d_view_frame, c->d_y, c->d_w, c->d_d, c->d_e, c->d_x,
                   c->d_stage, c->d_stage2, c->d_afirst, c->d_aval, c->d_tile_ptr,
                   c->d_tile_cols, c->d_launch_tiles, c->d_stats, c->d_partial, c->d_mask,
-                  c->d_aval16, c->d_ascale, c->d_ws};
+                  c->d_aval16, c->d_ascale, c->d_ws, c->d_rt};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
@@ -2280,6 +2350,173 @@ int mbir_hip_icd_equits(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t
                           c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
   }
+  return 0;
+}
+
+namespace {
+
+// Windows of the rectangular-patch offset constraint: triangular weights
+// over blocks of 2 N_r / round(sqrt N_r) channels by 2 N_t / round(sqrt N_t)
+// rows, half-overlapping, wrapped around both axes, a wrapped sample
+// overwriting (XT_OffsetError.c:46-96).  Returns the number of windows.
+int offset_windows(int N_r, int N_t, std::vector<double>* win) {
+  const int rs = 2 * N_r / (int)(std::sqrt((double)N_r) + 0.5);
+  const int ts = 2 * N_t / (int)(std::sqrt((double)N_t) + 0.5);
+  if (rs / 2 < 1 || ts / 2 < 1) return 0;
+  int m = 0;
+  for (int i = 0; i <= N_r - rs / 2; i += rs / 2)
+    for (int j = 0; j <= N_t - ts / 2; j += ts / 2) {
+      if (win) {
+        win->resize((size_t)(m + 1) * N_r * N_t, 0.0);
+        double* wm = win->data() + (size_t)m * N_r * N_t;
+        for (int k = i; k < i + rs; ++k)
+          for (int l = j; l < j + ts; ++l)
+            wm[(size_t)(k % N_r) * N_t + (l % N_t)] =
+                ((k - i) < rs / 2 ? (k - i + 1) : rs - (k - i)) *
+                (double)((l - j) < ts / 2 ? (l - j + 1) : ts - (l - j));
+      }
+      ++m;
+    }
+  return m;
+}
+
+// a := a^-1 (n x n, Gauss-Jordan with partial pivoting); false if singular
+bool invert_in_place(std::vector<double>& a, int n) {
+  std::vector<double> inv((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) inv[(size_t)i * n + i] = 1.0;
+  for (int c = 0; c < n; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < n; ++r)
+      if (std::fabs(a[(size_t)r * n + c]) > std::fabs(a[(size_t)piv * n + c])) piv = r;
+    if (a[(size_t)piv * n + c] == 0.0) return false;
+    if (piv != c)
+      for (int k = 0; k < n; ++k) {
+        std::swap(a[(size_t)c * n + k], a[(size_t)piv * n + k]);
+        std::swap(inv[(size_t)c * n + k], inv[(size_t)piv * n + k]);
+      }
+    const double p = a[(size_t)c * n + c];
+    for (int k = 0; k < n; ++k) {
+      a[(size_t)c * n + k] /= p;
+      inv[(size_t)c * n + k] /= p;
+    }
+    for (int r = 0; r < n; ++r) {
+      if (r == c) continue;
+      const double f = a[(size_t)r * n + c];
+      if (f == 0.0) continue;
+      for (int k = 0; k < n; ++k) {
+        a[(size_t)r * n + k] -= f * a[(size_t)c * n + k];
+        inv[(size_t)r * n + k] -= f * inv[(size_t)c * n + k];
+      }
+    }
+  }
+  a.swap(inv);
+  return true;
+}
+
+}  // namespace
+
+int mbir_hip_offset_update(mbir_hip_ctx* c, int32_t mode) {
+  if (!c) return fail("null context");
+  if (mode < 1 || mode > 3) return fail("offset mode must be 1, 2 or 3");
+  const mbir_hip_geom& g = c->g;
+  const int nrt = g.N_r * g.N_t;
+  if (!c->d_d) {
+    if (dev_alloc(&c->d_d, (size_t)nrt)) return -1;
+    HIP_OK(hipMemsetAsync(c->d_d, 0, (size_t)nrt * sizeof(float), c->stream));
+  }
+  if (!c->d_rt && dev_alloc(&c->d_rt, (size_t)3 * nrt)) return -1;
+  hipLaunchKernelGGL(k_offset_sums, dim3((nrt + 255) / 256), dim3(256), 0, c->stream, g.N_p, nrt,
+                     g.huber_T, g.huber_delta, c->d_e, c->d_w, c->d_d, mode == 3 ? 1 : 0,
+                     c->d_rt, c->d_rt + nrt);
+  HIP_OK(hipGetLastError());
+  std::vector<double> sums((size_t)2 * nrt), step(nrt);
+  HIP_OK(hipMemcpyAsync(sums.data(), c->d_rt, sums.size() * sizeof(double),
+                        hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  const double* num = sums.data();
+  const double* lam = sums.data() + nrt;
+  std::vector<float>& d = c->offset;
+  if (mode == 1) {  // update_d_offset_unconstrained, XT_ICD_update.c:866-911
+    for (int i = 0; i < nrt; ++i) step[i] = num[i] / lam[i];
+  } else if (mode == 2) {  // zero-mean offset, XT_ICD_update.c:813-863
+    double ns = 0.0, ds = 0.0;
+    for (int i = 0; i < nrt; ++i) {
+      ns += d[i] + num[i] / lam[i];
+      ds += 1.0 / lam[i];
+    }
+    const double gamma = ns / ds;
+    for (int i = 0; i < nrt; ++i) step[i] = (num[i] - gamma) / lam[i];
+  } else {  // windowed zero-sum constraints, XT_ICD_update.c:749-810
+    std::vector<double> A;
+    const int M = offset_windows(g.N_r, g.N_t, &A);
+    if (M == 0) return fail("detector too small for the offset windows");
+    std::vector<double> D((size_t)M * M), v(M), cm(M);
+    for (int i = 0; i < M; ++i)
+      for (int j = 0; j < M; ++j) {
+        double s = 0.0;
+        for (int k = 0; k < nrt; ++k) s += A[(size_t)i * nrt + k] * A[(size_t)j * nrt + k] / lam[k];
+        D[(size_t)i * M + j] = s;
+      }
+    if (!invert_in_place(D, M)) return fail("offset constraint matrix is singular");
+    for (int i = 0; i < M; ++i) {
+      double s = 0.0;
+      for (int k = 0; k < nrt; ++k) s += A[(size_t)i * nrt + k] * num[k] / lam[k];
+      v[i] = s;
+    }
+    for (int i = 0; i < M; ++i) {
+      double s = 0.0;
+      for (int j = 0; j < M; ++j) s += D[(size_t)i * M + j] * v[j];
+      cm[i] = s;
+    }
+    for (int k = 0; k < nrt; ++k) {
+      double s = 0.0;
+      for (int m = 0; m < M; ++m) s += A[(size_t)m * nrt + k] * cm[m];
+      step[k] = (num[k] - s) / lam[k] - d[k];  // new offset minus old
+    }
+  }
+  for (int i = 0; i < nrt; ++i) d[i] = (float)((double)d[i] + step[i]);
+  HIP_OK(hipMemcpyAsync(c->d_rt + 2 * nrt, step.data(), (size_t)nrt * sizeof(double),
+                        hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipMemcpyAsync(c->d_d, d.data(), (size_t)nrt * sizeof(float), hipMemcpyHostToDevice,
+                        c->stream));
+  const long long n = (long long)c->nsino;
+  hipLaunchKernelGGL(k_offset_apply, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                     n, nrt, c->d_rt + 2 * nrt, c->d_e);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mbir_hip_get_offset(mbir_hip_ctx* c, float* d) {
+  if (!c || !d) return fail("null argument");
+  memcpy(d, c->offset.data(), c->offset.size() * sizeof(float));
+  return 0;
+}
+
+int mbir_hip_variance_update(mbir_hip_ctx* c, double* var_est) {
+  if (!c) return fail("null context");
+  const long long n = (long long)c->nsino;
+  const unsigned nb = 1024;
+  if (ensure_partial(c, nb)) return -1;
+  const double var = c->g.var_est;
+  hipLaunchKernelGGL(k_var_sums, dim3(nb), dim3(256), 0, c->stream, n, c->g.huber_T,
+                     c->g.huber_delta, var, c->d_e, c->d_w, c->d_partial);
+  double s = 0.0;
+  if (sum_partials(c, nb, &s)) return -1;
+  const double vnew = s / (double)n;
+  if (!(vnew > 0.0)) return fail("variance estimate is not positive");
+  hipLaunchKernelGGL(k_var_apply, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, n,
+                     var, vnew, c->d_w);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(c->stream));
+  c->g.var_est = vnew;
+  if (var_est) *var_est = vnew;
+  return 0;
+}
+
+int mbir_hip_get_weight(mbir_hip_ctx* c, float* w) {
+  if (!c || !w) return fail("null argument");
+  HIP_OK(hipMemcpy(w, c->d_w, c->nsino * sizeof(float), hipMemcpyDeviceToHost));
   return 0;
 }
 

@@ -195,6 +195,26 @@ class Stage:
         _lib.check(self.L.mbir_hip_icd_graph(self.h, int(n), int(first_iter), int(seed)),
                    "icd_graph")
 
+    def offset_update(self, mode):
+        """One ring-offset update (mbir_hip_offset_update), mode 1, 2 or 3."""
+        _lib.check(self.L.mbir_hip_offset_update(self.h, int(mode)), "offset_update")
+
+    def get_offset(self):
+        d = np.zeros((self.g.N_r, self.g.N_t), np.float32)
+        _lib.check(self.L.mbir_hip_get_offset(self.h, _p(d)), "get_offset")
+        return d
+
+    def variance_update(self):
+        """The noise-variance re-estimate; returns the new variance."""
+        v = C.c_double()
+        _lib.check(self.L.mbir_hip_variance_update(self.h, C.byref(v)), "variance_update")
+        return v.value
+
+    def get_weight(self):
+        w = np.zeros(self.shape_e, np.float32)
+        _lib.check(self.L.mbir_hip_get_weight(self.h, _p(w)), "get_weight")
+        return w
+
     def cost(self, fresh=False, estimate=False):
         """[total, data, prior] FP64 MAP cost; fresh: e recomputed from x;
         estimate: maintained e with FP32 prior potentials (stopping tests)."""
