@@ -18,7 +18,8 @@
 //
 // Kernels
 //   k_conv       implicit-GEMM 3x3x3 convolution, v_mfma_f32_16x16x32_bf16:
-//                one wavefront = 16 voxels x all output channels; forward
+//                one wavefront = 16-64 voxels x all output channels, each
+//                weight fragment feeding every voxel fragment; forward
 //                (bias, ReLU) and backward-data (flipped weights)
 //   k_conv_dw    weight gradient: (16 out channels) x (16 im2col columns)
 //                per wavefront, reduced over a voxel chunk with MFMA, FP32
@@ -89,59 +90,80 @@ struct ConvArgs {
   int cin_pad, kpad, cout, relu;
 };
 
-// Output channel tiles of 16 per wavefront: NT.  Lane l: voxel row l % 16 of
-// the A operand with 8 consecutive k (one tap, 8 channels: a 16-byte load),
-// output channel l % 16 of the B operand; the result holds output channel
-// l % 16 of voxels 4 (l / 16) + r.
-template <int NT>
+// Output channel tiles of 16 per wavefront: NT; 16-voxel fragments per
+// wavefront: MV (each weight fragment feeds MV MFMAs).  Lane l: voxel row
+// l % 16 of an A fragment with 8 consecutive k (one tap, 8 channels: a
+// 16-byte load), output channel l % 16 of the B operand; a result holds
+// output channel l % 16 of voxels 4 (l / 16) + r of its fragment.  A
+// fragment past the last voxel feeds zeros and stores nothing.
+template <int NT, int MV>
 __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
   const int lane = threadIdx.x & 63;
   const long long V = (long long)a.N * a.D * a.H * a.W;
-  const long long v0 = ((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+  const long long v0 = ((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * (16 * MV);
   if (v0 >= V) return;
   const int m = lane & 15, kg = lane >> 4;
-  const long long v = v0 + m;
-  const int x = (int)(v % a.W);
-  long long r = v / a.W;
-  const int y = (int)(r % a.H);
-  r /= a.H;
-  const int z = (int)(r % a.D);
-  const long long n = r / a.D;
-  floatx4 acc[NT];
+  int x[MV], y[MV], z[MV];
+  long long n[MV];
+  bool in[MV];
 #pragma unroll
-  for (int t = 0; t < NT; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (int f = 0; f < MV; ++f) {
+    const long long v = v0 + f * 16 + m;
+    in[f] = v0 + f * 16 < V;
+    x[f] = (int)(v % a.W);
+    long long r = v / a.W;
+    y[f] = (int)(r % a.H);
+    r /= a.H;
+    z[f] = (int)(r % a.D);
+    n[f] = r / a.D;
+  }
+  floatx4 acc[MV][NT];
+#pragma unroll
+  for (int f = 0; f < MV; ++f)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[f][t] = floatx4{0.f, 0.f, 0.f, 0.f};
   for (int k0 = 0; k0 < a.kpad; k0 += 32) {
     const int k = k0 + kg * 8;
     const int tap = k / a.cin_pad, ci = k - tap * a.cin_pad;
-    uint4 raw = make_uint4(0u, 0u, 0u, 0u);
-    if (tap < 27) {
-      const int zz = z + tap / 9 - 1, yy = y + (tap / 3) % 3 - 1, xx = x + tap % 3 - 1;
-      if (zz >= 0 && zz < a.D && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) {
-        const long long vv = ((n * a.D + zz) * a.H + yy) * a.W + xx;
+    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+    bf16x8 av[MV], bv[NT];
+#pragma unroll
+    for (int f = 0; f < MV; ++f) {
+      uint4 raw = make_uint4(0u, 0u, 0u, 0u);
+      const int zz = z[f] + dz, yy = y[f] + dy, xx = x[f] + dx;
+      if (in[f] && tap < 27 && zz >= 0 && zz < a.D && yy >= 0 && yy < a.H && xx >= 0 &&
+          xx < a.W) {
+        const long long vv = ((n[f] * a.D + zz) * a.H + yy) * a.W + xx;
         raw = *reinterpret_cast<const uint4*>(a.in + vv * a.cs_in + a.co_in + ci);
       }
+      av[f] = __builtin_bit_cast(bf16x8, raw);
     }
-    const bf16x8 av = __builtin_bit_cast(bf16x8, raw);
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const uint4 wr =
-          *reinterpret_cast<const uint4*>(a.w + (size_t)(t * 16 + m) * a.kpad + k);
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, __builtin_bit_cast(bf16x8, wr),
-                                                       acc[t], 0, 0, 0);
-    }
+    for (int t = 0; t < NT; ++t)
+      bv[t] = __builtin_bit_cast(
+          bf16x8, *reinterpret_cast<const uint4*>(a.w + (size_t)(t * 16 + m) * a.kpad + k));
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int f = 0; f < MV; ++f)
+        acc[f][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[f], bv[t], acc[f][t], 0, 0, 0);
   }
 #pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    const int co = t * 16 + m;
-    if (co >= a.cout) continue;
-    const float b = a.bias ? a.bias[co] : 0.f;
+  for (int f = 0; f < MV; ++f) {
+    if (!in[f]) continue;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const long long vo = v0 + kg * 4 + q;
-      float o = acc[t][q] + b;
-      if (a.relu) o = fmaxf(o, 0.f);
-      if (a.out) a.out[vo * a.cs_out + a.co_out + co] = f2bf(o);
-      if (a.out32) a.out32[vo * a.cout + co] = o;
+    for (int t = 0; t < NT; ++t) {
+      const int co = t * 16 + m;
+      if (co >= a.cout) continue;
+      const float b = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const long long vo = v0 + f * 16 + kg * 4 + q;
+        float o = acc[f][t][q] + b;
+        if (a.relu) o = fmaxf(o, 0.f);
+        if (a.out) a.out[vo * a.cs_out + a.co_out + co] = f2bf(o);
+        if (a.out32) a.out32[vo * a.cout + co] = o;
+      }
     }
   }
 }
@@ -259,15 +281,18 @@ __global__ void k_pool_bwd(const u16* gout, int cs_o, int co_o, u16* gin, int cs
   gin[dst] = f2bf(bf2f(gin[dst]) + 0.125f * bf2f(gout[p * cs_o + co_o + c]));
 }
 
-// nearest 2x up-sampling: out (2D, 2H, 2W) channel range := in (D, H, W)
+// nearest 2x up-sampling: out (2D, 2H, 2W) channel range := in (D, H, W);
+// one thread per 8 channels (16-byte accesses; channel counts, strides and
+// offsets are multiples of 8)
 __global__ void k_up(const u16* in, int cs_in, int co_in, u16* out, int cs_out, int co_out, int C,
                      int N, int D, int H, int W) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int D2 = 2 * D, H2 = 2 * H, W2 = 2 * W;
-  const long long total = (long long)N * D2 * H2 * W2 * C;
+  const int C8 = C / 8;
+  const long long total = (long long)N * D2 * H2 * W2 * C8;
   if (i >= total) return;
-  const int c = (int)(i % C);
-  long long v = i / C;
+  const int c = (int)(i % C8) * 8;
+  long long v = i / C8;
   const int x = (int)(v % W2);
   v /= W2;
   const int y = (int)(v % H2);
@@ -275,7 +300,8 @@ __global__ void k_up(const u16* in, int cs_in, int co_in, u16* out, int cs_out, 
   const int z = (int)(v % D2);
   const long long n = v / D2;
   const long long src = ((n * D + z / 2) * H + y / 2) * W + x / 2;
-  out[(((n * D2 + z) * H2 + y) * W2 + x) * cs_out + co_out + c] = in[src * cs_in + co_in + c];
+  *reinterpret_cast<uint4*>(out + (((n * D2 + z) * H2 + y) * W2 + x) * cs_out + co_out + c) =
+      *reinterpret_cast<const uint4*>(in + src * cs_in + co_in + c);
 }
 
 // gradient of the up-sampling: gin (D, H, W) := sum of its 8 children
@@ -481,14 +507,17 @@ int launch_conv(hipStream_t s, const Chan& in, const u16* w, int cin_pad, int kp
   if (V % 16) return fail("voxel count must be a multiple of 16");
   if (in.cs % 8 || in.co % 8 || (out && (out->cs < out->co + cout)))
     return fail("channel strides and offsets must be multiples of 8");
-  const dim3 g(blocks_for(V / 16, 4));
+  // fragments per wavefront: 4 for up to 32 output channels, 2 for 48-64,
+  // 1 above (accumulators MV x NT x 4 VGPRs)
+  const int mv = nt <= 2 ? 4 : (nt <= 4 ? 2 : 1);
+  const dim3 g(blocks_for((V + 16 * mv - 1) / (16 * mv), 4));
   switch (nt) {
-    case 1: hipLaunchKernelGGL(k_conv<1>, g, dim3(256), 0, s, a); break;
-    case 2: hipLaunchKernelGGL(k_conv<2>, g, dim3(256), 0, s, a); break;
-    case 3: hipLaunchKernelGGL(k_conv<3>, g, dim3(256), 0, s, a); break;
-    case 4: hipLaunchKernelGGL(k_conv<4>, g, dim3(256), 0, s, a); break;
-    case 6: hipLaunchKernelGGL(k_conv<6>, g, dim3(256), 0, s, a); break;
-    case 8: hipLaunchKernelGGL(k_conv<8>, g, dim3(256), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_conv<1, 4>), g, dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_conv<2, 4>), g, dim3(256), 0, s, a); break;
+    case 3: hipLaunchKernelGGL((k_conv<3, 2>), g, dim3(256), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((k_conv<4, 2>), g, dim3(256), 0, s, a); break;
+    case 6: hipLaunchKernelGGL((k_conv<6, 1>), g, dim3(256), 0, s, a); break;
+    case 8: hipLaunchKernelGGL((k_conv<8, 1>), g, dim3(256), 0, s, a); break;
     default: return fail("unsupported channel count");
   }
   HIP_OK(hipGetLastError());
@@ -621,10 +650,10 @@ int forward(unet_ctx* c) {
   hipLaunchKernelGGL(k_pool, dim3(blocks_for(c->V3 * c2)), dim3(256), 0, c->s, S2.p, S2.cs, S2.co,
                      P2.p, P2.cs, P2.co, c2, N, D / 2, H / 2, W / 2);
   if (conv_layer(c, 4, P2, &A3, nullptr, 2) || conv_layer(c, 5, A3, &B3, nullptr, 2)) return -1;
-  hipLaunchKernelGGL(k_up, dim3(blocks_for(c->V2 * c3)), dim3(256), 0, c->s, B3.p, B3.cs, B3.co,
+  hipLaunchKernelGGL(k_up, dim3(blocks_for(c->V2 * c3 / 8)), dim3(256), 0, c->s, B3.p, B3.cs, B3.co,
                      U2.p, U2.cs, U2.co, c3, N, D / 4, H / 4, W / 4);
   if (conv_layer(c, 6, C2, &A4, nullptr, 1) || conv_layer(c, 7, A4, &B4, nullptr, 1)) return -1;
-  hipLaunchKernelGGL(k_up, dim3(blocks_for(c->V1 * c2)), dim3(256), 0, c->s, B4.p, B4.cs, B4.co,
+  hipLaunchKernelGGL(k_up, dim3(blocks_for(c->V1 * c2 / 8)), dim3(256), 0, c->s, B4.p, B4.cs, B4.co,
                      U1.p, U1.cs, U1.co, c2, N, D / 2, H / 2, W / 2);
   if (conv_layer(c, 8, C1, &A5, nullptr, 0) || conv_layer(c, 9, A5, &B5, nullptr, 0) ||
       conv_layer(c, 10, B5, nullptr, c->o32, 0))
