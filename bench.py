@@ -108,11 +108,20 @@ def algorithmic_bytes_per_equit(st):
 GRAPH = True  # equits as replays of a captured HIP graph (--no-graph: direct launches)
 
 
-def _equit(st, it, seed):
+NH = None  # (passes, fraction): non-homogeneous passes after each equit from the second
+
+
+def _equit(st, it, seed, nh=False):
+    nh = nh and NH
+    if nh and not getattr(st, "_nh_on", False):
+        st.option("nhicd", 1)
+        st._nh_on = True
     if GRAPH:
         st.icd_graph(1, it, seed)
     else:
         st.icd(1, it, seed)
+    if nh and it > 1:
+        st.nhicd(NH[0], NH[1], it, seed + 104729 * it)
 
 
 def to_target(st, target, max_equits, seed, refresh):
@@ -124,7 +133,7 @@ def to_target(st, target, max_equits, seed, refresh):
     cost = st.cost(estimate=True)[0]
     while it < max_equits:
         it += 1
-        _equit(st, it, seed)
+        _equit(st, it, seed, nh=True)
         if refresh and it % refresh == 0:
             st.init_error()
         cost = st.cost(estimate=True)[0]
@@ -492,6 +501,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every equit's sweeps directly instead of replaying a "
                          "captured HIP graph")
+    ap.add_argument("--nh", default="",
+                    help="non-homogeneous ICD after every full-resolution equit from the second: "
+                         "'passes,fraction' (e.g. '1,0.25'); default off")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep --tile / --coarse-tile instead of timing candidates per stage")
     ap.add_argument("--stages", default="",
@@ -531,8 +543,11 @@ def main():
     ap.add_argument("--pcmbir-views", type=int, default=360)
     ap.add_argument("--pcmbir-n", type=int, default=512)
     args = ap.parse_args()
-    global GRAPH
+    global GRAPH, NH
     GRAPH = not args.no_graph
+    if args.nh:
+        p_, f_ = args.nh.split(",")
+        NH = (int(p_), float(f_))
     if args.profile_unet and args.gpus == 1:
         profile_unet(args.profile_unet)
         return
@@ -705,6 +720,7 @@ def main():
                 if coarse else "single full-resolution stage from zero"),
             "refresh_every": args.refresh,
             "equit_launch": "captured HIP graph" if GRAPH else "direct launches",
+            "nhicd": ({"passes": NH[0], "fraction": NH[1]} if NH else None),
             "cstar": cstar,
             "cstar_source": csrc,
             "target_rel": REL_TOL,

@@ -76,7 +76,10 @@ void mbir_hip_destroy(mbir_hip_ctx* ctx);
  * FP16 values; the FP32 copy of A (mbir_hip_get_amatrix, the back-projector)
  * then holds the same rounded values.  The forward projector and the cost
  * compute A on the fly and stay exact.  The reference stores A in FP32
- * (XT_AMatrix.c:50-110); this knob has no counterpart there. */
+ * (XT_AMatrix.c:50-110); this knob has no counterpart there.  "nhicd" (0 or
+ * 1, default 0): every sweep records each visited column's sum of |dx| in a
+ * magnitude map (MagUpdateMap, XT_VoxUpdate.c:158,265), which
+ * mbir_hip_nhicd ranks. */
 int mbir_hip_set_option(mbir_hip_ctx* ctx, const char* key, double value);
 
 /* Object transfer (replaces the Object array of XT_Structures.h:215). */
@@ -149,6 +152,26 @@ int mbir_hip_icd_equits(mbir_hip_ctx* ctx, int32_t n, int32_t first_iter,
  * reference, which runs the same schedule as host loops
  * (XT_ICD_update.c:932-1092). */
 int mbir_hip_icd_graph(mbir_hip_ctx* ctx, int32_t n, int32_t first_iter, uint32_t seed);
+
+/* Non-homogeneous ICD passes (the NHICD loops of updateVoxelsTimeSlices,
+ * XT_ICD_update.c:1007-1046, with the voxel selection of
+ * VSC_based_Voxel_Line_Select, XT_NHICD.c:164-216).  Each pass filters the
+ * magnitude map with the 5x5 Hamming window (ComputeVSC, XT_NHICD.c:126-160)
+ * and then runs the sweep's 8 launches with every workgroup visiting only
+ * the floor(fraction n) (at least one) of its tile's n columns whose
+ * filtered magnitude is highest; the visited columns' magnitudes are
+ * refreshed for the next pass.  The reference selects the top fraction of
+ * each frame's lines with a quickselect threshold; the per-tile selection
+ * here keeps the workgroups of a launch equally long.  Needs option "nhicd"
+ * = 1 and a tile of at most 16 x 16 columns.  iter is the iteration number
+ * (zero skipping from 2); stats as for mbir_hip_icd_equits, over the
+ * passes. */
+int mbir_hip_nhicd(mbir_hip_ctx* ctx, int32_t passes, double fraction, int32_t iter,
+                   uint32_t seed, double* stats);
+
+/* The magnitude map [N_time][N_y][N_x] (filtered = 0) or its VSC filtering
+ * (filtered = 1).  Needs option "nhicd" = 1. */
+int mbir_hip_get_magnitudes(mbir_hip_ctx* ctx, int32_t filtered, float* out);
 
 /* One ring-offset update (update_Sinogram_Offset, XT_ICD_update.c:915-923)
  * of the offset d [N_r][N_t] and the error sinogram together (e + d is

@@ -241,6 +241,39 @@ def variance_update(var, w, e, sel, T, delta):
     return v.value, w, sel
 
 
+NHICD_ITERATIONS = 10  # NHICD_Iterations (XT_Init.c:464)
+
+
+def vsc(mag):
+    """ComputeVSC (XT_NHICD.c:126-160): the magnitude map(s) [..., N_y, N_x]
+    filtered with the 5 x 5 Hamming window, the outer product of the
+    5-point window 0.54 - 0.46 cos(2 pi n / 4), truncated at the edges;
+    accumulated in float64, stored as float32 as the reference does."""
+    mag = np.asarray(mag, np.float64)
+    h = 0.54 - 0.46 * np.cos(2.0 * np.pi * np.arange(5) / 4.0)
+    W = np.outer(h, h)
+    ny, nx = mag.shape[-2:]
+    pad = np.zeros(mag.shape[:-2] + (ny + 4, nx + 4))
+    pad[..., 2:-2, 2:-2] = mag
+    out = np.zeros_like(mag)
+    for p in range(5):
+        for q in range(5):
+            out += W[p, q] * pad[..., p:p + ny, q:q + nx]
+    return out.astype(np.float32)
+
+
+def vsc_select(filt, count, iterations=NHICD_ITERATIONS):
+    """VSC_based_Voxel_Line_Select (XT_NHICD.c:164-216) for one frame's
+    filtered map [N_y, N_x]: k = floor(count / iterations) lines, the
+    threshold the (N - k)-th smallest value of the whole map
+    (RandomizedSelect, XT_NHICD.c:60-80), every line strictly above it
+    selected (fewer than k when values tie at the threshold)."""
+    flat = np.asarray(filt).ravel()
+    k = count // iterations
+    thresh = np.sort(flat)[flat.size - k - 1] if k > 0 else np.inf
+    return np.asarray(filt) > thresh
+
+
 def reference_available():
     return os.path.exists(REF_SO)
 
@@ -391,6 +424,21 @@ class Reference:
         sel = np.array(sel, np.uint8)
         v = self.L.ref_variance_update(self.h, float(var), self._p(w), self._p(e), self._p(sel))
         return v, w, sel
+
+    def vsc(self, mag):
+        """ComputeVSC of one map [N_y, N_x]."""
+        mag = np.ascontiguousarray(mag, np.float32)
+        out = np.zeros_like(mag)
+        self.L.ref_vsc(self.h, self._p(mag), self._p(out))
+        return out
+
+    def vsc_select(self, mag, count):
+        """VSC_based_Voxel_Line_Select on maps [N_time, N_y, N_x] with
+        UpdateSelectNum = count; returns the selected lines as a bool map."""
+        mag = np.ascontiguousarray(mag, np.float32)
+        sel = np.zeros(mag.shape, np.uint8)
+        self.L.ref_vsc_select(self.h, self._p(mag), C.c_int32(count), self._p(sel))
+        return sel.astype(bool)
 
     def close(self):
         if self.h:

@@ -34,6 +34,7 @@
 #include "XT_AMatrix.h"
 #include "XT_ICD_update.h"
 #include "XT_Init.h"
+#include "XT_NHICD.h"
 #include "XT_Profile.h"
 #include "XT_VoxUpdate.h"
 #include "allocate.h"
@@ -49,6 +50,7 @@ int updateVoxelsTimeSlices(Sinogram*, ScannedObject*, TomoInputs*,
 void update_Sinogram_Offset(Sinogram*, TomoInputs*, Real_arr_t***);
 void update_variance_parameter(Sinogram*, TomoInputs*, Real_arr_t***);
 void gen_offset_constraint_windows(Sinogram*, TomoInputs*);
+void ComputeVSC(ScannedObject*, Real_arr_t**, Real_arr_t**, Real_arr_t**);
 int reconstruct(float**, float*, float*, float*, float*, float*, int32_t,
                 int32_t, int32_t, int32_t, float, float, float, float, float,
                 float, float, int32_t, int32_t, float, float, uint8_t, FILE*);
@@ -384,6 +386,45 @@ double ref_variance_update(void* h, double var, float* w, const float* e, uint8_
   update_variance_parameter(&c->S, &c->T, c->esino);
   ref_get_sino(c, w, NULL, sel);
   return c->T.var_est;
+}
+
+/* ComputeVSC (XT_NHICD.c:126-160) of one magnitude map [N_y][N_x] into
+ * out [N_y][N_x]. */
+void ref_vsc(void* h, const float* mag, float* out) {
+  ref_ctx* c = (ref_ctx*)h;
+  const size_t n = (size_t)c->O.N_y * c->O.N_x;
+  Real_arr_t** m = (Real_arr_t**)multialloc(sizeof(Real_arr_t), 2, c->O.N_y, c->O.N_x);
+  Real_arr_t** f = (Real_arr_t**)multialloc(sizeof(Real_arr_t), 2, c->O.N_y, c->O.N_x);
+  Real_arr_t** f2 = (Real_arr_t**)multialloc(sizeof(Real_arr_t), 2, c->O.N_y, c->O.N_x);
+  memcpy(&m[0][0], mag, sizeof(float) * n);
+  ComputeVSC(&c->O, m, f, f2);
+  memcpy(out, &f[0][0], sizeof(float) * n);
+  multifree(m, 2);
+  multifree(f, 2);
+  multifree(f2, 2);
+}
+
+/* VSC_based_Voxel_Line_Select (XT_NHICD.c:164-216) on the magnitude maps
+ * mag [N_time][N_y][N_x] (the same map in every z-block) with
+ * UpdateSelectNum = count lines per frame and block.  sel [N_time][N_y][N_x]
+ * receives 1 for every line z-block 0's lists select; returns the number
+ * selected in frame 0. */
+int32_t ref_vsc_select(void* h, const float* mag, int32_t count, uint8_t* sel) {
+  ref_ctx* c = (ref_ctx*)h;
+  const int32_t nt = c->O.N_time, nb = c->T.num_z_blocks, ny = c->O.N_y, nx = c->O.N_x;
+  Real_arr_t**** m = (Real_arr_t****)multialloc(sizeof(Real_arr_t), 4, nt, nb, ny, nx);
+  for (int32_t i = 0; i < nt; i++)
+    for (int32_t b = 0; b < nb; b++) {
+      memcpy(&m[i][b][0][0], mag + (size_t)i * ny * nx, sizeof(float) * ny * nx);
+      c->T.UpdateSelectNum[i][b] = count;
+    }
+  VSC_based_Voxel_Line_Select(&c->O, &c->T, m);
+  memset(sel, 0, (size_t)nt * ny * nx);
+  for (int32_t i = 0; i < nt; i++)
+    for (int32_t k = 0; k < c->T.NHICDSelectNum[i][0]; k++)
+      sel[((size_t)i * ny + c->T.y_NHICD_select[i][0][k]) * nx + c->T.x_NHICD_select[i][0][k]] = 1;
+  multifree(m, 4);
+  return c->T.NHICDSelectNum[0][0];
 }
 
 void ref_destroy(void* h) {

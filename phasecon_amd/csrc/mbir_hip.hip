@@ -1064,9 +1064,17 @@ struct SweepArgs {
   float alpha;
   float* x;
   double* stats;
+  // non-homogeneous ICD: mag, when set, receives each visited column's sum of
+  // |dx| ([N_time][N_y N_x]); nh_frac > 0 makes each workgroup visit only the
+  // floor(nh_frac n) (at least one) of its tile's n columns that rank highest
+  // in nh_rank (same layout), ties to the earlier list entry
+  float* mag;
+  const float* nh_rank;
+  float nh_frac;
 };
 
 constexpr int kSweepThreads = 2 * kWave;
+constexpr int kNhMax = 2 * kSweepThreads;  // tile columns a non-homogeneous pass can rank
 
 // One ICD pass over the voxel columns of the tiles in this launch (one
 // 2x2-checkerboard colour) for the frames of one parity, one workgroup of two
@@ -1105,7 +1113,13 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SwGeom G, SweepArgs A) 
   const int f = A.fparity + 2 * (wid / A.n_launch_tiles);
   if (f >= G.N_time) return;
   const int c0 = A.tile_ptr[tile];
-  const int n = A.tile_ptr[tile + 1] - c0;
+  const int n_all = A.tile_ptr[tile + 1] - c0;
+  int n = n_all;
+  if (A.nh_frac > 0.f) {
+    n = (int)(A.nh_frac * (float)n_all);
+    if (n < 1) n = 1;
+    if (n > n_all) n = n_all;
+  }
   const int s0 = A.frame_ptr[f];
   const int nv = A.frame_ptr[f + 1] - s0;
   if (n == 0 || nv == 0) return;
@@ -1139,8 +1153,37 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SwGeom G, SweepArgs A) 
     for (int u = 0; u < 8; ++u)
       if (j0 + u * kSweepThreads < nx) L.x[j0 + u * kSweepThreads] = v[u];
   }
-  for (int j = tid; j < n; j += kSweepThreads) L.cols[j] = A.tile_cols[c0 + j];
+  for (int j = tid; j < n_all; j += kSweepThreads) L.cols[j] = A.tile_cols[c0 + j];
   for (int j = tid; j < nv; j += kSweepThreads) L.views[j] = A.col.frame_views[s0 + j];
+  if (n < n_all) {
+    // non-homogeneous pass: rank the tile's columns by nh_rank (the count of
+    // columns ranked above, ties to the earlier entry, so the ranks are a
+    // permutation) and compact the n highest to the front of the list
+    const float* rk = A.nh_rank + (size_t)f * G.N_y * G.N_x;
+    for (int j = tid; j < n_all; j += kSweepThreads) s_row1[j] = rk[L.cols[j]];
+    __syncthreads();
+    int keep_c[kNhMax / kSweepThreads], keep_r[kNhMax / kSweepThreads];
+#pragma unroll
+    for (int u = 0; u < kNhMax / kSweepThreads; ++u) {
+      const int j = tid + u * kSweepThreads;
+      keep_r[u] = n_all;
+      keep_c[u] = 0;
+      if (j < n_all) {
+        const float v = s_row1[j];
+        int r = 0;
+        for (int i = 0; i < n_all; ++i) {
+          const float o = s_row1[i];
+          r += (o > v || (o == v && i < j)) ? 1 : 0;
+        }
+        keep_r[u] = r;
+        keep_c[u] = L.cols[j];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kNhMax / kSweepThreads; ++u)
+      if (keep_r[u] < n) L.cols[keep_r[u]] = keep_c[u];
+  }
   __syncthreads();
   int pos = pb;  // visiting order: pos_k = (pb + k pa) mod n
   // the theta pipeline applies when a column's window is one load batch per
@@ -1253,6 +1296,13 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SwGeom G, SweepArgs A) 
       col_solve(G, R, L.x, centre0, 1, iter, A.alpha, s_fnum, s_fden, s_fpos, s_row1, s_row2,
                 s_dx, st);
       __syncthreads();
+      if (A.mag) {
+        // the column's sum of |dx| (s_dx holds it until the next solve)
+        float m = 0.f;
+        for (int z = lane_id(); z < Nz; z += kWave) m += fabsf(s_dx[z]);
+        for (int off = 32; off > 0; off >>= 1) m += __shfl_xor(m, off);
+        if (lane_id() == 0) A.mag[(size_t)f * G.N_y * G.N_x + c] = m;
+      }
     }
   }
   __syncthreads();
@@ -1278,6 +1328,27 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SwGeom G, SweepArgs A) 
       atomicAdd(&A.stats[3], st.upd);
     }
   }
+}
+
+// Voxel-selection criterion of non-homogeneous ICD: the magnitude map
+// filtered with the separable 5x5 Hamming window h = 0.54 - 0.46 cos(2 pi n / 4)
+// = (0.08, 0.54, 1, 0.54, 0.08) per axis, truncated at the grid's edges
+// (XT_NHICD.c:126-160).
+__global__ __launch_bounds__(256) void k_vsc(int N_time, int N_y, int N_x,
+                                             const float* __restrict__ mag,
+                                             float* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (long long)N_time * N_y * N_x) return;
+  const int x = (int)(i % N_x), y = (int)((i / N_x) % N_y);
+  const float* m = mag + (i - ((long long)y * N_x + x));
+  const float h[5] = {0.08f, 0.54f, 1.f, 0.54f, 0.08f};
+  float acc = 0.f;
+  for (int p = -2; p <= 2; ++p) {
+    if (y + p < 0 || y + p >= N_y) continue;
+    for (int q = -2; q <= 2; ++q)
+      if (x + q >= 0 && x + q < N_x) acc += h[p + 2] * h[q + 2] * m[(y + p) * N_x + x + q];
+  }
+  out[i] = acc;
 }
 
 // theta1/theta2 of individual voxels through the sweep's column routine.
@@ -1768,8 +1839,12 @@ struct mbir_hip_ctx {
   int* d_iter = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
-  long long gkey[5] = {-1, -1, -1, -1, -1};
+  long long gkey[6] = {-1, -1, -1, -1, -1, -1};
   int glaunches = 0;
+  // non-homogeneous ICD (option "nhicd"): per (frame, column) sum of |dx| of
+  // the column's last visit and its VSC-filtered ranking, [N_time][N_y N_x]
+  bool nh = false;
+  float *d_mag = nullptr, *d_vsc = nullptr;
   int* d_afirst = nullptr;
   float* d_aval = nullptr;
   bool afp16 = false;                 // the sweep streams FP16 A + column scales
@@ -2028,7 +2103,7 @@ void mbir_hip_destroy(mbir_hip_ctx* c) {
                   c->d_view_frame, c->d_y, c->d_w, c->d_d, c->d_e, c->d_x,
                   c->d_stage, c->d_stage2, c->d_afirst, c->d_aval, c->d_tile_ptr,
                   c->d_tile_cols, c->d_launch_tiles, c->d_stats, c->d_partial, c->d_mask,
-                  c->d_aval16, c->d_ascale, c->d_ws, c->d_rt};
+                  c->d_aval16, c->d_ascale, c->d_ws, c->d_rt, c->d_mag, c->d_vsc};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
@@ -2059,6 +2134,14 @@ int mbir_hip_set_option(mbir_hip_ctx* c, const char* key, double value) {
     if (value != 0 && value != 1) return fail("afp16 must be 0 or 1");
     if (c->afp16 != (value == 1)) c->a_built = false;  // A must be built again
     c->afp16 = value == 1;
+  } else if (k == "nhicd") {
+    if (value != 0 && value != 1) return fail("nhicd must be 0 or 1");
+    if (value == 1 && !c->d_mag) {
+      const size_t nm = (size_t)c->ncol * c->g.N_time;
+      if (dev_alloc(&c->d_mag, nm) || dev_alloc(&c->d_vsc, nm)) return -1;
+      HIP_OK(hipMemset(c->d_mag, 0, nm * sizeof(float)));
+    }
+    c->nh = value == 1;
   } else {
     return fail("unknown option " + k);
   }
@@ -2264,8 +2347,13 @@ namespace {
 // timed; otherwise the iteration is *iter_dev + it (graph capture: no event
 // records).  Returns the number of launches.
 int sweep_launches(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t seed,
-                   const int* iter_dev, int* launches) {
+                   const int* iter_dev, int* launches, float nh_frac = 0.f) {
   SweepArgs A;
+  A.mag = c->nh ? c->d_mag : nullptr;
+  A.nh_rank = c->d_vsc;
+  A.nh_frac = nh_frac;
+  if (nh_frac > 0.f && (c->tile * c->tile > kNhMax || !c->nh))
+    return fail("non-homogeneous passes need option nhicd and a tile of at most 256 columns");
   A.col.aval = c->d_aval;
   A.col.aval16 = c->afp16 ? c->d_aval16 : nullptr;
   A.col.ascale = c->afp16 ? c->d_ascale : nullptr;
@@ -2520,14 +2608,56 @@ int mbir_hip_get_weight(mbir_hip_ctx* c, float* w) {
   return 0;
 }
 
+int mbir_hip_nhicd(mbir_hip_ctx* c, int32_t passes, double fraction, int32_t iter,
+                   uint32_t seed, double* stats) {
+  if (!c) return fail("null context");
+  if (!c->a_built) return fail("A not built");
+  if (!c->nh) return fail("option nhicd is off: no magnitude map");
+  if (passes < 0 || !(fraction > 0.0 && fraction <= 1.0))
+    return fail("passes must be >= 0 and fraction in (0, 1]");
+  if (build_tiles(c)) return -1;
+  HIP_OK(hipMemsetAsync(c->d_stats, 0, 4 * sizeof(double), c->stream));
+  const long long nm = (long long)c->ncol * c->g.N_time;
+  for (int p = 0; p < passes; ++p) {
+    hipLaunchKernelGGL(k_vsc, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, c->stream,
+                       c->g.N_time, c->g.N_y, c->g.N_x, c->d_mag, c->d_vsc);
+    int launches = 0;
+    if (sweep_launches(c, 1, iter, seed + 7919u * (unsigned)(p + 1), nullptr, &launches,
+                       (float)fraction))
+      return -1;
+  }
+  HIP_OK(hipGetLastError());
+  if (stats) {
+    HIP_OK(hipMemcpyAsync(stats, c->d_stats, 4 * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+  }
+  return 0;
+}
+
+int mbir_hip_get_magnitudes(mbir_hip_ctx* c, int32_t filtered, float* out) {
+  if (!c || !out) return fail("null argument");
+  if (!c->nh) return fail("option nhicd is off: no magnitude map");
+  const long long nm = (long long)c->ncol * c->g.N_time;
+  if (filtered)
+    hipLaunchKernelGGL(k_vsc, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, c->stream,
+                       c->g.N_time, c->g.N_y, c->g.N_x, c->d_mag, c->d_vsc);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(out, filtered ? c->d_vsc : c->d_mag, nm * sizeof(float),
+                        hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
 int mbir_hip_icd_graph(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t seed) {
   if (!c) return fail("null context");
   if (!c->a_built) return fail("A not built");
   if (n < 1) return fail("n must be positive");
   if (build_tiles(c)) return -1;
   if (!c->d_iter && dev_alloc(&c->d_iter, 1)) return -1;
-  const long long key[5] = {n, (long long)seed, c->tile, c->afp16 ? 1 : 0,
-                            (long long)((c->alpha > 0 ? c->alpha : c->g.alpha) * 1e9)};
+  const long long key[6] = {n, (long long)seed, c->tile, c->afp16 ? 1 : 0,
+                            (long long)((c->alpha > 0 ? c->alpha : c->g.alpha) * 1e9),
+                            c->nh ? 1 : 0};
   if (!c->gexec || memcmp(key, c->gkey, sizeof(key)) != 0) {
     if (c->gexec) HIP_OK(hipGraphExecDestroy(c->gexec));
     if (c->graph) HIP_OK(hipGraphDestroy(c->graph));

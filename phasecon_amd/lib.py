@@ -65,6 +65,9 @@ _SIGS = {
     "mbir_hip_backproject": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "mbir_hip_icd_equits": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_uint32, C.c_void_p]),
     "mbir_hip_icd_graph": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_uint32]),
+    "mbir_hip_nhicd": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_int32, C.c_uint32,
+                                 C.c_void_p]),
+    "mbir_hip_get_magnitudes": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     "mbir_hip_offset_update": (C.c_int, [C.c_void_p, C.c_int32]),
     "mbir_hip_get_offset": (C.c_int, [C.c_void_p, C.c_void_p]),
     "mbir_hip_variance_update": (C.c_int, [C.c_void_p, C.c_void_p]),

@@ -195,6 +195,23 @@ class Stage:
         _lib.check(self.L.mbir_hip_icd_graph(self.h, int(n), int(first_iter), int(seed)),
                    "icd_graph")
 
+    def nhicd(self, passes, fraction, iteration, seed=761521, stats=False):
+        """Non-homogeneous passes over the top `fraction` of each tile's
+        columns by filtered update magnitude (mbir_hip_nhicd); needs
+        option("nhicd", 1).  Asynchronous unless stats, which returns the
+        stats of icd()."""
+        st = np.zeros(4, np.float64) if stats else None
+        _lib.check(self.L.mbir_hip_nhicd(self.h, int(passes), float(fraction), int(iteration),
+                                         int(seed), _p(st) if stats else None), "nhicd")
+        return st
+
+    def magnitudes(self, filtered=False):
+        """The per-column |dx| map [N_time][N_y][N_x] or its VSC filtering."""
+        out = np.zeros((self.shape_x[0],) + self.shape_x[2:], np.float32)
+        _lib.check(self.L.mbir_hip_get_magnitudes(self.h, int(bool(filtered)), _p(out)),
+                   "get_magnitudes")
+        return out
+
     def offset_update(self, mode):
         """One ring-offset update (mbir_hip_offset_update), mode 1, 2 or 3."""
         _lib.check(self.L.mbir_hip_offset_update(self.h, int(mode)), "offset_update")

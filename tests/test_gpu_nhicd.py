@@ -1,0 +1,109 @@
+"""Non-homogeneous ICD on the GPU (option "nhicd", mbir_hip_nhicd,
+mbir_hip_get_magnitudes): the sweep's magnitude map against the object's
+actual change, its VSC filtering against the oracle (itself pinned to the
+reference's ComputeVSC), the columns a non-homogeneous pass visits against
+the per-tile top-k of that filtering, and convergence to the same MAP cost
+as the homogeneous sweep."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _stage(scan, tile, nh=True):
+    from phasecon_amd.mbir import Stage
+    st = Stage(scan, tile=tile, alpha=1.9)
+    if nh:
+        st.option("nhicd", 1)
+    st.set_object(np.zeros(st.shape_x, np.float32))
+    st.build_amatrix()
+    st.init_error()
+    return st
+
+
+def test_magnitude_map_is_each_columns_update(small4d_scan):
+    """One equit visits every column once: the map holds sum_z |dx|."""
+    st = _stage(small4d_scan, 32)
+    st.icd(2, 1, 23)
+    x0 = st.get_object()
+    st.icd(1, 3, 23)
+    x1 = st.get_object()
+    mag = st.magnitudes()
+    st.close()
+    expect = np.abs(x1.astype(np.float64) - x0).sum(axis=1)
+    print(f"map: max {mag.max():.4g}, max |diff| {np.abs(mag - expect).max():.3g}", flush=True)
+    assert mag.max() > 0
+    assert np.allclose(mag, expect, rtol=1e-5, atol=1e-6 * expect.max())
+
+
+def test_vsc_filter_matches_oracle(small4d_scan):
+    from oracle.oracle import vsc
+    st = _stage(small4d_scan, 8)
+    st.icd(3, 1, 23)
+    mag = st.magnitudes()
+    filt = st.magnitudes(filtered=True)
+    st.close()
+    ref = vsc(mag)
+    assert np.allclose(filt, ref, rtol=1e-5, atol=1e-6 * ref.max())
+
+
+@pytest.mark.parametrize("fraction", [0.25, 0.1])
+def test_nh_pass_visits_the_top_columns_of_each_tile(small4d_scan, fraction):
+    T = 8
+    st = _stage(small4d_scan, T)
+    st.icd(3, 1, 23)
+    mag0 = st.magnitudes()
+    filt = st.magnitudes(filtered=True)
+    x0 = st.get_object()
+    st.nhicd(1, fraction, 4, 99)
+    x1 = st.get_object()
+    mag1 = st.magnitudes()
+    mask = st.mask.astype(bool)
+    st.close()
+    changed = (x1 != x0).any(axis=1)
+    dsum = np.abs(x1.astype(np.float64) - x0).sum(axis=1)
+    nt, ny, nx = mag0.shape
+    visited = np.zeros_like(changed)
+    for t in range(nt):
+        for ty in range(0, ny, T):
+            for tx in range(0, nx, T):
+                yy, xx = np.nonzero(mask[ty:ty + T, tx:tx + T])   # row-major list order
+                if yy.size == 0:
+                    continue
+                yy, xx = yy + ty, xx + tx
+                k = max(1, int(np.float32(fraction) * np.float32(yy.size)))
+                v = filt[t, yy, xx]
+                order = np.lexsort((np.arange(v.size), -v))   # value desc, then list order
+                top = order[:k]
+                visited[t, yy[top], xx[top]] = True
+    print(f"fraction {fraction}: {visited.sum()} columns selected, {changed.sum()} changed",
+          flush=True)
+    assert not (changed & ~visited).any()
+    assert changed.sum() > 0.5 * visited.sum()
+    # the visited columns' magnitudes are refreshed, the others untouched
+    assert np.allclose(mag1[visited], dsum[visited], rtol=1e-5, atol=1e-6 * dsum.max())
+    assert np.array_equal(mag1[~visited], mag0[~visited])
+
+
+def test_nh_converges_to_the_homogeneous_optimum(small4d_scan):
+    """Graph-captured equits, each followed (from the second) by two
+    non-homogeneous passes over a quarter of every tile, reach the same MAP
+    cost as the homogeneous sweep (1e-6 after 120 equits)."""
+    a = _stage(small4d_scan, 8, nh=False)
+    b = _stage(small4d_scan, 8)
+    costs = []
+    for it in range(1, 121):
+        a.icd(1, it, 5)
+        b.icd_graph(1, it, 5)
+        if it > 1:
+            b.nhicd(2, 0.25, it, 1000 + it)
+        if it % 10 == 0:
+            a.init_error()
+            b.init_error()
+        if it in (5, 20):
+            costs.append((a.cost(fresh=True)[0], b.cost(fresh=True)[0]))
+    fa, fb = a.cost(fresh=True)[0], b.cost(fresh=True)[0]
+    a.close()
+    b.close()
+    print(f"equits 5, 20: homogeneous / with NH {costs}; 120: {fa:.12g} {fb:.12g}", flush=True)
+    assert abs(fa - fb) <= 1e-6 * abs(fa)
