@@ -26,6 +26,19 @@ extern "C" {
 int32_t mbir_multires_schedule(int32_t proj_rows, int32_t proj_cols,
                                int32_t* mult_xy, int32_t* mult_z);
 
+/* Time-interlaced view schedule of the APS driver
+ * (gen_interlaced_views_0_to_Inf,
+ * reconstruct/real_data/APS_Data_Format/XT_Views.c:7-46).  Candidate view i
+ * has index i K + bitrev(floor(i / L) mod K) (L = N_theta / K, the bit
+ * reversal over round(log2 K) bits), angle index * pi / N_theta and time
+ * index / K; a candidate is kept when its angle exceeds the last kept
+ * one's by more than rot_speed * min_acq_time (the first always is).
+ * Kept views proj_start .. proj_start + N_p - 1 go to views / times.
+ * Returns proj_start + N_p, or -1 on a bad argument. */
+int32_t mbir_interlaced_views(float* views, float* times, int32_t K, int32_t N_theta,
+                              int32_t proj_start, int32_t N_p, float min_acq_time,
+                              float rot_speed);
+
 /* View -> frame map (MBIR_4D/XT_Init.c:69-96).  frame_ptr[N_time+1],
  * frame_views[<= N_p].  Returns -1 if times decrease (XT_Init.c:109-113),
  * else the number of assigned views. */

@@ -169,6 +169,23 @@ int mbir_hip_icd_graph(mbir_hip_ctx* ctx, int32_t n, int32_t first_iter, uint32_
 int mbir_hip_nhicd(mbir_hip_ctx* ctx, int32_t passes, double fraction, int32_t iter,
                    uint32_t seed, double* stats);
 
+/* Flat-field normalisation of raw APS counts (read_data,
+ * reconstruct/real_data/APS_Data_Format/XT_HDFIO.c:138-178), on the GPU,
+ * from host buffers the caller has read from its HDF5 files (the
+ * reference's hyperslabs: white and dark frames 1 .. N-2, the rows from
+ * datafile_row0, the channels centred): proj [n_proj][rows ratio][cols
+ * ratio], white [n_white][..], dark [n_dark][..] uint16.  The white and
+ * dark frames are averaged per pixel; per output sample (view, row,
+ * channel), over its ratio x ratio block of pixels, weights = sum |proj -
+ * dark|, and with t = log(sum |white - dark| / weights), projections =
+ * bh_coef t^2 + t (BH_QUAD_COEF = 0.5 in the reference).  Outputs
+ * [n_proj][rows][cols] float, in the reference's arithmetic (float sums in
+ * its order, double log and polynomial). */
+int mbir_hip_aps_normalize(const uint16_t* proj, const uint16_t* white, const uint16_t* dark,
+                           int32_t n_proj, int32_t n_white, int32_t n_dark, int32_t rows,
+                           int32_t cols, int32_t ratio, double bh_coef, float* projections,
+                           float* weights);
+
 /* The magnitude map [N_time][N_y][N_x] (filtered = 0) or its VSC filtering
  * (filtered = 1).  Needs option "nhicd" = 1. */
 int mbir_hip_get_magnitudes(mbir_hip_ctx* ctx, int32_t filtered, float* out);

@@ -19,6 +19,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle_mbir.so")
 REF_SO = os.path.join(HERE, "_ref", "libmbir4d_ref.so")
+VIEWS_REF_SO = os.path.join(HERE, "_ref", "libviews_ref.so")
 NB = 65  # detector response table entries per view
 
 f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
@@ -272,6 +273,61 @@ def vsc_select(filt, count, iterations=NHICD_ITERATIONS):
     k = count // iterations
     thresh = np.sort(flat)[flat.size - k - 1] if k > 0 else np.inf
     return np.asarray(filt) > thresh
+
+
+def aps_normalize(proj, white, dark, ratio, bh_coef=0.5):
+    """read_data's flat-field step (reconstruct/real_data/APS_Data_Format/
+    XT_HDFIO.c:138-178) in the reference's arithmetic: float32 sums in its
+    order (frames; then block rows outer, channels inner), the quotient in
+    float32, log and the polynomial in float64.  Returns (projections,
+    weights) [views][rows][cols]."""
+    f32 = np.float32
+    proj, white, dark = (np.asarray(a, np.uint16) for a in (proj, white, dark))
+    mw = np.zeros(white.shape[1:], f32)
+    for fr in white:
+        mw = mw + fr.astype(f32)
+    mw = mw / f32(white.shape[0])
+    md = np.zeros(dark.shape[1:], f32)
+    for fr in dark:
+        md = md + fr.astype(f32)
+    md = md / f32(dark.shape[0])
+    n, R, Cc = proj.shape
+    rows, cols = R // ratio, Cc // ratio
+    w = np.zeros((n, rows, cols), f32)
+    wd = np.zeros((rows, cols), f32)
+    for m in range(ratio):
+        for q in range(ratio):
+            dk = md[m::ratio, q::ratio][:rows, :cols]
+            w = w + np.abs(proj[:, m::ratio, q::ratio][:, :rows, :cols].astype(f32) - dk)
+            wd = wd + np.abs(mw[m::ratio, q::ratio][:rows, :cols] - dk)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t = np.log((wd[None] / w).astype(np.float64)).astype(f32).astype(np.float64)
+    return (bh_coef * t * t + t).astype(f32), w
+
+
+def reference_interlaced_views(K, n_theta, n_views, proj_start=0, min_acq_time=0.0,
+                               rot_speed=0.0):
+    """The APS driver's own gen_interlaced_views_0_to_Inf (XT_Views.c:7-46),
+    from the reference build; returns (views, times, count)."""
+    L = _lib(VIEWS_REF_SO)
+    libc = C.CDLL(None)
+    libc.fopen.restype = C.c_void_p
+    libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+    libc.fclose.argtypes = [C.c_void_p]
+    fp = libc.fopen(b"/dev/null", b"w")
+    views = np.zeros(n_views, np.float32)
+    times = np.zeros(n_views, np.float32)
+    L.gen_interlaced_views_0_to_Inf.restype = C.c_int32
+    n = L.gen_interlaced_views_0_to_Inf(
+        views.ctypes.data_as(C.c_void_p), times.ctypes.data_as(C.c_void_p), C.c_int32(K),
+        C.c_int32(n_theta), C.c_int32(proj_start), C.c_int32(n_views), C.c_float(min_acq_time),
+        C.c_float(rot_speed), C.c_void_p(fp))
+    libc.fclose(fp)
+    return views, times, n
+
+
+def views_reference_available():
+    return os.path.exists(VIEWS_REF_SO)
 
 
 def reference_available():

@@ -66,6 +66,40 @@ int32_t mbir_multires_schedule(int32_t proj_rows, int32_t proj_cols,
   return n;
 }
 
+int32_t mbir_interlaced_views(float* views, float* times, int32_t K, int32_t N_theta,
+                              int32_t proj_start, int32_t N_p, float min_acq_time,
+                              float rot_speed) {
+  if (!views || !times || K < 1 || N_theta < K || proj_start < 0 || N_p < 1) return -1;
+  const int32_t L = N_theta / K;
+  const int32_t bits = (int32_t)floor(log((double)K) / log(2.0) + 0.5);
+  const float dtheta = (float)(M_PI / N_theta);
+  const float min_step = rot_speed * min_acq_time;
+  const int32_t total = proj_start + N_p;
+  float* ang = (float*)malloc(sizeof(float) * (size_t)total);
+  float* tim = (float*)malloc(sizeof(float) * (size_t)total);
+  if (!ang || !tim) {
+    free(ang);
+    free(tim);
+    return -1;
+  }
+  int32_t kept = 0;
+  for (int32_t i = 0; kept < total; ++i) {
+    const int32_t sub = (i / L) % K;
+    int32_t rev = 0;
+    for (int32_t b = 0; b < bits; ++b)
+      if (sub & (1 << b)) rev |= 1 << (bits - 1 - b);
+    const float index = (float)(i * K + rev);
+    ang[kept] = index * dtheta;
+    tim[kept] = index / (float)K;
+    if (kept == 0 || ang[kept] - ang[kept - 1] > min_step) ++kept;
+  }
+  memcpy(views, ang + proj_start, sizeof(float) * (size_t)N_p);
+  memcpy(times, tim + proj_start, sizeof(float) * (size_t)N_p);
+  free(ang);
+  free(tim);
+  return total;
+}
+
 int32_t mbir_view_map(int32_t N_p, const float* proj_times, int32_t N_time,
                       const float* recon_times, int32_t* frame_ptr,
                       int32_t* frame_views) {

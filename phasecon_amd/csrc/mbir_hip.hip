@@ -1351,6 +1351,50 @@ __global__ __launch_bounds__(256) void k_vsc(int N_time, int N_y, int N_x,
   out[i] = acc;
 }
 
+// Flat-field normalisation of the APS driver (read_data,
+// reconstruct/real_data/APS_Data_Format/XT_HDFIO.c:138-178).  First the
+// mean of the white (or dark) frames per detector pixel: summed in float in
+// frame order, divided by the frame count in float.
+__global__ __launch_bounds__(256) void k_aps_mean(const uint16_t* __restrict__ f, int nf,
+                                                  long long npix, float* __restrict__ out) {
+  const long long p = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (p >= npix) return;
+  float s = 0.f;
+  for (int i = 0; i < nf; ++i) s = __fadd_rn(s, (float)f[(size_t)i * npix + p]);
+  out[p] = __fdiv_rn(s, (float)nf);
+}
+
+// Then per output sample (view i, row j, channel k): |proj - dark| and
+// |white - dark| summed in float over its ratio x ratio block of detector
+// pixels (rows outer, channels inner); the weight is the first sum, the
+// projection b t^2 + t with t = log(second / first) (the quotient in
+// float, the log and the beam-hardening polynomial in double, unfused, the
+// result stored as float).
+__global__ __launch_bounds__(256) void k_aps_normalize(const uint16_t* __restrict__ proj,
+                                                       const float* __restrict__ white,
+                                                       const float* __restrict__ dark,
+                                                       int n_proj, int rows, int cols, int ratio,
+                                                       double bh, float* __restrict__ out_p,
+                                                       float* __restrict__ out_w) {
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long long)n_proj * rows * cols) return;
+  const int k = (int)(idx % cols), j = (int)((idx / cols) % rows);
+  const long long i = idx / ((long long)rows * cols);
+  const long long W = (long long)cols * ratio;
+  const long long frame = (long long)rows * ratio * W;
+  float w = 0.f, wd = 0.f;
+  for (int m = 0; m < ratio; ++m)
+    for (int n = 0; n < ratio; ++n) {
+      const long long q = (long long)(j * ratio + m) * W + (long long)k * ratio + n;
+      const float dk = dark[q];
+      w = __fadd_rn(w, fabsf(__fsub_rn((float)proj[i * frame + q], dk)));
+      wd = __fadd_rn(wd, fabsf(__fsub_rn(white[q], dk)));
+    }
+  const double t = (double)(float)log((double)__fdiv_rn(wd, w));
+  out_p[idx] = (float)__dadd_rn(__dmul_rn(__dmul_rn(bh, t), t), t);
+  out_w[idx] = w;
+}
+
 // theta1/theta2 of individual voxels through the sweep's column routine.
 template <int kW>
 __global__ __launch_bounds__(kWave) void k_thetas(SwGeom G, ColumnArgs A,
@@ -2814,6 +2858,42 @@ int mbir_hip_reset_kernel_time(mbir_hip_ctx* c) {
 int mbir_hip_synchronize(mbir_hip_ctx* c) {
   if (!c) return fail("null context");
   HIP_OK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mbir_hip_aps_normalize(const uint16_t* proj, const uint16_t* white, const uint16_t* dark,
+                           int32_t n_proj, int32_t n_white, int32_t n_dark, int32_t rows,
+                           int32_t cols, int32_t ratio, double bh_coef, float* projections,
+                           float* weights) {
+  if (!proj || !white || !dark || !projections || !weights) return fail("null argument");
+  if (n_proj < 1 || n_white < 1 || n_dark < 1 || rows < 1 || cols < 1 || ratio < 1)
+    return fail("counts, rows, cols and ratio must be positive");
+  const long long npix = (long long)rows * ratio * cols * ratio;  // detector pixels per frame
+  const long long nout = (long long)n_proj * rows * cols;
+  struct Bufs {
+    uint16_t *p = nullptr, *wh = nullptr, *dk = nullptr;
+    float *mw = nullptr, *md = nullptr, *op = nullptr, *ow = nullptr;
+    ~Bufs() {
+      for (void* b : {(void*)p, (void*)wh, (void*)dk, (void*)mw, (void*)md, (void*)op, (void*)ow})
+        if (b) (void)hipFree(b);
+    }
+  } B;
+  if (dev_alloc(&B.p, (size_t)(npix * n_proj)) || dev_alloc(&B.wh, (size_t)(npix * n_white)) ||
+      dev_alloc(&B.dk, (size_t)(npix * n_dark)) || dev_alloc(&B.mw, (size_t)npix) ||
+      dev_alloc(&B.md, (size_t)npix) || dev_alloc(&B.op, (size_t)nout) ||
+      dev_alloc(&B.ow, (size_t)nout))
+    return -1;
+  HIP_OK(hipMemcpy(B.p, proj, sizeof(uint16_t) * npix * n_proj, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(B.wh, white, sizeof(uint16_t) * npix * n_white, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(B.dk, dark, sizeof(uint16_t) * npix * n_dark, hipMemcpyHostToDevice));
+  const unsigned bp = (unsigned)((npix + 255) / 256), bo = (unsigned)((nout + 255) / 256);
+  hipLaunchKernelGGL(k_aps_mean, dim3(bp), dim3(256), 0, nullptr, B.wh, n_white, npix, B.mw);
+  hipLaunchKernelGGL(k_aps_mean, dim3(bp), dim3(256), 0, nullptr, B.dk, n_dark, npix, B.md);
+  hipLaunchKernelGGL(k_aps_normalize, dim3(bo), dim3(256), 0, nullptr, B.p, B.mw, B.md, n_proj,
+                     rows, cols, ratio, bh_coef, B.op, B.ow);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpy(projections, B.op, sizeof(float) * nout, hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(weights, B.ow, sizeof(float) * nout, hipMemcpyDeviceToHost));
   return 0;
 }
 

@@ -44,6 +44,40 @@ def view_map(proj_times, recon_times):
     return ptr, views[:n].copy()
 
 
+def interlaced_views(K, n_theta, n_views, proj_start=0, min_acq_time=0.0, rot_speed=0.0):
+    """Time-interlaced view angles and times (mbir_interlaced_views, the
+    APS driver's gen_interlaced_views_0_to_Inf)."""
+    L = _lib.load()
+    views = np.zeros(n_views, np.float32)
+    times = np.zeros(n_views, np.float32)
+    n = L.mbir_interlaced_views(_p(views), _p(times), int(K), int(n_theta), int(proj_start),
+                                int(n_views), float(min_acq_time), float(rot_speed))
+    if n < 0:
+        raise ValueError("bad interlaced-view arguments")
+    return views, times
+
+
+def aps_normalize(proj, white, dark, ratio, bh_coef=0.5):
+    """Projections and weights [views][rows][cols] from raw uint16 counts
+    (mbir_hip_aps_normalize, the APS driver's read_data flat-field step):
+    proj [views][rows*ratio][cols*ratio], white / dark frames of the same
+    frame shape."""
+    L = _lib.load()
+    proj = np.ascontiguousarray(proj, np.uint16)
+    white = np.ascontiguousarray(white, np.uint16)
+    dark = np.ascontiguousarray(dark, np.uint16)
+    n, R, Cc = proj.shape
+    if R % ratio or Cc % ratio or white.shape[1:] != (R, Cc) or dark.shape[1:] != (R, Cc):
+        raise ValueError("frame shapes must agree and divide by ratio")
+    rows, cols = R // ratio, Cc // ratio
+    out_p = np.zeros((n, rows, cols), np.float32)
+    out_w = np.zeros((n, rows, cols), np.float32)
+    _lib.check(L.mbir_hip_aps_normalize(_p(proj), _p(white), _p(dark), n, white.shape[0],
+                                        dark.shape[0], rows, cols, int(ratio), float(bh_coef),
+                                        _p(out_p), _p(out_w)), "aps_normalize")
+    return out_p, out_w
+
+
 class Stage:
     """One multi-resolution stage of one scan, resident on the GPU."""
 
