@@ -169,6 +169,20 @@ int mbir_hip_icd_graph(mbir_hip_ctx* ctx, int32_t n, int32_t first_iter, uint32_
 int mbir_hip_nhicd(mbir_hip_ctx* ctx, int32_t passes, double fraction, int32_t iter,
                    uint32_t seed, double* stats);
 
+/* Projection step of the phase-contrast phantom simulator (ForwardProject,
+ * PCMBIR_3D/XT_genSinogram.c:146-186, on the GPU): the absorption (mag)
+ * and refractive-index (phase) phantoms [n_z][n_xy][n_xy] projected at the
+ * phantom's own voxel pitch, detector N_r = n_xy channels by N_t = n_z rows
+ * (the geometry of initPhantomStructures, PCMBIR_3D/XT_Init.c:611-690, with
+ * n_xy / n_z in place of PHANTOM_XY_SIZE / PHANTOM_Z_SIZE): proj_real /
+ * proj_imag [n_p][n_z][n_xy] = sum over voxels of phantom x A x wavenumber
+ * x the voxel line response along z (PCMBIR_3D/XT_Profile.c:182-290).
+ * n_xy must be a multiple of proj_cols. */
+int mbir_hip_phantom_project(const float* mag, const float* phase, int32_t n_xy, int32_t n_z,
+                             int32_t n_p, const float* angles, int32_t proj_rows,
+                             int32_t proj_cols, double vox_wid, double rot_center,
+                             double wavenumber, float* proj_real, float* proj_imag);
+
 /* Flat-field normalisation of raw APS counts (read_data,
  * reconstruct/real_data/APS_Data_Format/XT_HDFIO.c:138-178), on the GPU,
  * from host buffers the caller has read from its HDF5 files (the

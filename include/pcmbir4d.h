@@ -25,8 +25,16 @@
  *   - recon_type 2 without readable pag_*_ret.bin files returns -1;
  *   - no TIFF files are written.
  *
- * phcontomo_forward_project() is declared with the reference's prototype;
- * this build returns -1 from it with a message (the simulator is not built).
+ * phcontomo_forward_project() (PCMBIR_3D/XT_Engine.c:199-247, the phantom
+ * simulator of XT_genSinogram.c:59-343) keeps the reference's prototype and
+ * reads the two 128 x 512 x 512 float phantoms from the files named by the
+ * environment variables PHCONTOMO_MAG_PHANTOM and PHCONTOMO_PHASE_PHANTOM
+ * (the reference's are the fixed paths MAG_PHANTOM_FILEPATH /
+ * PHASE_PHANTOM_FILEPATH, PCMBIR_3D/XT_Constants.h:91-92).  It writes
+ * SimMagProjs.bin, SimPhaseProjs.bin, mag_phantom.bin and phase_phantom.bin
+ * as the reference does; the measurement noise has the reference's
+ * distribution (m + sqrt(m) N(0, 1), absolute value) from another
+ * generator, so noisy measurements match in distribution, not bit for bit.
  */
 #ifndef PCMBIR4D_H_
 #define PCMBIR4D_H_
@@ -56,6 +64,19 @@ int phcontomo_forward_project(float** measurements, float** brights,
                               float vox_wid, float rot_center,
                               float obj2det_dist, float light_energy,
                               float pag_regparam, FILE* debug_msg_ptr);
+
+/* The simulator on caller-supplied phantoms mag / phase [n_z][n_xy][n_xy]
+ * (n_xy, n_z powers of two up to 1024 and multiples of proj_cols /
+ * proj_rows; the reference's are 512 and 128): measurements
+ * [proj_num][proj_rows][proj_cols] and brights [proj_rows][proj_cols];
+ * noise_seed 0 leaves the noise out.  No counterpart in the reference,
+ * whose simulator reads fixed files. */
+int phcontomo_forward_project_ex(float* measurements, float* brights, const float* mag,
+                                 const float* phase, int32_t n_xy, int32_t n_z,
+                                 const float* proj_angles, int32_t proj_rows,
+                                 int32_t proj_cols, int32_t proj_num, float vox_wid,
+                                 float rot_center, float obj2det_dist,
+                                 float light_energy, uint64_t noise_seed, FILE* dbg);
 
 #ifdef __cplusplus
 }

@@ -269,6 +269,120 @@ int32_t orc_amatrix_column(const orc_geom* g, const double* cosv,
   return n;
 }
 
+int32_t orc_phantom_project(const float* mag, const float* phase, int32_t n_xy,
+                            int32_t n_z, int32_t n_p, const float* angles,
+                            int32_t proj_rows, int32_t proj_cols, double vox_wid,
+                            double rot_center, double wavenumber, double* out_r,
+                            double* out_i) {
+  if (proj_cols <= 0 || n_xy % proj_cols != 0) return -1;
+  orc_geom g;
+  memset(&g, 0, sizeof g);
+  /* initPhantomStructures: the detector sampled at the phantom's voxels */
+  const double Length_R = vox_wid * proj_cols, Length_T = vox_wid * proj_rows;
+  const double rc = rot_center * (double)(n_xy / proj_cols);
+  g.N_r = n_xy;
+  g.N_t = n_z;
+  g.N_p = n_p;
+  g.delta_r = Length_R / g.N_r;
+  g.delta_t = Length_T / g.N_t;
+  g.R0 = -rc * g.delta_r;
+  g.RMax = (g.N_r - rc) * g.delta_r;
+  g.T0 = -Length_T / 2.0;
+  g.TMax = Length_T / 2.0;
+  g.N_x = g.N_y = n_xy;
+  g.N_z = n_z;
+  g.delta_xy = g.delta_r;
+  g.delta_z = g.delta_t;
+  g.x0 = g.R0;
+  g.z0 = g.T0;
+  g.y0 = -Length_R / 2.0;
+  g.OffsetR = (g.delta_xy / sqrt(2.0) + g.delta_r / 2.0) / ORC_RESPONSE_BINS;
+  g.OffsetT = ((g.delta_z / 2) + g.delta_t / 2) / ORC_RESPONSE_BINS;
+  g.W = 3;
+  /* z-line response table H_t (XT_Profile.c:182-227), float like the
+   * reference's Real_arr_t */
+  float Ht[ORC_RESPONSE_BINS + 1];
+  for (int32_t i = 0; i < ORC_RESPONSE_BINS; ++i) {
+    const double c = i * g.OffsetT;
+    double h;
+    if (g.delta_z >= g.delta_t)
+      h = (c <= g.delta_z / 2 - g.delta_t / 2) ? g.delta_t : -c + g.delta_z / 2 + g.delta_t / 2;
+    else
+      h = (c <= g.delta_t / 2 - g.delta_z / 2) ? g.delta_z : -c + g.delta_z / 2 + g.delta_t / 2;
+    if (h < 0) h = 0;
+    Ht[i] = (float)(h / g.delta_t);
+  }
+  Ht[ORC_RESPONSE_BINS] = 0.f;
+  /* voxel line response per slice (XT_Profile.c:229-290): up to 3 rows */
+  int32_t* lt = (int32_t*)malloc(sizeof(int32_t) * 3 * (size_t)n_z);
+  double* lv = (double*)malloc(sizeof(double) * 3 * (size_t)n_z);
+  int32_t* ln = (int32_t*)calloc((size_t)n_z, sizeof(int32_t));
+  for (int32_t i = 0; i < n_z; ++i) {
+    const double t = ((double)i + 0.5) * g.delta_z + g.z0;
+    const double tmin = (t - g.delta_z / 2) > g.T0 ? t - g.delta_z / 2 : g.T0;
+    const double tmax = (t + g.delta_z / 2) <= g.TMax ? t + g.delta_z / 2 : g.TMax;
+    int32_t lo = (int32_t)floor((tmin - g.T0) / g.delta_t);
+    int32_t hi = (int32_t)floor((tmax - g.T0) / g.delta_t);
+    if (lo < 0) lo = 0;
+    if (hi >= g.N_t) hi = g.N_t - 1;
+    for (int32_t it = lo; it <= hi; ++it) {
+      const double ct = ((double)it + 0.5) * g.delta_t + g.T0;
+      const double d = fabs(ct - t);
+      const int32_t k = (int32_t)floor(d / g.OffsetT);
+      double v = 0;
+      if (k < ORC_RESPONSE_BINS) {
+        const double w3 = d - (double)k * g.OffsetT;
+        const double w4 = ((double)k + 1) * g.OffsetT - d;
+        v = (w4 / g.OffsetT) * Ht[k] + (w3 / g.OffsetT) * Ht[k + 1];
+      }
+      if (v != 0 && ln[i] < 3) {
+        lt[3 * i + ln[i]] = it;
+        lv[3 * i + ln[i]] = v;
+        ++ln[i];
+      }
+    }
+  }
+  float* H = (float*)malloc(sizeof(float) * (size_t)n_p * (ORC_RESPONSE_BINS + 1));
+  double* cs = (double*)malloc(sizeof(double) * (size_t)n_p);
+  double* sn = (double*)malloc(sizeof(double) * (size_t)n_p);
+  orc_detector_response(&g, angles, H);
+  orc_sincos(n_p, angles, cs, sn);
+  const size_t per = (size_t)n_z * n_xy;
+  memset(out_r, 0, sizeof(double) * per * n_p);
+  memset(out_i, 0, sizeof(double) * per * n_p);
+#pragma omp parallel for schedule(dynamic)
+  for (int32_t p = 0; p < n_p; ++p) {
+    int32_t idx[64];
+    double val[64];
+    double* orp = out_r + (size_t)p * per;
+    double* oip = out_i + (size_t)p * per;
+    for (int32_t y = 0; y < n_xy; ++y)
+      for (int32_t x = 0; x < n_xy; ++x) {
+        const int32_t n = orc_amatrix_column(&g, cs, sn, H, y, x, p, idx, val);
+        for (int32_t z = 0; z < n_z; ++z) {
+          const double m = mag[((size_t)z * n_xy + y) * n_xy + x];
+          const double ph = phase[((size_t)z * n_xy + y) * n_xy + x];
+          if (m == 0 && ph == 0) continue;
+          for (int32_t a = 0; a < n; ++a) {
+            const double av = val[a] * wavenumber;
+            for (int32_t q = 0; q < ln[z]; ++q) {
+              const size_t o = (size_t)lt[3 * z + q] * n_xy + idx[a];
+              orp[o] += m * av * lv[3 * z + q];
+              oip[o] += ph * av * lv[3 * z + q];
+            }
+          }
+        }
+      }
+  }
+  free(lt);
+  free(lv);
+  free(ln);
+  free(H);
+  free(cs);
+  free(sn);
+  return 0;
+}
+
 void orc_amatrix_all(const orc_geom* g, const double* cosv, const double* sinv,
                      const float* H, int32_t* first, double* val) {
   const int32_t W = g->W;

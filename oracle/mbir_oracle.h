@@ -79,6 +79,22 @@ int32_t orc_amatrix_column(const orc_geom* g, const double* cosv,
                            int32_t col, int32_t view, int32_t* idx,
                            double* val);
 
+/* Projection of the PCMBIR simulator (ForwardProject,
+ * PCMBIR_3D/XT_genSinogram.c:146-186) in the geometry of
+ * initPhantomStructures (PCMBIR_3D/XT_Init.c:611-690) with a phantom of
+ * n_z x n_xy x n_xy voxels ([z][y][x]) in place of the 128 x 512 x 512 of
+ * PHANTOM_Z_SIZE / PHANTOM_XY_SIZE: out_r / out_i [n_p][n_z][n_xy] =
+ * sum over voxels of phantom x A(r) k x L(z, t), A the detector response
+ * column (XT_AMatrix.c, times the wavenumber k) and L the voxel line
+ * response (ZLineResponseProfile / storeVoxelLineResponse,
+ * PCMBIR_3D/XT_Profile.c:182-290), accumulated in FP64.  Returns 0, or -1
+ * if n_xy is not a multiple of proj_cols. */
+int32_t orc_phantom_project(const float* mag, const float* phase, int32_t n_xy,
+                            int32_t n_z, int32_t n_p, const float* angles,
+                            int32_t proj_rows, int32_t proj_cols, double vox_wid,
+                            double rot_center, double wavenumber, double* out_r,
+                            double* out_i);
+
 /* All columns for all views in window form. */
 void orc_amatrix_all(const orc_geom* g, const double* cosv, const double* sinv,
                      const float* H, int32_t* first, double* val);

@@ -242,6 +242,28 @@ def variance_update(var, w, e, sel, T, delta):
     return v.value, w, sel
 
 
+def phantom_project(mag, phase, angles, proj_rows, proj_cols, vox_wid, rot_center, wavenumber):
+    """orc_phantom_project: the phase-contrast simulator's projections of
+    the phantoms [n_z][n_xy][n_xy] (PCMBIR_3D/XT_genSinogram.c:146-186);
+    returns (real, imag) [n_p][n_z][n_xy] in float64."""
+    L = _orc()
+    mag = np.ascontiguousarray(mag, np.float32)
+    phase = np.ascontiguousarray(phase, np.float32)
+    ang = np.ascontiguousarray(angles, np.float32)
+    n_z, n_xy = mag.shape[0], mag.shape[-1]
+    out_r = np.zeros((ang.size, n_z, n_xy))
+    out_i = np.zeros_like(out_r)
+    rc = L.orc_phantom_project(
+        mag.ctypes.data_as(C.c_void_p), phase.ctypes.data_as(C.c_void_p), C.c_int32(n_xy),
+        C.c_int32(n_z), C.c_int32(ang.size), ang.ctypes.data_as(C.c_void_p),
+        C.c_int32(proj_rows), C.c_int32(proj_cols), C.c_double(vox_wid), C.c_double(rot_center),
+        C.c_double(wavenumber), out_r.ctypes.data_as(C.c_void_p),
+        out_i.ctypes.data_as(C.c_void_p))
+    if rc:
+        raise ValueError("n_xy must be a multiple of proj_cols")
+    return out_r, out_i
+
+
 NHICD_ITERATIONS = 10  # NHICD_Iterations (XT_Init.c:464)
 
 

@@ -77,6 +77,33 @@ def fresnel(x, d_rows, d_cols, lam, dist, win=None, herm=False):
     return np.fft.ifft2(np.fft.fft2(x) * f)
 
 
+EXPECTED_COUNT = 2500.0  # EXPECTED_COUNT_MEASUREMENT (PCMBIR_3D/XT_Constants.h:89)
+
+
+def simulate(mag, phase, angles, proj_rows, proj_cols, vox_wid, rot_center, obj2det_dist,
+             light_energy):
+    """Noise-free measurements of the phantom simulator (ForwardProject,
+    PCMBIR_3D/XT_genSinogram.c:146-264): the phantoms' projections
+    (oracle.phantom_project), the exit field sqrt(I0) exp(-p_mag - i
+    p_phase), Fresnel propagation with the simulator's window (sigma = 1 /
+    sqrt(lambda z / 2), PCMBIR_3D/XT_Init.c:682-686), the intensity averaged
+    over each detector pixel's samples.  Returns [n_p][proj_rows][proj_cols]."""
+    from . import oracle as O
+    lam = wavelength(light_energy)
+    k = 2 * np.pi / lam
+    n_z, n_xy = np.shape(mag)[0], np.shape(mag)[-1]
+    pr, pi = O.phantom_project(mag, phase, angles, proj_rows, proj_cols, vox_wid, rot_center, k)
+    d_r, d_t = vox_wid * proj_cols / n_xy, vox_wid * proj_rows / n_z
+    field = np.sqrt(EXPECTED_COUNT) * np.exp(-pr) * (np.cos(-pi) + 1j * np.sin(-pi))
+    field = np.swapaxes(field, 1, 2)                      # [view][r][t], the transform's layout
+    win = window(n_xy, n_z, d_r, d_t, 1.0 / np.sqrt(0.5 * lam * obj2det_dist))
+    y = fresnel(field, d_r, d_t, lam, obj2det_dist, win)
+    inten = np.abs(y) ** 2                                # [view][r][t]
+    rs, ts = n_xy // proj_cols, n_z // proj_rows
+    m = inten.reshape(len(angles), proj_cols, rs, proj_rows, ts).mean(axis=(2, 4))
+    return np.swapaxes(m, 1, 2)
+
+
 def paganin(meas, D, d_rows, d_cols, lam, dist, delta_over_beta):
     """Single-material Paganin retrieval of one view
     (paganins_1mat_phase_retrieval, XT_Paganin.c:77-134): intensity

@@ -46,6 +46,13 @@ LIBS = {
 }
 
 
+# extra link arguments: the phase-contrast simulator projects its phantoms
+# with the MBIR library's A rule (built first, found next to this one)
+LINK = {
+    "libpcmbir4d.so": ["-L", PKG, "-lmbir4d", "-Wl,-rpath,$ORIGIN"],
+}
+
+
 def sources(prefixes):
     return sorted(f for f in os.listdir(CSRC)
                   if f.endswith((".hip", ".c")) and f.startswith(prefixes))
@@ -65,7 +72,8 @@ def build(verbose=True):
             objs.append(obj)
         lib = os.path.join(PKG, lib_name)
         tmp = lib + ".tmp"
-        _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-lm"])
+        _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs,
+              *LINK.get(lib_name, []), "-lm"])
         os.replace(tmp, lib)
         out.append(lib)
     return out

@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "mbir_hip.h"
+#include "mbir_engine.h"
 
 namespace {
 
@@ -1349,6 +1350,73 @@ __global__ __launch_bounds__(256) void k_vsc(int N_time, int N_y, int N_x,
       if (x + q >= 0 && x + q < N_x) acc += h[p + 2] * h[q + 2] * m[(y + p) * N_x + x + q];
   }
   out[i] = acc;
+}
+
+// Projection of the phase-contrast simulator's phantoms (ForwardProject,
+// PCMBIR_3D/XT_genSinogram.c:146-186): for view p and detector sample
+// (t, r), the sums over voxels of phantom x A(r) k x L(z, t) for the
+// absorption and the refractive-index phantom, A the voxel's detector
+// response (the A rule, times the wavenumber k) and L the voxel line
+// response along z.  One workgroup per (view, chunk of detector rows): each
+// thread takes voxel columns, forms the chunk's z-weighted column values
+// (lz / lw: the slices feeding each row), and adds them times A into the
+// chunk's rows of both projections in LDS; the rows go out once.
+constexpr int kSimL = 3;    // voxel slices feeding one detector row
+constexpr int kSimTc = 16;  // detector rows per workgroup at most
+__global__ __launch_bounds__(256) void k_phantom_project(
+    DevGeom G, const float* __restrict__ H, const double* __restrict__ cosv,
+    const double* __restrict__ sinv, const float* __restrict__ mag,
+    const float* __restrict__ phase, const int* __restrict__ lz, const float* __restrict__ lw,
+    int n_z, int tchunk, double kwave, float* __restrict__ out_r, float* __restrict__ out_i) {
+  extern __shared__ __align__(16) float smem[];
+  __shared__ float sh[kNB];
+  const int p = blockIdx.x, t0 = blockIdx.y * tchunk;
+  const int tc = min(tchunk, n_z - t0);
+  const int tid = threadIdx.x;
+  float* ar = smem;
+  float* ai = smem + (size_t)G.N_r * tchunk;
+  for (int i = tid; i < 2 * G.N_r * tchunk; i += 256) smem[i] = 0.f;
+  for (int i = tid; i < kNB; i += 256) sh[i] = H[(size_t)p * kNB + i];
+  __syncthreads();
+  const double cs = cosv[p], sn = sinv[p];
+  const int ncol = G.N_x * G.N_y;
+  for (int c = tid; c < ncol; c += 256) {
+    double vr[kSimTc], vi[kSimTc];
+    bool any = false;
+#pragma unroll
+    for (int t = 0; t < kSimTc; ++t) {
+      vr[t] = 0.0;
+      vi[t] = 0.0;
+      if (t < tc) {
+        for (int q = 0; q < kSimL; ++q) {
+          const int z = lz[(t0 + t) * kSimL + q];
+          if (z < 0) continue;
+          const double w = lw[(t0 + t) * kSimL + q];
+          vr[t] += (double)mag[(size_t)z * ncol + c] * w;
+          vi[t] += (double)phase[(size_t)z * ncol + c] * w;
+        }
+        any |= vr[t] != 0.0 || vi[t] != 0.0;
+      }
+    }
+    if (!any) continue;
+    amatrix_visit(G, sh, cs, sn, c / G.N_x, c % G.N_x, [&](int j, int, double v) {
+      const double a = v * kwave;
+#pragma unroll
+      for (int t = 0; t < kSimTc; ++t) {
+        if (t < tc) {
+          if (vr[t] != 0.0) atomicAdd(&ar[j * tchunk + t], (float)(vr[t] * a));
+          if (vi[t] != 0.0) atomicAdd(&ai[j * tchunk + t], (float)(vi[t] * a));
+        }
+      }
+    });
+  }
+  __syncthreads();
+  for (int i = tid; i < G.N_r * tc; i += 256) {
+    const int t = i / G.N_r, r = i % G.N_r;
+    const size_t o = ((size_t)p * n_z + t0 + t) * G.N_r + r;
+    out_r[o] = ar[r * tchunk + t];
+    out_i[o] = ai[r * tchunk + t];
+  }
 }
 
 // Flat-field normalisation of the APS driver (read_data,
@@ -2858,6 +2926,117 @@ int mbir_hip_reset_kernel_time(mbir_hip_ctx* c) {
 int mbir_hip_synchronize(mbir_hip_ctx* c) {
   if (!c) return fail("null context");
   HIP_OK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mbir_hip_phantom_project(const float* mag, const float* phase, int32_t n_xy, int32_t n_z,
+                             int32_t n_p, const float* angles, int32_t proj_rows,
+                             int32_t proj_cols, double vox_wid, double rot_center,
+                             double wavenumber, float* proj_real, float* proj_imag) {
+  if (!mag || !phase || !angles || !proj_real || !proj_imag) return fail("null argument");
+  if (n_xy < 2 || n_z < 1 || n_p < 1 || proj_rows < 1 || proj_cols < 1 || n_xy % proj_cols)
+    return fail("phantom sizes must be positive and n_xy a multiple of proj_cols");
+  // geometry of initPhantomStructures (PCMBIR_3D/XT_Init.c:611-690): the
+  // detector sampled at the phantom's voxel pitch
+  const double len_r = vox_wid * proj_cols, len_t = vox_wid * proj_rows;
+  const double rc = rot_center * (double)(n_xy / proj_cols);
+  mbir_hip_geom g{};
+  g.N_r = n_xy;
+  g.N_t = n_z;
+  g.N_p = n_p;
+  g.N_x = g.N_y = n_xy;
+  g.N_z = n_z;
+  g.N_time = 1;
+  g.mult_xy = g.mult_z = 1;
+  g.delta_r = len_r / n_xy;
+  g.delta_t = len_t / n_z;
+  g.R0 = -rc * g.delta_r;
+  g.RMax = (n_xy - rc) * g.delta_r;
+  g.x0 = g.R0;
+  g.y0 = -len_r / 2.0;
+  g.delta_xy = g.delta_r;
+  g.OffsetR = (g.delta_xy / sqrt(2.0) + g.delta_r / 2.0) / kRespBins;
+  const double T0 = -len_t / 2.0, TMax = len_t / 2.0, dz = g.delta_t, dt = g.delta_t;
+  const double offT = (dz / 2 + dt / 2) / kRespBins;
+  // z-line response (ZLineResponseProfile, PCMBIR_3D/XT_Profile.c:182-227)
+  // and the rows each slice feeds (storeVoxelLineResponse, :229-290),
+  // inverted to the slices feeding each row
+  float ht[kRespBins + 1];
+  for (int i = 0; i < kRespBins; ++i) {
+    const double c = i * offT;
+    double h = (c <= dz / 2 - dt / 2) ? dt : -c + dz / 2 + dt / 2;
+    if (h < 0) h = 0;
+    ht[i] = (float)(h / dt);
+  }
+  ht[kRespBins] = 0.f;
+  std::vector<int> lz((size_t)n_z * kSimL, -1);
+  std::vector<float> lw((size_t)n_z * kSimL, 0.f);
+  for (int z = 0; z < n_z; ++z) {
+    const double t = ((double)z + 0.5) * dz + T0;
+    const double tmin = (t - dz / 2) > T0 ? t - dz / 2 : T0;
+    const double tmax = (t + dz / 2) <= TMax ? t + dz / 2 : TMax;
+    const int lo = std::max(0, (int)floor((tmin - T0) / dt));
+    const int hi = std::min(n_z - 1, (int)floor((tmax - T0) / dt));
+    for (int it = lo; it <= hi; ++it) {
+      const double d = fabs(((double)it + 0.5) * dt + T0 - t);
+      const int k = (int)floor(d / offT);
+      if (k >= kRespBins) continue;
+      const double v = ((k + 1) * offT - d) / offT * ht[k] + (d - k * offT) / offT * ht[k + 1];
+      if (v == 0) continue;
+      int q = 0;
+      while (q < kSimL && lz[(size_t)it * kSimL + q] >= 0) ++q;
+      if (q == kSimL) return fail("more than three slices feed one detector row");
+      lz[(size_t)it * kSimL + q] = z;
+      lw[(size_t)it * kSimL + q] = (float)v;
+    }
+  }
+  std::vector<float> H((size_t)n_p * kNB);
+  std::vector<double> cs(n_p), sn(n_p);
+  mbir_detector_response(&g, angles, H.data());
+  mbir_sincos(n_p, angles, cs.data(), sn.data());
+  DevGeom G{};
+  G.N_r = n_xy;
+  G.N_x = G.N_y = n_xy;
+  G.N_z = n_z;
+  G.N_p = n_p;
+  G.W = 3;
+  G.delta_r = g.delta_r;
+  G.R0 = g.R0;
+  G.RMax = g.RMax;
+  G.x0 = g.x0;
+  G.y0 = g.y0;
+  G.delta_xy = g.delta_xy;
+  G.OffsetR = g.OffsetR;
+  const int tchunk = std::max(1, std::min({kSimTc, n_z, (int)(48 * 1024 / (8 * n_xy))}));
+  const size_t nvox = (size_t)n_z * n_xy * n_xy, nout = (size_t)n_p * n_z * n_xy;
+  struct Bufs {
+    float *m = nullptr, *ph = nullptr, *h = nullptr, *w = nullptr, *orr = nullptr, *oi = nullptr;
+    double *c = nullptr, *s = nullptr;
+    int* z = nullptr;
+    ~Bufs() {
+      for (void* b : {(void*)m, (void*)ph, (void*)h, (void*)w, (void*)orr, (void*)oi, (void*)c,
+                      (void*)s, (void*)z})
+        if (b) (void)hipFree(b);
+    }
+  } B;
+  if (dev_alloc(&B.m, nvox) || dev_alloc(&B.ph, nvox) || dev_alloc(&B.h, H.size()) ||
+      dev_alloc(&B.w, lw.size()) || dev_alloc(&B.z, lz.size()) || dev_alloc(&B.c, cs.size()) ||
+      dev_alloc(&B.s, sn.size()) || dev_alloc(&B.orr, nout) || dev_alloc(&B.oi, nout))
+    return -1;
+  HIP_OK(hipMemcpy(B.m, mag, sizeof(float) * nvox, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(B.ph, phase, sizeof(float) * nvox, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(B.h, H.data(), sizeof(float) * H.size(), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(B.w, lw.data(), sizeof(float) * lw.size(), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(B.z, lz.data(), sizeof(int) * lz.size(), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(B.c, cs.data(), sizeof(double) * cs.size(), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(B.s, sn.data(), sizeof(double) * sn.size(), hipMemcpyHostToDevice));
+  const dim3 grid((unsigned)n_p, (unsigned)((n_z + tchunk - 1) / tchunk));
+  const size_t lds = sizeof(float) * 2 * (size_t)n_xy * tchunk;
+  hipLaunchKernelGGL(k_phantom_project, grid, dim3(256), lds, nullptr, G, B.h, B.c, B.s, B.m,
+                     B.ph, B.z, B.w, n_z, tchunk, wavenumber, B.orr, B.oi);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpy(proj_real, B.orr, sizeof(float) * nout, hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(proj_imag, B.oi, sizeof(float) * nout, hipMemcpyDeviceToHost));
   return 0;
 }
 

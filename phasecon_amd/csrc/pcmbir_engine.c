@@ -21,6 +21,7 @@
 #include <time.h>
 #include <unistd.h>
 
+#include "mbir_hip.h"
 #include "pcmbir4d.h"
 #include "pcmbir_hip.h"
 
@@ -34,6 +35,10 @@
 #define PC_PLANCKS_CONSTANT 4.135668e-19
 #define PC_LIGHT_SPEED 299792458e6
 #define PC_MRF_P 1.2                  /* :146 */
+#define PC_PHANTOM_XY_SIZE 512        /* :203 */
+#define PC_PHANTOM_Z_SIZE 128         /* :204 */
+#define PC_EXPECTED_COUNT 2500.0      /* EXPECTED_COUNT_MEASUREMENT, :89 */
+#define PC_NOISE_SEED 761521          /* srandom2 seed, PCMBIR_3D/XT_Engine.c:212 */
 /* iteration limits, thresholds and ADMM weights (PCMBIR_3D/XT_Init.c:494-521) */
 #define PC_PRET_MAXITER 100
 #define PC_SD_MAXITER 100
@@ -479,22 +484,191 @@ int phcontomo_reconstruct(float** magobject, float** phaseobject, float* measure
   return 0;
 }
 
+/* Standard normal deviates for the measurement noise: the polar Box-Muller
+ * method over splitmix64.  The reference draws them from its randlib
+ * normal(); the sequence differs, the distribution is the same. */
+typedef struct {
+  uint64_t s;
+  double spare;
+  int has;
+} pc_rng;
+
+static double pc_uniform(pc_rng* r) {
+  uint64_t z = (r->s += 0x9e3779b97f4a7c15ULL);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  z ^= z >> 31;
+  return ((double)(z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+static double pc_normal(pc_rng* r) {
+  if (r->has) {
+    r->has = 0;
+    return r->spare;
+  }
+  double u, v, q;
+  do {
+    u = 2.0 * pc_uniform(r) - 1.0;
+    v = 2.0 * pc_uniform(r) - 1.0;
+    q = u * u + v * v;
+  } while (q >= 1.0 || q == 0.0);
+  const double f = sqrt(-2.0 * log(q) / q);
+  r->spare = v * f;
+  r->has = 1;
+  return u * f;
+}
+
+int phcontomo_forward_project_ex(float* measurements, float* brights, const float* mag,
+                                 const float* phase, int32_t n_xy, int32_t n_z,
+                                 const float* proj_angles, int32_t proj_rows, int32_t proj_cols,
+                                 int32_t proj_num, float vox_wid, float rot_center,
+                                 float obj2det_dist, float light_energy, uint64_t noise_seed,
+                                 FILE* dbg) {
+  if (!measurements || !brights || !mag || !phase || !proj_angles) {
+    say(dbg, "ERROR", "phcontomo_forward_project: null argument.\n");
+    return -1;
+  }
+  /* the reference's argument checks (PCMBIR_3D/XT_Engine.c:214-217, one node) */
+  if (proj_rows < PC_MIN_PROJECTION_ROWS || proj_rows % 2 || proj_cols % 2 || proj_num < 1) {
+    say(dbg, "ERROR", "proj_rows must be even and >= %d, proj_cols even.\n",
+        PC_MIN_PROJECTION_ROWS);
+    return -1;
+  }
+  if (n_xy % proj_cols || n_z % proj_rows || !is_pow2(n_xy) || !is_pow2(n_z)) {
+    say(dbg, "ERROR", "phantom %d x %d: sizes must be powers of two up to 1024 (the GPU "
+        "transforms are radix-2) and multiples of proj_cols / proj_rows.\n", n_xy, n_z);
+    return -1;
+  }
+  const int32_t N_r = n_xy, N_t = n_z, rs = n_xy / proj_cols, ts = n_z / proj_rows;
+  const size_t per = (size_t)N_r * N_t, nall = per * proj_num;
+  /* XT_Init.c:616-619: wavelength, wavenumber; :623-628, 679: the sampling */
+  const double wavelength = PC_PLANCKS_CONSTANT * PC_LIGHT_SPEED / light_energy;
+  const double wavenumber = 2 * M_PI / wavelength;
+  const double d_r = (double)vox_wid * proj_cols / N_r, d_t = (double)vox_wid * proj_rows / N_t;
+  int rc = -1;
+  float *pr = NULL, *pi = NULL, *field = NULL, *win = NULL;
+  pcmbir_hip_ctx* ctx = NULL;
+  pr = (float*)malloc(sizeof(float) * nall);
+  pi = (float*)malloc(sizeof(float) * nall);
+  field = (float*)malloc(sizeof(float) * 2 * nall);
+  win = (float*)malloc(sizeof(float) * per);
+  if (!pr || !pi || !field || !win) {
+    say(dbg, "ERROR", "Out of memory.\n");
+    goto done;
+  }
+  /* projections of both phantoms (XT_genSinogram.c:146-186) */
+  if (mbir_hip_phantom_project(mag, phase, n_xy, n_z, proj_num, proj_angles, proj_rows,
+                               proj_cols, vox_wid, rot_center, wavenumber, pr, pi)) {
+    say(dbg, "ERROR", "Phantom projection failed: %s\n", mbir_hip_last_error());
+    goto done;
+  }
+  if (write_bin("SimMagProjs", pr, sizeof(float), nall, "wb", dbg) ||
+      write_bin("SimPhaseProjs", pi, sizeof(float), nall, "wb", dbg) ||
+      write_bin("mag_phantom", mag, sizeof(float), (size_t)n_z * n_xy * n_xy, "wb", dbg) ||
+      write_bin("phase_phantom", phase, sizeof(float), (size_t)n_z * n_xy * n_xy, "wb", dbg))
+    goto done;
+  /* exit field sqrt(I0) exp(-p_mag - i p_phase) in the transform's [r][t]
+   * layout (:214-228) */
+  {
+    const double amp = sqrt(PC_EXPECTED_COUNT);
+    for (int32_t i = 0; i < proj_num; ++i)
+      for (int32_t t = 0; t < N_t; ++t)
+        for (int32_t r = 0; r < N_r; ++r) {
+          const size_t src = (size_t)i * per + (size_t)t * N_r + r;
+          const size_t dst = (size_t)i * per + (size_t)r * N_t + t;
+          const double e = exp(-(double)pr[src]);
+          field[2 * dst] = (float)(e * cos(-(double)pi[src]) * amp);
+          field[2 * dst + 1] = (float)(e * sin(-(double)pi[src]) * amp);
+        }
+  }
+  /* Fresnel propagation with the simulator's window, sigma = 1 /
+   * sqrt(lambda z / 2) (XT_Init.c:682-686, create_FresnelTranWindow
+   * :210-231) */
+  {
+    const double sig = 1.0 / sqrt(0.5 * wavelength * obj2det_dist);
+    for (int32_t j = 0; j < N_r; ++j)
+      for (int32_t k = 0; k < N_t; ++k) {
+        const double u = (j >= N_r / 2 ? -(double)(N_r - j) : (double)j) / (N_r * d_r);
+        const double v = (k >= N_t / 2 ? -(double)(N_t - k) : (double)k) / (N_t * d_t);
+        win[(size_t)j * N_t + k] = (float)exp(-(u * u + v * v) / (2 * sig * sig));
+      }
+  }
+  if (pcmbir_hip_create(&ctx, proj_num, N_r, N_t, d_r, d_t, wavelength, obj2det_dist, win) ||
+      pcmbir_hip_fresnel(ctx, 0, field, field)) {
+    say(dbg, "ERROR", "Fresnel transform failed: %s\n", pcmbir_hip_last_error());
+    goto done;
+  }
+  /* intensity averaged over each detector pixel's ts x rs samples, plus
+   * the noise |m + sqrt(|m|) n| (:241-264); the brights are the expected
+   * count */
+  {
+    pc_rng rng = {noise_seed, 0.0, 0};
+    for (int32_t i = 0; i < proj_num; ++i)
+      for (int32_t row = 0; row < proj_rows; ++row)
+        for (int32_t col = 0; col < proj_cols; ++col) {
+          double m = 0.0;
+          for (int32_t a = 0; a < ts; ++a)
+            for (int32_t b = 0; b < rs; ++b) {
+              const size_t q = (size_t)i * per + (size_t)(col * rs + b) * N_t + row * ts + a;
+              const double re = field[2 * q], im = field[2 * q + 1];
+              m += re * re + im * im;
+            }
+          m /= (double)(ts * rs);
+          if (noise_seed) m = fabs(m + sqrt(fabs(m)) * pc_normal(&rng));
+          measurements[((size_t)i * proj_rows + row) * proj_cols + col] = (float)m;
+        }
+    for (size_t q = 0; q < (size_t)proj_rows * proj_cols; ++q) brights[q] = (float)PC_EXPECTED_COUNT;
+  }
+  rc = 0;
+done:
+  if (ctx) pcmbir_hip_destroy(ctx);
+  free(pr);
+  free(pi);
+  free(field);
+  free(win);
+  return rc;
+}
+
 int phcontomo_forward_project(float** measurements, float** brights, float* proj_angles,
                               int32_t proj_rows, int32_t proj_cols, int32_t proj_num,
                               float vox_wid, float rot_center, float obj2det_dist,
                               float light_energy, float pag_regparam, FILE* debug_msg_ptr) {
-  (void)measurements;
-  (void)brights;
-  (void)proj_angles;
-  (void)proj_rows;
-  (void)proj_cols;
-  (void)proj_num;
-  (void)vox_wid;
-  (void)rot_center;
-  (void)obj2det_dist;
-  (void)light_energy;
-  (void)pag_regparam;
-  say(debug_msg_ptr, "ERROR", "phcontomo_forward_project: the phantom simulator "
-      "(PCMBIR_3D/XT_genSinogram.c:59-343) is not available in this build.\n");
-  return -1;
+  (void)pag_regparam; /* stored by the reference, unused by its simulator */
+  if (!measurements || !brights) return -1;
+  /* the phantoms the reference reads from MAG_PHANTOM_FILEPATH /
+   * PHASE_PHANTOM_FILEPATH (PCMBIR_3D/XT_Constants.h:91-92), here from the
+   * files named by PHCONTOMO_MAG_PHANTOM / PHCONTOMO_PHASE_PHANTOM */
+  const char* mpath = getenv("PHCONTOMO_MAG_PHANTOM");
+  const char* ppath = getenv("PHCONTOMO_PHASE_PHANTOM");
+  const size_t nv = (size_t)PC_PHANTOM_Z_SIZE * PC_PHANTOM_XY_SIZE * PC_PHANTOM_XY_SIZE;
+  float* mag = (float*)malloc(sizeof(float) * nv);
+  float* phase = (float*)malloc(sizeof(float) * nv);
+  int rc = -1;
+  FILE* fp = NULL;
+  if (!mag || !phase || !mpath || !ppath) {
+    say(debug_msg_ptr, "ERROR", "Set PHCONTOMO_MAG_PHANTOM and PHCONTOMO_PHASE_PHANTOM to the "
+        "%d x %d x %d float phantom files.\n", PC_PHANTOM_Z_SIZE, PC_PHANTOM_XY_SIZE,
+        PC_PHANTOM_XY_SIZE);
+    goto done;
+  }
+  fp = fopen(mpath, "rb");
+  if (!fp || fread(mag, sizeof(float), nv, fp) != nv) {
+    say(debug_msg_ptr, "ERROR", "Error in reading file %s\n", mpath);
+    goto done;
+  }
+  fclose(fp);
+  fp = fopen(ppath, "rb");
+  if (!fp || fread(phase, sizeof(float), nv, fp) != nv) {
+    say(debug_msg_ptr, "ERROR", "Error in reading file %s\n", ppath);
+    goto done;
+  }
+  rc = phcontomo_forward_project_ex(*measurements, *brights, mag, phase, PC_PHANTOM_XY_SIZE,
+                                    PC_PHANTOM_Z_SIZE, proj_angles, proj_rows, proj_cols,
+                                    proj_num, vox_wid, rot_center, obj2det_dist, light_energy,
+                                    PC_NOISE_SEED, debug_msg_ptr);
+done:
+  if (fp) fclose(fp);
+  free(mag);
+  free(phase);
+  return rc;
 }

@@ -57,6 +57,22 @@ def interlaced_views(K, n_theta, n_views, proj_start=0, min_acq_time=0.0, rot_sp
     return views, times
 
 
+def phantom_project(mag, phase, angles, proj_rows, proj_cols, vox_wid, rot_center, wavenumber):
+    """The phase-contrast simulator's phantom projections on the GPU
+    (mbir_hip_phantom_project): (real, imag) [views][n_z][n_xy]."""
+    L = _lib.load()
+    mag = np.ascontiguousarray(mag, np.float32)
+    phase = np.ascontiguousarray(phase, np.float32)
+    ang = np.ascontiguousarray(angles, np.float32)
+    n_z, n_xy = mag.shape[0], mag.shape[-1]
+    pr = np.zeros((ang.size, n_z, n_xy), np.float32)
+    pi = np.zeros_like(pr)
+    _lib.check(L.mbir_hip_phantom_project(_p(mag), _p(phase), n_xy, n_z, ang.size, _p(ang),
+                                          proj_rows, proj_cols, float(vox_wid), float(rot_center),
+                                          float(wavenumber), _p(pr), _p(pi)), "phantom_project")
+    return pr, pi
+
+
 def aps_normalize(proj, white, dark, ratio, bh_coef=0.5):
     """Projections and weights [views][rows][cols] from raw uint16 counts
     (mbir_hip_aps_normalize, the APS driver's read_data flat-field step):

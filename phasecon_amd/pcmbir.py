@@ -214,6 +214,9 @@ _DROPIN = {
                                                                     C.c_void_p]),
     "phcontomo_forward_project": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p] +
                                   [C.c_int32] * 3 + [C.c_float] * 5 + [C.c_void_p]),
+    "phcontomo_forward_project_ex": (C.c_int, [C.c_void_p] * 4 + [C.c_int32] * 2 +
+                                     [C.c_void_p] + [C.c_int32] * 3 + [C.c_float] * 4 +
+                                     [C.c_uint64, C.c_void_p]),
 }
 
 
@@ -259,3 +262,62 @@ def phcontomo_reconstruct(measurements, brights, angles, proj_times, recon_times
     if rc == 0 and (mag.value or ph.value):
         raise RuntimeError("phcontomo_reconstruct returned object pointers; expected NULL")
     return rc
+
+
+def _dropin():
+    L = load()
+    for name, (res, args) in _DROPIN.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    return L
+
+
+def forward_project(mag, phase, angles, proj_rows, proj_cols, vox_wid, rot_center, obj2det_dist,
+                    light_energy, noise_seed=0):
+    """The phantom simulator on given phantoms [n_z][n_xy][n_xy]
+    (phcontomo_forward_project_ex): returns (measurements [views][proj_rows]
+    [proj_cols], brights [proj_rows][proj_cols]).  Writes SimMagProjs.bin,
+    SimPhaseProjs.bin, mag_phantom.bin and phase_phantom.bin in the CWD, as
+    the reference's simulator does.  noise_seed 0: no noise."""
+    L = _dropin()
+    mag = np.ascontiguousarray(mag, np.float32)
+    phase = np.ascontiguousarray(phase, np.float32)
+    ang = np.ascontiguousarray(angles, np.float32)
+    n_z, n_xy = mag.shape[0], mag.shape[-1]
+    meas = np.zeros((ang.size, proj_rows, proj_cols), np.float32)
+    br = np.zeros((proj_rows, proj_cols), np.float32)
+    f = C.c_float
+    rc = L.phcontomo_forward_project_ex(_p(meas), _p(br), _p(mag), _p(phase), n_xy, n_z, _p(ang),
+                                        proj_rows, proj_cols, ang.size, f(vox_wid),
+                                        f(rot_center), f(obj2det_dist), f(light_energy),
+                                        int(noise_seed), None)
+    if rc != 0:
+        raise RuntimeError("phcontomo_forward_project_ex failed")
+    return meas, br
+
+
+def phcontomo_forward_project(angles, proj_rows, proj_cols, vox_wid, rot_center, obj2det_dist,
+                              light_energy, pag_regparam=0.0, log_path=None):
+    """The drop-in phcontomo_forward_project (include/pcmbir4d.h) with the
+    reference's arguments (phantoms from the files named by
+    PHCONTOMO_MAG_PHANTOM / PHCONTOMO_PHASE_PHANTOM); returns (rc,
+    measurements, brights)."""
+    L = _dropin()
+    ang = np.ascontiguousarray(angles, np.float32)
+    meas = np.zeros((ang.size, proj_rows, proj_cols), np.float32)
+    br = np.zeros((proj_rows, proj_cols), np.float32)
+    pm, pb = C.c_void_p(meas.ctypes.data), C.c_void_p(br.ctypes.data)
+    libc = C.CDLL(None)
+    libc.fopen.restype = C.c_void_p
+    libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+    libc.fclose.argtypes = [C.c_void_p]
+    fp = libc.fopen((log_path or "/dev/null").encode(), b"w")
+    f = C.c_float
+    try:
+        rc = L.phcontomo_forward_project(C.byref(pm), C.byref(pb), _p(ang), proj_rows, proj_cols,
+                                         ang.size, f(vox_wid), f(rot_center), f(obj2det_dist),
+                                         f(light_energy), f(pag_regparam), C.c_void_p(fp))
+    finally:
+        libc.fclose(C.c_void_p(fp))
+    return rc, meas, br

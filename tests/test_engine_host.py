@@ -84,7 +84,12 @@ def test_pcmbir_dropin_rejects_bad_sizes_before_any_gpu_call(rows, cols, why, tm
 
 
 @pytest.mark.skipif(not os.path.exists(PCLIB), reason="libpcmbir4d.so not built")
-def test_pcmbir_forward_project_reports_that_it_is_absent(tmp_path):
+def test_pcmbir_forward_project_needs_its_phantom_files(tmp_path, monkeypatch):
+    """Without the phantom files (the reference reads fixed paths; here the
+    environment names them) the simulator returns -1 and says what to set,
+    before any GPU work."""
+    monkeypatch.delenv("PHCONTOMO_MAG_PHANTOM", raising=False)
+    monkeypatch.delenv("PHCONTOMO_PHASE_PHANTOM", raising=False)
     L = C.CDLL(PCLIB)
     libc = C.CDLL(None)
     libc.fopen.restype = C.c_void_p
@@ -97,7 +102,7 @@ def test_pcmbir_forward_project_reports_that_it_is_absent(tmp_path):
                                      17.0, 50.0, C.c_void_p(fp))
     libc.fclose(C.c_void_p(fp))
     assert rc == -1
-    assert "not available" in (tmp_path / "log").read_text()
+    assert "PHCONTOMO_MAG_PHANTOM" in (tmp_path / "log").read_text()
 
 
 @needs_lib
