@@ -109,10 +109,11 @@ GRAPH = True  # equits as replays of a captured HIP graph (--no-graph: direct la
 
 
 NH = None  # (passes, fraction): non-homogeneous passes after each equit from the second
+NH_STAGES = "full"  # stages that run them: "full", "coarse" or "all"
 
 
 def _equit(st, it, seed, nh=False):
-    nh = nh and NH
+    nh = NH and (NH_STAGES == "all" or (NH_STAGES == "full") == nh)
     if nh and not getattr(st, "_nh_on", False):
         st.option("nhicd", 1)
         st._nh_on = True
@@ -504,6 +505,7 @@ def main():
     ap.add_argument("--nh", default="",
                     help="non-homogeneous ICD after every full-resolution equit from the second: "
                          "'passes,fraction' (e.g. '1,0.25'); default off")
+    ap.add_argument("--nh-stages", choices=("full", "coarse", "all"), default="full")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep --tile / --coarse-tile instead of timing candidates per stage")
     ap.add_argument("--stages", default="",
@@ -543,8 +545,9 @@ def main():
     ap.add_argument("--pcmbir-views", type=int, default=360)
     ap.add_argument("--pcmbir-n", type=int, default=512)
     args = ap.parse_args()
-    global GRAPH, NH
+    global GRAPH, NH, NH_STAGES
     GRAPH = not args.no_graph
+    NH_STAGES = args.nh_stages
     if args.nh:
         p_, f_ = args.nh.split(",")
         NH = (int(p_), float(f_))
@@ -720,7 +723,7 @@ def main():
                 if coarse else "single full-resolution stage from zero"),
             "refresh_every": args.refresh,
             "equit_launch": "captured HIP graph" if GRAPH else "direct launches",
-            "nhicd": ({"passes": NH[0], "fraction": NH[1]} if NH else None),
+            "nhicd": ({"passes": NH[0], "fraction": NH[1], "stages": NH_STAGES} if NH else None),
             "cstar": cstar,
             "cstar_source": csrc,
             "target_rel": REL_TOL,
