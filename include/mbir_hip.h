@@ -183,6 +183,18 @@ int mbir_hip_phantom_project(const float* mag, const float* phase, int32_t n_xy,
                              int32_t proj_cols, double vox_wid, double rot_center,
                              double wavenumber, float* proj_real, float* proj_imag);
 
+/* Down-sampling of already normalised data, the standard-format driver's
+ * read_ProjWeightData (reconstruct/real_data/Standard_Data_Format/
+ * XT_HDFIO.c:90-111), on the GPU from the caller's HDF5 hyperslabs:
+ * proj / weights_in [n_proj][rows ratio][cols ratio] float.  Per output
+ * sample, t = the mean of its ratio x ratio block of projections and
+ * projections = bh_coef t^2 + t (BH_QUAD_COEF = 0 for this driver),
+ * weights = the block's sum of weights; outputs [n_proj][rows][cols], in
+ * the reference's float / double arithmetic. */
+int mbir_hip_std_downsample(const float* proj, const float* weights_in, int32_t n_proj,
+                            int32_t rows, int32_t cols, int32_t ratio, double bh_coef,
+                            float* projections, float* weights);
+
 /* Flat-field normalisation of raw APS counts (read_data,
  * reconstruct/real_data/APS_Data_Format/XT_HDFIO.c:138-178), on the GPU,
  * from host buffers the caller has read from its HDF5 files (the

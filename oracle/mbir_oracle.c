@@ -24,6 +24,13 @@
  *   orc_cost              reference build computeCost; hand-computed 2x2x2
  *   orc_icd_equit         reference ICD cost trajectory is monotone and
  *                         reaches the same converged cost (convexity)
+ *   orc_offset_update,    reference build update_Sinogram_Offset /
+ *   orc_variance_update   update_variance_parameter; the constraints each
+ *                         mode must satisfy; the all-selected variance
+ *   orc_phantom_project   the simulator's projection (PCMBIR, whose build
+ *                         needs FFTW): identity z response on matched
+ *                         slices, mass conservation per view and row,
+ *                         linearity (tests/test_simulator_oracle.py)
  * orc_icd_equit_tiled is a schedule study tool with no reference counterpart.
  */
 #include "mbir_oracle.h"

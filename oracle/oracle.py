@@ -327,6 +327,26 @@ def aps_normalize(proj, white, dark, ratio, bh_coef=0.5):
     return (bh_coef * t * t + t).astype(f32), w
 
 
+def std_downsample(proj, weights, ratio, bh_coef=0.0):
+    """read_ProjWeightData's arithmetic (reconstruct/real_data/
+    Standard_Data_Format/XT_HDFIO.c:90-111): float32 block sums (block rows
+    outer, channels inner), the mean divided in float32, the polynomial in
+    float64.  Returns (projections, weights) [views][rows][cols]."""
+    f32 = np.float32
+    proj = np.asarray(proj, f32)
+    weights = np.asarray(weights, f32)
+    n, R, Cc = proj.shape
+    rows, cols = R // ratio, Cc // ratio
+    t = np.zeros((n, rows, cols), f32)
+    w = np.zeros((n, rows, cols), f32)
+    for m in range(ratio):
+        for q in range(ratio):
+            t = t + proj[:, m::ratio, q::ratio]
+            w = w + weights[:, m::ratio, q::ratio]
+    td = (t / f32(ratio * ratio)).astype(np.float64)
+    return (bh_coef * td * td + td).astype(f32), w
+
+
 def reference_interlaced_views(K, n_theta, n_views, proj_start=0, min_acq_time=0.0,
                                rot_speed=0.0):
     """The APS driver's own gen_interlaced_views_0_to_Inf (XT_Views.c:7-46),

@@ -1463,6 +1463,36 @@ __global__ __launch_bounds__(256) void k_aps_normalize(const uint16_t* __restric
   out_w[idx] = w;
 }
 
+// Down-sampling of the standard-format driver (read_ProjWeightData,
+// reconstruct/real_data/Standard_Data_Format/XT_HDFIO.c:90-111): per output
+// sample, the mean of its ratio x ratio block of projections (float sum in
+// block-row-outer order, divided in float) through the beam-hardening
+// polynomial b t^2 + t (double, unfused), and the sum of the block's
+// weights.
+__global__ __launch_bounds__(256) void k_std_downsample(const float* __restrict__ proj,
+                                                        const float* __restrict__ wts,
+                                                        int n_proj, int rows, int cols,
+                                                        int ratio, double bh,
+                                                        float* __restrict__ out_p,
+                                                        float* __restrict__ out_w) {
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long long)n_proj * rows * cols) return;
+  const int k = (int)(idx % cols), j = (int)((idx / cols) % rows);
+  const long long i = idx / ((long long)rows * cols);
+  const long long W = (long long)cols * ratio;
+  const long long base = i * rows * ratio * W;
+  float t = 0.f, w = 0.f;
+  for (int m = 0; m < ratio; ++m)
+    for (int n = 0; n < ratio; ++n) {
+      const long long q = base + (long long)(j * ratio + m) * W + (long long)k * ratio + n;
+      t = __fadd_rn(t, proj[q]);
+      w = __fadd_rn(w, wts[q]);
+    }
+  const double td = (double)__fdiv_rn(t, (float)(ratio * ratio));
+  out_p[idx] = (float)__dadd_rn(__dmul_rn(__dmul_rn(bh, td), td), td);
+  out_w[idx] = w;
+}
+
 // theta1/theta2 of individual voxels through the sweep's column routine.
 template <int kW>
 __global__ __launch_bounds__(kWave) void k_thetas(SwGeom G, ColumnArgs A,
@@ -3037,6 +3067,34 @@ int mbir_hip_phantom_project(const float* mag, const float* phase, int32_t n_xy,
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpy(proj_real, B.orr, sizeof(float) * nout, hipMemcpyDeviceToHost));
   HIP_OK(hipMemcpy(proj_imag, B.oi, sizeof(float) * nout, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int mbir_hip_std_downsample(const float* proj, const float* weights_in, int32_t n_proj,
+                            int32_t rows, int32_t cols, int32_t ratio, double bh_coef,
+                            float* projections, float* weights) {
+  if (!proj || !weights_in || !projections || !weights) return fail("null argument");
+  if (n_proj < 1 || rows < 1 || cols < 1 || ratio < 1)
+    return fail("counts, rows, cols and ratio must be positive");
+  const size_t nin = (size_t)n_proj * rows * ratio * cols * ratio;
+  const size_t nout = (size_t)n_proj * rows * cols;
+  struct Bufs {
+    float *p = nullptr, *w = nullptr, *op = nullptr, *ow = nullptr;
+    ~Bufs() {
+      for (void* b : {(void*)p, (void*)w, (void*)op, (void*)ow})
+        if (b) (void)hipFree(b);
+    }
+  } B;
+  if (dev_alloc(&B.p, nin) || dev_alloc(&B.w, nin) || dev_alloc(&B.op, nout) ||
+      dev_alloc(&B.ow, nout))
+    return -1;
+  HIP_OK(hipMemcpy(B.p, proj, sizeof(float) * nin, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(B.w, weights_in, sizeof(float) * nin, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_std_downsample, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0,
+                     nullptr, B.p, B.w, n_proj, rows, cols, ratio, bh_coef, B.op, B.ow);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpy(projections, B.op, sizeof(float) * nout, hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(weights, B.ow, sizeof(float) * nout, hipMemcpyDeviceToHost));
   return 0;
 }
 

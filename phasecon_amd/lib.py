@@ -70,6 +70,8 @@ _SIGS = {
     "mbir_hip_get_magnitudes": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     "mbir_hip_phantom_project": (C.c_int, [C.c_void_p] * 2 + [C.c_int32] * 3 + [C.c_void_p]
                                  + [C.c_int32] * 2 + [C.c_double] * 3 + [C.c_void_p] * 2),
+    "mbir_hip_std_downsample": (C.c_int, [C.c_void_p] * 2 + [C.c_int32] * 4
+                                + [C.c_double, C.c_void_p, C.c_void_p]),
     "mbir_hip_aps_normalize": (C.c_int, [C.c_void_p] * 3 + [C.c_int32] * 6
                                + [C.c_double, C.c_void_p, C.c_void_p]),
     "mbir_interlaced_views": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,

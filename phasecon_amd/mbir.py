@@ -73,6 +73,24 @@ def phantom_project(mag, phase, angles, proj_rows, proj_cols, vox_wid, rot_cente
     return pr, pi
 
 
+def std_downsample(proj, weights, ratio, bh_coef=0.0):
+    """Projections and weights [views][rows][cols] from normalised data
+    [views][rows*ratio][cols*ratio] (mbir_hip_std_downsample, the
+    standard-format driver's read_ProjWeightData)."""
+    L = _lib.load()
+    proj = np.ascontiguousarray(proj, np.float32)
+    weights = np.ascontiguousarray(weights, np.float32)
+    n, R, Cc = proj.shape
+    if R % ratio or Cc % ratio or weights.shape != proj.shape:
+        raise ValueError("shapes must agree and divide by ratio")
+    rows, cols = R // ratio, Cc // ratio
+    out_p = np.zeros((n, rows, cols), np.float32)
+    out_w = np.zeros((n, rows, cols), np.float32)
+    _lib.check(L.mbir_hip_std_downsample(_p(proj), _p(weights), n, rows, cols, int(ratio),
+                                         float(bh_coef), _p(out_p), _p(out_w)), "std_downsample")
+    return out_p, out_w
+
+
 def aps_normalize(proj, white, dark, ratio, bh_coef=0.5):
     """Projections and weights [views][rows][cols] from raw uint16 counts
     (mbir_hip_aps_normalize, the APS driver's read_data flat-field step):

@@ -71,3 +71,18 @@ def test_aps_ratio_one_and_frame_means():
         tt = np.log(np.float32(2989.0) / w.astype(np.float64))
     assert np.array_equal(np.isfinite(p), np.isfinite(tt))
     assert np.allclose(p[np.isfinite(p)], tt[np.isfinite(tt)], rtol=1e-6, atol=5e-7)
+
+
+def test_std_downsample_closed_form():
+    """Standard format: blocks of equal values keep their value (b = 0),
+    the weights add up; with b the polynomial b t^2 + t."""
+    rng = np.random.default_rng(2)
+    block = rng.random((3, 4, 5)).astype(np.float32)
+    wblk = rng.random((3, 4, 5)).astype(np.float32)
+    up = lambda a: np.repeat(np.repeat(a, 2, 1), 2, 2)
+    p, w = O.std_downsample(up(block), up(wblk), 2)
+    assert np.array_equal(p, block)
+    assert np.allclose(w, 4 * wblk, rtol=1e-6)
+    p, _ = O.std_downsample(up(block), up(wblk), 2, bh_coef=0.5)
+    t = block.astype(np.float64)
+    assert np.allclose(p, 0.5 * t * t + t, rtol=1e-6)

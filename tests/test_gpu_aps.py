@@ -36,3 +36,20 @@ def test_aps_normalize_matches_oracle(n, rows, cols, ratio, nw, nd):
     print(f"projections bit-exact {exact:.6f}", flush=True)
     np.testing.assert_array_max_ulp(p[fin], rp[fin], maxulp=1)
     assert exact >= 0.999
+
+
+@pytest.mark.parametrize("n,rows,cols,ratio,bh", [(4, 8, 12, 1, 0.0), (20, 16, 32, 4, 0.0),
+                                                   (6, 8, 16, 2, 0.5)])
+def test_std_downsample_matches_oracle(n, rows, cols, ratio, bh):
+    """The standard-format down-sampling: bit-exact (float sums in the
+    reference's order, an exact double polynomial rounded once)."""
+    from oracle.oracle import std_downsample as ref
+    from phasecon_amd.mbir import std_downsample
+    rng = np.random.default_rng(n)
+    proj = rng.random((n, rows * ratio, cols * ratio)).astype(np.float32) * 3
+    wts = rng.random((n, rows * ratio, cols * ratio)).astype(np.float32)
+    print(f"std n={n} rows={rows} cols={cols} ratio={ratio} bh={bh}", flush=True)
+    p, w = std_downsample(proj, wts, ratio, bh)
+    rp, rw = ref(proj, wts, ratio, bh)
+    assert np.array_equal(w, rw)
+    assert np.array_equal(p, rp)
