@@ -316,24 +316,28 @@ __global__ __launch_bounds__(256) void k_prior_cost(DevGeom G,
                                                     const float* __restrict__ x,
                                                     double* __restrict__ partial) {
   __shared__ double red[256];
-  const int fwd[13][3] = {{0, 0, 1},  {0, 1, -1},  {0, 1, 0}, {0, 1, 1},
-                          {1, -1, 0}, {1, 0, 0},   {1, 1, 0}, {1, -1, -1},
-                          {1, -1, 1}, {1, 0, -1},  {1, 1, -1}, {1, 1, 1},
-                          {1, 0, 1}};
-  const size_t vol = (size_t)G.N_y * G.N_x * G.N_z;
-  const size_t n = vol * G.N_time;
+  constexpr int fwd[13][3] = {{0, 0, 1},  {0, 1, -1},  {0, 1, 0}, {0, 1, 1},
+                              {1, -1, 0}, {1, 0, 0},   {1, 1, 0}, {1, -1, -1},
+                              {1, -1, 1}, {1, 0, -1},  {1, 1, -1}, {1, 1, 1},
+                              {1, 0, 1}};
+  // one thread per voxel (consecutive threads walk a column's z, so the
+  // neighbour loads coalesce); 32-bit index arithmetic (the voxel count is
+  // below 2^31, checked on the host)
+  const unsigned Nz = (unsigned)G.N_z, Nx = (unsigned)G.N_x;
+  const unsigned ncol = (unsigned)G.N_y * Nx;
+  const unsigned vol = ncol * Nz;
+  const unsigned n = vol * (unsigned)G.N_time;
   double local = 0.0;
-  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * 256) {
-    const int z = (int)(i % G.N_z);
-    const size_t c = (i / G.N_z) % ((size_t)G.N_y * G.N_x);
-    const int f = (int)(i / vol);
-    const int yy = (int)(c / G.N_x), xx = (int)(c % G.N_x);
+  for (unsigned i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+    const unsigned col = i / Nz, z = i - col * Nz;
+    const unsigned f = col / ncol, c = col - f * ncol;
+    const int yy = (int)(c / Nx), xx = (int)(c - (unsigned)yy * Nx);
     const double v = x[i];
+#pragma unroll
     for (int k = 0; k < 13; ++k) {
-      const int z2 = z + fwd[k][0], y2 = yy + fwd[k][1], x2 = xx + fwd[k][2];
+      const int z2 = (int)z + fwd[k][0], y2 = yy + fwd[k][1], x2 = xx + fwd[k][2];
       if (z2 >= G.N_z || y2 < 0 || y2 >= G.N_y || x2 < 0 || x2 >= G.N_x) continue;
-      const double nv = x[(size_t)f * vol + ((size_t)y2 * G.N_x + x2) * G.N_z + z2];
+      const double nv = x[f * vol + ((unsigned)y2 * Nx + (unsigned)x2) * Nz + (unsigned)z2];
       const int fi = ((fwd[k][0] + 1) * 3 + (fwd[k][1] + 1)) * 3 + fwd[k][2] + 1;
       if (kFast)
         local += (double)(G.filt[fi] * qg_value32((float)(v - nv), G.q - G.p_s, G.c_s,
@@ -341,7 +345,7 @@ __global__ __launch_bounds__(256) void k_prior_cost(DevGeom G,
       else
         local += G.dfilt[fi] * qg_value64(v - nv, G.dq, G.dp_s, G.dc_s, G.dsq_s, G.dsqp_s);
     }
-    if (f + 1 < G.N_time) {
+    if (f + 1 < (unsigned)G.N_time) {
       const double d = v - (double)x[i + vol];
       if (kFast)
         local += (double)(G.filt_t *
@@ -2850,6 +2854,8 @@ int mbir_hip_cost(mbir_hip_ctx* c, int32_t fresh, double* out) {
   data /= 2.0;
   data += (double)g.N_p * (double)g.N_r * (double)g.N_t * log(g.var_est) / 2;
   {
+    if ((long long)g.N_y * g.N_x * g.N_z * g.N_time >= 0x7fffffffLL)
+      return fail("too many voxels for the prior cost");
     const unsigned blocks = 1024;
     if (ensure_partial(c, blocks)) return -1;
     if (ev_begin(c, MBIR_K_COST)) return -1;
