@@ -2096,6 +2096,11 @@ size_t fwd_tchunk(const mbir_hip_geom& g, size_t acc_bytes) {
   const size_t cap = 56 * 1024;  // LDS budget per workgroup (dynamic part)
   size_t tc = cap / (acc_bytes * (size_t)g.N_r);
   if (tc > (size_t)g.N_t) tc = g.N_t;
+  // at least about 1024 workgroups (four per CU; 2048 measured slower): a
+  // view's rows are split further when there are few views and rows, at the
+  // price of computing each column's A window once per chunk
+  const size_t par = ((size_t)g.N_t * g.N_p + 1023) / 1024;
+  if (tc > par) tc = par;
   if (tc < 1) tc = 1;
   return tc;
 }
