@@ -107,3 +107,25 @@ def test_nh_converges_to_the_homogeneous_optimum(small4d_scan):
     b.close()
     print(f"equits 5, 20: homogeneous / with NH {costs}; 120: {fa:.12g} {fb:.12g}", flush=True)
     assert abs(fa - fb) <= 1e-6 * abs(fa)
+
+
+def test_nh_rejects_what_it_cannot_run(small4d_scan):
+    """Without the magnitude map, with a tile of more than 256 columns or a
+    fraction outside (0, 1] the call fails with a message and changes
+    nothing; zero passes change nothing either."""
+    st = _stage(small4d_scan, 8, nh=False)
+    st.icd(2, 1, 23)
+    x0 = st.get_object()
+    with pytest.raises(RuntimeError, match="nhicd"):
+        st.nhicd(1, 0.25, 3)
+    st.option("nhicd", 1)
+    for bad in (0.0, 1.5):
+        with pytest.raises(RuntimeError, match="fraction"):
+            st.nhicd(1, bad, 3)
+    st.nhicd(0, 0.25, 3)
+    st.option("tile", 32)
+    with pytest.raises(RuntimeError, match="256 columns"):
+        st.nhicd(1, 0.25, 3)
+    x1 = st.get_object()
+    st.close()
+    assert np.array_equal(x0, x1)
