@@ -69,7 +69,7 @@ METRIC = "sec to MAP cost ≤1e-5 rel of FP64 oracle; equits/s; HBM GB/s; learne
 PEAK_HBM_GBPS = 8000.0
 REL_TOL = 1e-5
 
-# HBM bytes per k_sweep launch measured with rocprofv3 on this build:
+# HBM bytes per k_sweep launch measured with rocprofv3 on this code:
 # FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (KiB x 1024), the factors read off a
 # cold 1 GiB device copy (bench.py --profile-equits 3 --calibrate).
 MEASURED_TRAFFIC = {("ch4d", 8, False): 145060.6 * 1024 * 2.0 + 120815.1 * 1024 * 1.0}

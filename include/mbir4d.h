@@ -11,7 +11,7 @@
  * definition (MBIR_4D/XT_Engine.c:81) returns int, as does this one.
  * The caller owns every input and frees *object with free().
  *
- * Differences from the reference, by design of this build:
+ * Differences from the reference, by design of this library:
  *   - single process, one GPU: no MPI; proj_rows is not split across ranks;
  *   - the ICD voxel order differs (parallel ICD on the GPU), so results are
  *     judged on the converged MAP cost, not per-iteration equality;

@@ -16,7 +16,7 @@
  *                 phase_tomo_aux.bin and the auxiliary files).
  * The caller owns every input.
  *
- * Differences from the reference, by design of this build:
+ * Differences from the reference, by design of this library:
  *   - single process, one GPU: no MPI; proj_rows is not split across ranks;
  *   - proj_rows and proj_cols must be powers of two up to 1024 (the GPU
  *     transforms are radix-2; the reference's FFTW takes any size);

@@ -1,7 +1,7 @@
 /*
  * mbir_oracle.c -- CPU restatement of the reference MBIR_4D hot path, in FP64.
  *
- * TEST INFRASTRUCTURE ONLY: the checker for the HIP path.  Only tests/,
+ * TEST INFRASTRUCTURE ONLY: the reference the HIP path is tested against.  Only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the
  * product library never does.  All file:line citations are relative to the
  * reference's Source_Code/ directory.

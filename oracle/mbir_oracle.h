@@ -1,7 +1,7 @@
 /*
  * mbir_oracle.h -- CPU restatement of the reference MBIR_4D hot path, in FP64.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
+ * TEST INFRASTRUCTURE ONLY.  This library is what the HIP path is tested against:
  * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
  * load it.  The product library (phasecon_amd/libmbir4d.so) never links or
  * calls it.

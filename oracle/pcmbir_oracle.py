@@ -1,5 +1,5 @@
 """FP64 numpy restatement of the PCMBIR phase-retrieval path (TEST
-INFRASTRUCTURE ONLY: imported by tests/ as the checker, never by the product).
+INFRASTRUCTURE ONLY: imported by tests/ to test against, never by the product).
 
 Follows, per function, the reference file:line it cites (paths relative to the
 reference's Source_Code/PCMBIR_3D/).  The reference computes its FFTs with

@@ -2,8 +2,9 @@
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, never by the product.
 
-The network has no counterpart in the reference (it is this build's warm
-start); the oracle restates its definition in numpy float64:
+The network has no counterpart in the reference (it is the learned warm
+start of this library); the oracle restates its definition in numpy
+float64:
 
   * conv3d / conv3d_bwd_data / conv3d_bwd_weight: 3x3x3 cross-correlation
     with zero padding, weights [cout][27][cin], tap = (dz+1)*9 + (dy+1)*3 +

@@ -1,5 +1,5 @@
 """Input stages of the APS driver: the time-interlaced view schedule
-(mbir_interlaced_views, host C) against the driver's own
+(mbir_interlaced_views, host C) against the APS program's own
 gen_interlaced_views_0_to_Inf (XT_Views.c, the reference build), and the
 oracle's flat-field normalisation (read_data, XT_HDFIO.c:138-178; that file
 needs libhdf5, absent here, so the oracle is pinned by closed forms) --
