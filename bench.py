@@ -56,6 +56,7 @@ from __future__ import annotations
 import argparse
 import hashlib
 import json
+import math
 import os
 import sys
 import time
@@ -125,24 +126,44 @@ def _equit(st, it, seed, nh=False):
         st.nhicd(NH[0], NH[1], it, seed + 104729 * it)
 
 
+PREDICT_CHECKS = True  # --check-every-equit turns the scheduled cost checks off
+
+
 def to_target(st, target, max_equits, seed, refresh):
     """Full-resolution equits from the stage's current object until the
-    FP64 MAP cost is <= target; returns (equits, cost)."""
+    FP64 MAP cost is <= target; returns (equits, cost).  Each cost check
+    (a prior-cost pass and a host synchronisation) schedules the next one:
+    from the decrease of C - C* since the last check, taken as geometric,
+    the equits still needed are predicted and the next check comes after
+    half of them (at least one equit later); with PREDICT_CHECKS off, after
+    every equit."""
     st.build_amatrix()
     st.init_error()
+    cstar = target / (1.0 + REL_TOL)
     it = 0
     cost = st.cost(estimate=True)[0]
+    prev = (0, cost - cstar)
+    check_at = 1
     while it < max_equits:
         it += 1
         _equit(st, it, seed, nh=True)
         if refresh and it % refresh == 0:
             st.init_error()
+        if it < check_at:
+            continue
         cost = st.cost(estimate=True)[0]
         if cost <= target:
             fresh = st.cost(fresh=True)[0]   # FP64 check from a recomputed e
             if fresh <= target:
                 cost = fresh
                 break
+        check_at = it + 1
+        gap = cost - cstar
+        if PREDICT_CHECKS and 0 < gap < prev[1]:
+            rate = (gap / prev[1]) ** (1.0 / (it - prev[0]))
+            need = math.log((target - cstar) / gap) / math.log(rate)
+            check_at = it + max(1, int(0.5 * need))
+        prev = (it, gap)
     return it, cost
 
 
@@ -506,6 +527,8 @@ def main():
                     help="non-homogeneous ICD after every full-resolution equit from the second: "
                          "'passes,fraction' (e.g. '1,0.25'); default off")
     ap.add_argument("--nh-stages", choices=("full", "coarse", "all"), default="full")
+    ap.add_argument("--check-every-equit", action="store_true",
+                    help="evaluate the stopping cost after every full-resolution equit")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep --tile / --coarse-tile instead of timing candidates per stage")
     ap.add_argument("--stages", default="",
@@ -545,8 +568,9 @@ def main():
     ap.add_argument("--pcmbir-views", type=int, default=360)
     ap.add_argument("--pcmbir-n", type=int, default=512)
     args = ap.parse_args()
-    global GRAPH, NH, NH_STAGES
+    global GRAPH, NH, NH_STAGES, PREDICT_CHECKS
     GRAPH = not args.no_graph
+    PREDICT_CHECKS = not args.check_every_equit
     NH_STAGES = args.nh_stages
     if args.nh:
         p_, f_ = args.nh.split(",")
@@ -723,6 +747,8 @@ def main():
                 if coarse else "single full-resolution stage from zero"),
             "refresh_every": args.refresh,
             "equit_launch": "captured HIP graph" if GRAPH else "direct launches",
+            "cost_checks": ("scheduled from the observed decrease" if PREDICT_CHECKS
+                            else "after every equit"),
             "nhicd": ({"passes": NH[0], "fraction": NH[1], "stages": NH_STAGES} if NH else None),
             "cstar": cstar,
             "cstar_source": csrc,
