@@ -535,7 +535,9 @@ def main():
                     help="explicit coarse stages, coarsest first: mult_xy:mult_z:equits,...")
     ap.add_argument("--max-equits", type=int, default=3000)
     ap.add_argument("--seed", type=int, default=761521)
-    ap.add_argument("--refresh", type=int, default=10)
+    ap.add_argument("--refresh", type=int, default=40,
+                    help="equits between exact error-sinogram refreshes (10, 20, 40 measured "
+                         "on ch4d: 0.500, 0.492, 0.488 s, the same equits to target)")
     ap.add_argument("--alpha", type=float, default=1.9,
                     help="over-relaxation factor of the full-resolution sweeps "
                          "(the reference's is 1.5)")
