@@ -3049,6 +3049,8 @@ int mbir_hip_phantom_project(const float* mag, const float* phase, int32_t n_xy,
   G.delta_xy = g.delta_xy;
   G.OffsetR = g.OffsetR;
   const int tchunk = std::max(1, std::min({kSimTc, n_z, (int)(48 * 1024 / (8 * n_xy))}));
+  if ((size_t)8 * n_xy * tchunk > 48 * 1024)
+    return fail("phantom too wide: one detector row of both projections must fit 48 KiB of LDS");
   const size_t nvox = (size_t)n_z * n_xy * n_xy, nout = (size_t)n_p * n_z * n_xy;
   struct Bufs {
     float *m = nullptr, *ph = nullptr, *h = nullptr, *w = nullptr, *orr = nullptr, *oi = nullptr;
