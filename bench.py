@@ -538,10 +538,10 @@ def main():
     ap.add_argument("--refresh", type=int, default=40,
                     help="equits between exact error-sinogram refreshes (10, 20, 40 measured "
                          "on ch4d: 0.500, 0.492, 0.488 s, the same equits to target)")
-    ap.add_argument("--alpha", type=float, default=1.9,
+    ap.add_argument("--alpha", type=float, default=1.97,
                     help="over-relaxation factor of the full-resolution sweeps "
                          "(the reference's is 1.5)")
-    ap.add_argument("--coarse-alpha", type=float, default=1.9)
+    ap.add_argument("--coarse-alpha", type=float, default=1.97)
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned warm start leg")
     ap.add_argument("--learned-scans", type=int, default=8)
