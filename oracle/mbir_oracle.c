@@ -36,6 +36,7 @@
 #include "mbir_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -923,6 +924,329 @@ void orc_icd_equit_tiled_sync(const orc_geom* g, const double* cosv,
   free(esnap);
   free(lists);
   free(lens);
+}
+
+/* ------------------------------------ asynchronous sweep model (study tool)
+ * The GPU sweep's schedule restated in FP64 to study its convergence on the
+ * CPU (no reference counterpart; it follows k_sweep / sweep_launches in
+ * phasecon_amd/csrc/mbir_hip.hip).  Per equit 8 launches: 2x2 tile colour,
+ * then frame parity.  A launch's work items are (tile, frame) pairs, each
+ * placed on one of `groups` cache groups (the GPU's XCDs, each with its own
+ * L2).  Every item visits its tile's masked columns in the order
+ * k -> (pb + k pa) mod n, pa and pb hashed from the launch seed, the tile and
+ * the frame as the kernel hashes them, and solves each column in two z-parity
+ * halves.  All items of a launch advance one column per step (lock-step).
+ * A column's theta is computed from the error sinogram minus the updates
+ * its item cannot see yet: an update made at step j by another item becomes
+ * visible at step j + lag_same + 1 when both items share a cache group, and
+ * at step j + lag_cross + 1 otherwise (lag_cross < 0: when the launch ends);
+ * an item always sees its own updates.  With lag_same = lag_cross = 0 the
+ * items of one step still do not see each other (lock-step Jacobi); with one
+ * item per frame per launch that is the sequential ICD in the GPU's order. */
+typedef struct {
+  int32_t tl, fl, grp;  /* tile index within the launch, frame index, group */
+} orc_item;
+
+static int32_t orc_gcd(int32_t a, int32_t b) {
+  while (b) {
+    int32_t t = a % b;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+/* Work items of a launch of ntl tiles x nfl frames.  placement 0: the
+ * tile-major grid (workgroup id = frame x ntl + tile, group = id mod
+ * groups); 1: frames bound to groups -- each frame split into
+ * r = groups / gcd(nfl, groups) parts (tiles t = part mod r), unit
+ * u = part x nfl + frame placed on group u mod groups. */
+static int32_t orc_place(int32_t placement, int32_t groups, int32_t ntl,
+                         int32_t nfl, orc_item* out) {
+  int32_t n = 0;
+  if (placement == 0) {
+    for (int32_t wid = 0; wid < ntl * nfl; ++wid) {
+      out[n].tl = wid % ntl;
+      out[n].fl = wid / ntl;
+      out[n].grp = wid % groups;
+      ++n;
+    }
+    return n;
+  }
+  const int32_t r = groups / orc_gcd(nfl, groups);
+  for (int32_t u = 0; u < nfl * r; ++u)
+    for (int32_t t = u / nfl; t < ntl; t += r) {
+      out[n].tl = t;
+      out[n].fl = u % nfl;
+      out[n].grp = u % groups;
+      ++n;
+    }
+  return n;
+}
+
+void orc_icd_equit_async(const orc_geom* g, const double* cosv,
+                         const double* sinv, const float* H,
+                         const int32_t* frame_ptr, const int32_t* frame_views,
+                         const uint8_t* mask, const orc_async_opts* op,
+                         double* x, double* e, const double* w,
+                         double* stats) {
+  orc_geom gg = *g;
+  gg.alpha = op->alpha;
+  const int32_t Nx = g->N_x, Ny = g->N_y, Nz = g->N_z, Nf = g->N_time;
+  const int32_t T = op->T, X = op->groups > 0 ? op->groups : 1;
+  const int32_t TX = (Nx + T - 1) / T, TY = (Ny + T - 1) / T;
+  const int32_t cap = g->W, zo = g->mult_z;
+  const size_t vol = (size_t)Nz * Ny * Nx;
+  const size_t ns = (size_t)g->N_p * g->N_r * g->N_t;
+  const int lag_s = op->lag_same, lag_c = op->lag_cross;
+  const int hidden = lag_s != 0 || lag_c != 0;
+  int32_t nvmax = 0;
+  for (int32_t f = 0; f < Nf; ++f)
+    if (frame_ptr[f + 1] - frame_ptr[f] > nvmax) nvmax = frame_ptr[f + 1] - frame_ptr[f];
+  /* tiles as build_tiles: masked columns in raster order, grouped by colour */
+  int32_t* tptr = (int32_t*)calloc((size_t)TX * TY + 1, sizeof(int32_t));
+  int32_t* tcols = (int32_t*)malloc(sizeof(int32_t) * ((size_t)Nx * Ny + 1));
+  int32_t* order = (int32_t*)malloc(sizeof(int32_t) * ((size_t)TX * TY + 1));
+  int32_t off[5], nt = 0, nc = 0;
+  for (int32_t ty = 0; ty < TY; ++ty)
+    for (int32_t tx = 0; tx < TX; ++tx) {
+      for (int32_t yy = ty * T; yy < ty * T + T && yy < Ny; ++yy)
+        for (int32_t xx = tx * T; xx < tx * T + T && xx < Nx; ++xx)
+          if (mask[yy * Nx + xx]) tcols[nc++] = yy * Nx + xx;
+      tptr[ty * TX + tx + 1] = nc;
+    }
+  for (int colour = 0; colour < 4; ++colour) {
+    off[colour] = nt;
+    for (int32_t ty = 0; ty < TY; ++ty)
+      for (int32_t tx = 0; tx < TX; ++tx) {
+        int32_t id = ty * TX + tx;
+        if ((tx % 2) + 2 * (ty % 2) == colour && tptr[id + 1] > tptr[id]) order[nt++] = id;
+      }
+  }
+  off[4] = nt;
+  const int32_t max_items = nt * ((Nf + 1) / 2) * 1 + 1;
+  orc_item* items = (orc_item*)malloc(sizeof(orc_item) * (size_t)max_items);
+  /* per item: visiting order parameters and the column's update entries */
+  int32_t* pa = (int32_t*)malloc(sizeof(int32_t) * (size_t)max_items);
+  int32_t* pb = (int32_t*)malloc(sizeof(int32_t) * (size_t)max_items);
+  const size_t ment = (size_t)Nz * nvmax * cap * zo;  /* entries per column */
+  const int ring = (lag_s > (lag_c > 0 ? lag_c : 0) ? lag_s : (lag_c > 0 ? lag_c : 0)) + 1;
+  size_t* didx = (size_t*)malloc(sizeof(size_t) * ring * (size_t)max_items * ment);
+  double* dval = (double*)malloc(sizeof(double) * ring * (size_t)max_items * ment);
+  int32_t* dcnt = (int32_t*)calloc((size_t)ring * max_items, sizeof(int32_t));
+  double *hs = NULL, *hc = NULL, *htot = NULL, *own_all = NULL;
+  if (hidden) {
+    own_all = (double*)calloc((size_t)omp_get_max_threads() * ns, sizeof(double));
+    hs = (double*)calloc((size_t)X * ns, sizeof(double));
+    hc = (double*)calloc((size_t)X * ns, sizeof(double));
+    htot = (double*)calloc(ns, sizeof(double));
+  }
+  double dsum = 0, xsum = 0, skipped = 0;
+  for (int colour = 0; colour < 4; ++colour)
+    for (int fpar = 0; fpar < 2; ++fpar) {
+      const int32_t ntl = off[colour + 1] - off[colour];
+      const int32_t nfl = (Nf - fpar + 1) / 2;
+      if (ntl == 0 || nfl <= 0) continue;
+      const int32_t ni = orc_place(op->placement, X, ntl, nfl, items);
+      const uint32_t lseed = op->seed * 2654435761U +
+                             (uint32_t)(op->call_index * 8 + colour * 2 + fpar) * 40503U;
+      int32_t steps = 0;
+      for (int32_t i = 0; i < ni; ++i) {
+        const int32_t tile = order[off[colour] + items[i].tl];
+        const int32_t f = fpar + 2 * items[i].fl;
+        const int32_t n = tptr[tile + 1] - tptr[tile];
+        const uint32_t h = orc_hash(lseed ^ orc_hash((uint32_t)tile * 131071U + (uint32_t)f * 524287U));
+        pa[i] = 1;
+        pb[i] = 0;
+        if (n > 1) {
+          pa[i] = 1 + (int32_t)(h % (uint32_t)(n - 1));
+          while (orc_gcd(pa[i], n) != 1) pa[i] = pa[i] % (n - 1) + 1;
+          pb[i] = (int32_t)(orc_hash(h + 0x9e3779b9U) % (uint32_t)n);
+        }
+        if (n > steps) steps = n;
+      }
+      memset(dcnt, 0, sizeof(int32_t) * (size_t)ring * max_items);
+      for (int32_t k = 0; k < steps; ++k) {
+        const int slot = k % ring;
+#pragma omp parallel reduction(+ : dsum, xsum, skipped)
+        {
+          const int tid = omp_get_thread_num();
+          double* own = hidden ? own_all + (size_t)tid * ns : NULL;
+          int32_t* cnt = (int32_t*)calloc(nvmax + 1, sizeof(int32_t));
+          int32_t* cidx = (int32_t*)calloc((size_t)(nvmax + 1) * cap, sizeof(int32_t));
+          double* cval = (double*)calloc((size_t)(nvmax + 1) * cap, sizeof(double));
+          double* th1 = (double*)malloc(sizeof(double) * Nz);
+          double* th2 = (double*)malloc(sizeof(double) * Nz);
+#pragma omp for schedule(dynamic, 4)
+          for (int32_t i = 0; i < ni; ++i) {
+            const int32_t tile = order[off[colour] + items[i].tl];
+            const int32_t f = fpar + 2 * items[i].fl, grp = items[i].grp;
+            const int32_t n = tptr[tile + 1] - tptr[tile];
+            size_t* my_idx = didx + ((size_t)slot * max_items + i) * ment;
+            double* my_val = dval + ((size_t)slot * max_items + i) * ment;
+            dcnt[(size_t)slot * max_items + i] = 0;
+            if (k >= n) continue;
+            const int32_t c = tcols[tptr[tile] + (int32_t)(((int64_t)pb[i] + (int64_t)k * pa[i]) % n)];
+            const int32_t yy = c / Nx, xx = c % Nx;
+            orc_line_columns(g, cosv, sinv, H, f, yy, xx, frame_ptr, frame_views,
+                             cnt, cidx, cval);
+            /* own recent updates, visible to their item */
+            if (hidden)
+              for (int back = 1; back <= lag_s && back <= k; ++back) {
+                const size_t b = ((size_t)((k - back) % ring) * max_items + i);
+                for (int32_t q = 0; q < dcnt[b]; ++q) own[didx[b * ment + q]] += dval[b * ment + q];
+              }
+            /* theta per z from the sinogram this item sees */
+            const int32_t s0 = frame_ptr[f], nv = frame_ptr[f + 1] - s0;
+            for (int32_t z = 0; z < Nz; ++z) {
+              double t1 = 0, t2 = 0;
+              for (int32_t si = 0; si < nv; ++si) {
+                const int32_t p = frame_views[s0 + si];
+                for (int32_t m = 0; m < cnt[si]; ++m) {
+                  const double a = cval[si * cap + m] * g->delta_t;
+                  const int32_t r = cidx[si * cap + m];
+                  for (int32_t kk = 0; kk < zo; ++kk) {
+                    const size_t o = ((size_t)p * g->N_r + r) * g->N_t + z * zo + kk;
+                    double evv = e[o];
+                    if (hidden)
+                      evv -= (hs[(size_t)grp * ns + o] - own[o]) + (htot[o] - hc[(size_t)grp * ns + o]);
+                    const double wv = w[o];
+                    if (fabs(evv * sqrt(wv)) < g->huber_T) {
+                      t2 += a * a * wv;
+                      t1 -= evv * a * wv;
+                    } else {
+                      const double hw = g->huber_T * g->huber_delta * sqrt(wv) / fabs(evv);
+                      t2 += a * a * hw;
+                      t1 -= evv * a * hw;
+                    }
+                  }
+                }
+              }
+              th1[z] = t1;
+              th2[z] = t2;
+            }
+            if (hidden)
+              for (int back = 1; back <= lag_s && back <= k; ++back) {
+                const size_t b = ((size_t)((k - back) % ring) * max_items + i);
+                for (int32_t q = 0; q < dcnt[b]; ++q) own[didx[b * ment + q]] = 0.0;
+              }
+            /* the column's two z-parity halves */
+            int32_t ne = 0;
+            for (int par = 0; par < 2; ++par)
+              for (int32_t z = par; z < Nz; z += 2) {
+                const size_t o = (((size_t)f * Nz + z) * Ny + yy) * Nx + xx;
+                double nb[27], tn[2];
+                uint8_t nbf[27], tf[2];
+                for (int dz = -1; dz <= 1; ++dz)
+                  for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                      const int kq = ((dz + 1) * 3 + dy + 1) * 3 + dx + 1;
+                      const int32_t z2 = z + dz, y2 = yy + dy, x2 = xx + dx;
+                      const int in = z2 >= 0 && z2 < Nz && y2 >= 0 && y2 < Ny && x2 >= 0 && x2 < Nx;
+                      nb[kq] = in ? x[(((size_t)f * Nz + z2) * Ny + y2) * Nx + x2] : 0;
+                      nbf[kq] = (uint8_t)in;
+                    }
+                nb[13] = 0;
+                tf[0] = f > 0;
+                tn[0] = tf[0] ? x[o - vol] : 0;
+                tf[1] = f + 1 < Nf;
+                tn[1] = tf[1] ? x[o + vol] : 0;
+                const double V = x[o];
+                if (V == 0.0 && op->iter > 1) {
+                  int any = (tn[0] > 0.0 || tn[1] > 0.0);
+                  for (int kq = 0; kq < 27 && !any; ++kq)
+                    if (nb[kq] > 0.0) any = 1;
+                  if (!any) {
+                    skipped += 1;
+                    continue;
+                  }
+                }
+                const double xn = orc_substitution(&gg, V, th1[z], th2[z], nb, nbf, tn, tf);
+                x[o] = xn;
+                dsum += fabs(xn - V);
+                xsum += fabs(xn);
+                if (xn == V) continue;
+                for (int32_t si = 0; si < nv; ++si) {
+                  const int32_t p = frame_views[s0 + si];
+                  for (int32_t m = 0; m < cnt[si]; ++m) {
+                    const double a = cval[si * cap + m] * g->delta_t;
+                    const int32_t r = cidx[si * cap + m];
+                    for (int32_t kk = 0; kk < zo; ++kk) {
+                      my_idx[ne] = ((size_t)p * g->N_r + r) * g->N_t + z * zo + kk;
+                      my_val[ne] = -a * (xn - V);
+                      ++ne;
+                    }
+                  }
+                }
+              }
+            dcnt[(size_t)slot * max_items + i] = ne;
+          }
+          free(cnt);
+          free(cidx);
+          free(cval);
+          free(th1);
+          free(th2);
+        }
+        /* apply the step's updates; items of one frame touch only that
+         * frame's views, so frames are applied in parallel */
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int32_t fl = 0; fl < nfl; ++fl)
+          for (int32_t i = 0; i < ni; ++i) {
+            if (items[i].fl != fl) continue;
+            const int32_t grp = items[i].grp;
+            const size_t b = (size_t)slot * max_items + i;
+            for (int32_t q = 0; q < dcnt[b]; ++q) {
+              const size_t o = didx[b * ment + q];
+              const double v = dval[b * ment + q];
+              e[o] += v;
+              if (hidden) {
+                if (lag_s > 0) hs[(size_t)grp * ns + o] += v;
+                if (lag_c != 0) {
+                  hc[(size_t)grp * ns + o] += v;
+                  htot[o] += v;
+                }
+              }
+            }
+            /* updates leaving the hidden windows */
+            if (hidden && lag_s > 0 && k - lag_s >= 0) {
+              const size_t bo = (size_t)((k - lag_s) % ring) * max_items + i;
+              for (int32_t q = 0; q < dcnt[bo]; ++q)
+                hs[(size_t)grp * ns + didx[bo * ment + q]] -= dval[bo * ment + q];
+            }
+            if (hidden && lag_c > 0 && k - lag_c >= 0) {
+              const size_t bo = (size_t)((k - lag_c) % ring) * max_items + i;
+              for (int32_t q = 0; q < dcnt[bo]; ++q) {
+                hc[(size_t)grp * ns + didx[bo * ment + q]] -= dval[bo * ment + q];
+                htot[didx[bo * ment + q]] -= dval[bo * ment + q];
+              }
+            }
+          }
+      }
+      /* a launch ends with every update visible */
+      if (hidden) {
+        memset(hs, 0, sizeof(double) * (size_t)X * ns);
+        memset(hc, 0, sizeof(double) * (size_t)X * ns);
+        memset(htot, 0, sizeof(double) * ns);
+      }
+    }
+  free(hs);
+  free(hc);
+  free(htot);
+  free(own_all);
+  free(didx);
+  free(dval);
+  free(dcnt);
+  free(pa);
+  free(pb);
+  free(items);
+  free(order);
+  free(tcols);
+  free(tptr);
+  if (stats) {
+    stats[0] = dsum;
+    stats[1] = xsum;
+    stats[2] = skipped;
+  }
 }
 
 void orc_sino_from_file(int32_t N_p, int32_t N_t, int32_t N_r,

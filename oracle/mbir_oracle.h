@@ -174,6 +174,33 @@ void orc_icd_equit_tiled_sync(const orc_geom* g, const double* cosv,
                               int32_t iter, int32_t m, double* x, double* e,
                               const double* w);
 
+/* Model of the GPU sweep (test-only study tool, no reference counterpart):
+ * the kernel's launch order, tiles, work-item placement on cache groups and
+ * column visiting order, all items of a launch in lock-step, and a column's
+ * theta computed without the other items' updates of the last lag_same
+ * steps (same group) or lag_cross steps (other groups; < 0: the whole
+ * launch so far).  lag_same = lag_cross = 0 with one item per frame per
+ * launch is the sequential ICD in the kernel's order.  stats as
+ * orc_icd_equit. */
+typedef struct {
+  int32_t T;           /* tile edge */
+  double alpha;        /* over-relaxation */
+  uint32_t seed;       /* the sweep call's seed */
+  int32_t iter;        /* iteration number (zero skipping from 2) */
+  int32_t call_index;  /* equit index within the sweep call */
+  int32_t lag_same, lag_cross;
+  int32_t groups;      /* cache groups (XCDs) */
+  int32_t placement;   /* 0: tile-major grid, group = id mod groups;
+                          1: frames bound to groups */
+} orc_async_opts;
+
+void orc_icd_equit_async(const orc_geom* g, const double* cosv,
+                         const double* sinv, const float* H,
+                         const int32_t* frame_ptr, const int32_t* frame_views,
+                         const uint8_t* mask, const orc_async_opts* op,
+                         double* x, double* e, const double* w,
+                         double* stats);
+
 /* File-layout <-> internal sinogram transpose (XT_Init.c:294-301):
  * file [p][t][r] -> internal [p][r][t]. */
 void orc_sino_from_file(int32_t N_p, int32_t N_t, int32_t N_r,

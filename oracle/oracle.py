@@ -40,6 +40,12 @@ class Geom(C.Structure):
          ("huber_T", C.c_double), ("huber_delta", C.c_double), ("var_est", C.c_double)]
 
 
+class AsyncOpts(C.Structure):
+    _fields_ = [("T", C.c_int32), ("alpha", C.c_double), ("seed", C.c_uint32),
+                ("iter", C.c_int32), ("call_index", C.c_int32), ("lag_same", C.c_int32),
+                ("lag_cross", C.c_int32), ("groups", C.c_int32), ("placement", C.c_int32)]
+
+
 def _lib(path):
     if not os.path.exists(path):
         raise FileNotFoundError(f"{path} is not built (make -C oracle)")
@@ -179,6 +185,24 @@ class Oracle:
                                    self._p(self.frame_views), self._p(self.mask), T,
                                    C.c_double(alpha), C.c_uint32(seed), iteration, self._p(x),
                                    self._p(e), self._p(self.w))
+
+    def icd_equit_async(self, x, e, iteration, T, alpha, seed, call_index=0, lag_same=0,
+                        lag_cross=0, groups=8, placement=0):
+        """One equit of the GPU sweep's schedule (orc_icd_equit_async): x
+        [N_time][N_z][N_y][N_x] and e [N_p][N_r][N_t] (float64) updated in
+        place; returns stats (sum |dx|, sum |x|, skipped)."""
+        if not (lag_cross < 0 or lag_cross >= lag_same) or lag_same < 0:
+            raise ValueError("need lag_same >= 0 and lag_cross < 0 or >= lag_same")
+        assert x.dtype == np.float64 and x.flags.c_contiguous and x.size == np.prod(self.shape_x)
+        assert e.dtype == np.float64 and e.flags.c_contiguous and e.shape == self.y.shape
+        op = AsyncOpts(T, alpha, seed & 0xFFFFFFFF, iteration, call_index, lag_same, lag_cross,
+                       groups, placement)
+        stats = np.zeros(3, np.float64)
+        self.L.orc_icd_equit_async(C.byref(self.g), self._p(self.cos), self._p(self.sin),
+                                   self._p(self.H), self._p(self.frame_ptr),
+                                   self._p(self.frame_views), self._p(self.mask), C.byref(op),
+                                   self._p(x), self._p(e), self._p(self.w), self._p(stats))
+        return stats
 
     def srandom2(self, seed):
         self.L.orc_srandom2(C.c_uint32(seed))

@@ -17,11 +17,15 @@ zero distance, the transmittance against numpy's complex exponential, the
 Paganin filter against its published form 1 / (1 + pi lambda z (delta/beta)
 |u|^2) and a uniform slab's thickness).
 
-Parity unpinned: the order of the steps inside estimate_complex_projection's
-loop, its dual update and its residual normalisations follow
-XT_CmplxProjEst.c:105-229 as read; each step is pinned above, the loop as a
-whole only by its own definition (no FFTW build and no reference fixture for
-it exist here).  nms_pass(order="colour") is the GPU's four-colour visiting
+estimate_complex_projection (XT_CmplxProjEst.c:105-229) is pinned by what
+the ADMM splitting fixes: the exact solution of a noise-free uniform view is
+a fixed point; after every iteration the scaled dual equals the data term's
+gradient over nu (grad f(w) = nu v, which a flipped dual sign, another
+w-step target or another step order breaks); without a prior the z-step
+lands on exp(-z) = w - v; and the residuals fall on a noise-free view.  Its
+two residual normalisations (the RMS of exp(-z) - w over the pixels, and the
+change of w relative to |v|) are pinned only by their reading of the
+reference.  nms_pass(order="colour") is the GPU's four-colour visiting
 order and has no reference counterpart; order="raster" is the reference's.
 
 Arrays are per view [rows][cols] = [N_r][N_t], complex128 (real/imag pairs
@@ -427,7 +431,9 @@ def estimate_complex_projection(meas, om, D, zr, zi, Lam, w, pr, geo, nu, pret_i
     costs = [pret_cost(meas, D, Lam, zr, zi, pr, geo)]
     prim, dual, nms_its, sd_its = [], [], [], []
     it = 0
+    w_prev, v_prev = w, v
     for it in range(pret_iters):
+        w_prev, v_prev = w, v
         zr, zi, nit = nms_pass(zr, zi, w - v, nu, pr, order=order, **nms_kw)
         nms_its.append(nit)
         wold = w.copy()
@@ -447,4 +453,4 @@ def estimate_complex_projection(meas, om, D, zr, zi, Lam, w, pr, geo, nu, pret_i
     else:
         it = pret_iters
     return dict(zr=zr, zi=zi, w=w, v=v, om=om, iters=it, costs=costs, primal=prim, dual=dual,
-                nms_iters=nms_its, sd_iters=sd_its)
+                nms_iters=nms_its, sd_iters=sd_its, w_prev=w_prev, v_prev=v_prev)

@@ -178,3 +178,29 @@ def test_interlaced_views_rule():
     assert idx[0] == 0 and idx[64] == 4 * 64 + 2 and idx[128] == 4 * 128 + 1 and idx[192] == 4 * 192 + 3
     assert np.all(np.diff(t) > 0)
     assert sorted(set(idx % 256 // 1)) == list(range(256))
+
+
+@needs_lib
+def test_cost_guard_actions():
+    """The equit loops' cost-rise guard (mbir_guard_check): accepts
+    decreases (asking for a save at each new lowest cost) and rises within
+    the relative tolerance, asks for a recomputed error sinogram before it
+    calls a larger rise real, recovers at most max_recoveries times, then
+    fails as the reference does at any rise (XT_ICD_update.c:1171)."""
+    from phasecon_amd import lib
+    L = lib.load()
+    g = lib.GuardState()
+    L.mbir_guard_init(C.byref(g), 100.0, 1e-6, 2)
+    chk = lambda c, exact=0: L.mbir_guard_check(C.byref(g), c, exact)
+    assert chk(90.0) == lib.GUARD_SAVE and g.best == 90.0
+    assert chk(90.0) == lib.GUARD_OK
+    assert chk(90.0 * (1 + 0.5e-6)) == lib.GUARD_OK            # rounding noise
+    assert chk(80.0, 1) == lib.GUARD_SAVE and g.best == 80.0
+    assert chk(80.1) == lib.GUARD_CONFIRM                       # maintained-e rise
+    assert chk(80.0 * (1 + 1e-7), 1) == lib.GUARD_OK            # gone after a refresh
+    assert chk(85.0) == lib.GUARD_CONFIRM
+    assert chk(85.0, 1) == lib.GUARD_RECOVER and g.recoveries == 1 and g.best == 80.0
+    assert chk(float("nan")) == lib.GUARD_RECOVER               # NaN needs no confirmation
+    assert chk(float("inf"), 1) == lib.GUARD_FAIL
+    assert abs(L.mbir_guard_alpha(1.97) - 1.485) < 1e-12
+    assert abs(L.mbir_guard_alpha(0.8) - 0.4) < 1e-12

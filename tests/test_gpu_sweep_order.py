@@ -1,0 +1,79 @@
+"""The production sweep, voxel by voxel, against the oracle's restatement of
+its schedule.
+
+With a tile as wide as half the grid, each 2x2 colour holds one tile, so a
+launch runs one workgroup per frame of one parity; those frames share no
+views and are not temporal neighbours, so the launch is deterministic and
+the oracle's schedule model (orc_icd_equit_async with no hidden updates:
+the kernel's launch order, tile lists, hashed visiting order
+k -> (pb + k pa) mod n, two z-parity halves per column) is the sequential ICD
+the GPU runs.  One equit then pins, voxel by voxel, the production column
+step: theta from the pipelined loads with the previous column's own update
+subtracted (theta_issue / theta_finish, or col_thetas on the non-pipelined
+path), the out-of-column surrogate terms (col_fixed / col_fixed_tab), the
+solve with zero skipping and over-relaxation (col_solve), and the error
+update (col_update) -- against compute_voxel_update_Atten
+(MBIR_4D/XT_VoxUpdate.c:45-117) and CE_FunctionalSubstitution
+(MBIR_4D/XT_ICD_update.c:133-182) as the oracle restates them.
+
+Tolerance: FP32 kernels against the FP64 restatement after one equit of
+sequential updates: each voxel within 2e-5 of the largest |x| and the
+maintained error sinogram within 2e-5 of the largest |y|.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 2e-5
+
+
+def _start(o, scan, seed):
+    """A non-trivial start: two oracle equits from zero, rounded to FP32,
+    with a zero block whose neighbours are all <= 0 (zero skipping)."""
+    x = np.zeros(o.shape_x, np.float64)
+    e = o.init_error(x)
+    for it in (1, 2):
+        o.icd_equit_async(x, e, it, 32, 1.5, seed + it, 0, 0, 0)
+    x = x.astype(np.float32).astype(np.float64)
+    T, Z, Y, X = x.shape
+    cy, cx = Y // 2 - 4, X // 2 - 4
+    x[:, :, cy - 1:cy + 5, cx - 1:cx + 5] = -1e-3
+    x[:, :, cy:cy + 4, cx:cx + 4] = 0.0
+    return x
+
+
+@pytest.mark.parametrize("name,placement,alpha,p_t", [
+    ("ch4d_small", 1, 1.97, None), ("ch4d_small", 0, 1.5, None),
+    ("ch4d_small", 1, 1.97, 1.1), ("tall3d", 1, 1.97, None), ("tiny3d", 1, 1.5, None)])
+def test_sweep_matches_oracle_schedule(name, placement, alpha, p_t):
+    from oracle.oracle import Oracle
+    from phasecon_amd.mbir import Stage
+    from phasecon_amd.synth import make_scan
+    scan = make_scan(name, seed=3 if name == "ch4d_small" else 0)
+    o = Oracle(scan, p_t=p_t)
+    T = max(o.g.N_x, o.g.N_y) // 2
+    print(f"{name}: tile {T}, placement {placement}, alpha {alpha}, p_t {p_t}", flush=True)
+    x0 = _start(o, scan, 17)
+    st = Stage(scan, tile=T, alpha=alpha, p_t=p_t)
+    st.option("placement", placement)
+    st.set_object(x0.astype(np.float32))
+    st.build_amatrix()
+    st.init_error()
+    seed, it = 4242, 3
+    st.icd(1, it, seed)
+    xg = st.get_object().astype(np.float64)
+    eg = st.get_error().astype(np.float64)
+    st.close()
+    xo = x0.copy()
+    eo = o.init_error(xo)
+    stats = o.icd_equit_async(xo, eo, it, T, alpha, seed, 0, 0, 0, 8, placement)
+    assert stats[2] > 0, "the zero block should be skipped"
+    dx = np.abs(xg - xo).max() / np.abs(xo).max()
+    de = np.abs(eg - eo).max() / np.abs(o.y).max()
+    moved = np.abs(xo - x0).max() / np.abs(xo).max()
+    print(f"{name}: max |x_gpu - x_oracle| / max|x| = {dx:.3g} (the equit moved x by "
+          f"{moved:.3g}), e {de:.3g}, skipped {int(stats[2])}", flush=True)
+    assert moved > 100 * TOL   # the equit changed the object well beyond the tolerance
+    assert dx < TOL
+    assert de < TOL

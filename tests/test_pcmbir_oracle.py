@@ -332,3 +332,85 @@ def test_paganin_filter_is_the_published_form(k0, l0):
     thick = -np.log(1 + eps * f * np.cos(ph))
     assert np.allclose(zr, thick / 2, rtol=0, atol=1e-13)
     assert np.allclose(zi, thick * dob / 2, rtol=0, atol=1e-11)
+
+
+def _admm_problem(n=8, seed=0, uniform=False):
+    """A noise-free view: y = |H exp(-z*)| (D = 1, Lambda = 1)."""
+    r = np.random.default_rng(seed)
+    if uniform:
+        zr_t, zi_t = np.full((n, n), 0.025), np.full((n, n), 0.35)
+    else:
+        zr_t, zi_t = 0.02 + 0.01 * r.random((n, n)), 0.3 + 0.1 * r.random((n, n))
+    geo = (1.0, 1.0, P.wavelength(17.0), 1e5, None)
+    y = np.abs(P.fresnel(P.transmittance(zr_t, zi_t), *geo[:4])).astype(complex)
+    return zr_t, zi_t, y, np.ones((n, n), complex), np.ones((n, n)), geo
+
+
+def test_admm_solution_is_a_fixed_point():
+    """estimate_complex_projection started at the exact solution of a
+    noise-free uniform view (where the prior is flat) stays there: z, w and
+    the dual v unchanged, primal residual 0, and the pretcost 0."""
+    zr_t, zi_t, y, D, Lam, geo = _admm_problem(uniform=True)
+    pr = P.NmsPrior(**PRIOR, decorrelate=True)
+    w0 = P.transmittance(zr_t, zi_t)
+    om0 = P.PhaseProblem(y, D, Lam, None, 0.5, *geo).phase_projection(w0)
+    with np.errstate(invalid="ignore"):   # dual residual 0 / |v| = 0 / 0, as in the reference
+        res = P.estimate_complex_projection(y, om0, D, zr_t.copy(), zi_t.copy(), Lam, w0, pr, geo,
+                                            0.5, 3, 0.0, 5, 0.0)
+    assert np.array_equal(res["zr"], zr_t) and np.array_equal(res["zi"], zi_t)
+    assert np.max(np.abs(res["w"] - w0)) < 1e-12
+    assert np.max(np.abs(res["v"])) < 1e-12
+    assert max(res["primal"]) < 1e-12
+    assert res["costs"][-1] < 1e-20
+
+
+def test_admm_dual_is_the_scaled_gradient_of_the_data_term():
+    """Scaled ADMM for min_w f(w) + g(z) s.t. w = exp(-z), with the z-step on
+    b = w - v, the w-step to the target exp(-z) + v and v += exp(-z) - w:
+    when the w-step is solved exactly, its optimality condition
+    grad f(w) + nu (w - exp(-z) - v_old) = 0 and the dual update give
+    grad f(w) = nu v after every iteration, f(w) = |y - Omega H D w|^2_Lam / 2
+    at the phase projection Omega of w.  A flipped dual sign gives
+    grad f = -nu v (relative error 2), and the steps in another order break
+    the identity.  The residuals fall on this noise-free view."""
+    zr_t, zi_t, y, D, Lam, geo = _admm_problem(seed=0)
+    r = np.random.default_rng(1)
+    zr0 = zr_t + 0.01 * r.standard_normal(zr_t.shape)
+    zi0 = zi_t + 0.05 * r.standard_normal(zi_t.shape)
+    pr = P.NmsPrior(phase_sig=1.0, phase_c=0.01, mag_sig=1.0, mag_c=0.01, delta_over_beta=50.0,
+                    decorrelate=False)
+    nu = 0.5
+    w0 = P.transmittance(zr0, zi0)
+    pp = P.PhaseProblem(y, D, Lam, None, nu, *geo)
+    res = P.estimate_complex_projection(y, pp.phase_projection(w0), D, zr0, zi0, Lam, w0, pr, geo,
+                                        nu, 20, 0.0, 50, 0.0,
+                                        nms_kw=dict(thresh=1e-12, max_iter=400))
+    w, v = res["w"], res["v"]
+    om = pp.phase_projection(w)
+    gf = pp.herm_omega_hd(om, (pp.omega_hd(om, w) - y) * Lam)
+    assert np.linalg.norm(gf - nu * v) < 1e-9 * np.linalg.norm(nu * v)
+    prim = np.array(res["primal"])
+    assert prim[-1] < 0.01 * prim[0]
+    assert res["dual"][-1] < 0.01 * res["dual"][0]
+    assert res["costs"][-1] < 0.1 * res["costs"][0]
+
+
+def test_admm_z_step_targets_w_minus_v():
+    """Without a prior (sigmas 1e6) the z-step's NMS cost is
+    nu/2 |exp(-z) - b|^2, minimised where exp(-z) = b; after the loop the last
+    z must therefore satisfy exp(-z) = w_prev - v_prev, the state that
+    iteration started from (XT_CmplxProjEst.c:121-122: b = w - v)."""
+    zr_t, zi_t, y, D, Lam, geo = _admm_problem(seed=0)
+    r = np.random.default_rng(1)
+    zr0 = zr_t + 0.01 * r.standard_normal(zr_t.shape)
+    zi0 = zi_t + 0.05 * r.standard_normal(zi_t.shape)
+    pr = P.NmsPrior(phase_sig=1e6, phase_c=0.01, mag_sig=1e6, mag_c=0.01, delta_over_beta=50.0,
+                    decorrelate=False)
+    w0 = P.transmittance(zr0, zi0)
+    pp = P.PhaseProblem(y, D, Lam, None, 0.5, *geo)
+    res = P.estimate_complex_projection(y, pp.phase_projection(w0), D, zr0, zi0, Lam, w0, pr, geo,
+                                        0.5, 3, 0.0, 50, 0.0,
+                                        nms_kw=dict(thresh=1e-14, max_iter=1000))
+    b = res["w_prev"] - res["v_prev"]
+    assert np.abs(res["v_prev"]).max() > 1e-3          # the dual is not trivially zero
+    assert np.abs(P.transmittance(res["zr"], res["zi"]) - b).max() < 1e-8
