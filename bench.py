@@ -127,6 +127,10 @@ def _equit(st, it, seed, nh=False):
 
 
 PREDICT_CHECKS = True  # --check-every-equit turns the scheduled cost checks off
+GUARD_RECOVER = 3      # mbir_guard_check's recovery code (phasecon_amd.lib.GUARD_RECOVER)
+
+
+GUARD_EVENTS = []  # recoveries of the cost-rise guard in the current step
 
 
 def to_target(st, target, max_equits, seed, refresh):
@@ -136,12 +140,16 @@ def to_target(st, target, max_equits, seed, refresh):
     from the decrease of C - C* since the last check, taken as geometric,
     the equits still needed are predicted and the next check comes after
     half of them (at least one equit later); with PREDICT_CHECKS off, after
-    every equit."""
+    every equit.  Every check goes through the cost-rise guard (Guard): a
+    confirmed rise restores the object of the lowest checked cost and halves
+    alpha - 1 (recorded in GUARD_EVENTS; the time stays in the step)."""
+    from phasecon_amd.mbir import Guard
     st.build_amatrix()
     st.init_error()
     cstar = target / (1.0 + REL_TOL)
     it = 0
     cost = st.cost(estimate=True)[0]
+    guard = Guard(st, cost, max_recoveries=GUARD_RECOVERIES)
     prev = (0, cost - cstar)
     check_at = 1
     while it < max_equits:
@@ -152,6 +160,14 @@ def to_target(st, target, max_equits, seed, refresh):
         if it < check_at:
             continue
         cost = st.cost(estimate=True)[0]
+        act, cost = guard.check(cost, where=f"{st.g.N_x}x{st.g.N_y}x{st.g.N_z} equit {it}")
+        if guard.failed:
+            break
+        if act == GUARD_RECOVER:
+            GUARD_EVENTS.extend(guard.events[-1:])
+            prev = (it, cost - cstar)
+            check_at = it + 1
+            continue
         if cost <= target:
             fresh = st.cost(fresh=True)[0]   # FP64 check from a recomputed e
             if fresh <= target:
@@ -174,6 +190,10 @@ def run_learned(st, net, target, max_equits, seed, refresh, source="fbp2", coars
     coarse_equits > 0), then full resolution to the target; returns
     (seconds, equits, cost, seconds to the start)."""
     from phasecon_amd.warmstart import learned_start
+    GUARD_EVENTS.clear()
+    for s_ in (st, coarse):
+        if s_ is not None and s_.alpha != s_.alpha0:
+            s_.option("alpha", s_.alpha0)
     st.synchronize()
     t0 = time.perf_counter()
     st.build_amatrix()
@@ -184,12 +204,47 @@ def run_learned(st, net, target, max_equits, seed, refresh, source="fbp2", coars
     return time.perf_counter() - t0, it, cost, t_start
 
 
+GUARD_RECOVERIES = 3
+COARSE_CHECK = 20  # equits between the cost-rise guard's checks in a coarse stage
+
+
+def coarse_stage(cs, n, seed, refresh):
+    """n equits of a coarse stage from its current object, the cost-rise
+    guard checking every COARSE_CHECK equits and at the end (a recovery
+    restores the object of the lowest checked cost with alpha - 1 halved
+    and runs the equits since that check again)."""
+    from phasecon_amd.mbir import Guard
+    cs.build_amatrix()
+    cs.init_error()
+    guard = Guard(cs, cs.cost(estimate=True)[0], max_recoveries=GUARD_RECOVERIES)
+    it, good = 0, 0
+    while it < n:
+        it += 1
+        _equit(cs, it, seed)
+        if refresh and it % refresh == 0:
+            cs.init_error()
+        if it % COARSE_CHECK == 0 or it == n:
+            act, _ = guard.check(cs.cost(estimate=True)[0],
+                                 where=f"{cs.g.N_x}x{cs.g.N_y}x{cs.g.N_z} equit {it}")
+            if guard.failed:
+                raise RuntimeError(f"coarse stage diverged: {guard.events}")
+            if act == GUARD_RECOVER:
+                GUARD_EVENTS.extend(guard.events[-1:])
+                it = good
+            else:
+                good = it
+
+
 def run_once(st, target, max_equits, seed, refresh, coarse=()):
     """One timed reconstruction from zero; returns (seconds, equits, cost).
     coarse: (stage, equits) pairs of the multi-resolution stages that run
     before st, coarsest first.  Every `refresh` equits the error sinogram is
     recomputed exactly (FP64 accumulation) so FP32 drift cannot bias the
-    fixed point or the test."""
+    fixed point or the test.  Each stage starts at its configured alpha."""
+    GUARD_EVENTS.clear()
+    for s_ in [cs for cs, _ in coarse] + [st]:
+        if s_.alpha != s_.alpha0:
+            s_.option("alpha", s_.alpha0)
     first = coarse[0][0] if coarse else st
     first.set_object(np.zeros(first.shape_x, np.float32))
     first.synchronize()
@@ -201,12 +256,7 @@ def run_once(st, target, max_equits, seed, refresh, coarse=()):
         prev = cs
         if n == 0:
             continue
-        cs.build_amatrix()
-        cs.init_error()
-        for it in range(1, n + 1):
-            _equit(cs, it, seed)
-            if refresh and it % refresh == 0:
-                cs.init_error()
+        coarse_stage(cs, n, seed, refresh)
     if prev is not None:
         st.upsample_from(prev, masked=True)
     it, cost = to_target(st, target, max_equits, seed, refresh)
@@ -558,6 +608,12 @@ def main():
     ap.add_argument("--learned-input", choices=("fbp3", "fbp2", "fbp", "bp"), default="fbp2")
     ap.add_argument("--afp16", action="store_true",
                     help="store A in FP16 with a per-column FP32 scale")
+    ap.add_argument("--placement", type=int, choices=(0, 1), default=0,
+                    help="sweep work items on the XCDs: 0 tile-major, 1 each frame on one XCD")
+    ap.add_argument("--local-atomics", action="store_true",
+                    help="error-sinogram updates in the XCD's L2 (placement 1, one XCD per frame)")
+    ap.add_argument("--xcd-check", action="store_true",
+                    help="count sweep work items that ran off their intended XCD")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-equits", type=int, default=2)
     ap.add_argument("--longrun", type=int, default=0)
@@ -630,6 +686,12 @@ def main():
                 coarse.append((Stage(scan, mult_xy=mxy, mult_z=mz, tile=args.coarse_tile,
                                      afp16=args.afp16, alpha=args.coarse_alpha), n))
 
+    for s_ in [cs for cs, _ in coarse] + [st]:
+        s_.option("placement", args.placement)
+        s_.option("local_atomics", int(args.local_atomics))
+        if args.xcd_check:
+            s_.option("xcd_check", 1)
+
     tuned = {}
 
     def tile_of(s_):
@@ -698,13 +760,15 @@ def main():
         run_once(st, target, args.max_equits, args.seed, args.refresh, coarse)
     st.synchronize()
     st.reset_kernel_time()
-    times, equits, costs = [], [], []
+    times, equits, costs, recoveries = [], [], [], []
     for k in range(args.steps):
         dt, n, c = run_once(st, target, args.max_equits, args.seed + k, args.refresh, coarse)
         times.append(dt)
         equits.append(n)
         costs.append(c)
-        print(f"[bench] step {k}: {dt:.3f}s, {n} equits, cost {c:.10g} (target {target:.10g})",
+        recoveries.extend({"step": k, **ev} for ev in GUARD_EVENTS)
+        print(f"[bench] step {k}: {dt:.3f}s, {n} equits, cost {c:.10g} (target {target:.10g})"
+              + (f", guard recoveries {GUARD_EVENTS}" if GUARD_EVENTS else ""),
               file=sys.stderr, flush=True)
     st.synchronize()
     sweep_ms, sweep_n = st.kernel_time(K_SWEEP)
@@ -758,6 +822,11 @@ def main():
         },
         "reached_target": reached,
         "equits_to_target": equits,
+        "placement": {"frames_on_xcds": args.placement == 1, "local_atomics": args.local_atomics,
+                      "xcd_check": ({f"{s_.g.N_x}x{s_.g.N_y}x{s_.g.N_z}": s_.xcd_check()
+                                     for s_ in [cs for cs, _ in coarse] + [st]}
+                                    if args.xcd_check else None)},
+        "guard_recoveries": recoveries,
         "coarse_equits": [n for _, n in coarse],
         "equits_per_s": total_equits / (sweep_ms * 1e-3),
         "hbm_gbps": achieved,
@@ -778,7 +847,8 @@ def main():
         try:
             out["learned"] = learned_leg(args, st, train_scans, target)
             lv = out["learned"]["value"]
-            if lv and value:
+            # ratios only between legs whose every step reached the target
+            if lv and reached:
                 out["learned"]["vs_multires"] = value / lv
             z = out["learned"]["zero_start"]["value"]
             if lv and z:
@@ -788,7 +858,7 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scan, args.cpu_equits)
         cb = out["cpu_baseline"]
-        if cb.get("value"):
+        if cb.get("value") and reached:
             cb["s_to_target_est"] = cb["value"] * float(np.mean(equits))
             cb["note"] = ("s_to_target_est = measured s/equit x the GPU run's full-resolution "
                           "equits to target (coarse stages not counted; the reference's own "

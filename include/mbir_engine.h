@@ -88,6 +88,39 @@ typedef struct {
 
 void mbir_default_options(mbir_options* opt);
 
+/* Cost-rise guard of an equit loop.  The reference stops with -1 at any
+ * rise of its FP64 cost (check_error, MBIR_4D/XT_ICD_update.c:1171); the
+ * parallel sweep's cost is usually evaluated from the error sinogram its
+ * FP32 atomics maintain, which carries rounding noise, and its asynchronous
+ * updates can overshoot when over-relaxed close to 2.  The guard accepts a
+ * cost within a relative tol of the lowest accepted one, asks for a cost
+ * from a recomputed error sinogram before it calls a larger one a rise, and
+ * on a confirmed rise asks the caller to restore the object saved at the
+ * lowest cost and halve alpha - 1, at most max_recoveries times; after that
+ * it fails (the reference's -1). */
+typedef struct {
+  double best;             /* lowest accepted cost (that of the saved object) */
+  double tol;              /* relative rise taken as rounding noise           */
+  int32_t recoveries;      /* recoveries made                                 */
+  int32_t max_recoveries;
+} mbir_guard;
+
+enum {
+  MBIR_GUARD_OK = 0,       /* accepted                                        */
+  MBIR_GUARD_SAVE = 1,     /* accepted, lowest so far: save the object        */
+  MBIR_GUARD_CONFIRM = 2,  /* call again with the cost of a recomputed e      */
+  MBIR_GUARD_RECOVER = 3,  /* restore the saved object, recompute e, lower alpha */
+  MBIR_GUARD_FAIL = 4      /* rise after max_recoveries: stop with an error   */
+};
+
+void mbir_guard_init(mbir_guard* gd, double cost0, double tol, int32_t max_recoveries);
+/* exact = 0: cost from the maintained error sinogram; 1: from a recomputed
+ * one.  Returns one of the MBIR_GUARD_ codes. */
+int32_t mbir_guard_check(mbir_guard* gd, double cost, int32_t exact);
+/* The over-relaxation after a recovery: 1 + (alpha - 1) / 2 (alpha > 1), else
+ * alpha / 2. */
+double mbir_guard_alpha(double alpha);
+
 /* reconstruct() with options.  equits_out / cost_out (may be NULL) receive
  * the equits run in the last stage and the final FP64 cost. */
 int reconstruct_ex(float** object, float* projections, float* weights,

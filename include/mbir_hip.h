@@ -79,8 +79,22 @@ void mbir_hip_destroy(mbir_hip_ctx* ctx);
  * (XT_AMatrix.c:50-110); this knob has no counterpart there.  "nhicd" (0 or
  * 1, default 0): every sweep records each visited column's sum of |dx| in a
  * magnitude map (MagUpdateMap, XT_VoxUpdate.c:158,265), which
- * mbir_hip_nhicd ranks. */
+ * mbir_hip_nhicd ranks.  "placement" (default 0): how a sweep launch's
+ * (tile, frame) work items map to workgroups -- 0 tile-major, 1 each frame
+ * bound to one XCD (the hardware deals workgroups to the 8 XCDs
+ * round-robin; with 8 frames per launch every frame's error-sinogram
+ * traffic then stays in one L2; with other frame counts a frame spans
+ * 8 / gcd(frames, 8) XCDs).  "local_atomics" (0 or 1, default 0; placement
+ * 1 with one XCD per frame only): the sweep's error-sinogram updates are
+ * performed in the XCD's L2 instead of at the memory side.  "xcd_check"
+ * (0 or 1): the sweep counts its work items and those that ran on another
+ * XCD than the placement intended (mbir_hip_xcd_check).  None of these has
+ * a counterpart in the reference. */
 int mbir_hip_set_option(mbir_hip_ctx* ctx, const char* key, double value);
+
+/* Counters of option "xcd_check" since it was set: sweep work items run and
+ * those that ran on another XCD than their placement intended. */
+int mbir_hip_xcd_check(mbir_hip_ctx* ctx, int64_t* items, int64_t* off_xcd);
 
 /* Object transfer (replaces the Object array of XT_Structures.h:215). */
 int mbir_hip_set_object(mbir_hip_ctx* ctx, const float* x);
@@ -96,6 +110,13 @@ int mbir_hip_get_object(mbir_hip_ctx* ctx, float* x);
  * the fine stage's ICD solves the same problem as a start from zero. */
 int mbir_hip_upsample_object(mbir_hip_ctx* fine, mbir_hip_ctx* coarse,
                              int32_t masked);
+
+/* Saves (restore = 0) or restores (restore = 1) the object on the device,
+ * for the cost-rise guard of the equit loops (mbir_guard_check in
+ * mbir_engine.h); no reference counterpart (the reference stops at a cost
+ * rise, XT_ICD_update.c:1171).  The error sinogram is not saved: recompute
+ * it with mbir_hip_init_error after a restore. */
+int mbir_hip_object_snapshot(mbir_hip_ctx* ctx, int32_t restore);
 
 /* Builds the stored system matrix for every (column, view) pair;
  * replaces the per-visit calcAMatrixColumnforAngle (XT_AMatrix.c:50-110).

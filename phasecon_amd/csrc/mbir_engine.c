@@ -272,6 +272,11 @@ void mbir_sino_to_internal(int32_t N_p, int32_t N_t, int32_t N_r,
         out[((size_t)p * N_r + r) * N_t + t] = in[((size_t)p * N_t + t) * N_r + r];
 }
 
+/* cost-rise guard of the drop-in's equit loop: relative FP32 noise
+ * tolerance and recoveries before the reference's error */
+#define GUARD_TOL 1e-6
+#define GUARD_RECOVERIES 2
+
 void mbir_default_options(mbir_options* o) {
   o->p_s = 1.2; /* MRF_P, XT_Constants.h:62 */
   o->p_t = 1.2;
@@ -284,6 +289,34 @@ void mbir_default_options(mbir_options* o) {
   o->seed = 761521; /* XT_Engine.c:99 */
   o->refresh_every = 10;
 }
+
+/* ------------------------------------------------------- cost guard */
+void mbir_guard_init(mbir_guard* gd, double cost0, double tol, int32_t max_recoveries) {
+  gd->best = cost0;
+  gd->tol = tol;
+  gd->recoveries = 0;
+  gd->max_recoveries = max_recoveries;
+}
+
+int32_t mbir_guard_check(mbir_guard* gd, double cost, int32_t exact) {
+  /* NaN and infinity are rises whatever the tolerance */
+  const int finite = cost == cost && cost < HUGE_VAL && cost > -HUGE_VAL;
+  if (finite && cost <= gd->best + gd->tol * fabs(gd->best)) {
+    if (cost < gd->best) {
+      gd->best = cost;
+      return MBIR_GUARD_SAVE;
+    }
+    return MBIR_GUARD_OK;
+  }
+  if (!exact && finite) return MBIR_GUARD_CONFIRM;
+  if (gd->recoveries < gd->max_recoveries) {
+    gd->recoveries++;
+    return MBIR_GUARD_RECOVER;
+  }
+  return MBIR_GUARD_FAIL;
+}
+
+double mbir_guard_alpha(double alpha) { return alpha > 1.0 ? 1.0 + 0.5 * (alpha - 1.0) : 0.5 * alpha; }
 
 /* ------------------------------------------------------------ files */
 static int write_bin(const char* name, const void* data, size_t elem,
@@ -621,8 +654,12 @@ int reconstruct_ex(float** object, float* projections, float* weights,
         goto stage_fail;
       }
       double cost[3], last;
+      double alpha_now = opt.alpha > 0 ? opt.alpha : g->alpha;
+      mbir_guard guard;
       if (mbir_hip_cost(ctx, 0, cost)) goto gpu_fail;
       last = cost[0];
+      mbir_guard_init(&guard, cost[0], GUARD_TOL, GUARD_RECOVERIES);
+      if (mbir_hip_object_snapshot(ctx, 0)) goto gpu_fail;
       say(log, "INFO", "------------- Iteration 0, Cost = %f------------\n", cost[0]);
       if (opt.write_files && write_bin("cost", &cost[0], sizeof(double), 1, "wb", log)) goto stage_fail;
       int32_t it;
@@ -647,9 +684,28 @@ int reconstruct_ex(float** object, float* projections, float* weights,
         say(log, "INFO", "------------- Iteration = %d, Cost = %f, update = %f%% ------------\n",
             it, cost[0], pct);
         if (opt.write_files && write_bin("cost", &cost[0], sizeof(double), 1, "ab", log)) goto stage_fail;
-        if (cost[0] > last)
-          say(log, "WARN", "Cost value increased by %g (relative %g).\n", cost[0] - last,
-              (cost[0] - last) / fabs(last));
+        /* a rise beyond rounding noise: confirmed from a recomputed e, then
+         * undone (object restored, alpha lowered) or, once the recoveries
+         * are spent, the reference's error (XT_ICD_update.c:1171) */
+        int32_t act = mbir_guard_check(&guard, cost[0], 0);
+        if (act == MBIR_GUARD_CONFIRM) {
+          if (mbir_hip_init_error(ctx) || mbir_hip_cost(ctx, 0, cost)) goto gpu_fail;
+          act = mbir_guard_check(&guard, cost[0], 1);
+        }
+        if (act == MBIR_GUARD_SAVE) {
+          if (mbir_hip_object_snapshot(ctx, 0)) goto gpu_fail;
+        } else if (act == MBIR_GUARD_RECOVER) {
+          const double a2 = mbir_guard_alpha(alpha_now);
+          say(log, "WARN", "Cost value increased to %f (lowest %f); restoring that object and "
+              "lowering the over-relaxation from %g to %g.\n", cost[0], guard.best, alpha_now, a2);
+          alpha_now = a2;
+          if (mbir_hip_object_snapshot(ctx, 1) || mbir_hip_init_error(ctx) ||
+              mbir_hip_set_option(ctx, "alpha", alpha_now) || mbir_hip_cost(ctx, 0, cost))
+            goto gpu_fail;
+        } else if (act == MBIR_GUARD_FAIL) {
+          say(log, "ERROR", "Cost value increased.\n");
+          goto stage_fail;
+        }
         last = cost[0];
         if (opt.stop_cost > 0 && cost[0] <= opt.stop_cost) break;
         if (pct < convg_thresh && it > 1) {

@@ -430,8 +430,11 @@ __device__ __forceinline__ float e_load(const float* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-__device__ __forceinline__ void e_add(float* p, float v) {
-  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void e_add(float* p, float v, bool local) {
+  if (local)
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  else
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Wave-scope ordering: one wavefront per workgroup, and a wavefront's LDS
@@ -492,6 +495,9 @@ struct ColumnArgs {
   const int* frame_views;
   const float* w;
   float* e;
+  // error-sinogram updates at workgroup scope, performed in the XCD's L2:
+  // valid only while all items of a frame run on one XCD
+  int local_atomics;
 };
 
 constexpr int kMaxViews = 512;  // views per frame held in LDS
@@ -806,6 +812,7 @@ template <int kW>
 __device__ __forceinline__ void col_update(const SwGeom& G, const ColumnArgs& A, const Win& win,
                                            int nv, const float* dxz) {
   const Lanes L(G.N_t);
+  const bool local = __builtin_amdgcn_readfirstlane(A.local_atomics) != 0;
   for (int t0 = 0; t0 < G.N_t; t0 += L.rpl) {
     const int row = t0 + L.rr;
     if (L.rr >= L.rpl || row >= G.N_t) continue;
@@ -817,7 +824,7 @@ __device__ __forceinline__ void col_update(const SwGeom& G, const ColumnArgs& A,
 #pragma unroll
       for (int k = 0; k < kW; ++k) {
         const float a = win.av[sl * kW + k];
-        if (a != 0.f) e_add(A.e + base + k * G.N_t + row, -(a * G.delta_t) * dz);
+        if (a != 0.f) e_add(A.e + base + k * G.N_t + row, -(a * G.delta_t) * dz, local);
       }
     }
   }
@@ -1076,9 +1083,19 @@ struct SweepArgs {
   float* mag;
   const float* nh_rank;
   float nh_frac;
+  // placement of the launch's (tile, frame) items on the XCDs (workgroup w
+  // runs on XCD w mod 8 when the dispatcher deals workgroups round-robin):
+  // place_r = 0: tile-major grid (w = frame x n_launch_tiles + tile);
+  // place_r > 0: each of the nfl frames split into place_r parts (tiles
+  // t = part mod place_r), unit u = part x nfl + frame on XCD u mod 8, the
+  // unit's items in unit_slots consecutive slots of that XCD
+  int place_r, nfl, unit_slots;
+  unsigned* xcd_check;  // [0] items run, [1] items off their XCD (may be null)
 };
 
 constexpr int kSweepThreads = 2 * kWave;
+constexpr int kXcds = 8;          // XCDs of an MI355X, each with its own L2
+constexpr int kHwRegXccId = 20;   // s_getreg id of XCC_ID (gfx940 family)
 constexpr int kNhMax = 2 * kSweepThreads;  // tile columns a non-homogeneous pass can rank
 
 // One ICD pass over the voxel columns of the tiles in this launch (one
@@ -1114,9 +1131,25 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SwGeom G, SweepArgs A) 
   const int wid = blockIdx.x;
   const int iter = A.iter_dev ? *A.iter_dev + A.iter : A.iter;
   load_prior(G, &s_prior);
-  const int tile = A.launch_tiles[wid % A.n_launch_tiles];
-  const int f = A.fparity + 2 * (wid / A.n_launch_tiles);
+  int tl, fl;
+  if (A.place_r == 0) {
+    tl = wid % A.n_launch_tiles;
+    fl = wid / A.n_launch_tiles;
+  } else {
+    const int s = wid / kXcds, u = wid % kXcds + kXcds * (s / A.unit_slots);
+    fl = u % A.nfl;
+    tl = u / A.nfl + A.place_r * (s % A.unit_slots);
+    if (tl >= A.n_launch_tiles) return;
+  }
+  const int tile = A.launch_tiles[tl];
+  const int f = A.fparity + 2 * fl;
   if (f >= G.N_time) return;
+  if (A.xcd_check && tid == 0) {
+    // the XCD this workgroup runs on (hardware register XCC_ID, bits 3:0)
+    const unsigned xcc = (unsigned)__builtin_amdgcn_s_getreg((3 << 11) | kHwRegXccId) & 15u;
+    atomicAdd(&A.xcd_check[0], 1u);
+    if (xcc != (unsigned)(wid % kXcds)) atomicAdd(&A.xcd_check[1], 1u);
+  }
   const int c0 = A.tile_ptr[tile];
   const int n_all = A.tile_ptr[tile + 1] - c0;
   int n = n_all;
@@ -1985,7 +2018,7 @@ struct mbir_hip_ctx {
   int* d_iter = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
-  long long gkey[6] = {-1, -1, -1, -1, -1, -1};
+  long long gkey[7] = {-1, -1, -1, -1, -1, -1, -1};
   int glaunches = 0;
   // non-homogeneous ICD (option "nhicd"): per (frame, column) sum of |dx| of
   // the column's last visit and its VSC-filtered ranking, [N_time][N_y N_x]
@@ -2002,6 +2035,13 @@ struct mbir_hip_ctx {
   int tile = 8, tiles_for = -1;
   double alpha = -1.0;
   size_t lds_max = 64 * 1024;  // LDS bytes one workgroup may allocate
+  // sweep placement (option "placement"): 0 tile-major grid, 1 frames bound
+  // to XCDs; option "local_atomics" (placement 1, one XCD per frame):
+  // error-sinogram updates performed in the XCD's L2
+  int placement = 0, local_atomics = 0;
+  unsigned* d_xcd = nullptr;  // [items run, items off their XCD] (option "xcd_check")
+  float* d_snap = nullptr;    // saved object (mbir_hip_object_snapshot)
+  bool snap_valid = false;
   double* d_stats = nullptr;
   double* d_partial = nullptr;
   size_t partial_cap = 0;
@@ -2254,7 +2294,8 @@ void mbir_hip_destroy(mbir_hip_ctx* c) {
                   c->d_view_frame, c->d_y, c->d_w, c->d_d, c->d_e, c->d_x,
                   c->d_stage, c->d_stage2, c->d_afirst, c->d_aval, c->d_tile_ptr,
                   c->d_tile_cols, c->d_launch_tiles, c->d_stats, c->d_partial, c->d_mask,
-                  c->d_aval16, c->d_ascale, c->d_ws, c->d_rt, c->d_mag, c->d_vsc};
+                  c->d_aval16, c->d_ascale, c->d_ws, c->d_rt, c->d_mag, c->d_vsc,
+                  c->d_xcd, c->d_snap};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
@@ -2293,6 +2334,24 @@ int mbir_hip_set_option(mbir_hip_ctx* c, const char* key, double value) {
       HIP_OK(hipMemset(c->d_mag, 0, nm * sizeof(float)));
     }
     c->nh = value == 1;
+  } else if (k == "placement") {
+    if (value != 0 && value != 1) return fail("placement must be 0 or 1");
+    c->placement = (int)value;
+  } else if (k == "local_atomics") {
+    if (value != 0 && value != 1) return fail("local_atomics must be 0 or 1");
+    c->local_atomics = (int)value;
+  } else if (k == "xcd_check") {
+    // count the sweep's work items and those that ran off their XCD
+    if (value != 0 && value != 1) return fail("xcd_check must be 0 or 1");
+    if (value == 1 && !c->d_xcd) {
+      if (dev_alloc(&c->d_xcd, 2)) return -1;
+      HIP_OK(hipMemset(c->d_xcd, 0, 2 * sizeof(unsigned)));
+    }
+    if (value == 0 && c->d_xcd) {
+      HIP_OK(hipStreamSynchronize(c->stream));
+      HIP_OK(hipFree(c->d_xcd));
+      c->d_xcd = nullptr;
+    }
   } else {
     return fail("unknown option " + k);
   }
@@ -2337,6 +2396,22 @@ int mbir_hip_get_object(mbir_hip_ctx* c, float* x) {
   HIP_OK(hipMemcpyAsync(x, c->d_stage, c->nvox * sizeof(float), hipMemcpyDeviceToHost,
                         c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mbir_hip_object_snapshot(mbir_hip_ctx* c, int32_t restore) {
+  if (!c) return fail("null context");
+  if (restore != 0 && restore != 1) return fail("restore must be 0 or 1");
+  if (restore) {
+    if (!c->snap_valid) return fail("no saved object to restore");
+    HIP_OK(hipMemcpyAsync(c->d_x, c->d_snap, c->nvox * sizeof(float), hipMemcpyDeviceToDevice,
+                          c->stream));
+    return 0;
+  }
+  if (!c->d_snap && dev_alloc(&c->d_snap, c->nvox)) return -1;
+  HIP_OK(hipMemcpyAsync(c->d_snap, c->d_x, c->nvox * sizeof(float), hipMemcpyDeviceToDevice,
+                        c->stream));
+  c->snap_valid = true;
   return 0;
 }
 
@@ -2512,6 +2587,8 @@ int sweep_launches(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t seed
   A.col.frame_views = c->d_frame_views;
   A.col.w = c->d_w;
   A.col.e = c->d_e;
+  A.col.local_atomics = 0;
+  A.xcd_check = c->d_xcd;
   A.frame_ptr = c->d_frame_ptr;
   A.tile_ptr = c->d_tile_ptr;
   A.tile_cols = c->d_tile_cols;
@@ -2547,8 +2624,27 @@ int sweep_launches(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t seed
         if (frames <= 0) continue;
         A.fparity = fpar;
         A.seed = seed * 2654435761u + (unsigned)(it * 8 + colour * 2 + fpar) * 40503u;
+        unsigned nwg = (unsigned)(ntl * frames);
+        A.place_r = 0;
+        A.nfl = frames;
+        A.unit_slots = 0;
+        A.col.local_atomics = 0;
+        if (c->placement == 1) {
+          // frames bound to XCDs: r parts per frame, lcm(frames, 8) units
+          int a = frames, b = kXcds;
+          while (b) {
+            const int t = a % b;
+            a = b;
+            b = t;
+          }
+          A.place_r = kXcds / a;
+          A.unit_slots = (ntl + A.place_r - 1) / A.place_r;
+          nwg = (unsigned)(frames * A.place_r * A.unit_slots);
+          // one XCD per frame: its updates can stay in that XCD's L2
+          A.col.local_atomics = (c->local_atomics && A.place_r == 1) ? 1 : 0;
+        }
         if (timed && ev_begin(c, MBIR_K_SWEEP)) return -1;
-        const dim3 grid((unsigned)(ntl * frames)), block(kSweepThreads);
+        const dim3 grid(nwg), block(kSweepThreads);
         if (small && c->afp16) {
           MBIR_DISPATCH_W(c->G.W, hipLaunchKernelGGL((k_sweep<kW, true, true>), grid, block,
                                                      xbytes, c->stream, c->SG, A));
@@ -2806,9 +2902,11 @@ int mbir_hip_icd_graph(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t 
   if (n < 1) return fail("n must be positive");
   if (build_tiles(c)) return -1;
   if (!c->d_iter && dev_alloc(&c->d_iter, 1)) return -1;
-  const long long key[6] = {n, (long long)seed, c->tile, c->afp16 ? 1 : 0,
+  const long long key[7] = {n, (long long)seed, c->tile, c->afp16 ? 1 : 0,
                             (long long)((c->alpha > 0 ? c->alpha : c->g.alpha) * 1e9),
-                            c->nh ? 1 : 0};
+                            (c->nh ? 1 : 0) | (c->placement << 1) | (c->local_atomics << 2) |
+                                (c->d_xcd ? 8 : 0),
+                            (long long)(uintptr_t)c->d_xcd};
   if (!c->gexec || memcmp(key, c->gkey, sizeof(key)) != 0) {
     if (c->gexec) HIP_OK(hipGraphExecDestroy(c->gexec));
     if (c->graph) HIP_OK(hipGraphDestroy(c->graph));
@@ -2933,6 +3031,7 @@ int mbir_hip_voxel_thetas(mbir_hip_ctx* c, int32_t n, const int32_t* tzyx, float
   A.frame_views = c->d_frame_views;
   A.w = c->d_w;
   A.e = c->d_e;
+  A.local_atomics = 0;
   const size_t shm = sizeof(float) * sweep_lds_words(0, c->nvmax, c->G.W, 0);
   MBIR_DISPATCH_W(c->G.W, hipLaunchKernelGGL(k_thetas<kW>, dim3(n), dim3(kWave), shm, c->stream,
                                              c->SG, A, c->d_frame_ptr, dq, c->nvmax, d1, d2));
@@ -2951,6 +3050,17 @@ int mbir_hip_kernel_time(mbir_hip_ctx* c, int32_t k, double* ms, int64_t* launch
   if (ev_collect(c)) return -1;
   if (ms) *ms = c->ms[k];
   if (launches) *launches = c->launches[k];
+  return 0;
+}
+
+int mbir_hip_xcd_check(mbir_hip_ctx* c, int64_t* items, int64_t* off_xcd) {
+  if (!c || !items || !off_xcd) return fail("null argument");
+  if (!c->d_xcd) return fail("option xcd_check is not set");
+  unsigned h[2] = {0, 0};
+  HIP_OK(hipMemcpyAsync(h, c->d_xcd, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  *items = h[0];
+  *off_xcd = h[1];
   return 0;
 }
 

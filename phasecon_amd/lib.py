@@ -27,6 +27,15 @@ class HipGeom(C.Structure):
          ("huber_T", C.c_double), ("huber_delta", C.c_double), ("var_est", C.c_double)]
 
 
+class GuardState(C.Structure):
+    """mirror of mbir_guard (include/mbir_engine.h)"""
+    _fields_ = [("best", C.c_double), ("tol", C.c_double), ("recoveries", C.c_int32),
+                ("max_recoveries", C.c_int32)]
+
+
+GUARD_OK, GUARD_SAVE, GUARD_CONFIRM, GUARD_RECOVER, GUARD_FAIL = range(5)
+
+
 class Options(C.Structure):
     """mirror of mbir_options (include/mbir_engine.h)"""
     _fields_ = [("p_s", C.c_double), ("p_t", C.c_double), ("tile", C.c_int32),
@@ -85,6 +94,11 @@ _SIGS = {
     "mbir_hip_voxel_thetas": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "mbir_hip_kernel_time": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     "mbir_hip_reset_kernel_time": (C.c_int, [C.c_void_p]),
+    "mbir_hip_xcd_check": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mbir_hip_object_snapshot": (C.c_int, [C.c_void_p, C.c_int32]),
+    "mbir_guard_init": (None, [C.POINTER(GuardState), C.c_double, C.c_double, C.c_int32]),
+    "mbir_guard_check": (C.c_int32, [C.POINTER(GuardState), C.c_double, C.c_int32]),
+    "mbir_guard_alpha": (C.c_double, [C.c_double]),
     "mbir_hip_synchronize": (C.c_int, [C.c_void_p]),
     "mbir_hip_bp_normalized": (C.c_int, [C.c_void_p, C.c_void_p]),
     "mbir_hip_fbp": (C.c_int, [C.c_void_p, C.c_void_p]),

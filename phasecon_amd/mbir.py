@@ -152,6 +152,8 @@ class Stage:
                    "mbir_hip_create")
         self.h = h
         self.option("tile", tile)
+        self.alpha = float(g.alpha) if alpha is None else float(alpha)
+        self.alpha0 = self.alpha  # the configured value (the guard may lower alpha)
         if alpha is not None:
             self.option("alpha", alpha)
         self.afp16 = bool(afp16)
@@ -184,6 +186,8 @@ class Stage:
     # -- calls
     def option(self, key, value):
         _lib.check(self.L.mbir_hip_set_option(self.h, key.encode(), float(value)), "set_option")
+        if key == "alpha":
+            self.alpha = float(value)
 
     def set_object(self, x):
         x = np.ascontiguousarray(x, np.float32).reshape(self.shape_x)
@@ -321,6 +325,17 @@ class Stage:
         _lib.check(self.L.mbir_hip_kernel_time(self.h, k, C.byref(ms), C.byref(n)), "kernel_time")
         return ms.value, n.value
 
+    def snapshot(self, restore=False):
+        """Saves the object on the device, or restores the saved one."""
+        _lib.check(self.L.mbir_hip_object_snapshot(self.h, int(bool(restore))), "object_snapshot")
+
+    def xcd_check(self):
+        """(work items run, items that ran off their XCD) since option
+        xcd_check was set."""
+        items, off = C.c_int64(), C.c_int64()
+        _lib.check(self.L.mbir_hip_xcd_check(self.h, C.byref(items), C.byref(off)), "xcd_check")
+        return items.value, off.value
+
     def reset_kernel_time(self):
         _lib.check(self.L.mbir_hip_reset_kernel_time(self.h), "reset_kernel_time")
 
@@ -337,6 +352,49 @@ class Stage:
             self.close()
         except Exception:
             pass
+
+
+class Guard:
+    """The engine's cost-rise guard (mbir_guard_check, include/mbir_engine.h)
+    over one Stage: call check() with each cost from the maintained error
+    sinogram.  It saves the object at each new lowest cost, confirms a rise
+    beyond tol from a recomputed error sinogram, and on a confirmed rise
+    restores the saved object, recomputes e and lowers the stage's alpha;
+    failed is set once max_recoveries are spent."""
+
+    def __init__(self, stage, cost0, tol=1e-6, max_recoveries=3):
+        self.L = _lib.load()
+        self.st = stage
+        self.s = _lib.GuardState()
+        self.L.mbir_guard_init(C.byref(self.s), float(cost0), float(tol), int(max_recoveries))
+        self.failed = False
+        self.events = []
+        stage.snapshot()
+
+    @property
+    def best(self):
+        return self.s.best
+
+    def check(self, cost, where=None):
+        """Returns (action, cost): the MBIR_GUARD_ code and the cost the
+        stage now has (the saved object's after a recovery)."""
+        act = self.L.mbir_guard_check(C.byref(self.s), float(cost), 0)
+        if act == _lib.GUARD_CONFIRM:
+            cost = self.st.cost(fresh=True)[0]
+            act = self.L.mbir_guard_check(C.byref(self.s), float(cost), 1)
+        if act == _lib.GUARD_SAVE:
+            self.st.snapshot()
+        elif act == _lib.GUARD_RECOVER:
+            a2 = self.L.mbir_guard_alpha(self.st.alpha)
+            self.events.append({"where": where, "cost": float(cost), "best": self.s.best,
+                                "alpha": self.st.alpha, "new_alpha": a2})
+            self.st.snapshot(restore=True)
+            self.st.init_error()
+            self.st.option("alpha", a2)
+            cost = self.s.best
+        elif act == _lib.GUARD_FAIL:
+            self.failed = True
+        return act, cost
 
 
 def substitution_batch(g, V, th1, th2, nb, nbf, tn, tf):
