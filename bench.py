@@ -83,6 +83,15 @@ def scan_digest(scan):
     return h.hexdigest()
 
 
+def load_cstar(workload, p_t=None):
+    """The FP64 oracle's C* record of tests/golden/make_cstar.py for the
+    workload (with a temporal p other than the workload's: its _pt file),
+    or None."""
+    name = f"cstar_{workload}.json" if p_t is None else f"cstar_{workload}_pt{p_t:g}.json"
+    path = os.path.join(ROOT, "tests", "golden", name)
+    return json.load(open(path)) if os.path.exists(path) else None
+
+
 def algorithmic_bytes_per_equit(st):
     """Bytes one equit must move for the updated voxels: per (column, frame)
     visit the A window (first index + W values per view, the values in FP16
@@ -451,34 +460,116 @@ def config4(args):
     print(json.dumps(out))
 
 
-def cpu_baseline(scan, equits=2):
-    """The reference's own ICD (oracle/_ref) on this host, full resolution,
-    bounded sample.  The reference has one qGGMRF p (Q6), so it runs p=1.2
-    for both priors; the work per equit is the same."""
+def _up2(a, axis):
+    """2x along axis by the reference's rule for the next stage's initial
+    object (XT_ICD_update.c:358-390): ends copied, interior samples 3/4 and
+    1/4 of the two nearest coarse samples."""
+    a = np.moveaxis(a.astype(np.float32), axis, -1)
+    n = a.shape[-1]
+    out = np.empty(a.shape[:-1] + (2 * n,), np.float32)
+    out[..., 0], out[..., -1] = a[..., 0], a[..., -1]
+    out[..., 1] = (3 * a[..., 0] + a[..., 1]) / 4
+    out[..., -2] = (a[..., -2] + 3 * a[..., -1]) / 4
+    out[..., 2:-2:2] = (a[..., :-2] + 3 * a[..., 1:-1]) / 4
+    out[..., 3:-2:2] = (3 * a[..., 1:-1] + a[..., 2:]) / 4
+    return np.moveaxis(out, -1, axis)
+
+
+def _cpu_model():
     try:
-        from oracle.oracle import Reference, reference_available
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(scan, budget_s=120.0, coarse_cap=100, convg=1.0):
+    """The reference's own ICD (oracle/_ref, its sources compiled in place)
+    on this host's cores, timed to its own target: its multi-resolution
+    schedule (XT_Engine.c:111-141: each coarse stage from the previous one's
+    up-sampled object until its convergence test at convg_thresh = convg %,
+    at most coarse_cap equits), then full resolution until the FP64 MAP cost
+    is <= (1 + 1e-5) C*_ref, or budget_s of reference time.  The reference
+    has one qGGMRF p (MRF_P = 1.2, Q6), so it solves the workload with p = 1.2
+    for both priors and C*_ref is the FP64 oracle's converged cost of that
+    problem (tests/golden/cstar_<workload>_pt1.2.json).  Voxels outside each
+    stage's update mask are zeroed after up-sampling, as on the GPU (the
+    reference leaves them at their interpolated values and so converges to a
+    slightly different cost).  Time = stage set-up (initStructures, error
+    sinogram, up-sampling) + the equits; the reference's per-equit cost
+    evaluations, which its shipped build does not run (NO_COST_CALCULATE,
+    MBIR_4D/Makefile:3), are excluded."""
+    try:
+        from oracle.oracle import Oracle, Reference, reference_available
     except Exception as exc:  # pragma: no cover
-        return {"value": None, "unit": "s/equit", "cores": 0, "kind": "reference",
+        return {"value": None, "unit": "s", "cores": 0, "kind": "reference",
                 "sample": f"unavailable: {exc}"}
     if not reference_available():
-        return {"value": None, "unit": "s/equit", "cores": 0, "kind": "reference",
+        return {"value": None, "unit": "s", "cores": 0, "kind": "reference",
                 "sample": "oracle/_ref not built"}
-    cores = min(os.cpu_count() or 1, 16)
+    from phasecon_amd.mbir import multires_schedule
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     s = dict(scan)
     s["p_t"] = s["p_s"]
-    t0 = time.perf_counter()
-    ref = Reference(s, threads=cores)
-    ref.set_object(np.zeros(ref.shape_x, np.float32))
-    ref.init_error()
-    setup = time.perf_counter() - t0
-    costs, conv, secs = ref.icd_run(equits)
-    ref.close()
-    per = float(np.mean(secs))
-    return {"value": per, "unit": "s/equit", "cores": cores, "kind": "reference",
-            "sample": f"{equits} reference HICD+NHICD equits of this workload at full "
-                      f"resolution (OpenMP {cores} threads, p=1.2 for both priors), "
-                      f"setup {setup:.1f}s excluded",
-            "equit_seconds": [float(v) for v in secs]}
+    rec = load_cstar(s["name"], s["p_t"])
+    if rec is None or rec.get("digest") != scan_digest(scan):
+        return {"value": None, "unit": "s", "cores": threads, "kind": "reference",
+                "sample": "no FP64 C* for the reference's p = 1.2 problem"}
+    target = rec["cstar"] * (1.0 + REL_TOL)
+    stages = multires_schedule(s["proj_rows"], s["proj_cols"])
+    spent, x, per_stage, reached, trace = 0.0, None, [], False, []
+    for k, (mxy, mz) in enumerate(stages):
+        last = k == len(stages) - 1
+        t0 = time.perf_counter()
+        ref = Reference(s, mult_xy=mxy, mult_z=mz, convg=convg, threads=threads)
+        if x is None:
+            x = np.zeros(ref.shape_x, np.float32)
+        else:
+            x = _up2(_up2(x, 3), 2)
+            if x.shape[1] != ref.shape_x[1]:
+                x = _up2(x, 1)
+            x[..., ~Oracle(s, mult_xy=mxy, mult_z=mz).mask.astype(bool)] = 0.0
+        ref.set_object(x)
+        ref.init_error()
+        spent += time.perf_counter() - t0
+        n = 0
+        while spent < budget_s:
+            costs, conv, secs = ref.icd_run(1, first_iter=n + 1)
+            n += 1
+            spent += float(secs[0])
+            if last:
+                trace.append((round(spent, 3), float(costs[0])))
+                if costs[0] <= target:
+                    reached = True
+                    break
+            elif (conv[0] and n > 1) or n >= coarse_cap:
+                break
+        per_stage.append({"grid": f"{ref.N_x}x{ref.N_y}x{ref.N_z}", "equits": n})
+        x = ref.get_object()
+        ref.close()
+        if spent >= budget_s:
+            break
+    out = {"value": spent if reached else None, "unit": "s", "cores": threads,
+           "cpu_model": _cpu_model(), "host_cpus": os.cpu_count(), "kind": "reference",
+           "reached_target": reached, "stages": per_stage, "budget_s": budget_s,
+           "cstar_ref": rec["cstar"],
+           "sample": ("the reference's ICD (OpenMP, its HICD + NHICD schedule, alpha 1.5) "
+                      "from zero through its multi-resolution stages to (1 + 1e-5) C* of the "
+                      "same scan with p = 1.2 for both priors (the reference has one p), "
+                      f"within a budget of {budget_s:g} s")}
+    if not reached and len(trace) >= 3:
+        # geometric extrapolation of the last full-resolution equits (labelled)
+        (t1, c1), (t2, c2) = trace[-3], trace[-1]
+        g1, g2 = c1 - rec["cstar"], c2 - rec["cstar"]
+        if 0 < g2 < g1:
+            rate = (g2 / g1) ** (1.0 / 2)
+            need = math.log((target - rec["cstar"]) / g2) / math.log(rate)
+            out["s_to_target_est"] = spent + need * (t2 - t1) / 2
+            out["note"] = ("budget spent before the target: s_to_target_est extrapolates the "
+                           "last equits' geometric cost decrease")
+    return out
 
 
 def calibration_copy(nbytes=1 << 30):
@@ -610,12 +701,14 @@ def main():
                     help="store A in FP16 with a per-column FP32 scale")
     ap.add_argument("--placement", type=int, choices=(0, 1), default=0,
                     help="sweep work items on the XCDs: 0 tile-major, 1 each frame on one XCD")
-    ap.add_argument("--local-atomics", action="store_true",
-                    help="error-sinogram updates in the XCD's L2 (placement 1, one XCD per frame)")
     ap.add_argument("--xcd-check", action="store_true",
                     help="count sweep work items that ran off their intended XCD")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-equits", type=int, default=2)
+    ap.add_argument("--cpu-budget", type=float, default=120.0,
+                    help="seconds of reference CPU time for the CPU baseline's run to its target")
+    ap.add_argument("--p-t", type=float, default=None,
+                    help="temporal qGGMRF p instead of the workload's (C* from "
+                         "tests/golden/cstar_<workload>_pt<p>.json)")
     ap.add_argument("--longrun", type=int, default=0)
     ap.add_argument("--dump-x", default=None)
     ap.add_argument("--profile-equits", type=int, default=0)
@@ -664,6 +757,8 @@ def main():
               file=sys.stderr, flush=True)
     t0 = time.time()
     scan = make_scan(args.workload, seed=0)
+    if args.p_t is not None:
+        scan["p_t"] = args.p_t
     print(f"[bench] scan {args.workload} generated in {time.time() - t0:.1f}s", file=sys.stderr,
           flush=True)
     st = Stage(scan, tile=args.tile, afp16=args.afp16, alpha=args.alpha)
@@ -688,7 +783,6 @@ def main():
 
     for s_ in [cs for cs, _ in coarse] + [st]:
         s_.option("placement", args.placement)
-        s_.option("local_atomics", int(args.local_atomics))
         if args.xcd_check:
             s_.option("xcd_check", 1)
 
@@ -741,11 +835,10 @@ def main():
         print(json.dumps({"longrun_equits": args.longrun, "fresh_cost": c, "digest": scan_digest(scan)}))
         return
 
-    cpath = os.path.join(ROOT, "tests", "golden", f"cstar_{args.workload}.json")
     cstar, csrc = None, None
-    if os.path.exists(cpath):
-        rec = json.load(open(cpath))
-        if rec.get("digest") == scan_digest(scan):
+    rec = load_cstar(args.workload, args.p_t)
+    if rec is not None:
+        if rec.get("digest") == scan_digest(scan) and rec.get("p_t") == scan["p_t"]:
             cstar, csrc = rec["cstar"], "fp64-oracle"
     if cstar is None:
         # no matching oracle value: fall back to a long GPU run (reported as such)
@@ -800,7 +893,7 @@ def main():
         "config": {
             "workload": (f"{args.workload}: {w['cols']}x{w['cols']}x{w['rows']} voxels x {w['frames']} "
                          f"frames, {w['views']} views" + (f" interlaced K={w['K']}" if w['K'] else "")
-                         + f", I0={w['I0']:g}, qGGMRF p_s={w['p_s']} p_t={w['p_t']} q=2, from zero"),
+                         + f", I0={w['I0']:g}, qGGMRF p_s={w['p_s']} p_t={scan['p_t']} q=2, from zero"),
             "tile": tile_of(st),
             "autotuned_tiles": tuned or None,
             "alpha": args.alpha,
@@ -822,7 +915,7 @@ def main():
         },
         "reached_target": reached,
         "equits_to_target": equits,
-        "placement": {"frames_on_xcds": args.placement == 1, "local_atomics": args.local_atomics,
+        "placement": {"frames_on_xcds": args.placement == 1,
                       "xcd_check": ({f"{s_.g.N_x}x{s_.g.N_y}x{s_.g.N_z}": s_.xcd_check()
                                      for s_ in [cs for cs, _ in coarse] + [st]}
                                     if args.xcd_check else None)},
@@ -856,13 +949,10 @@ def main():
         except Exception as exc:  # the headline line must still be printed
             out["learned"] = {"error": f"{type(exc).__name__}: {exc}"}
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(scan, args.cpu_equits)
+        out["cpu_baseline"] = cpu_baseline(scan, args.cpu_budget)
         cb = out["cpu_baseline"]
         if cb.get("value") and reached:
-            cb["s_to_target_est"] = cb["value"] * float(np.mean(equits))
-            cb["note"] = ("s_to_target_est = measured s/equit x the GPU run's full-resolution "
-                          "equits to target (coarse stages not counted; the reference's own "
-                          "equit count to this target was not measured)")
+            cb["gpu_speedup"] = cb["value"] / value
     print(json.dumps(out))
 
 

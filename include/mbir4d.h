@@ -15,7 +15,17 @@
  *   - single process, one GPU: no MPI; proj_rows is not split across ranks;
  *   - the ICD voxel order differs (parallel ICD on the GPU), so results are
  *     judged on the converged MAP cost, not per-iteration equality;
- *   - remove_rings > 0 (ring offset estimation) is rejected with -1.
+ *   - remove_rings 1-3 (ring offset estimation) and, with quad_convex = 0,
+ *     the noise-variance estimate are updated once per equit from the
+ *     second stage on (the reference: the offset after every half-sweep of
+ *     its NHICD subsets);
+ *   - a cost rise is handled as the reference does (-1 and "Cost value
+ *     increased.", MBIR_4D/XT_ICD_update.c:1171) only when it exceeds 1e-6
+ *     relative (the FP32 rounding noise of the maintained error sinogram),
+ *     is confirmed from a recomputed error sinogram, and persists after two
+ *     recoveries; each recovery restores the object of the lowest cost in
+ *     the stage and halves the over-relaxation's excess over 1, with a WARN
+ *     line in the log.
  */
 #ifndef MBIR4D_H_
 #define MBIR4D_H_

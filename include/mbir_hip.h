@@ -84,12 +84,10 @@ void mbir_hip_destroy(mbir_hip_ctx* ctx);
  * bound to one XCD (the hardware deals workgroups to the 8 XCDs
  * round-robin; with 8 frames per launch every frame's error-sinogram
  * traffic then stays in one L2; with other frame counts a frame spans
- * 8 / gcd(frames, 8) XCDs).  "local_atomics" (0 or 1, default 0; placement
- * 1 with one XCD per frame only): the sweep's error-sinogram updates are
- * performed in the XCD's L2 instead of at the memory side.  "xcd_check"
- * (0 or 1): the sweep counts its work items and those that ran on another
- * XCD than the placement intended (mbir_hip_xcd_check).  None of these has
- * a counterpart in the reference. */
+ * 8 / gcd(frames, 8) XCDs; measured slower on the 4D benchmark).
+ * "xcd_check" (0 or 1): the sweep counts its work items and those that ran
+ * on another XCD than workgroup id mod 8 (mbir_hip_xcd_check).  None of
+ * these has a counterpart in the reference. */
 int mbir_hip_set_option(mbir_hip_ctx* ctx, const char* key, double value);
 
 /* Counters of option "xcd_check" since it was set: sweep work items run and

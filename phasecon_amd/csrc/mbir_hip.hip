@@ -430,11 +430,8 @@ __device__ __forceinline__ float e_load(const float* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-__device__ __forceinline__ void e_add(float* p, float v, bool local) {
-  if (local)
-    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-  else
-    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void e_add(float* p, float v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Wave-scope ordering: one wavefront per workgroup, and a wavefront's LDS
@@ -495,9 +492,6 @@ struct ColumnArgs {
   const int* frame_views;
   const float* w;
   float* e;
-  // error-sinogram updates at workgroup scope, performed in the XCD's L2:
-  // valid only while all items of a frame run on one XCD
-  int local_atomics;
 };
 
 constexpr int kMaxViews = 512;  // views per frame held in LDS
@@ -812,7 +806,6 @@ template <int kW>
 __device__ __forceinline__ void col_update(const SwGeom& G, const ColumnArgs& A, const Win& win,
                                            int nv, const float* dxz) {
   const Lanes L(G.N_t);
-  const bool local = __builtin_amdgcn_readfirstlane(A.local_atomics) != 0;
   for (int t0 = 0; t0 < G.N_t; t0 += L.rpl) {
     const int row = t0 + L.rr;
     if (L.rr >= L.rpl || row >= G.N_t) continue;
@@ -824,7 +817,7 @@ __device__ __forceinline__ void col_update(const SwGeom& G, const ColumnArgs& A,
 #pragma unroll
       for (int k = 0; k < kW; ++k) {
         const float a = win.av[sl * kW + k];
-        if (a != 0.f) e_add(A.e + base + k * G.N_t + row, -(a * G.delta_t) * dz, local);
+        if (a != 0.f) e_add(A.e + base + k * G.N_t + row, -(a * G.delta_t) * dz);
       }
     }
   }
@@ -2036,9 +2029,8 @@ struct mbir_hip_ctx {
   double alpha = -1.0;
   size_t lds_max = 64 * 1024;  // LDS bytes one workgroup may allocate
   // sweep placement (option "placement"): 0 tile-major grid, 1 frames bound
-  // to XCDs; option "local_atomics" (placement 1, one XCD per frame):
-  // error-sinogram updates performed in the XCD's L2
-  int placement = 0, local_atomics = 0;
+  // to XCDs
+  int placement = 0;
   unsigned* d_xcd = nullptr;  // [items run, items off their XCD] (option "xcd_check")
   float* d_snap = nullptr;    // saved object (mbir_hip_object_snapshot)
   bool snap_valid = false;
@@ -2337,9 +2329,6 @@ int mbir_hip_set_option(mbir_hip_ctx* c, const char* key, double value) {
   } else if (k == "placement") {
     if (value != 0 && value != 1) return fail("placement must be 0 or 1");
     c->placement = (int)value;
-  } else if (k == "local_atomics") {
-    if (value != 0 && value != 1) return fail("local_atomics must be 0 or 1");
-    c->local_atomics = (int)value;
   } else if (k == "xcd_check") {
     // count the sweep's work items and those that ran off their XCD
     if (value != 0 && value != 1) return fail("xcd_check must be 0 or 1");
@@ -2587,7 +2576,6 @@ int sweep_launches(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t seed
   A.col.frame_views = c->d_frame_views;
   A.col.w = c->d_w;
   A.col.e = c->d_e;
-  A.col.local_atomics = 0;
   A.xcd_check = c->d_xcd;
   A.frame_ptr = c->d_frame_ptr;
   A.tile_ptr = c->d_tile_ptr;
@@ -2628,7 +2616,6 @@ int sweep_launches(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t seed
         A.place_r = 0;
         A.nfl = frames;
         A.unit_slots = 0;
-        A.col.local_atomics = 0;
         if (c->placement == 1) {
           // frames bound to XCDs: r parts per frame, lcm(frames, 8) units
           int a = frames, b = kXcds;
@@ -2640,8 +2627,6 @@ int sweep_launches(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t seed
           A.place_r = kXcds / a;
           A.unit_slots = (ntl + A.place_r - 1) / A.place_r;
           nwg = (unsigned)(frames * A.place_r * A.unit_slots);
-          // one XCD per frame: its updates can stay in that XCD's L2
-          A.col.local_atomics = (c->local_atomics && A.place_r == 1) ? 1 : 0;
         }
         if (timed && ev_begin(c, MBIR_K_SWEEP)) return -1;
         const dim3 grid(nwg), block(kSweepThreads);
@@ -2904,8 +2889,7 @@ int mbir_hip_icd_graph(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t 
   if (!c->d_iter && dev_alloc(&c->d_iter, 1)) return -1;
   const long long key[7] = {n, (long long)seed, c->tile, c->afp16 ? 1 : 0,
                             (long long)((c->alpha > 0 ? c->alpha : c->g.alpha) * 1e9),
-                            (c->nh ? 1 : 0) | (c->placement << 1) | (c->local_atomics << 2) |
-                                (c->d_xcd ? 8 : 0),
+                            (c->nh ? 1 : 0) | (c->placement << 1) | (c->d_xcd ? 4 : 0),
                             (long long)(uintptr_t)c->d_xcd};
   if (!c->gexec || memcmp(key, c->gkey, sizeof(key)) != 0) {
     if (c->gexec) HIP_OK(hipGraphExecDestroy(c->gexec));
@@ -3031,7 +3015,6 @@ int mbir_hip_voxel_thetas(mbir_hip_ctx* c, int32_t n, const int32_t* tzyx, float
   A.frame_views = c->d_frame_views;
   A.w = c->d_w;
   A.e = c->d_e;
-  A.local_atomics = 0;
   const size_t shm = sizeof(float) * sweep_lds_words(0, c->nvmax, c->G.W, 0);
   MBIR_DISPATCH_W(c->G.W, hipLaunchKernelGGL(k_thetas<kW>, dim3(n), dim3(kWave), shm, c->stream,
                                              c->SG, A, c->d_frame_ptr, dq, c->nvmax, d1, d2));

@@ -33,8 +33,16 @@ def scan_digest(scan):
     return h.hexdigest()
 
 
-def main(name, seed=0, max_equits=1500, tol=1e-11, init=None):
+def cstar_path(name, p_t=None):
+    """tests/golden/cstar_<workload>.json, or cstar_<workload>_pt<p_t>.json
+    for a temporal p other than the workload's."""
+    return os.path.join(HERE, f"cstar_{name}.json" if p_t is None else f"cstar_{name}_pt{p_t:g}.json")
+
+
+def main(name, seed=0, max_equits=1500, tol=1e-11, init=None, p_t=None):
     scan = make_scan(name, seed=seed)
+    if p_t is not None:
+        scan["p_t"] = p_t
     o = Oracle(scan)
     x = np.zeros(o.shape_x) if init is None else np.load(init).astype(np.float64).reshape(o.shape_x)
     e = o.init_error(x)
@@ -60,7 +68,7 @@ def main(name, seed=0, max_equits=1500, tol=1e-11, init=None):
                init="zero" if init is None else os.path.basename(init),
                p_s=scan["p_s"], p_t=scan["p_t"], sig_s=scan["sig_s"], sig_t=scan["sig_t"],
                c_s=scan["c_s"], c_t=scan["c_t"])
-    path = os.path.join(HERE, f"cstar_{name}.json")
+    path = cstar_path(name, p_t)
     with open(path, "w") as f:
         json.dump(out, f)
     print(f"C* = {cstar:.12g} after {len(costs) - 1} equits -> {path}")
@@ -72,5 +80,8 @@ if __name__ == "__main__":
     ap.add_argument("workload", nargs="?", default="ch4d")
     ap.add_argument("--init", default=None)
     ap.add_argument("--max-equits", type=int, default=1500)
+    ap.add_argument("--p-t", type=float, default=None,
+                    help="temporal qGGMRF p instead of the workload's (1.2: the reference's "
+                         "single MRF_P, for timing the reference to its own C*)")
     a = ap.parse_args()
-    main(a.workload, init=a.init, max_equits=a.max_equits)
+    main(a.workload, init=a.init, max_equits=a.max_equits, p_t=a.p_t)

@@ -28,39 +28,50 @@ pytestmark = pytest.mark.gpu
 TOL = 2e-5
 
 
-def _start(o, scan, seed):
-    """A non-trivial start: two oracle equits from zero, rounded to FP32,
-    with a zero block whose neighbours are all <= 0 (zero skipping)."""
+def _start(o, n, block):
+    """A start after n oracle equits from zero (alpha 1.5), rounded to FP32;
+    block: a zero block whose neighbours are all <= 0 (zero skipping)."""
     x = np.zeros(o.shape_x, np.float64)
     e = o.init_error(x)
-    for it in (1, 2):
-        o.icd_equit_async(x, e, it, 32, 1.5, seed + it, 0, 0, 0)
+    for it in range(1, n + 1):
+        o.icd_equit_async(x, e, it, 32, 1.5, 17 + it, 0, 0, 0)
     x = x.astype(np.float32).astype(np.float64)
-    T, Z, Y, X = x.shape
-    cy, cx = Y // 2 - 4, X // 2 - 4
-    x[:, :, cy - 1:cy + 5, cx - 1:cx + 5] = -1e-3
-    x[:, :, cy:cy + 4, cx:cx + 4] = 0.0
+    if block:
+        T, Z, Y, X = x.shape
+        cy, cx = Y // 2 - 4, X // 2 - 4
+        x[:, :, cy - 1:cy + 5, cx - 1:cx + 5] = -1e-3
+        x[:, :, cy:cy + 4, cx:cx + 4] = 0.0
     return x
 
 
-@pytest.mark.parametrize("name,placement,alpha,p_t", [
-    ("ch4d_small", 1, 1.97, None), ("ch4d_small", 0, 1.5, None),
-    ("ch4d_small", 1, 1.97, 1.1), ("tall3d", 1, 1.97, None), ("tiny3d", 1, 1.5, None)])
-def test_sweep_matches_oracle_schedule(name, placement, alpha, p_t):
+# One equit of ICD far from convergence amplifies a perturbation of its
+# input by up to 1e6 (measured with the oracle alone: 1e-9 relative noise on
+# the input of ch4d_small after 2 equits moves the output by 1.6e-3), so
+# FP32 rounding alone would exceed any useful tolerance there.  From 20
+# equits the amplification is about 10 (ch4d_small, tall3d), and the equit
+# still moves the object by about 1 % of its largest value, 500 times the
+# tolerance; the zero block (a large disturbance) is used on tiny3d, whose
+# amplification stays small.
+@pytest.mark.parametrize("name,placement,alpha,p_t,n0,block", [
+    ("ch4d_small", 1, 1.97, None, 20, False), ("ch4d_small", 0, 1.5, None, 20, False),
+    ("ch4d_small", 1, 1.97, 1.1, 20, False), ("tall3d", 1, 1.97, None, 20, False),
+    ("tiny3d", 1, 1.5, None, 2, True)])
+def test_sweep_matches_oracle_schedule(name, placement, alpha, p_t, n0, block):
     from oracle.oracle import Oracle
     from phasecon_amd.mbir import Stage
     from phasecon_amd.synth import make_scan
     scan = make_scan(name, seed=3 if name == "ch4d_small" else 0)
     o = Oracle(scan, p_t=p_t)
     T = max(o.g.N_x, o.g.N_y) // 2
-    print(f"{name}: tile {T}, placement {placement}, alpha {alpha}, p_t {p_t}", flush=True)
-    x0 = _start(o, scan, 17)
+    print(f"{name}: tile {T}, placement {placement}, alpha {alpha}, p_t {p_t}, start after "
+          f"{n0} equits{', zero block' if block else ''}", flush=True)
+    x0 = _start(o, n0, block)
     st = Stage(scan, tile=T, alpha=alpha, p_t=p_t)
     st.option("placement", placement)
     st.set_object(x0.astype(np.float32))
     st.build_amatrix()
     st.init_error()
-    seed, it = 4242, 3
+    seed, it = 4242, n0 + 1
     st.icd(1, it, seed)
     xg = st.get_object().astype(np.float64)
     eg = st.get_error().astype(np.float64)
@@ -68,7 +79,8 @@ def test_sweep_matches_oracle_schedule(name, placement, alpha, p_t):
     xo = x0.copy()
     eo = o.init_error(xo)
     stats = o.icd_equit_async(xo, eo, it, T, alpha, seed, 0, 0, 0, 8, placement)
-    assert stats[2] > 0, "the zero block should be skipped"
+    if block:
+        assert stats[2] > 0, "the zero block should be skipped"
     dx = np.abs(xg - xo).max() / np.abs(xo).max()
     de = np.abs(eg - eo).max() / np.abs(o.y).max()
     moved = np.abs(xo - x0).max() / np.abs(xo).max()
