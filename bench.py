@@ -83,6 +83,20 @@ def scan_digest(scan):
     return h.hexdigest()
 
 
+def coarse_alphas(spec, n):
+    """Per coarse stage over-relaxation from '--coarse-alpha': one value for
+    all n stages, or one per stage (coarsest first); with n = 1 and a list,
+    the finest coarse stage's."""
+    vals = [float(v) for v in str(spec).split(",")]
+    if len(vals) == 1:
+        return vals * n
+    if n == 1:
+        return vals[-1:]
+    if len(vals) != n:
+        sys.exit(f"bench.py: --coarse-alpha needs 1 or {n} values")
+    return vals
+
+
 def load_cstar(workload, p_t=None):
     """The FP64 oracle's C* record of tests/golden/make_cstar.py for the
     workload (with a temporal p other than the workload's: its _pt file),
@@ -297,7 +311,8 @@ def learned_leg(args, st, train_scans, target):
     t_train = time.perf_counter() - t0
     from phasecon_amd.mbir import Stage, multires_schedule
     mxy, mz = multires_schedule(st.scan["proj_rows"], st.scan["proj_cols"])[-2]
-    coarse = Stage(st.scan, mult_xy=mxy, mult_z=mz, tile=args.coarse_tile, alpha=args.coarse_alpha)
+    coarse = Stage(st.scan, mult_xy=mxy, mult_z=mz, tile=args.coarse_tile,
+                   alpha=coarse_alphas(args.coarse_alpha, 1)[-1])
     variants = []
     for nc in [int(v) for v in str(args.learned_coarse_equits).split(",")]:
         run_learned(st, net, target, args.max_equits, args.seed, args.refresh, src, coarse, nc)
@@ -401,8 +416,9 @@ def config4(args):
     results = {}
     ref_x = None
     for afp16 in (False, True):
-        coarse = [(Stage(scan, mult_xy=mx, mult_z=mz, tile=8, alpha=args.coarse_alpha,
-                         afp16=afp16), n) for (mx, mz), n in zip(sched[:-1], counts)]
+        coarse = [(Stage(scan, mult_xy=mx, mult_z=mz, tile=8, alpha=a, afp16=afp16), n)
+                  for (mx, mz), n, a in zip(sched[:-1], counts,
+                                            coarse_alphas(args.coarse_alpha, len(sched) - 1))]
         st = Stage(scan, tile=8, alpha=args.alpha, afp16=afp16)
         tiles = {}
         for s_ in [cs for cs, _ in coarse] + [st]:
@@ -682,7 +698,9 @@ def main():
     ap.add_argument("--alpha", type=float, default=1.97,
                     help="over-relaxation factor of the full-resolution sweeps "
                          "(the reference's is 1.5)")
-    ap.add_argument("--coarse-alpha", type=float, default=1.97)
+    ap.add_argument("--coarse-alpha", default="1.97",
+                    help="over-relaxation of the coarse stages, coarsest first: one value "
+                         "for every stage or a comma list")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned warm start leg")
     ap.add_argument("--learned-scans", type=int, default=8)
@@ -774,12 +792,13 @@ def main():
         counts = (counts * len(stages))[:len(stages)] if len(counts) == 1 else counts
         if len(counts) != len(stages):
             sys.exit(f"bench.py: --coarse-equits needs 1 or {len(stages)} counts")
-        for (mxy, mz), n in zip(stages, counts):
+        alphas = coarse_alphas(args.coarse_alpha, len(stages))
+        for (mxy, mz), n, a in zip(stages, counts, alphas):
             # a count of 0 leaves a leading stage out; a later one only
             # passes the object through (up-sampling doubles the grid)
             if n > 0 or coarse:
                 coarse.append((Stage(scan, mult_xy=mxy, mult_z=mz, tile=args.coarse_tile,
-                                     afp16=args.afp16, alpha=args.coarse_alpha), n))
+                                     afp16=args.afp16, alpha=a), n))
 
     for s_ in [cs for cs, _ in coarse] + [st]:
         s_.option("placement", args.placement)
@@ -897,7 +916,7 @@ def main():
             "tile": tile_of(st),
             "autotuned_tiles": tuned or None,
             "alpha": args.alpha,
-            "coarse_alpha": args.coarse_alpha,
+            "coarse_alpha": [cs.alpha0 for cs, _ in coarse],
             "a_storage": "fp16 + fp32 column scale" if args.afp16 else "fp32",
             "schedule": (("multires: " + " -> ".join(
                 f"{cs.g.N_x}x{cs.g.N_y}x{cs.g.N_z} ({n} equits, tile {tile_of(cs)})"
