@@ -204,3 +204,21 @@ def test_cost_guard_actions():
     assert chk(float("inf"), 1) == lib.GUARD_FAIL
     assert abs(L.mbir_guard_alpha(1.97) - 1.485) < 1e-12
     assert abs(L.mbir_guard_alpha(0.8) - 0.4) < 1e-12
+
+
+DRIVER = os.path.join(ROOT, "oracle", "_ref", "xt_main_basic")
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="oracle/_ref/xt_main_basic not built")
+def test_reference_driver_links_to_this_library():
+    """The reference's basic driver, compiled against include/mbir4d.h,
+    takes reconstruct() from phasecon_amd/libmbir4d.so (the relink of the
+    integration notes); the GPU test runs it."""
+    import subprocess
+    und = subprocess.run(["nm", "-D", "--undefined-only", DRIVER], capture_output=True,
+                         text=True, check=True).stdout
+    assert re.search(r"\breconstruct\b", und)
+    ldd = subprocess.run(["ldd", DRIVER], capture_output=True, text=True, check=True).stdout
+    assert "not found" not in ldd
+    line = next(ln for ln in ldd.splitlines() if "libmbir4d.so" in ln)
+    assert os.path.realpath(line.split("=>")[1].split("(")[0].strip()) == os.path.realpath(LIB)

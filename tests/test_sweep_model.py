@@ -1,0 +1,84 @@
+"""The over-relaxed parallel sweep's stability on the benchmark's coarsest
+stage, in the oracle's model of the GPU schedule (orc_icd_equit_async).
+
+The 4D benchmark scan's 64x64x2 stage (voxels of 4x4 detector channels and
+4 rows, tile 4: about 50 concurrent tiles per frame) is the stage that
+diverged on the GPU at alpha 1.97 (bench step with seed offset 4, guard
+recovery at its equit 40).  The model runs every tile of a launch in
+lock-step, one column per step, and gives each column's theta the error
+sinogram without the other tiles' updates of the previous step (the GPU
+issues a column's theta loads while the previous column is solved).  In
+that model the stage diverges at alpha 1.97 and converges monotonically at
+1.8 (the benchmark's over-relaxation of this stage); under the cost-rise
+guard of the equit loops (mbir_guard_check) the divergence at 1.97 is
+caught, the object of the lowest cost restored, alpha lowered to 1.485, and
+the stage then converges.  CPU only.
+"""
+import numpy as np
+import pytest
+
+SEED = 761525  # the benchmark's seed offset 4
+
+
+@pytest.fixture(scope="module")
+def coarse_stage():
+    from oracle.oracle import Oracle
+    from phasecon_amd.synth import make_scan
+    return Oracle(make_scan("ch4d", seed=0), mult_xy=4, mult_z=4)
+
+
+def _equit(o, x, e, it, alpha):
+    # tile 4, one column of lag within a frame, each frame its own cache group
+    o.icd_equit_async(x, e, it, 4, alpha, SEED + 7919 * it, 0, 1, -1, 8, 1)
+    return o.cost(e, x)[0]
+
+
+def test_model_diverges_at_197_and_converges_at_18(coarse_stage):
+    o = coarse_stage
+    runs = {}
+    for alpha in (1.97, 1.8):
+        x = np.zeros(o.shape_x)
+        e = o.init_error(x)
+        c = [o.cost(e, x)[0]]
+        for it in range(1, 13):
+            c.append(_equit(o, x, e, it, alpha))
+            if not np.isfinite(c[-1]) or c[-1] > 10 * c[0]:
+                break
+        runs[alpha] = np.array(c)
+        print(f"alpha {alpha}: " + " ".join(f"{v:.4g}" for v in c), flush=True)
+    c = runs[1.97]
+    assert np.any(np.diff(c) > 0) and c[-1] > c[1]          # rises, and ends above equit 1
+    c = runs[1.8]
+    assert len(c) == 13 and np.all(np.diff(c) < 0)           # monotone over 12 equits
+
+
+def test_guard_recovers_the_model_divergence(coarse_stage):
+    import ctypes as C
+    from phasecon_amd import lib
+    L = lib.load()
+    o = coarse_stage
+    x = np.zeros(o.shape_x)
+    e = o.init_error(x)
+    g = lib.GuardState()
+    L.mbir_guard_init(C.byref(g), o.cost(e, x)[0], 1e-6, 3)
+    saved = (x.copy(), e.copy())
+    alpha, recovered_at, after = 1.97, None, []
+    for it in range(1, 21):
+        cost = _equit(o, x, e, it, alpha)
+        act = L.mbir_guard_check(C.byref(g), cost, 1)   # the model's e is exact FP64
+        assert act != lib.GUARD_FAIL
+        if act == lib.GUARD_RECOVER:
+            assert recovered_at is None, "one recovery suffices"
+            recovered_at = it
+            x[...], e[...] = saved
+            alpha = L.mbir_guard_alpha(alpha)
+            after.append(o.cost(e, x)[0])
+            continue
+        if act == lib.GUARD_SAVE:
+            saved = (x.copy(), e.copy())
+        if recovered_at is not None:
+            after.append(cost)
+    print(f"recovered at equit {recovered_at}, alpha then {alpha:.4g}; costs after: "
+          + " ".join(f"{v:.4g}" for v in after), flush=True)
+    assert recovered_at is not None and abs(alpha - 1.485) < 1e-12
+    assert len(after) >= 8 and np.all(np.diff(after) < 0)
