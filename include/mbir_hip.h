@@ -275,6 +275,21 @@ int mbir_hip_substitution_batch(const mbir_hip_geom* g, int32_t n,
                                 const uint8_t* nbf, const float* tn,
                                 const uint8_t* tf, float* out);
 
+/* mbir_hip_substitution_batch without any allocation, for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc); no counterpart in the reference.
+ * The inputs and the output are staged through one fixed device array per
+ * GPU (8 MiB, held by the library from module load to process end), so n is
+ * at most MBIR_SUBST_NOALLOC_MAX (else -1 with a message).  Calls that
+ * overlap on one GPU (several host threads) share that array and race;
+ * sequential calls are independent (every staged byte is rewritten). */
+#define MBIR_SUBST_NOALLOC_MAX 49152
+int mbir_hip_substitution_batch_noalloc(const mbir_hip_geom* g, int32_t n,
+                                        const float* V, const float* theta1,
+                                        const float* theta2, const float* nb,
+                                        const uint8_t* nbf, const float* tn,
+                                        const uint8_t* tf, float* out);
+
 /* theta1/theta2 of n voxels (t, z, y, x quadruples) from the stored A, the
  * current error sinogram and weights, FP32 accumulation
  * (XT_VoxUpdate.c:56-85). */

@@ -30,6 +30,7 @@
 #include <type_traits>
 #include <algorithm>
 #include <vector>
+#include <mutex>
 
 #include "mbir_hip.h"
 #include "mbir_engine.h"
@@ -2989,6 +2990,95 @@ int mbir_hip_substitution_batch(const mbir_hip_geom* g, int32_t n, const float* 
   HIP_OK(hipMemcpy(out, dout, n * 4, hipMemcpyDeviceToHost));
   void* bufs[] = {dV, d1, d2, dnb, dtn, dout, dnbf, dtf};
   for (void* b : bufs) HIP_OK(hipFree(b));
+  return 0;
+}
+
+}  // extern "C"
+
+// Device scratch of mbir_hip_substitution_batch_noalloc: the staging copies
+// of its inputs and output, in one array per GPU that the library holds
+// from module load until the process ends (no allocation per call).
+constexpr size_t kNoAllocBytes = 8u << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's address of `scratch` (hipGetSymbolAddress, once per
+// device with that device current); host code uses only this address.
+int scratch_address(void** out) {
+  int dev = -1;
+  HIP_OK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail("device index out of range for the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail("hipGetSymbolAddress returned a null address for scratch");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Byte offsets of the staged arrays of n voxels in the scratch array
+// (V, theta1, theta2, nb, tn, out as floats, then nbf, tf as bytes); returns
+// the total.
+size_t noalloc_layout(size_t n, size_t off[8]) {
+  const size_t sz[8] = {4 * n, 4 * n, 4 * n, 4 * 27 * n, 4 * 2 * n, 4 * n, 27 * n, 2 * n};
+  size_t at = 0;
+  for (int k = 0; k < 8; ++k) {
+    off[k] = at;
+    at += align256(sz[k]);
+  }
+  return at;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mbir_hip_substitution_batch_noalloc(const mbir_hip_geom* g, int32_t n, const float* V,
+                                        const float* theta1, const float* theta2,
+                                        const float* nb, const uint8_t* nbf, const float* tn,
+                                        const uint8_t* tf, float* out) {
+  if (!g || n < 0 || (n > 0 && (!V || !theta1 || !theta2 || !nb || !nbf || !tn || !tf || !out)))
+    return fail("bad argument");
+  if (n > MBIR_SUBST_NOALLOC_MAX)
+    return fail("n exceeds MBIR_SUBST_NOALLOC_MAX (" + std::to_string(MBIR_SUBST_NOALLOC_MAX) +
+                " voxels) of the no-allocation substitution");
+  if (n == 0) return 0;
+  if (g->q != 2.0) return fail("q must be 2 (the qGGMRF of the sweep kernels)");
+  size_t off[8];
+  if (noalloc_layout((size_t)n, off) > kNoAllocBytes)
+    return fail("staging layout exceeds the scratch array");
+  unsigned char* base = nullptr;
+  if (scratch_address((void**)&base)) return -1;
+  float* dV = (float*)(base + off[0]);
+  float* d1 = (float*)(base + off[1]);
+  float* d2 = (float*)(base + off[2]);
+  float* dnb = (float*)(base + off[3]);
+  float* dtn = (float*)(base + off[4]);
+  float* dout = (float*)(base + off[5]);
+  unsigned char* dnbf = base + off[6];
+  unsigned char* dtf = base + off[7];
+  const SwGeom G = make_sw_geom(*g, 0);
+  HIP_OK(hipMemcpy(dV, V, (size_t)n * 4, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(d1, theta1, (size_t)n * 4, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(d2, theta2, (size_t)n * 4, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(dnb, nb, (size_t)n * 27 * 4, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(dtn, tn, (size_t)n * 2 * 4, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(dnbf, nbf, (size_t)n * 27, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(dtf, tf, (size_t)n * 2, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_subst, dim3((unsigned)((n + 3) / 4)), dim3(kWave), 0, 0, G,
+                     (float)g->alpha, n, dV, d1, d2, dnb, dnbf, dtn, dtf, dout);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpy(out, dout, (size_t)n * 4, hipMemcpyDeviceToHost));
   return 0;
 }
 

@@ -91,6 +91,7 @@ _SIGS = {
     "mbir_hip_get_weight": (C.c_int, [C.c_void_p, C.c_void_p]),
     "mbir_hip_cost": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     "mbir_hip_substitution_batch": (C.c_int, None),
+    "mbir_hip_substitution_batch_noalloc": (C.c_int, None),
     "mbir_hip_voxel_thetas": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "mbir_hip_kernel_time": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     "mbir_hip_reset_kernel_time": (C.c_int, [C.c_void_p]),
