@@ -13,13 +13,19 @@ each stage's object is up-sampled on the device into the next with the
 voxels outside the next stage's update mask zeroed (so the full-resolution
 stage solves the same problem as a start from zero), and the full-resolution
 stage runs to the target; --schedule single runs the full-resolution stage
-alone from zero.  Every stage over-relaxes its voxel updates by --alpha /
---coarse-alpha (default 1.9; the reference uses 1.5, OVER_RELAXATION_FACTOR
-in XT_Constants.h): any factor in (0, 2) keeps the same fixed point, and 1.9
-with 60 + 100 coarse equits was the fastest measured setting.  Every stage's A build and error sinogram are inside the
-timed region, as are the parallel ICD equits and a cost check after each
-full-resolution equit (maintained error sinogram, FP32 prior potentials
-summed in FP64).  Each equit's 8 sweep launches are one replay of a captured
+alone from zero.  The stages over-relax their voxel updates by --alpha
+(full resolution, default 1.97) and --coarse-alpha (per coarse stage,
+default 1.8 for 64x64x2 and 1.97 for 128x128x4; the reference uses 1.5,
+OVER_RELAXATION_FACTOR in XT_Constants.h): any factor in (0, 2) keeps the
+same fixed point for exact sequential updates, but the parallel sweep's
+concurrent updates diverge on the coarsest stage at 1.97 (measured on the
+GPU and reproduced by the oracle's model of the sweep), where 1.8 converges
+and was also the faster setting.  Every cost check goes through the
+cost-rise guard (a confirmed rise restores the object of the lowest cost
+and lowers alpha; recoveries are reported in the line).  Every stage's A
+build and error sinogram are inside the timed region, as are the parallel
+ICD equits and the cost checks (maintained error sinogram, FP32 prior
+potentials summed in FP64).  Each equit's 8 sweep launches are one replay of a captured
 HIP graph (mbir_hip_icd_graph; --no-graph launches them directly).  Inputs are resident in HBM before the timed region.
 Prints one JSON line.
 
@@ -555,6 +561,9 @@ def cpu_baseline(scan, budget_s=120.0, coarse_cap=100, convg=1.0):
             costs, conv, secs = ref.icd_run(1, first_iter=n + 1)
             n += 1
             spent += float(secs[0])
+            print(f"[bench] cpu reference {ref.N_x}x{ref.N_y}x{ref.N_z} equit {n}: "
+                  f"{secs[0]:.2f}s, cost {costs[0]:.10g}, {spent:.1f}s so far",
+                  file=sys.stderr, flush=True)
             if last:
                 trace.append((round(spent, 3), float(costs[0])))
                 if costs[0] <= target:
@@ -698,7 +707,7 @@ def main():
     ap.add_argument("--alpha", type=float, default=1.97,
                     help="over-relaxation factor of the full-resolution sweeps "
                          "(the reference's is 1.5)")
-    ap.add_argument("--coarse-alpha", default="1.97",
+    ap.add_argument("--coarse-alpha", default="1.8,1.97",
                     help="over-relaxation of the coarse stages, coarsest first: one value "
                          "for every stage or a comma list")
     ap.add_argument("--no-learned", action="store_true",
