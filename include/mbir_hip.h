@@ -296,6 +296,45 @@ int mbir_hip_substitution_batch_noalloc(const mbir_hip_geom* g, int32_t n,
 int mbir_hip_voxel_thetas(mbir_hip_ctx* ctx, int32_t n, const int32_t* tzyx,
                           float* theta1, float* theta2);
 
+/* ---- PCMBIR tomographic ICD (the ENABLE_TOMO_RECONS path of PCMBIR_3D,
+ * compiled out of the shipped reference).  A context switched to this mode
+ * holds two objects, delta (PhaseObject) and beta (MagObject), and their
+ * error sinograms e = target - k A x (A the MBIR rule of the context's
+ * geometry, k the wavenumber: PCMBIR_3D/XT_AMatrix.c:103), and runs the
+ * tomographic ICD: theta1 = -mu sum e k a, theta2 = mu sum (k a)^2
+ * (PCMBIR_3D/XT_VoxUpdate.c:23-80), the coupled substitution of both
+ * objects (FunctionalSubstitution, PCMBIR_3D/XT_Prior.c:137-239), zero
+ * skipping when both are zero.  Index 0 of the pairs is the phase (delta)
+ * prior, 1 the magnitude (beta) prior; T is DecorrTran row-major
+ * (PCMBIR_3D/XT_Init.c:234-246); sigmas already raised to q and q - p and
+ * scaled as PCMBIR_3D/XT_Init.c:482-489 does; filt / filt_t as initFilter
+ * (XT_Init.c:144-176: 1/distance normalised to 1, temporal 0). */
+typedef struct {
+  double mu, wavenumber, alpha;
+  double T[4];
+  double c_s[2], sig_s_q[2], sig_s_qp[2], c_t[2], sig_t_q[2], sig_t_qp[2];
+  double p, q;
+  double filt[27], filt_t;
+} mbir_hip_pc_prior;
+
+int mbir_hip_pc_enable(mbir_hip_ctx* ctx, const mbir_hip_pc_prior* prior);
+/* Objects [N_time][N_z][N_y][N_x]. */
+int mbir_hip_pc_set_objects(mbir_hip_ctx* ctx, const float* delta, const float* beta);
+int mbir_hip_pc_get_objects(mbir_hip_ctx* ctx, float* delta, float* beta);
+/* e = target - k A x of both objects (internal layout [N_p][N_r][N_t]),
+ * FP64 accumulation rounded once. */
+int mbir_hip_pc_init_error(mbir_hip_ctx* ctx, const float* target_delta,
+                           const float* target_beta);
+int mbir_hip_pc_get_error(mbir_hip_ctx* ctx, float* e_delta, float* e_beta);
+/* n equits (iterations first_iter ..); stats (may be NULL) receives
+ * [sum |dx| over both objects, voxels updated, voxels skipped]. */
+int mbir_hip_pc_equits(mbir_hip_ctx* ctx, int32_t n, int32_t first_iter, uint32_t seed,
+                       double* stats);
+/* FP64 cost: out = [total, mu/2 sum(e_delta^2 + e_beta^2), prior] (the
+ * reference's computeCost squares mu e, PCMBIR_3D/XT_ICD_update.c:92-102;
+ * the same at mu = 1). */
+int mbir_hip_pc_cost(mbir_hip_ctx* ctx, double* out);
+
 /* Device time of the kernels of one id since the last reset, from HIP
  * events recorded on the context's stream around every launch. */
 int mbir_hip_kernel_time(mbir_hip_ctx* ctx, int32_t kernel_id, double* ms,

@@ -1249,6 +1249,272 @@ void orc_icd_equit_async(const orc_geom* g, const double* cosv,
   }
 }
 
+/* ------------------------------ PCMBIR tomographic ICD (row a14), FP64
+ * The tomographic part of PCMBIR_3D, compiled out of the shipped reference
+ * (ENABLE_TOMO_RECONS is never defined): two objects, delta (PhaseObject)
+ * and beta (MagObject), on the MBIR A rule times the wavenumber
+ * (PCMBIR_3D/XT_AMatrix.c:103), each with its own error sinogram, coupled
+ * through the decorrelating transform of the prior. */
+
+/* b(d) = rho'(d) / d of one qGGMRF potential, rho''(0) at d = 0
+ * (PCMBIR_3D/XT_Prior.c:27-118). */
+static double orc_pc_b(double d, double c, double sq, double sqp, double p, double q) {
+  if (d == 0) return q / (sq * c);
+  double ratio = pow(fabs(d), q - p) / sqp;
+  double den = c + ratio;
+  double mag = pow(fabs(d), q - 1);
+  double deriv = (mag / (den * sq)) * (q - ((q - p) * ratio) / den);
+  return (d < 0 ? -deriv : deriv) / d;
+}
+
+static double orc_pc_rho(double d, double c, double sq, double sqp, double p, double q) {
+  double ad = fabs(d);
+  return (pow(ad, q) / sq) / (c + pow(ad, q - p) / sqp);
+}
+
+/* Accumulates one neighbour's terms of the coupled surrogate
+ * (XT_Prior.c:148-176): with D = (Vd - nd, Vb - nb), the two decorrelated
+ * differences T0 D (phase prior) and T1 D (magnitude prior) give the
+ * weights b0, b1; Q = T^T diag(b0, b1) T; u += w Q (nd, nb), M += w Q. */
+static void orc_pc_pair(const orc_pc_prior* P, int temporal, double w, double Vd, double Vb,
+                        double nd, double nb, double u[2], double M[4]) {
+  const double* T = P->T;
+  double Dd = Vd - nd, Db = Vb - nb;
+  double d0 = T[0] * Dd + T[1] * Db, d1 = T[2] * Dd + T[3] * Db;
+  double b0 = temporal ? orc_pc_b(d0, P->c_t[0], P->sq_t[0], P->sqp_t[0], P->p, P->q)
+                       : orc_pc_b(d0, P->c_s[0], P->sq_s[0], P->sqp_s[0], P->p, P->q);
+  double b1 = temporal ? orc_pc_b(d1, P->c_t[1], P->sq_t[1], P->sqp_t[1], P->p, P->q)
+                       : orc_pc_b(d1, P->c_s[1], P->sq_s[1], P->sqp_s[1], P->p, P->q);
+  /* Q[i][j] = T[0][i] b0 T[0][j] + T[1][i] b1 T[1][j] */
+  double Q[4];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) Q[i * 2 + j] = T[i] * b0 * T[j] + T[2 + i] * b1 * T[2 + j];
+  u[0] += w * (Q[0] * nd + Q[1] * nb);
+  u[1] += w * (Q[2] * nd + Q[3] * nb);
+  for (int k = 0; k < 4; ++k) M[k] += w * Q[k];
+}
+
+void orc_pc_substitution(const orc_pc_prior* P, double* Vd, double* Vb, double th1d,
+                         double th1b, double th2d, double th2b, const double* nbd,
+                         const double* nbb, const uint8_t* nbf, const double* tnd,
+                         const double* tnb, const uint8_t* tf) {
+  /* FunctionalSubstitution (XT_Prior.c:137-239): the 2x2 system
+   * (diag(theta2) + M) v = u + theta2 V - theta1, then over-relaxation */
+  double u[2] = {0, 0}, M[4] = {0, 0, 0, 0};
+  const double V0 = *Vd, V1 = *Vb;
+  for (int k = 0; k < 27; ++k)
+    if (k != 13 && nbf[k]) orc_pc_pair(P, 0, P->filt[k], V0, V1, nbd[k], nbb[k], u, M);
+  for (int k = 0; k < 2; ++k)
+    if (tf[k]) orc_pc_pair(P, 1, P->filt_t, V0, V1, tnd[k], tnb[k], u, M);
+  double a = th2d + M[0], b = M[1], c = M[2], d = th2b + M[3];
+  double inv = 1.0 / (a * d - c * b);
+  double r0 = u[0] + th2d * V0 - th1d, r1 = u[1] + th2b * V1 - th1b;
+  double nd = inv * (d * r0 - b * r1), nb = inv * (-c * r0 + a * r1);
+  *Vd = V0 + P->alpha * (nd - V0);
+  *Vb = V1 + P->alpha * (nb - V1);
+}
+
+double orc_pc_cost(const orc_geom* g, const orc_pc_prior* P, const double* ed,
+                   const double* eb, const double* xd, const double* xb, double* out) {
+  /* mu/2 sum(e_d^2 + e_b^2), the objective the ICD minimises (the
+   * reference's computeCost squares mu e, XT_ICD_update.c:92-102, which
+   * agrees at mu = 1), plus the decorrelated pair potentials of
+   * XT_ICD_update.c:103-270 over the 13 forward spatial and the (t, t+1)
+   * pairs */
+  const size_t ns = (size_t)g->N_p * g->N_r * g->N_t;
+  double data = 0;
+  for (size_t i = 0; i < ns; ++i) data += ed[i] * ed[i] + eb[i] * eb[i];
+  data *= P->mu / 2;
+  const int32_t Nx = g->N_x, Ny = g->N_y, Nz = g->N_z, Nf = g->N_time;
+  double prior = 0;
+  for (int32_t f = 0; f < Nf; ++f)
+    for (int32_t z = 0; z < Nz; ++z)
+      for (int32_t yy = 0; yy < Ny; ++yy)
+        for (int32_t xx = 0; xx < Nx; ++xx) {
+          size_t o = (((size_t)f * Nz + z) * Ny + yy) * Nx + xx;
+          for (int k = 14; k < 27; ++k) {
+            int dz = k / 9 - 1, dy = (k / 3) % 3 - 1, dx = k % 3 - 1;
+            int32_t z2 = z + dz, y2 = yy + dy, x2 = xx + dx;
+            if (z2 < 0 || z2 >= Nz || y2 < 0 || y2 >= Ny || x2 < 0 || x2 >= Nx) continue;
+            size_t o2 = (((size_t)f * Nz + z2) * Ny + y2) * Nx + x2;
+            double Dd = xd[o] - xd[o2], Db = xb[o] - xb[o2];
+            prior += P->filt[k] *
+                     (orc_pc_rho(P->T[0] * Dd + P->T[1] * Db, P->c_s[0], P->sq_s[0], P->sqp_s[0], P->p, P->q) +
+                      orc_pc_rho(P->T[2] * Dd + P->T[3] * Db, P->c_s[1], P->sq_s[1], P->sqp_s[1], P->p, P->q));
+          }
+          if (f + 1 < Nf && P->filt_t != 0) {
+            size_t o2 = o + (size_t)Nz * Ny * Nx;
+            double Dd = xd[o] - xd[o2], Db = xb[o] - xb[o2];
+            prior += P->filt_t *
+                     (orc_pc_rho(P->T[0] * Dd + P->T[1] * Db, P->c_t[0], P->sq_t[0], P->sqp_t[0], P->p, P->q) +
+                      orc_pc_rho(P->T[2] * Dd + P->T[3] * Db, P->c_t[1], P->sq_t[1], P->sqp_t[1], P->p, P->q));
+          }
+        }
+  if (out) {
+    out[0] = data;
+    out[1] = prior;
+  }
+  return data + prior;
+}
+
+void orc_pc_equit(const orc_geom* g, const double* cosv, const double* sinv, const float* H,
+                  const int32_t* frame_ptr, const int32_t* frame_views, const uint8_t* mask,
+                  const orc_pc_prior* P, int32_t T, uint32_t seed, int32_t iter,
+                  int32_t call_index, double* xd, double* xb, double* ed, double* eb,
+                  double* stats) {
+  /* One equit of the GPU sweep's schedule (tiles, colours, frame parity,
+   * hashed column order, two z-parity halves per column), run serially:
+   * the voxel update of PCMBIR_3D/XT_VoxUpdate.c:23-80 (theta of both error
+   * sinograms over the column, theta2 shared), the neighbourhood and zero
+   * skipping of XT_VoxUpdate.c:84-230 and the coupled substitution.  Each
+   * launch's items touch disjoint views and no common neighbours, so this
+   * order is the deterministic GPU sweep's when every launch holds one item
+   * per frame. */
+  const int32_t Nx = g->N_x, Ny = g->N_y, Nz = g->N_z, Nf = g->N_time;
+  const int32_t TX = (Nx + T - 1) / T, TY = (Ny + T - 1) / T, cap = g->W, zo = g->mult_z;
+  const size_t vol = (size_t)Nz * Ny * Nx;
+  const double kw = P->wavenumber, mu = P->mu;
+  int32_t nvmax = 0;
+  for (int32_t f = 0; f < Nf; ++f)
+    if (frame_ptr[f + 1] - frame_ptr[f] > nvmax) nvmax = frame_ptr[f + 1] - frame_ptr[f];
+  int32_t* tptr = (int32_t*)calloc((size_t)TX * TY + 1, sizeof(int32_t));
+  int32_t* tcols = (int32_t*)malloc(sizeof(int32_t) * ((size_t)Nx * Ny + 1));
+  int32_t* order = (int32_t*)malloc(sizeof(int32_t) * ((size_t)TX * TY + 1));
+  orc_item* items = (orc_item*)malloc(sizeof(orc_item) * ((size_t)TX * TY * Nf + 1));
+  int32_t* cnt = (int32_t*)calloc(nvmax + 1, sizeof(int32_t));
+  int32_t* cidx = (int32_t*)calloc((size_t)(nvmax + 1) * cap, sizeof(int32_t));
+  double* cval = (double*)calloc((size_t)(nvmax + 1) * cap, sizeof(double));
+  double* t1d = (double*)malloc(sizeof(double) * Nz);
+  double* t1b = (double*)malloc(sizeof(double) * Nz);
+  int32_t off[5], nt = 0, nc = 0;
+  double dsum = 0, skipped = 0;
+  for (int32_t ty = 0; ty < TY; ++ty)
+    for (int32_t tx = 0; tx < TX; ++tx) {
+      for (int32_t yy = ty * T; yy < ty * T + T && yy < Ny; ++yy)
+        for (int32_t xx = tx * T; xx < tx * T + T && xx < Nx; ++xx)
+          if (mask[yy * Nx + xx]) tcols[nc++] = yy * Nx + xx;
+      tptr[ty * TX + tx + 1] = nc;
+    }
+  for (int colour = 0; colour < 4; ++colour) {
+    off[colour] = nt;
+    for (int32_t ty = 0; ty < TY; ++ty)
+      for (int32_t tx = 0; tx < TX; ++tx) {
+        int32_t id = ty * TX + tx;
+        if ((tx % 2) + 2 * (ty % 2) == colour && tptr[id + 1] > tptr[id]) order[nt++] = id;
+      }
+  }
+  off[4] = nt;
+  for (int colour = 0; colour < 4; ++colour)
+    for (int fpar = 0; fpar < 2; ++fpar) {
+      const int32_t ntl = off[colour + 1] - off[colour], nfl = (Nf - fpar + 1) / 2;
+      if (ntl == 0 || nfl <= 0) continue;
+      const int32_t ni = orc_place(0, 8, ntl, nfl, items);
+      const uint32_t lseed = seed * 2654435761U +
+                             (uint32_t)(call_index * 8 + colour * 2 + fpar) * 40503U;
+      for (int32_t i = 0; i < ni; ++i) {
+        const int32_t tile = order[off[colour] + items[i].tl];
+        const int32_t f = fpar + 2 * items[i].fl;
+        const int32_t n = tptr[tile + 1] - tptr[tile];
+        const uint32_t h = orc_hash(lseed ^ orc_hash((uint32_t)tile * 131071U + (uint32_t)f * 524287U));
+        int32_t pa = 1, pb = 0;
+        if (n > 1) {
+          pa = 1 + (int32_t)(h % (uint32_t)(n - 1));
+          while (orc_gcd(pa, n) != 1) pa = pa % (n - 1) + 1;
+          pb = (int32_t)(orc_hash(h + 0x9e3779b9U) % (uint32_t)n);
+        }
+        const int32_t s0 = frame_ptr[f], nv = frame_ptr[f + 1] - s0;
+        for (int32_t k = 0; k < n; ++k) {
+          const int32_t c = tcols[tptr[tile] + (int32_t)(((int64_t)pb + (int64_t)k * pa) % n)];
+          const int32_t yy = c / Nx, xx = c % Nx;
+          orc_line_columns(g, cosv, sinv, H, f, yy, xx, frame_ptr, frame_views, cnt, cidx, cval);
+          double th2 = 0;
+          for (int32_t z = 0; z < Nz; ++z) t1d[z] = t1b[z] = 0;
+          for (int32_t si = 0; si < nv; ++si) {
+            const int32_t pv = frame_views[s0 + si];
+            for (int32_t m = 0; m < cnt[si]; ++m) {
+              const double a = cval[si * cap + m] * kw;
+              const int32_t r = cidx[si * cap + m];
+              th2 += mu * a * a * zo;
+              for (int32_t z = 0; z < Nz; ++z)
+                for (int32_t kk = 0; kk < zo; ++kk) {
+                  const size_t o = ((size_t)pv * g->N_r + r) * g->N_t + z * zo + kk;
+                  t1d[z] -= ed[o] * a * mu;
+                  t1b[z] -= eb[o] * a * mu;
+                }
+            }
+          }
+          for (int par = 0; par < 2; ++par)
+            for (int32_t z = par; z < Nz; z += 2) {
+              const size_t o = (((size_t)f * Nz + z) * Ny + yy) * Nx + xx;
+              double nbd[27], nbb[27], tnd[2], tnb[2];
+              uint8_t nbf[27], tf[2];
+              int any = 0;
+              for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy)
+                  for (int dx = -1; dx <= 1; ++dx) {
+                    const int kq = ((dz + 1) * 3 + dy + 1) * 3 + dx + 1;
+                    const int32_t z2 = z + dz, y2 = yy + dy, x2 = xx + dx;
+                    const int in = z2 >= 0 && z2 < Nz && y2 >= 0 && y2 < Ny && x2 >= 0 && x2 < Nx;
+                    const size_t o2 = (((size_t)f * Nz + z2) * Ny + y2) * Nx + x2;
+                    nbd[kq] = in ? xd[o2] : 0;
+                    nbb[kq] = in ? xb[o2] : 0;
+                    nbf[kq] = (uint8_t)in;
+                  }
+              nbd[13] = nbb[13] = 0;
+              tf[0] = f > 0;
+              tf[1] = f + 1 < Nf;
+              tnd[0] = tf[0] ? xd[o - vol] : 0;
+              tnb[0] = tf[0] ? xb[o - vol] : 0;
+              tnd[1] = tf[1] ? xd[o + vol] : 0;
+              tnb[1] = tf[1] ? xb[o + vol] : 0;
+              const double Vd = xd[o], Vb = xb[o];
+              /* zero skipping (XT_VoxUpdate.c:208-228) */
+              if (Vd == 0.0 && Vb == 0.0 && iter > 1) {
+                any = tnd[0] > 0 || tnd[1] > 0 || tnb[0] > 0 || tnb[1] > 0;
+                for (int kq = 0; kq < 27 && !any; ++kq)
+                  if (nbd[kq] > 0 || nbb[kq] > 0) any = 1;
+                if (!any) {
+                  skipped += 1;
+                  continue;
+                }
+              }
+              double nd = Vd, nb = Vb;
+              orc_pc_substitution(P, &nd, &nb, t1d[z], t1b[z], th2, th2, nbd, nbb, nbf, tnd, tnb, tf);
+              xd[o] = nd;
+              xb[o] = nb;
+              dsum += sqrt((nd - Vd) * (nd - Vd) + (nb - Vb) * (nb - Vb));
+              for (int32_t si = 0; si < nv; ++si) {
+                const int32_t pv = frame_views[s0 + si];
+                for (int32_t m = 0; m < cnt[si]; ++m) {
+                  const double a = cval[si * cap + m] * kw;
+                  const int32_t r = cidx[si * cap + m];
+                  for (int32_t kk = 0; kk < zo; ++kk) {
+                    const size_t oe = ((size_t)pv * g->N_r + r) * g->N_t + z * zo + kk;
+                    ed[oe] -= a * (nd - Vd);
+                    eb[oe] -= a * (nb - Vb);
+                  }
+                }
+              }
+            }
+        }
+      }
+    }
+  free(tptr);
+  free(tcols);
+  free(order);
+  free(items);
+  free(cnt);
+  free(cidx);
+  free(cval);
+  free(t1d);
+  free(t1b);
+  if (stats) {
+    stats[0] = dsum;
+    stats[1] = 0;
+    stats[2] = skipped;
+  }
+}
+
 void orc_sino_from_file(int32_t N_p, int32_t N_t, int32_t N_r,
                         const float* in, double* out) {
   /* XT_Init.c:294-301: file index p*N_t*N_r + t*N_r + r */

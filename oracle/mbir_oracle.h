@@ -201,6 +201,33 @@ void orc_icd_equit_async(const orc_geom* g, const double* cosv,
                          double* x, double* e, const double* w,
                          double* stats);
 
+/* PCMBIR tomographic ICD (row a14; PCMBIR_3D code compiled out of the
+ * shipped reference).  Index 0 of the paired arrays is the phase (delta)
+ * prior, 1 the magnitude (beta) prior; T is DecorrTran row-major. */
+typedef struct {
+  double mu, wavenumber;
+  double T[4];
+  double c_s[2], sq_s[2], sqp_s[2], c_t[2], sq_t[2], sqp_t[2];
+  double p, q;
+  double filt[27], filt_t;
+  double alpha;
+} orc_pc_prior;
+
+/* FunctionalSubstitution (PCMBIR_3D/XT_Prior.c:137-239): updates *Vd, *Vb. */
+void orc_pc_substitution(const orc_pc_prior* P, double* Vd, double* Vb, double th1d,
+                         double th1b, double th2d, double th2b, const double* nbd,
+                         const double* nbb, const uint8_t* nbf, const double* tnd,
+                         const double* tnb, const uint8_t* tf);
+/* mu/2 sum(e_d^2 + e_b^2) + the decorrelated prior; out = [data, prior]. */
+double orc_pc_cost(const orc_geom* g, const orc_pc_prior* P, const double* ed,
+                   const double* eb, const double* xd, const double* xb, double* out);
+/* One equit of the GPU sweep's order, serial (PCMBIR_3D/XT_VoxUpdate.c). */
+void orc_pc_equit(const orc_geom* g, const double* cosv, const double* sinv, const float* H,
+                  const int32_t* frame_ptr, const int32_t* frame_views, const uint8_t* mask,
+                  const orc_pc_prior* P, int32_t T, uint32_t seed, int32_t iter,
+                  int32_t call_index, double* xd, double* xb, double* ed, double* eb,
+                  double* stats);
+
 /* File-layout <-> internal sinogram transpose (XT_Init.c:294-301):
  * file [p][t][r] -> internal [p][r][t]. */
 void orc_sino_from_file(int32_t N_p, int32_t N_t, int32_t N_r,

@@ -40,6 +40,42 @@ class Geom(C.Structure):
          ("huber_T", C.c_double), ("huber_delta", C.c_double), ("var_est", C.c_double)]
 
 
+class PcPrior(C.Structure):
+    """mirror of orc_pc_prior (oracle/mbir_oracle.h)"""
+    _fields_ = [("mu", C.c_double), ("wavenumber", C.c_double), ("T", C.c_double * 4)] + \
+        [(n, C.c_double * 2) for n in ("c_s", "sq_s", "sqp_s", "c_t", "sq_t", "sqp_t")] + \
+        [("p", C.c_double), ("q", C.c_double), ("filt", C.c_double * 27), ("filt_t", C.c_double),
+         ("alpha", C.c_double)]
+
+
+def pc_prior(phase, mag, mult_xy, delta_recon, wavenumber, delta_over_beta=None, mu=1.0,
+             alpha=1.5, p=1.2, q=2.0):
+    """The PCMBIR tomographic prior and data weights of one stage
+    (PCMBIR_3D/XT_Init.c:144-176, 234-246, 482-502): phase / mag = (sig_s,
+    sig_t, c_s, c_t); sigmas scaled by mult_xy (spatial) and delta_recon
+    (temporal); the 26-neighbour filter 1/distance normalised to sum 1 and
+    the temporal filter 0 (initFilter); DecorrTran [[1, d/b], [1, -d/b]]
+    with a delta_over_beta (recon_type 2), else the identity."""
+    P = PcPrior()
+    P.mu, P.wavenumber, P.p, P.q, P.alpha = mu, wavenumber, p, q, alpha
+    T = [1.0, 0.0, 0.0, 1.0] if delta_over_beta is None else \
+        [1.0, delta_over_beta, 1.0, -delta_over_beta]
+    for i, v in enumerate(T):
+        P.T[i] = v
+    for k, (ss, st, cs, ct) in enumerate((phase, mag)):
+        P.c_s[k], P.c_t[k] = cs, ct
+        P.sq_s[k], P.sqp_s[k] = (ss * mult_xy) ** q, (ss * mult_xy) ** (q - p)
+        P.sq_t[k], P.sqp_t[k] = (st * delta_recon) ** q, (st * delta_recon) ** (q - p)
+    d = np.array([[[np.sqrt((i - 1.0) ** 2 + (j - 1.0) ** 2 + (k - 1.0) ** 2)
+                    for k in range(3)] for j in range(3)] for i in range(3)]).ravel()
+    f = np.where(d > 0, 1.0 / np.where(d > 0, d, 1.0), 0.0)
+    f /= f.sum()
+    for k in range(27):
+        P.filt[k] = f[k]
+    P.filt_t = 0.0
+    return P
+
+
 class AsyncOpts(C.Structure):
     _fields_ = [("T", C.c_int32), ("alpha", C.c_double), ("seed", C.c_uint32),
                 ("iter", C.c_int32), ("call_index", C.c_int32), ("lag_same", C.c_int32),
@@ -202,6 +238,47 @@ class Oracle:
                                    self._p(self.H), self._p(self.frame_ptr),
                                    self._p(self.frame_views), self._p(self.mask), C.byref(op),
                                    self._p(x), self._p(e), self._p(self.w), self._p(stats))
+        return stats
+
+    # --- PCMBIR tomographic ICD (row a14)
+    def pc_substitution(self, P, Vd, Vb, th1d, th1b, th2d, th2b, nbd, nbb, nbf, tnd, tnb, tf):
+        """orc_pc_substitution on one voxel; returns (delta, beta)."""
+        vd, vb = C.c_double(Vd), C.c_double(Vb)
+        arr = lambda a, t=np.float64: np.ascontiguousarray(a, t).ravel()
+        a = [arr(nbd), arr(nbb), arr(nbf, np.uint8), arr(tnd), arr(tnb), arr(tf, np.uint8)]
+        self.L.orc_pc_substitution(C.byref(P), C.byref(vd), C.byref(vb), C.c_double(th1d),
+                                   C.c_double(th1b), C.c_double(th2d), C.c_double(th2b),
+                                   *[self._p(v) for v in a])
+        return vd.value, vb.value
+
+    def pc_init_error(self, P, target_d, target_b, xd, xb):
+        """Error sinograms e = target - k A x of both objects (internal
+        layout [N_p][N_r][N_t])."""
+        k = P.wavenumber
+        return (np.asarray(target_d, np.float64) - k * self.forward_project(xd),
+                np.asarray(target_b, np.float64) - k * self.forward_project(xb))
+
+    def pc_cost(self, P, ed, eb, xd, xb):
+        out = np.zeros(2, np.float64)
+        c = [np.ascontiguousarray(a, np.float64) for a in (ed, eb, xd, xb)]
+        self.L.orc_pc_cost.restype = C.c_double
+        tot = self.L.orc_pc_cost(C.byref(self.g), C.byref(P), *[self._p(v) for v in c],
+                                 self._p(out))
+        return tot, out
+
+    def pc_equit(self, P, xd, xb, ed, eb, iteration, T, seed, call_index=0, mask=None):
+        """One equit of the GPU sweep's order on both objects (float64,
+        updated in place); mask None: every column (PCMBIR updates all
+        voxels, Q18).  Returns stats (sum |dx|, 0, skipped)."""
+        for a in (xd, xb, ed, eb):
+            assert a.dtype == np.float64 and a.flags.c_contiguous
+        m = np.ones_like(self.mask) if mask is None else np.ascontiguousarray(mask, np.uint8)
+        stats = np.zeros(3, np.float64)
+        self.L.orc_pc_equit(C.byref(self.g), self._p(self.cos), self._p(self.sin),
+                            self._p(self.H), self._p(self.frame_ptr), self._p(self.frame_views),
+                            self._p(m), C.byref(P), T, C.c_uint32(seed & 0xFFFFFFFF), iteration,
+                            call_index, self._p(xd), self._p(xb), self._p(ed), self._p(eb),
+                            self._p(stats))
         return stats
 
     def srandom2(self, seed):

@@ -239,13 +239,17 @@ class NmsPrior:
         return (ad * ad / sq) / (c + ad ** (2 - self.p) / sqp)
 
 
-def cost_nms(zr, zi, br, bi, nu, zr_nb, zi_nb, flags, pr):
-    """Cost_NMS without ENABLE_TOMO_RECONS (XT_Search.c:8-43, Q12):
-    nu/2 |exp(-z) - b|^2 plus the decorrelated qGGMRF terms of the 3x3
-    neighbours."""
+def cost_nms(zr, zi, br, bi, nu, zr_nb, zi_nb, flags, pr, tomo=None):
+    """Cost_NMS (XT_Search.c:8-43): nu/2 |exp(-z) - b|^2 plus, without
+    ENABLE_TOMO_RECONS (Q12), the decorrelated qGGMRF terms of the 3x3
+    neighbours, or with it (tomo = (mu, ar, ai), XT_Search.c:18-19) the
+    proximal term mu/2 |z - a|^2 towards the tomographic projection."""
     e = np.exp(-zr)
     cv, sv = e * np.cos(-zi), e * np.sin(-zi)
     cost = ((cv - br) * (cv - br) + (sv - bi) * (sv - bi)) * nu / 2
+    if tomo is not None:
+        mu, ar, ai = tomo
+        return cost + ((zr - ar) * (zr - ar) + (zi - ai) * (zi - ai)) * mu / 2
     T = pr.T
     for i in range(3):
         for j in range(3):
@@ -261,11 +265,11 @@ NMS_DEFAULTS = dict(rho=1.0, chi=2.0, gamma=0.5, sigma=0.5, thresh=0.005, max_it
 
 
 def nms_search(zr1, zi1, br, bi, nu, zr_nb, zi_nb, flags, pr, rho=1.0, chi=2.0, gamma=0.5,
-               sigma=0.5, thresh=0.005, max_iter=50):
+               sigma=0.5, thresh=0.005, max_iter=50, tomo=None):
     """Nelder_Mead_Simplex_2DSearch (XT_Search.c:100-215, including its
     simplex initialisation (2 zr, zi), (zr, 2 zi)): returns the best vertex
     and the iteration count."""
-    f = lambda a, b: cost_nms(a, b, br, bi, nu, zr_nb, zi_nb, flags, pr)  # noqa: E731
+    f = lambda a, b: cost_nms(a, b, br, bi, nu, zr_nb, zi_nb, flags, pr, tomo)  # noqa: E731
     zr = [0.0, zr1, 2 * zr1, zr1]
     zi = [0.0, zi1, zi1, 2 * zi1]
     c = [0.0, f(zr[1], zi[1]), f(zr[2], zi[2]), f(zr[3], zi[3])]
@@ -361,12 +365,14 @@ def neighbours(zr, zi, j, k):
     return nr, ni, fl
 
 
-def nms_pass(zr, zi, b, nu, pr, order="raster", **kw):
+def nms_pass(zr, zi, b, nu, pr, order="raster", tomo=None, **kw):
     """One NMS pass over a view's pixels with b = w - v: the reference's raster
     order (XT_CmplxProjEst.c:118-170, each pixel sees its already-updated
     neighbours), or the four-colour order of the GPU kernel ((j%2, k%2) =
-    (0,0), (0,1), (1,0), (1,1), pixels of a colour independent).  Returns the
-    new (zr, zi) and the summed iteration counts."""
+    (0,0), (0,1), (1,0), (1,1), pixels of a colour independent).  tomo =
+    (mu, a), a complex [rows][cols] (real: magnitude, imag: phase): the
+    tomographic coupling of cost_nms.  Returns the new (zr, zi) and the summed
+    iteration counts."""
     zr, zi = zr.copy(), zi.copy()
     rows, cols = zr.shape
     if order == "raster":
@@ -377,8 +383,9 @@ def nms_pass(zr, zi, b, nu, pr, order="raster", **kw):
     total = 0
     for j, k in pix:
         nr, ni, fl = neighbours(zr, zi, j, k)
+        tk = None if tomo is None else (tomo[0], tomo[1][j, k].real, tomo[1][j, k].imag)
         zr[j, k], zi[j, k], it = nms_search(zr[j, k], zi[j, k], b[j, k].real, b[j, k].imag, nu,
-                                            nr, ni, fl, pr, **kw)
+                                            nr, ni, fl, pr, tomo=tk, **kw)
         total += it
     return zr, zi, total
 
@@ -389,13 +396,17 @@ def transmittance(zr, zi):
     return e * np.cos(-zi) + 1j * e * np.sin(-zi)
 
 
-def pret_cost(meas, D, Lam, zr, zi, pr, geo):
-    """compute_pretcost without ENABLE_TOMO_RECONS (XT_CmplxProjEst.c:15-103):
-    sum Lam (|y| - |H(D exp(-z))|)^2 / 2 over the real part of the
-    measurements, plus the decorrelated prior over the right, down,
-    down-right and down-left pairs."""
+def pret_cost(meas, D, Lam, zr, zi, pr, geo, tomo=None):
+    """compute_pretcost (XT_CmplxProjEst.c:15-103): sum Lam (|y| -
+    |H(D exp(-z))|)^2 / 2 over the real part of the measurements, plus
+    without ENABLE_TOMO_RECONS the decorrelated prior over the right, down,
+    down-right and down-left pairs, or with it (tomo = (mu, a),
+    XT_CmplxProjEst.c:38-41) sum mu/2 |z - a|^2."""
     h = fresnel(transmittance(zr, zi) * D, *geo)
     fwd = np.sum((meas.real - np.abs(h)) ** 2 * Lam / 2)
+    if tomo is not None:
+        mu, a = tomo
+        return fwd + np.sum(mu * ((zr - a.real) ** 2 + (zi - a.imag) ** 2) / 2)
     rows, cols = zr.shape
     T, f = pr.T, pr.filt
     prior = 0.0
@@ -419,22 +430,25 @@ def pret_cost(meas, D, Lam, zr, zi, pr, geo):
 
 
 def estimate_complex_projection(meas, om, D, zr, zi, Lam, w, pr, geo, nu, pret_iters,
-                                pret_thresh, sd_iters, sd_thresh, order="raster", nms_kw=None):
+                                pret_thresh, sd_iters, sd_thresh, order="raster", nms_kw=None,
+                                tomo=None):
     """One view's ADMM loop (estimate_complex_projection,
-    XT_CmplxProjEst.c:105-229, Q12: no tomographic coupling): NMS pass on z
-    with b = w - v, steepest descent on w towards exp(-z) + v, dual update
-    v += exp(-z) - w, residuals, pretcost, stop on the dual residual.
-    Returns a dict of the final state and the per-iteration histories."""
+    XT_CmplxProjEst.c:105-229): NMS pass on z with b = w - v, steepest
+    descent on w towards exp(-z) + v, dual update v += exp(-z) - w,
+    residuals, pretcost, stop on the dual residual.  tomo = (mu, a): the
+    ENABLE_TOMO_RECONS coupling of the NMS and of the pretcost to the
+    tomographic projection a (None: Q12, the shipped build).  Returns a dict
+    of the final state and the per-iteration histories."""
     nms_kw = nms_kw or {}
     v = np.zeros_like(w)
     pp = PhaseProblem(meas, D, Lam, None, nu, *geo)
-    costs = [pret_cost(meas, D, Lam, zr, zi, pr, geo)]
+    costs = [pret_cost(meas, D, Lam, zr, zi, pr, geo, tomo)]
     prim, dual, nms_its, sd_its = [], [], [], []
     it = 0
     w_prev, v_prev = w, v
     for it in range(pret_iters):
         w_prev, v_prev = w, v
-        zr, zi, nit = nms_pass(zr, zi, w - v, nu, pr, order=order, **nms_kw)
+        zr, zi, nit = nms_pass(zr, zi, w - v, nu, pr, order=order, tomo=tomo, **nms_kw)
         nms_its.append(nit)
         wold = w.copy()
         pp.v = transmittance(zr, zi) + v
@@ -446,7 +460,7 @@ def estimate_complex_projection(meas, om, D, zr, zi, Lam, w, pr, geo, nu, pret_i
         du_res = np.sqrt(np.sum(np.abs(w - wold) ** 2) / np.sum(np.abs(v) ** 2))
         prim.append(pr_res)
         dual.append(du_res)
-        costs.append(pret_cost(meas, D, Lam, zr, zi, pr, geo))
+        costs.append(pret_cost(meas, D, Lam, zr, zi, pr, geo, tomo))
         if du_res < pret_thresh and it > 1:
             it += 1
             break

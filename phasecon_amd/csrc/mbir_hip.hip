@@ -199,7 +199,7 @@ __global__ __launch_bounds__(256) void k_fwdproj(
     const int* __restrict__ view_frame, const float* __restrict__ x,
     const float* __restrict__ y, const float* __restrict__ w,
     const float* __restrict__ d, float* __restrict__ out,
-    double* __restrict__ partial) {
+    double* __restrict__ partial, double xscale) {
   extern __shared__ __align__(16) float smem[];
   Acc* acc = reinterpret_cast<Acc*>(smem);
   __shared__ float sh[kNB];
@@ -226,7 +226,7 @@ __global__ __launch_bounds__(256) void k_fwdproj(
         Acc* row = acc + (size_t)j * tc;
         for (int t = 0; t < tc; ++t) {
           const float xv = xc[(t0 + t) / G.zo];
-          if (xv != 0.f) atomicAdd(&row[t], (Acc)((double)xv * v));
+          if (xv != 0.f) atomicAdd(&row[t], (Acc)((double)xv * v * xscale));
         }
       });
     }
@@ -1362,6 +1362,337 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SwGeom G, SweepArgs A) 
   }
 }
 
+// ----------------------------------------------------------------------
+// PCMBIR tomographic ICD (the ENABLE_TOMO_RECONS path of PCMBIR_3D,
+// compiled out of the shipped reference): two objects, delta and beta, on
+// the A rule times the wavenumber (PCMBIR_3D/XT_AMatrix.c:103), each with its
+// error sinogram, updated together by the coupled substitution of
+// PCMBIR_3D/XT_Prior.c:137-239.  Index 0 of the paired constants is the
+// phase (delta) prior, 1 the magnitude (beta) prior.
+struct PcTab {
+  float T[4];                    // DecorrTran, row-major
+  float qmp;                     // q - p
+  float c[2][2], inv_sq[2][2], inv_sqp[2][2], b0[2][2];  // [spatial, temporal][phase, mag]
+  float filt[28];                // 27 spatial ([dz][dy][dx]) + temporal
+  float mu, kw, alpha;
+};
+
+struct PcArgs {
+  const float* aval;
+  const int* afirst;
+  const int* frame_views;
+  const int* frame_ptr;
+  const int* tile_ptr;
+  const int* tile_cols;
+  const int* launch_tiles;
+  int n_launch_tiles, tiles_x, tile, nvmax, fparity, iter, W;
+  unsigned seed;
+  float *xd, *xb, *ed, *eb;
+  double* stats;  // [sum |dx|, updated, skipped]
+};
+
+// Adds one neighbour's share of the coupled surrogate: D = V - n, the
+// decorrelated differences T0 D (phase prior) and T1 D (magnitude prior)
+// give b0, b1, and Q = T^T diag(b0, b1) T; u += w Q n, M += w Q.  Q and the
+// sums are FP64: with DecorrTran for delta/beta = d, T^T T = diag(2, 2 d^2)
+// has condition number d^2 (2500 at d = 50), which FP32 sums would lose.
+__device__ __forceinline__ void pc_pair(const PcTab& P, int kind, float w, float Vd, float Vb,
+                                        float nd, float nb, double u[2], double M[3]) {
+  const float Dd = Vd - nd, Db = Vb - nb;
+  const float d0 = P.T[0] * Dd + P.T[1] * Db, d1 = P.T[2] * Dd + P.T[3] * Db;
+  const double b0 = w * qg_b(d0, P.qmp, P.c[kind][0], P.inv_sq[kind][0], P.inv_sqp[kind][0],
+                             P.b0[kind][0]);
+  const double b1 = w * qg_b(d1, P.qmp, P.c[kind][1], P.inv_sq[kind][1], P.inv_sqp[kind][1],
+                             P.b0[kind][1]);
+  const double t0 = P.T[0], t1 = P.T[1], t2 = P.T[2], t3 = P.T[3];
+  const double q00 = t0 * b0 * t0 + t2 * b1 * t2;
+  const double q01 = t0 * b0 * t1 + t2 * b1 * t3;
+  const double q11 = t1 * b0 * t1 + t3 * b1 * t3;
+  u[0] += q00 * nd + q01 * nb;
+  u[1] += q01 * nd + q11 * nb;
+  M[0] += q00;
+  M[1] += q01;
+  M[2] += q11;
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// One pass of the tomographic ICD over the columns of the launch's tiles
+// (one 2x2 tile colour, the frames of one parity), one wavefront per (tile,
+// frame): per column the A window in LDS, theta2 = mu zo sum (k a)^2 (shared
+// by both objects) and theta1 = -mu sum e k a of each error sinogram per
+// voxel (PCMBIR_3D/XT_VoxUpdate.c:23-52), then the two z-parity halves of
+// coupled substitutions with zero skipping (XT_VoxUpdate.c:208-228), then
+// both error sinograms updated with atomics (XT_VoxUpdate.c:57-79).
+__global__ __launch_bounds__(kWave) void k_pc_sweep(SwGeom G, PcTab P, PcArgs A) {
+  extern __shared__ __align__(16) float smem[];
+  __shared__ float s_t1d[kNZMax], s_t1b[kNZMax], s_dxd[kNZMax], s_dxb[kNZMax];
+  __shared__ float s_tn[4][kNZMax];  // delta t-1, delta t+1, beta t-1, beta t+1
+  const int lane = threadIdx.x;
+  const int wid = blockIdx.x;
+  const int tile = A.launch_tiles[wid % A.n_launch_tiles];
+  const int f = A.fparity + 2 * (wid / A.n_launch_tiles);
+  if (f >= G.N_time) return;
+  const int c0 = A.tile_ptr[tile], n = A.tile_ptr[tile + 1] - c0;
+  const int s0 = A.frame_ptr[f], nv = A.frame_ptr[f + 1] - s0;
+  if (n == 0 || nv == 0) return;
+  const int Nz = G.N_z, T = A.tile, BW = T + 2, W = A.W;
+  const int nx = BW * BW * Nz;
+  float* sd = smem;
+  float* sb = sd + nx;
+  int* views = reinterpret_cast<int*>(sb + nx);
+  int* cols = views + A.nvmax;
+  int* wbase = cols + T * T;
+  float* wav = reinterpret_cast<float*>(wbase + A.nvmax);
+  const int oy = (tile / A.tiles_x) * T - 1, ox = (tile % A.tiles_x) * T - 1;
+  const size_t vol = (size_t)G.N_y * G.N_x * Nz;
+  float* xdf = A.xd + (size_t)f * vol;
+  float* xbf = A.xb + (size_t)f * vol;
+  for (int j = lane; j < nx; j += kWave) {
+    const int z = j % Nz, b = j / Nz;
+    const int yy = oy + b / BW, xx = ox + b % BW;
+    const bool in = yy >= 0 && yy < G.N_y && xx >= 0 && xx < G.N_x;
+    const size_t o = in ? ((size_t)yy * G.N_x + xx) * Nz + z : 0;
+    sd[j] = in ? xdf[o] : 0.f;
+    sb[j] = in ? xbf[o] : 0.f;
+  }
+  for (int j = lane; j < n; j += kWave) cols[j] = A.tile_cols[c0 + j];
+  for (int j = lane; j < nv; j += kWave) views[j] = A.frame_views[s0 + j];
+  // seeded visiting order k -> (pb + k pa) mod n, as k_sweep
+  const unsigned h = mix32(A.seed ^ mix32((unsigned)tile * 131071u + (unsigned)f * 524287u));
+  int pa = 1, pb = 0;
+  if (n > 1) {
+    pa = 1 + (int)(h % (unsigned)(n - 1));
+    while (gcd_i(pa, n) != 1) pa = pa % (n - 1) + 1;
+    pb = (int)(mix32(h + 0x9e3779b9u) % (unsigned)n);
+  }
+  __syncthreads();
+  const int Wd = G.zo;  // detector rows per voxel slice
+  float sdx = 0.f, supd = 0.f, sskip = 0.f;
+  int pos = pb;
+  for (int k = 0; k < n; ++k) {
+    const int c = cols[pos];
+    pos += pa;
+    if (pos >= n) pos -= n;
+    const int cy = c / G.N_x, cx = c % G.N_x;
+    const int ly = cy - oy, lx = cx - ox;
+    const int centre0 = (ly * BW + lx) * Nz;
+    // the column's A window (values times the wavenumber) and e offsets
+    for (int sl = lane; sl < nv; sl += kWave) {
+      const int first = A.afirst[(size_t)c * G.nslots + s0 + sl];
+      wbase[sl] = first < 0 ? -1 : (views[sl] * G.N_r + first) * G.N_t;
+    }
+    for (int i = lane; i < nv * W; i += kWave) {
+      const int sl = i / W, jj = i % W;
+      wav[i] = A.aval[((size_t)c * G.nslots + s0 + sl) * W + jj] * P.kw;
+    }
+    for (int z = lane; z < Nz; z += kWave) {
+      const size_t tc = (size_t)c * Nz + z;
+      s_tn[0][z] = f > 0 ? A.xd[(size_t)(f - 1) * vol + tc] : 0.f;
+      s_tn[1][z] = f + 1 < G.N_time ? A.xd[(size_t)(f + 1) * vol + tc] : 0.f;
+      s_tn[2][z] = f > 0 ? A.xb[(size_t)(f - 1) * vol + tc] : 0.f;
+      s_tn[3][z] = f + 1 < G.N_time ? A.xb[(size_t)(f + 1) * vol + tc] : 0.f;
+    }
+    __syncthreads();
+    // theta2, shared by all voxels of the column and both objects
+    float t2 = 0.f;
+    for (int i = lane; i < nv * W; i += kWave) {
+      const float a = wbase[i / W] >= 0 ? wav[i] : 0.f;
+      t2 += a * a;
+    }
+    const float th2 = P.mu * (float)Wd * wave_sum(t2);
+    // theta1 of each voxel from both error sinograms
+    const int E = nv * W * Wd;
+    for (int z = 0; z < Nz; ++z) {
+      float a1 = 0.f, a2 = 0.f;
+      for (int i = lane; i < E; i += kWave) {
+        const int sl = i / (W * Wd), rem = i % (W * Wd);
+        const int jj = rem / Wd, r = rem % Wd;
+        const int base = wbase[sl];
+        const float a = base >= 0 ? wav[sl * W + jj] : 0.f;
+        if (a != 0.f) {
+          const int o = base + jj * G.N_t + z * Wd + r;
+          a1 -= A.ed[o] * a;
+          a2 -= A.eb[o] * a;
+        }
+      }
+      a1 = wave_sum(a1);
+      a2 = wave_sum(a2);
+      if (lane == 0) {
+        s_t1d[z] = P.mu * a1;
+        s_t1b[z] = P.mu * a2;
+      }
+    }
+    __syncthreads();
+    // the two z-parity halves
+    for (int par = 0; par < 2; ++par) {
+      const int z = par + 2 * lane;
+      if (z < Nz) {
+        const int ctr = centre0 + z;
+        const float Vd = sd[ctr], Vb = sb[ctr];
+        double u[2] = {0.0, 0.0}, M[3] = {0.0, 0.0, 0.0};
+        bool pos_nb = false;
+        for (int j = 0; j < 27; ++j) {
+          if (j == 13) continue;
+          const int dz = j / 9 - 1, dy = (j / 3) % 3 - 1, dx = j % 3 - 1;
+          const bool ok = z + dz >= 0 && z + dz < Nz && cy + dy >= 0 && cy + dy < G.N_y &&
+                          cx + dx >= 0 && cx + dx < G.N_x;
+          if (!ok) continue;
+          const int o = ctr + (dy * BW + dx) * Nz + dz;
+          const float nd = sd[o], nb = sb[o];
+          pos_nb |= nd > 0.f || nb > 0.f;
+          pc_pair(P, 0, P.filt[j], Vd, Vb, nd, nb, u, M);
+        }
+        for (int t = 0; t < 2; ++t) {
+          const bool ok = t == 0 ? f > 0 : f + 1 < G.N_time;
+          if (!ok) continue;
+          const float nd = s_tn[t][z], nb = s_tn[2 + t][z];
+          pos_nb |= nd > 0.f || nb > 0.f;
+          if (P.filt[27] != 0.f) pc_pair(P, 1, P.filt[27], Vd, Vb, nd, nb, u, M);
+        }
+        const bool skip = Vd == 0.f && Vb == 0.f && A.iter > 1 && !pos_nb;
+        float nd = Vd, nb = Vb;
+        if (!skip) {
+          // the 2x2 solve (th2 I + M) v = u + th2 V - th1 in FP64
+          const double a = th2 + M[0], b = M[1], d = th2 + M[2];
+          const double inv = 1.0 / (a * d - b * b);
+          const double r0 = u[0] + (double)th2 * Vd - s_t1d[z];
+          const double r1 = u[1] + (double)th2 * Vb - s_t1b[z];
+          const double vd = inv * (d * r0 - b * r1), vb = inv * (a * r1 - b * r0);
+          nd = (float)(Vd + P.alpha * (vd - Vd));
+          nb = (float)(Vb + P.alpha * (vb - Vb));
+        }
+        sd[ctr] = nd;
+        sb[ctr] = nb;
+        s_dxd[z] = nd - Vd;
+        s_dxb[z] = nb - Vb;
+        sdx += sqrtf((nd - Vd) * (nd - Vd) + (nb - Vb) * (nb - Vb));
+        supd += skip ? 0.f : 1.f;
+        sskip += skip ? 1.f : 0.f;
+      }
+      __syncthreads();
+    }
+    // both error sinograms: e -= k a dx along the column
+    for (int i = lane; i < Nz * E; i += kWave) {
+      const int z = i / E, e = i % E;
+      const float dd = s_dxd[z], db = s_dxb[z];
+      if (dd == 0.f && db == 0.f) continue;
+      const int sl = e / (W * Wd), rem = e % (W * Wd);
+      const int jj = rem / Wd, r = rem % Wd;
+      const int base = wbase[sl];
+      const float a = base >= 0 ? wav[sl * W + jj] : 0.f;
+      if (a == 0.f) continue;
+      const int o = base + jj * G.N_t + z * Wd + r;
+      if (dd != 0.f) e_add(A.ed + o, -a * dd);
+      if (db != 0.f) e_add(A.eb + o, -a * db);
+    }
+    __syncthreads();
+  }
+  // the tile back (interior of the box, inside the object)
+  for (int j = lane; j < nx; j += kWave) {
+    const int z = j % Nz, b = j / Nz;
+    const int by = b / BW, bx = b % BW;
+    if (by == 0 || bx == 0 || by == BW - 1 || bx == BW - 1) continue;
+    const int yy = oy + by, xx = ox + bx;
+    if (yy < G.N_y && xx < G.N_x) {
+      const size_t o = ((size_t)yy * G.N_x + xx) * Nz + z;
+      xdf[o] = sd[j];
+      xbf[o] = sb[j];
+    }
+  }
+  sdx = wave_sum(sdx);
+  supd = wave_sum(supd);
+  sskip = wave_sum(sskip);
+  if (lane == 0) {
+    atomicAdd(&A.stats[0], (double)sdx);
+    atomicAdd(&A.stats[1], (double)supd);
+    atomicAdd(&A.stats[2], (double)sskip);
+  }
+}
+
+// FP64 constants of the tomographic cost.
+struct PcDbl {
+  double T[4], c[2][2], sq[2][2], sqp[2][2], p, q, filt[28], mu;
+};
+
+__device__ __forceinline__ double pc_rho(double d, double c, double sq, double sqp, double p,
+                                         double q) {
+  const double a = fabs(d);
+  return (pow(a, q) / sq) / (c + pow(a, q - p) / sqp);
+}
+
+// Prior term of the tomographic cost: over each voxel's 13 forward spatial
+// pairs and its (t, t+1) pair, filter weight times rho_phase(T0 D) +
+// rho_mag(T1 D), D the pair's (delta, beta) difference
+// (PCMBIR_3D/XT_ICD_update.c:103-270); objects column-major [t][col][z].
+__global__ __launch_bounds__(256) void k_pc_prior_cost(int Nt, int Ny, int Nx, int Nz, PcDbl D,
+                                                       const float* __restrict__ xd,
+                                                       const float* __restrict__ xb,
+                                                       double* __restrict__ partial) {
+  __shared__ double red[256];
+  const long long nv = (long long)Nt * Ny * Nx * Nz;
+  double acc = 0.0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < nv;
+       i += (long long)gridDim.x * 256) {
+    const int z = (int)(i % Nz);
+    const long long cz = i / Nz;
+    const int col = (int)(cz % ((long long)Ny * Nx));
+    const int t = (int)(cz / ((long long)Ny * Nx));
+    const int yy = col / Nx, xx = col % Nx;
+    const double vd = xd[i], vb = xb[i];
+    for (int k = 14; k < 28; ++k) {
+      long long o2;
+      int kind = 0;
+      if (k < 27) {
+        const int dz = k / 9 - 1, dy = (k / 3) % 3 - 1, dx = k % 3 - 1;
+        const int z2 = z + dz, y2 = yy + dy, x2 = xx + dx;
+        if (z2 < 0 || z2 >= Nz || y2 < 0 || y2 >= Ny || x2 < 0 || x2 >= Nx) continue;
+        o2 = (((long long)t * Ny + y2) * Nx + x2) * Nz + z2;
+      } else {
+        if (t + 1 >= Nt || D.filt[27] == 0.0) continue;
+        o2 = i + (long long)Ny * Nx * Nz;
+        kind = 1;
+      }
+      const double Dd = vd - (double)xd[o2], Db = vb - (double)xb[o2];
+      acc += D.filt[k] *
+             (pc_rho(D.T[0] * Dd + D.T[1] * Db, D.c[kind][0], D.sq[kind][0], D.sqp[kind][0],
+                     D.p, D.q) +
+              pc_rho(D.T[2] * Dd + D.T[3] * Db, D.c[kind][1], D.sq[kind][1], D.sqp[kind][1],
+                     D.p, D.q));
+    }
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// Data term mu/2 sum(e_d^2 + e_b^2) (PCMBIR_3D/XT_ICD_update.c:92-102 with
+// mu, not mu^2: the objective the ICD minimises).
+__global__ __launch_bounds__(256) void k_pc_data_cost(long long n, double mu,
+                                                      const float* __restrict__ ed,
+                                                      const float* __restrict__ eb,
+                                                      double* __restrict__ partial) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (long long)gridDim.x * 256)
+    acc += (double)ed[i] * ed[i] + (double)eb[i] * eb[i];
+  red[threadIdx.x] = acc * mu * 0.5;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
 // Voxel-selection criterion of non-homogeneous ICD: the magnitude map
 // filtered with the separable 5x5 Hamming window h = 0.54 - 0.46 cos(2 pi n / 4)
 // = (0.08, 0.54, 1, 0.54, 0.08) per axis, truncated at the grid's edges
@@ -2035,6 +2366,12 @@ struct mbir_hip_ctx {
   unsigned* d_xcd = nullptr;  // [items run, items off their XCD] (option "xcd_check")
   float* d_snap = nullptr;    // saved object (mbir_hip_object_snapshot)
   bool snap_valid = false;
+  // PCMBIR tomographic mode (mbir_hip_pc_enable): d_x holds delta, d_x2
+  // beta; d_e / d_e2 their error sinograms; d_tgt staging for targets
+  bool pc = false;
+  float *d_x2 = nullptr, *d_e2 = nullptr;
+  PcTab ptab;
+  PcDbl pdbl;
   double* d_stats = nullptr;
   double* d_partial = nullptr;
   size_t partial_cap = 0;
@@ -2138,7 +2475,12 @@ size_t fwd_tchunk(const mbir_hip_geom& g, size_t acc_bytes) {
   return tc;
 }
 
-int run_fwdproj(mbir_hip_ctx* c, int mode, const float* x, float* out) {
+// mode 0: out = y - A x - d, mode 1: data cost partials, mode 2: out = A x.
+// y (mode 0) replaces the scan's projections and drops the offset d, and
+// xscale multiplies A (the tomographic ICD's error sinograms: y the target,
+// xscale the wavenumber).
+int run_fwdproj(mbir_hip_ctx* c, int mode, const float* x, float* out,
+                const float* y = nullptr, double xscale = 1.0) {
   const mbir_hip_geom& g = c->g;
   // FP64 accumulation for the error sinogram and the cost, so e is the
   // correctly rounded y - A x - d however long the sweeps have run
@@ -2153,12 +2495,14 @@ int run_fwdproj(mbir_hip_ctx* c, int mode, const float* x, float* out) {
   if (ev_begin(c, mode == 1 ? MBIR_K_COST : MBIR_K_FWDPROJ)) return -1;
   if (dbl)
     hipLaunchKernelGGL(k_fwdproj<double>, grid, dim3(256), shm, c->stream, c->G, mode,
-                       (int)tc, c->d_H, c->d_cos, c->d_sin, c->d_view_frame, x, c->d_y,
-                       c->d_w, c->d_d, out, c->d_partial);
+                       (int)tc, c->d_H, c->d_cos, c->d_sin, c->d_view_frame, x,
+                       y ? y : c->d_y, c->d_w, y ? nullptr : c->d_d, out, c->d_partial,
+                       xscale);
   else
     hipLaunchKernelGGL(k_fwdproj<float>, grid, dim3(256), shm, c->stream, c->G, mode,
-                       (int)tc, c->d_H, c->d_cos, c->d_sin, c->d_view_frame, x, c->d_y,
-                       c->d_w, c->d_d, out, c->d_partial);
+                       (int)tc, c->d_H, c->d_cos, c->d_sin, c->d_view_frame, x,
+                       y ? y : c->d_y, c->d_w, y ? nullptr : c->d_d, out, c->d_partial,
+                       xscale);
   return ev_end(c, mode == 1 ? MBIR_K_COST : MBIR_K_FWDPROJ);
 }
 
@@ -2288,7 +2632,7 @@ void mbir_hip_destroy(mbir_hip_ctx* c) {
                   c->d_stage, c->d_stage2, c->d_afirst, c->d_aval, c->d_tile_ptr,
                   c->d_tile_cols, c->d_launch_tiles, c->d_stats, c->d_partial, c->d_mask,
                   c->d_aval16, c->d_ascale, c->d_ws, c->d_rt, c->d_mag, c->d_vsc,
-                  c->d_xcd, c->d_snap};
+                  c->d_xcd, c->d_snap, c->d_x2, c->d_e2};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
@@ -3115,6 +3459,176 @@ int mbir_hip_voxel_thetas(mbir_hip_ctx* c, int32_t n, const int32_t* tzyx, float
   HIP_OK(hipFree(dq));
   HIP_OK(hipFree(d1));
   HIP_OK(hipFree(d2));
+  return 0;
+}
+
+int mbir_hip_pc_enable(mbir_hip_ctx* c, const mbir_hip_pc_prior* pr) {
+  if (!c || !pr) return fail("null argument");
+  if (pr->q != 2.0) return fail("q must be 2 (the qGGMRF of the tomographic kernels)");
+  if (pr->mu <= 0 || pr->wavenumber <= 0) return fail("mu and the wavenumber must be positive");
+  if (c->g.N_z > kWave) return fail("the tomographic sweep takes at most 64 voxels per column");
+  if (!c->d_x2 && (dev_alloc(&c->d_x2, c->nvox) || dev_alloc(&c->d_e2, c->nsino))) return -1;
+  HIP_OK(hipMemset(c->d_x2, 0, c->nvox * sizeof(float)));
+  HIP_OK(hipMemset(c->d_e2, 0, c->nsino * sizeof(float)));
+  PcTab& P = c->ptab;
+  PcDbl& D = c->pdbl;
+  memset(&P, 0, sizeof(P));
+  memset(&D, 0, sizeof(D));
+  for (int i = 0; i < 4; ++i) P.T[i] = (float)(D.T[i] = pr->T[i]);
+  P.qmp = (float)(pr->q - pr->p);
+  D.p = pr->p;
+  D.q = pr->q;
+  for (int o = 0; o < 2; ++o) {
+    const double cs[2] = {pr->c_s[o], pr->c_t[o]}, sq[2] = {pr->sig_s_q[o], pr->sig_t_q[o]};
+    const double sqp[2] = {pr->sig_s_qp[o], pr->sig_t_qp[o]};
+    for (int kind = 0; kind < 2; ++kind) {
+      P.c[kind][o] = (float)cs[kind];
+      P.inv_sq[kind][o] = (float)(1.0 / sq[kind]);
+      P.inv_sqp[kind][o] = (float)(1.0 / sqp[kind]);
+      P.b0[kind][o] = (float)(pr->q / (sq[kind] * cs[kind]));
+      D.c[kind][o] = cs[kind];
+      D.sq[kind][o] = sq[kind];
+      D.sqp[kind][o] = sqp[kind];
+    }
+  }
+  for (int k = 0; k < 27; ++k) P.filt[k] = (float)(D.filt[k] = pr->filt[k]);
+  P.filt[27] = (float)(D.filt[27] = pr->filt_t);
+  P.mu = (float)(D.mu = pr->mu);
+  P.kw = (float)pr->wavenumber;
+  P.alpha = (float)pr->alpha;
+  c->pc = true;
+  return 0;
+}
+
+int mbir_hip_pc_set_objects(mbir_hip_ctx* c, const float* delta, const float* beta) {
+  if (!c || !delta || !beta) return fail("null argument");
+  if (!c->pc) return fail("tomographic mode not enabled (mbir_hip_pc_enable)");
+  if (mbir_hip_set_object(c, delta)) return -1;
+  HIP_OK(hipMemcpyAsync(c->d_stage, beta, c->nvox * sizeof(float), hipMemcpyHostToDevice,
+                        c->stream));
+  const unsigned blocks = (unsigned)((c->nvox + 255) / 256);
+  hipLaunchKernelGGL(k_to_colmajor, dim3(blocks), dim3(256), 0, c->stream, c->g.N_time,
+                     c->g.N_z, c->ncol, c->d_stage, c->d_x2);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mbir_hip_pc_get_objects(mbir_hip_ctx* c, float* delta, float* beta) {
+  if (!c || !delta || !beta) return fail("null argument");
+  if (!c->pc) return fail("tomographic mode not enabled (mbir_hip_pc_enable)");
+  if (mbir_hip_get_object(c, delta)) return -1;
+  const unsigned blocks = (unsigned)((c->nvox + 255) / 256);
+  hipLaunchKernelGGL(k_from_colmajor, dim3(blocks), dim3(256), 0, c->stream, c->g.N_time,
+                     c->g.N_z, c->ncol, c->d_x2, c->d_stage);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(beta, c->d_stage, c->nvox * sizeof(float), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mbir_hip_pc_init_error(mbir_hip_ctx* c, const float* target_delta, const float* target_beta) {
+  if (!c || !target_delta || !target_beta) return fail("null argument");
+  if (!c->pc) return fail("tomographic mode not enabled (mbir_hip_pc_enable)");
+  if (!c->a_built) return fail("A not built");
+  const double k = (double)c->ptab.kw;
+  const float* tg[2] = {target_delta, target_beta};
+  const float* xs[2] = {c->d_x, c->d_x2};
+  float* es[2] = {c->d_e, c->d_e2};
+  for (int o = 0; o < 2; ++o) {
+    HIP_OK(hipMemcpyAsync(c->d_stage2, tg[o], c->nsino * sizeof(float), hipMemcpyHostToDevice,
+                          c->stream));
+    if (run_fwdproj(c, 0, xs[o], es[o], c->d_stage2, k)) return -1;
+  }
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mbir_hip_pc_get_error(mbir_hip_ctx* c, float* e_delta, float* e_beta) {
+  if (!c || !e_delta || !e_beta) return fail("null argument");
+  if (!c->pc) return fail("tomographic mode not enabled (mbir_hip_pc_enable)");
+  HIP_OK(hipMemcpyAsync(e_delta, c->d_e, c->nsino * sizeof(float), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIP_OK(hipMemcpyAsync(e_beta, c->d_e2, c->nsino * sizeof(float), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mbir_hip_pc_equits(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t seed,
+                       double* stats) {
+  if (!c) return fail("null context");
+  if (!c->pc) return fail("tomographic mode not enabled (mbir_hip_pc_enable)");
+  if (!c->a_built) return fail("A not built");
+  if (c->afp16) return fail("the tomographic sweep reads the FP32 A");
+  if (build_tiles(c)) return -1;
+  const size_t nx = (size_t)(c->tile + 2) * (c->tile + 2) * c->g.N_z;
+  const size_t words = 2 * nx + (size_t)c->nvmax * (2 + c->G.W) + (size_t)c->tile * c->tile;
+  if (words * 4 + 4 * 1024 + 6 * kNZMax * 4 > c->lds_max)
+    return fail("tile slabs of both objects exceed the LDS of one workgroup");
+  HIP_OK(hipMemsetAsync(c->d_stats, 0, 4 * sizeof(double), c->stream));
+  PcArgs A;
+  A.aval = c->d_aval;
+  A.afirst = c->d_afirst;
+  A.frame_views = c->d_frame_views;
+  A.frame_ptr = c->d_frame_ptr;
+  A.tile_ptr = c->d_tile_ptr;
+  A.tile_cols = c->d_tile_cols;
+  A.tiles_x = (c->g.N_x + c->tile - 1) / c->tile;
+  A.tile = c->tile;
+  A.nvmax = c->nvmax;
+  A.W = c->G.W;
+  A.xd = c->d_x;
+  A.xb = c->d_x2;
+  A.ed = c->d_e;
+  A.eb = c->d_e2;
+  A.stats = c->d_stats;
+  const int nf = c->g.N_time;
+  for (int it = 0; it < n; ++it) {
+    A.iter = first_iter + it;
+    for (int colour = 0; colour < 4; ++colour) {
+      const int ntl = c->launch_off[colour + 1] - c->launch_off[colour];
+      if (ntl == 0) continue;
+      A.launch_tiles = c->d_launch_tiles + c->launch_off[colour];
+      A.n_launch_tiles = ntl;
+      for (int fpar = 0; fpar < 2; ++fpar) {
+        const int frames = (nf - fpar + 1) / 2;
+        if (frames <= 0) continue;
+        A.fparity = fpar;
+        A.seed = seed * 2654435761u + (unsigned)(it * 8 + colour * 2 + fpar) * 40503u;
+        if (ev_begin(c, MBIR_K_SWEEP)) return -1;
+        hipLaunchKernelGGL(k_pc_sweep, dim3((unsigned)(ntl * frames)), dim3(kWave),
+                           words * sizeof(float), c->stream, c->SG, c->ptab, A);
+        if (ev_end(c, MBIR_K_SWEEP)) return -1;
+      }
+    }
+  }
+  if (stats) {
+    HIP_OK(hipMemcpyAsync(stats, c->d_stats, 3 * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+  }
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mbir_hip_pc_cost(mbir_hip_ctx* c, double* out) {
+  if (!c || !out) return fail("null argument");
+  if (!c->pc) return fail("tomographic mode not enabled (mbir_hip_pc_enable)");
+  const unsigned blocks = 1024;
+  double data = 0.0, prior = 0.0;
+  if (ensure_partial(c, blocks)) return -1;
+  hipLaunchKernelGGL(k_pc_data_cost, dim3(blocks), dim3(256), 0, c->stream,
+                     (long long)c->nsino, c->pdbl.mu, c->d_e, c->d_e2, c->d_partial);
+  HIP_OK(hipGetLastError());
+  if (sum_partials(c, blocks, &data)) return -1;
+  hipLaunchKernelGGL(k_pc_prior_cost, dim3(blocks), dim3(256), 0, c->stream, c->g.N_time,
+                     c->g.N_y, c->g.N_x, c->g.N_z, c->pdbl, c->d_x, c->d_x2, c->d_partial);
+  HIP_OK(hipGetLastError());
+  if (sum_partials(c, blocks, &prior)) return -1;
+  out[0] = data + prior;
+  out[1] = data;
+  out[2] = prior;
   return 0;
 }
 

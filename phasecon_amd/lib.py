@@ -36,6 +36,15 @@ class GuardState(C.Structure):
 GUARD_OK, GUARD_SAVE, GUARD_CONFIRM, GUARD_RECOVER, GUARD_FAIL = range(5)
 
 
+class PcPrior(C.Structure):
+    """mirror of mbir_hip_pc_prior (include/mbir_hip.h)"""
+    _fields_ = [("mu", C.c_double), ("wavenumber", C.c_double), ("alpha", C.c_double),
+                ("T", C.c_double * 4)] + \
+        [(n, C.c_double * 2) for n in ("c_s", "sig_s_q", "sig_s_qp", "c_t", "sig_t_q",
+                                       "sig_t_qp")] + \
+        [("p", C.c_double), ("q", C.c_double), ("filt", C.c_double * 27), ("filt_t", C.c_double)]
+
+
 class Options(C.Structure):
     """mirror of mbir_options (include/mbir_engine.h)"""
     _fields_ = [("p_s", C.c_double), ("p_t", C.c_double), ("tile", C.c_int32),
@@ -97,6 +106,13 @@ _SIGS = {
     "mbir_hip_reset_kernel_time": (C.c_int, [C.c_void_p]),
     "mbir_hip_xcd_check": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "mbir_hip_object_snapshot": (C.c_int, [C.c_void_p, C.c_int32]),
+    "mbir_hip_pc_enable": (C.c_int, [C.c_void_p, C.POINTER(PcPrior)]),
+    "mbir_hip_pc_set_objects": (C.c_int, [C.c_void_p] * 3),
+    "mbir_hip_pc_get_objects": (C.c_int, [C.c_void_p] * 3),
+    "mbir_hip_pc_init_error": (C.c_int, [C.c_void_p] * 3),
+    "mbir_hip_pc_get_error": (C.c_int, [C.c_void_p] * 3),
+    "mbir_hip_pc_equits": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_uint32, C.c_void_p]),
+    "mbir_hip_pc_cost": (C.c_int, [C.c_void_p, C.c_void_p]),
     "mbir_guard_init": (None, [C.POINTER(GuardState), C.c_double, C.c_double, C.c_int32]),
     "mbir_guard_check": (C.c_int32, [C.POINTER(GuardState), C.c_double, C.c_int32]),
     "mbir_guard_alpha": (C.c_double, [C.c_double]),

@@ -117,7 +117,7 @@ class Stage:
 
     def __init__(self, scan, mult_xy=1, mult_z=1, quad_convex=1, huber_delta=0.5,
                  huber_T=4.0, p_s=None, p_t=None, tile=16, alpha=None, var_est=1.0,
-                 afp16=False):
+                 afp16=False, full_mask=False):
         L = self.L = _lib.load()
         s = self.scan = scan
         N_p, N_t, N_r = s["proj_num"], s["proj_rows"], s["proj_cols"]
@@ -144,7 +144,13 @@ class Stage:
         self.y = self._internal(s["projections"])
         self.w = (self._internal(s["weights"]) / np.float32(var_est)).astype(np.float32)
         self.mask = np.zeros((g.N_y, g.N_x), np.uint8)
-        L.mbir_update_mask(C.byref(g), C.c_double(float(s["vox_wid"]) * N_r / 2), _p(self.mask))
+        # full_mask: every column updated, as PCMBIR's radius_obj = vox_wid
+        # proj_cols (PCMBIR_3D/XT_Init.c:303) circumscribes the grid
+        if full_mask:
+            self.mask[:] = 1
+        else:
+            L.mbir_update_mask(C.byref(g), C.c_double(float(s["vox_wid"]) * N_r / 2),
+                               _p(self.mask))
         h = C.c_void_p()
         _lib.check(L.mbir_hip_create(C.byref(h), C.byref(g), _p(self.H), _p(self.cos),
                                      _p(self.sin), _p(self.frame_ptr), _p(self.frame_views),
@@ -395,6 +401,88 @@ class Guard:
         elif act == _lib.GUARD_FAIL:
             self.failed = True
         return act, cost
+
+
+def tomo_prior(phase, mag, mult_xy, delta_recon, wavenumber, delta_over_beta=None, mu=1.0,
+               alpha=1.5, p=1.2, q=2.0):
+    """The PCMBIR tomographic prior of one stage as mbir_hip_pc_enable takes
+    it (PCMBIR_3D/XT_Init.c:144-176, 234-246, 482-502).  phase / mag =
+    (sig_s, sig_t, c_s, c_t) of the phase (delta) and magnitude (beta)
+    priors; spatial sigmas scaled by mult_xy, temporal by delta_recon, each
+    raised to q and q - p; the 26-neighbour filter 1/distance normalised to
+    sum 1 and the temporal filter 0 (initFilter); DecorrTran
+    [[1, d/b], [1, -d/b]] for a delta_over_beta (recon_type 2), else the
+    identity."""
+    P = _lib.PcPrior()
+    P.mu, P.wavenumber, P.alpha, P.p, P.q = mu, wavenumber, alpha, p, q
+    T = (1.0, 0.0, 0.0, 1.0) if delta_over_beta is None else \
+        (1.0, delta_over_beta, 1.0, -delta_over_beta)
+    for i, v in enumerate(T):
+        P.T[i] = v
+    for k, (ss, st, cs, ct) in enumerate((phase, mag)):
+        P.c_s[k], P.c_t[k] = cs, ct
+        P.sig_s_q[k], P.sig_s_qp[k] = (ss * mult_xy) ** q, (ss * mult_xy) ** (q - p)
+        P.sig_t_q[k], P.sig_t_qp[k] = (st * delta_recon) ** q, (st * delta_recon) ** (q - p)
+    o = np.indices((3, 3, 3)).reshape(3, 27) - 1.0
+    dist = np.sqrt((o ** 2).sum(0))
+    filt = np.divide(1.0, dist, out=np.zeros(27), where=dist > 0)
+    filt /= filt.sum()
+    for k in range(27):
+        P.filt[k] = filt[k]
+    P.filt_t = 0.0
+    return P
+
+
+class TomoStage(Stage):
+    """A Stage switched to the PCMBIR tomographic ICD (mbir_hip_pc_*): two
+    objects, delta and beta, fitted to two target sinograms (the ADMM's
+    phase and magnitude projections plus duals) through A times the
+    wavenumber.  The scan supplies the geometry; its projections and
+    weights are not used."""
+
+    def __init__(self, scan, prior, mult_xy=1, mult_z=1, tile=16):
+        super().__init__(scan, mult_xy=mult_xy, mult_z=mult_z, tile=tile, alpha=prior.alpha,
+                         full_mask=True)
+        self.prior = prior
+        _lib.check(self.L.mbir_hip_pc_enable(self.h, C.byref(prior)), "mbir_hip_pc_enable")
+        self.build_amatrix()
+
+    def set_objects(self, delta, beta):
+        d = np.ascontiguousarray(delta, np.float32).reshape(self.shape_x)
+        b = np.ascontiguousarray(beta, np.float32).reshape(self.shape_x)
+        _lib.check(self.L.mbir_hip_pc_set_objects(self.h, _p(d), _p(b)), "pc_set_objects")
+
+    def get_objects(self):
+        d = np.zeros(self.shape_x, np.float32)
+        b = np.zeros(self.shape_x, np.float32)
+        _lib.check(self.L.mbir_hip_pc_get_objects(self.h, _p(d), _p(b)), "pc_get_objects")
+        return d, b
+
+    def init_errors(self, target_delta, target_beta):
+        """e = target - k A x for both objects; targets in the internal
+        sinogram layout [N_p][N_r][N_t]."""
+        td = np.ascontiguousarray(target_delta, np.float32).reshape(self.shape_e)
+        tb = np.ascontiguousarray(target_beta, np.float32).reshape(self.shape_e)
+        _lib.check(self.L.mbir_hip_pc_init_error(self.h, _p(td), _p(tb)), "pc_init_error")
+
+    def get_errors(self):
+        ed = np.zeros(self.shape_e, np.float32)
+        eb = np.zeros(self.shape_e, np.float32)
+        _lib.check(self.L.mbir_hip_pc_get_error(self.h, _p(ed), _p(eb)), "pc_get_error")
+        return ed, eb
+
+    def equits(self, n, first_iter, seed=761521):
+        """n equits; returns [sum |dx|, voxels updated, voxels skipped]."""
+        st = np.zeros(3, np.float64)
+        _lib.check(self.L.mbir_hip_pc_equits(self.h, int(n), int(first_iter), seed & 0xFFFFFFFF,
+                                             _p(st)), "pc_equits")
+        return st
+
+    def tomo_cost(self):
+        """FP64 [total, data, prior]."""
+        out = np.zeros(3, np.float64)
+        _lib.check(self.L.mbir_hip_pc_cost(self.h, _p(out)), "pc_cost")
+        return out
 
 
 def substitution_batch(g, V, th1, th2, nb, nbf, tn, tf):
