@@ -330,6 +330,25 @@ int mbir_hip_pc_get_error(mbir_hip_ctx* ctx, float* e_delta, float* e_beta);
  * [sum |dx| over both objects, voxels updated, voxels skipped]. */
 int mbir_hip_pc_equits(mbir_hip_ctx* ctx, int32_t n, int32_t first_iter, uint32_t seed,
                        double* stats);
+/* ADMM head loop of the tomographic mode (PCMBIR_3D/XT_ICD_update.c:1037-1151
+ * with ENABLE_TOMO_RECONS).  The context keeps, per object, the phase
+ * retrieval's projection estimate Z, the scaled dual V and the previous
+ * estimate P (FP32, internal layout [N_p][N_r][N_t] = the phase retrieval's
+ * [n_views][rows][cols]); the error sinograms then hold e = Z - V - k A x.
+ * init (initErrorSinogam, XT_ICD_update.c:726-760): P = z, Z = z or, with
+ * from_object (recon_type 2), Z = k A x of the current objects, V = 0.
+ * target: a = Z - e = k A x + V, the retrieval's proximal point
+ * (XT_ICD_update.c:1089-1093).  update with the retrieval's new estimate z
+ * (XT_ICD_update.c:1095-1140): V' = V + k A x - z, e = z - V' - k A x, P = Z
+ * = z; residuals (may be NULL) = [beta primal, beta dual, delta primal,
+ * delta dual], primal = RMS of z - k A x, dual = |z - P| / |V'|. */
+int mbir_hip_pc_admm_init(mbir_hip_ctx* ctx, const double* z_delta, const double* z_beta,
+                          int32_t from_object);
+int mbir_hip_pc_admm_target(mbir_hip_ctx* ctx, double* a_delta, double* a_beta);
+int mbir_hip_pc_admm_update(mbir_hip_ctx* ctx, const double* z_delta, const double* z_beta,
+                            double* residuals);
+/* which: 0 Z, 1 V, 2 P. */
+int mbir_hip_pc_admm_state(mbir_hip_ctx* ctx, int32_t which, float* delta, float* beta);
 /* FP64 cost: out = [total, mu/2 sum(e_delta^2 + e_beta^2), prior] (the
  * reference's computeCost squares mu e, PCMBIR_3D/XT_ICD_update.c:92-102;
  * the same at mu = 1). */

@@ -80,6 +80,16 @@ int pcmbir_hip_set_nms_prior(pcmbir_hip_ctx* ctx, double phase_sig,
                              double delta_over_beta, int32_t decorrelate,
                              double p, double nu);
 
+/* The tomographic coupling of the phase retrieval (the ENABLE_TOMO_RECONS
+ * branches of Cost_NMS, XT_Search.c:18-19, and compute_pretcost,
+ * XT_CmplxProjEst.c:38-41, compiled out of the shipped build, Q12): with a
+ * target set, the simplex search and the pretcost use mu/2 |z - a|^2 with a
+ * = (a_mag, a_phase) [n_views][rows][cols] FP64 (the ADMM head loop's
+ * projection plus dual, PCMBIR_3D/XT_ICD_update.c:1089-1093) in place of the
+ * projection-domain prior.  a_mag or a_phase NULL switches the coupling off. */
+int pcmbir_hip_set_tomo_target(pcmbir_hip_ctx* ctx, double mu, const double* a_mag,
+                               const double* a_phase);
+
 /* One Nelder-Mead pass (Nelder_Mead_Simplex_2DSearch, XT_Search.c:100-215)
  * over every pixel of every view with b = w - v given: zr, zi
  * [n_views][rows][cols] FP64 in and out, b complex.  Pixels are visited in

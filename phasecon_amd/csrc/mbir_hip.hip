@@ -1673,6 +1673,76 @@ __global__ __launch_bounds__(256) void k_pc_prior_cost(int Nt, int Ny, int Nx, i
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// ADMM state of the tomographic mode, per object (ENABLE_TOMO_RECONS,
+// PCMBIR_3D/XT_ICD_update.c:726-760): the projection estimate Z of the phase
+// retrieval, the scaled dual V, the previous estimate P.  Start: P = z,
+// Z = z (or k A x, from_object: recon_type 2), V = 0, e = Z - V - k A x;
+// nkax holds -k A x.
+__global__ __launch_bounds__(256) void k_pc_admm_init(long long n, const double* __restrict__ z,
+                                                      const float* __restrict__ nkax,
+                                                      int from_object, float* __restrict__ Z,
+                                                      float* __restrict__ V,
+                                                      float* __restrict__ P,
+                                                      float* __restrict__ e) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (long long)gridDim.x * 256) {
+    const float kax = -nkax[i];
+    const float zz = from_object ? kax : (float)z[i];
+    P[i] = (float)z[i];
+    Z[i] = zz;
+    V[i] = 0.f;
+    e[i] = zz - kax;
+  }
+}
+
+// The phase retrieval's target a = Z - e = k A x + V
+// (PCMBIR_3D/XT_ICD_update.c:1089-1093).
+__global__ __launch_bounds__(256) void k_pc_admm_target(long long n, const float* __restrict__ Z,
+                                                        const float* __restrict__ e,
+                                                        double* __restrict__ a) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (long long)gridDim.x * 256)
+    a[i] = (double)Z[i] - (double)e[i];
+}
+
+// After the phase retrieval returned znew (PCMBIR_3D/XT_ICD_update.c:1095-1140):
+// e1 = znew - a, V' = -e1, e' = e1 + V - V' (so e' = Z' - V' - k A x again
+// and V' = V + k A x - Z'); sums of (V' + e')^2 = (Z' - k A x)^2, of
+// (Z' - P)^2 and of V'^2 per block into partial[3 block + 0..2]; P = Z = znew.
+__global__ __launch_bounds__(256) void k_pc_admm_update(long long n,
+                                                        const double* __restrict__ znew,
+                                                        float* __restrict__ Z,
+                                                        float* __restrict__ e,
+                                                        float* __restrict__ V,
+                                                        float* __restrict__ P,
+                                                        double* __restrict__ partial) {
+  __shared__ double red[3][256];
+  double pr = 0.0, du = 0.0, vs = 0.0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (long long)gridDim.x * 256) {
+    const double zn = znew[i];
+    const double a = (double)Z[i] - (double)e[i];
+    const double e1 = zn - a, vold = V[i], vnew = -e1, en = e1 + vold - vnew;
+    pr += (vnew + en) * (vnew + en);
+    du += (zn - (double)P[i]) * (zn - (double)P[i]);
+    vs += vnew * vnew;
+    V[i] = (float)vnew;
+    e[i] = (float)en;
+    P[i] = (float)zn;
+    Z[i] = (float)zn;
+  }
+  red[0][threadIdx.x] = pr;
+  red[1][threadIdx.x] = du;
+  red[2][threadIdx.x] = vs;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st)
+      for (int k = 0; k < 3; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) partial[3 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
 // Data term mu/2 sum(e_d^2 + e_b^2) (PCMBIR_3D/XT_ICD_update.c:92-102 with
 // mu, not mu^2: the objective the ICD minimises).
 __global__ __launch_bounds__(256) void k_pc_data_cost(long long n, double mu,
@@ -2372,6 +2442,11 @@ struct mbir_hip_ctx {
   float *d_x2 = nullptr, *d_e2 = nullptr;
   PcTab ptab;
   PcDbl pdbl;
+  // its ADMM state (mbir_hip_pc_admm_*): Z, V, P of delta then beta, 2 nsino
+  // each; d_pk -k A x of one object; d_pdbl an FP64 sinogram of staging
+  float *d_pz = nullptr, *d_pv = nullptr, *d_pp = nullptr, *d_pk = nullptr;
+  double* d_pdbl = nullptr;
+  bool admm = false;
   double* d_stats = nullptr;
   double* d_partial = nullptr;
   size_t partial_cap = 0;
@@ -2632,7 +2707,8 @@ void mbir_hip_destroy(mbir_hip_ctx* c) {
                   c->d_stage, c->d_stage2, c->d_afirst, c->d_aval, c->d_tile_ptr,
                   c->d_tile_cols, c->d_launch_tiles, c->d_stats, c->d_partial, c->d_mask,
                   c->d_aval16, c->d_ascale, c->d_ws, c->d_rt, c->d_mag, c->d_vsc,
-                  c->d_xcd, c->d_snap, c->d_x2, c->d_e2};
+                  c->d_xcd, c->d_snap, c->d_x2, c->d_e2, c->d_pz, c->d_pv, c->d_pp,
+                  c->d_pk, c->d_pdbl};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
@@ -3608,6 +3684,104 @@ int mbir_hip_pc_equits(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t 
     HIP_OK(hipMemcpyAsync(stats, c->d_stats, 3 * sizeof(double), hipMemcpyDeviceToHost,
                           c->stream));
   }
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mbir_hip_pc_admm_init(mbir_hip_ctx* c, const double* z_delta, const double* z_beta,
+                          int32_t from_object) {
+  if (!c || !z_delta || !z_beta) return fail("null argument");
+  if (!c->pc) return fail("tomographic mode not enabled (mbir_hip_pc_enable)");
+  if (!c->a_built) return fail("A not built");
+  const size_t n = c->nsino;
+  if (!c->d_pz && (dev_alloc(&c->d_pz, 2 * n) || dev_alloc(&c->d_pv, 2 * n) ||
+                   dev_alloc(&c->d_pp, 2 * n) || dev_alloc(&c->d_pk, n) ||
+                   dev_alloc(&c->d_pdbl, n)))
+    return -1;
+  const double k = (double)c->ptab.kw;
+  const double* zs[2] = {z_delta, z_beta};
+  const float* xs[2] = {c->d_x, c->d_x2};
+  float* es[2] = {c->d_e, c->d_e2};
+  const unsigned blocks = 1024;
+  HIP_OK(hipMemsetAsync(c->d_stage2, 0, n * sizeof(float), c->stream));
+  for (int o = 0; o < 2; ++o) {
+    if (run_fwdproj(c, 0, xs[o], c->d_pk, c->d_stage2, k)) return -1;  // -k A x
+    HIP_OK(hipMemcpyAsync(c->d_pdbl, zs[o], n * sizeof(double), hipMemcpyHostToDevice,
+                          c->stream));
+    hipLaunchKernelGGL(k_pc_admm_init, dim3(blocks), dim3(256), 0, c->stream, (long long)n,
+                       c->d_pdbl, c->d_pk, from_object ? 1 : 0, c->d_pz + o * n,
+                       c->d_pv + o * n, c->d_pp + o * n, es[o]);
+    HIP_OK(hipGetLastError());
+  }
+  HIP_OK(hipStreamSynchronize(c->stream));
+  c->admm = true;
+  return 0;
+}
+
+int mbir_hip_pc_admm_target(mbir_hip_ctx* c, double* a_delta, double* a_beta) {
+  if (!c || !a_delta || !a_beta) return fail("null argument");
+  if (!c->admm) return fail("ADMM state not initialised (mbir_hip_pc_admm_init)");
+  const size_t n = c->nsino;
+  double* as[2] = {a_delta, a_beta};
+  const float* es[2] = {c->d_e, c->d_e2};
+  for (int o = 0; o < 2; ++o) {
+    hipLaunchKernelGGL(k_pc_admm_target, dim3(1024), dim3(256), 0, c->stream, (long long)n,
+                       c->d_pz + o * n, es[o], c->d_pdbl);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(as[o], c->d_pdbl, n * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+  }
+  return 0;
+}
+
+int mbir_hip_pc_admm_update(mbir_hip_ctx* c, const double* z_delta, const double* z_beta,
+                            double* residuals) {
+  if (!c || !z_delta || !z_beta) return fail("null argument");
+  if (!c->admm) return fail("ADMM state not initialised (mbir_hip_pc_admm_init)");
+  const size_t n = c->nsino;
+  const unsigned blocks = 1024;
+  if (ensure_partial(c, 3 * blocks)) return -1;
+  const double* zs[2] = {z_delta, z_beta};
+  float* es[2] = {c->d_e, c->d_e2};
+  double res[2][2];
+  for (int o = 0; o < 2; ++o) {
+    HIP_OK(hipMemcpyAsync(c->d_pdbl, zs[o], n * sizeof(double), hipMemcpyHostToDevice,
+                          c->stream));
+    hipLaunchKernelGGL(k_pc_admm_update, dim3(blocks), dim3(256), 0, c->stream, (long long)n,
+                       c->d_pdbl, c->d_pz + o * n, es[o], c->d_pv + o * n, c->d_pp + o * n,
+                       c->d_partial);
+    HIP_OK(hipGetLastError());
+    std::vector<double> h(3 * blocks);
+    HIP_OK(hipMemcpyAsync(h.data(), c->d_partial, h.size() * sizeof(double),
+                          hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    double pr = 0.0, du = 0.0, vs = 0.0;
+    for (unsigned b = 0; b < blocks; ++b) {
+      pr += h[3 * b];
+      du += h[3 * b + 1];
+      vs += h[3 * b + 2];
+    }
+    res[o][0] = sqrt(pr / (double)n);
+    res[o][1] = sqrt(du / vs);
+  }
+  if (residuals) {  // the reference's order: magnitude (beta) first
+    residuals[0] = res[1][0];
+    residuals[1] = res[1][1];
+    residuals[2] = res[0][0];
+    residuals[3] = res[0][1];
+  }
+  return 0;
+}
+
+int mbir_hip_pc_admm_state(mbir_hip_ctx* c, int32_t which, float* delta, float* beta) {
+  if (!c || !delta || !beta) return fail("null argument");
+  if (!c->admm) return fail("ADMM state not initialised (mbir_hip_pc_admm_init)");
+  if (which < 0 || which > 2) return fail("which must be 0 (Z), 1 (V) or 2 (P)");
+  const float* src = which == 0 ? c->d_pz : which == 1 ? c->d_pv : c->d_pp;
+  const size_t n = c->nsino;
+  HIP_OK(hipMemcpyAsync(delta, src, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipMemcpyAsync(beta, src + n, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
   return 0;
 }

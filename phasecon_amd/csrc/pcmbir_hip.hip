@@ -314,7 +314,9 @@ __global__ void k_pr_stop(int n_views, const double* __restrict__ scal, int* __r
 
 // ---------------------------------------------------------------- NMS
 // Parameters of the projection-domain prior of Cost_NMS (XT_Search.c:8-43,
-// without ENABLE_TOMO_RECONS, Q12) and of the simplex search.
+// without ENABLE_TOMO_RECONS, Q12) and of the simplex search; tomo selects
+// the ENABLE_TOMO_RECONS cost, mu/2 |z - a|^2 towards the tomographic
+// projection a in place of the prior (XT_Search.c:18-19).
 struct NmsParams {
   double T[2][2];     // decorrelating transform: row 0 phase, row 1 magnitude
   double filt[9];     // in-plane 3x3 neighbour weights
@@ -323,6 +325,8 @@ struct NmsParams {
   double nu;
   double rho, chi, gamma, sigma, thresh;
   int max_iter;
+  int tomo;
+  double mu;
 };
 
 // One pixel's data: b = w - v and its 3x3 neighbourhood (flags bit k set
@@ -330,6 +334,7 @@ struct NmsParams {
 struct NmsPix {
   double nr[9], ni[9];
   double br, bi;
+  double ar, ai;  // tomographic projection (tomo only)
   unsigned flags;
 };
 
@@ -345,6 +350,7 @@ __device__ double nms_cost(double zr, double zi, const NmsPix& X, const NmsParam
   const double e = exp(-zr);
   const double cv = e * cos(-zi), sv = e * sin(-zi);
   double cost = ((cv - X.br) * (cv - X.br) + (sv - X.bi) * (sv - X.bi)) * P.nu / 2;
+  if (P.tomo) return cost + ((zr - X.ar) * (zr - X.ar) + (zi - X.ai) * (zi - X.ai)) * P.mu / 2;
   for (int k = 0; k < 9; ++k) {
     if (!((X.flags >> k) & 1u)) continue;
     const double db = zr - X.nr[k], dd = zi - X.ni[k];
@@ -441,6 +447,8 @@ __global__ __launch_bounds__(kThreads) void k_nms_colour(NmsParams P, double* __
                                                          const float2* __restrict__ w,
                                                          const float2* __restrict__ vd,
                                                          const int* __restrict__ active,
+                                                         const double* __restrict__ ar,
+                                                         const double* __restrict__ ai,
                                                          int n_views, int rows, int cols, int cj,
                                                          int ck, int* __restrict__ iters) {
   const int hr = (rows - cj + 1) / 2, hc = (cols - ck + 1) / 2;
@@ -465,6 +473,8 @@ __global__ __launch_bounds__(kThreads) void k_nms_colour(NmsParams P, double* __
   const long long p = o + (long long)j * cols + k;
   X.br = (double)w[p].x - (double)vd[p].x;
   X.bi = (double)w[p].y - (double)vd[p].y;
+  X.ar = P.tomo ? ar[p] : 0.0;
+  X.ai = P.tomo ? ai[p] : 0.0;
   double r = zr[p], i = zi[p];
   const int it = nms_search(r, i, X, P);
   zr[p] = r;
@@ -504,13 +514,15 @@ __global__ void k_sd_target(const double* __restrict__ zr, const double* __restr
 
 // Per view: the pretcost (XT_CmplxProjEst.c:15-103) -- the forward term from
 // h = H(D exp(-z)) in hb and the prior over right / down / down-right /
-// down-left pairs.
+// down-left pairs, or with tomo mu/2 |z - a|^2 (XT_CmplxProjEst.c:38-41).
 __global__ __launch_bounds__(kThreads) void k_pret_cost(NmsParams P, const double* __restrict__ zr,
                                                         const double* __restrict__ zi,
                                                         const float2* __restrict__ y,
                                                         const float* __restrict__ lam,
                                                         const float2* __restrict__ hb,
-                                                        const int* __restrict__ active, int rows,
+                                                        const int* __restrict__ active,
+                                                        const double* __restrict__ ar,
+                                                        const double* __restrict__ ai, int rows,
                                                         int cols, double* __restrict__ out) {
   const int view = blockIdx.x;
   if (!active[view]) return;
@@ -521,6 +533,11 @@ __global__ __launch_bounds__(kThreads) void k_pret_cost(NmsParams P, const doubl
     const float2 h = hb[o + p];
     const double r = (double)y[o + p].x - sqrt((double)h.x * h.x + (double)h.y * h.y);
     fwd += r * r * lam[o + p] / 2;
+    if (P.tomo) {
+      const double dr = zr[o + p] - ar[o + p], di = zi[o + p] - ai[o + p];
+      prior += P.mu * (dr * dr + di * di) / 2;
+      continue;
+    }
     const int j = (int)(p / cols), k = (int)(p % cols);
     for (int e = 0; e < 4; ++e) {
       const int a = j + dj[e], b = k + dk[e];
@@ -634,6 +651,10 @@ struct pcmbir_hip_ctx {
   double *zr = nullptr, *zi = nullptr, *res = nullptr;
   float2 *vd = nullptr, *wold = nullptr;
   int *pix_iters = nullptr, *admm = nullptr;
+  // tomographic projection a of the ENABLE_TOMO_RECONS coupling
+  double *ar = nullptr, *ai = nullptr;
+  bool tomo = false;
+  double tomo_mu = 0.0;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
   size_t ev_used = 0;
   double fft_ms = 0.0;
@@ -792,8 +813,8 @@ int nms_pass_dev(pcmbir_hip_ctx* c, const float2* w, const float2* vd, const int
     const long long per =
         (long long)((c->rows - cj + 1) / 2) * ((c->cols - ck + 1) / 2) * c->nv;
     hipLaunchKernelGGL(k_nms_colour, dim3((unsigned)((per + kThreads - 1) / kThreads)),
-                       dim3(kThreads), 0, c->stream, c->nms, c->zr, c->zi, w, vd, mask, c->nv,
-                       c->rows, c->cols, cj, ck, c->pix_iters);
+                       dim3(kThreads), 0, c->stream, c->nms, c->zr, c->zi, w, vd, mask, c->ar,
+                       c->ai, c->nv, c->rows, c->cols, cj, ck, c->pix_iters);
   }
   HIP_OK(hipGetLastError());
   return 0;
@@ -806,7 +827,7 @@ int pret_cost_dev(pcmbir_hip_ctx* c, const int* mask, double* out) {
                      c->npix, c->nall, c->b);
   if (fresnel_dev(c, c->b, c->nv, 0)) return -1;
   hipLaunchKernelGGL(k_pret_cost, dim3(c->nv), dim3(kThreads), 0, c->stream, c->nms, c->zr, c->zi,
-                     c->y, c->lam, c->b, mask, c->rows, c->cols, out);
+                     c->y, c->lam, c->b, mask, c->ar, c->ai, c->rows, c->cols, out);
   HIP_OK(hipGetLastError());
   return 0;
 }
@@ -878,7 +899,7 @@ void pcmbir_hip_destroy(pcmbir_hip_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void* bufs[] = {c->tw_r, c->tw_c, c->fac_h, c->fac_hh, c->fac_pag, c->y, c->D, c->v, c->w, c->om,
                   c->g, c->b, c->lam, c->active, c->iters, c->scal, c->cost,
-                  c->zr, c->zi, c->res, c->vd, c->wold, c->pix_iters, c->admm};
+                  c->zr, c->zi, c->res, c->vd, c->wold, c->pix_iters, c->admm, c->ar, c->ai};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (auto& pr : c->ev) {
@@ -1006,7 +1027,30 @@ int pcmbir_hip_set_nms_prior(pcmbir_hip_ctx* c, double phase_sig, double phase_c
   P.sigma = 0.5;
   P.thresh = 0.005;
   P.max_iter = 50;
+  P.tomo = c->tomo ? 1 : 0;
+  P.mu = c->tomo_mu;
   c->nms_set = true;
+  return 0;
+}
+
+int pcmbir_hip_set_tomo_target(pcmbir_hip_ctx* c, double mu, const double* a_mag,
+                               const double* a_phase) {
+  if (!c) return fail("null context");
+  if (!a_mag || !a_phase) {
+    c->tomo = false;
+    c->nms.tomo = 0;
+    return 0;
+  }
+  if (!(mu > 0)) return fail("mu must be positive");
+  if (!c->ar && (dev_alloc(&c->ar, c->nall) || dev_alloc(&c->ai, c->nall))) return -1;
+  const size_t db = (size_t)c->nall * sizeof(double);
+  HIP_OK(hipMemcpyAsync(c->ar, a_mag, db, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipMemcpyAsync(c->ai, a_phase, db, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  c->tomo = true;
+  c->tomo_mu = mu;
+  c->nms.tomo = 1;
+  c->nms.mu = mu;
   return 0;
 }
 

@@ -113,6 +113,10 @@ _SIGS = {
     "mbir_hip_pc_get_error": (C.c_int, [C.c_void_p] * 3),
     "mbir_hip_pc_equits": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_uint32, C.c_void_p]),
     "mbir_hip_pc_cost": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "mbir_hip_pc_admm_init": (C.c_int, [C.c_void_p] * 3 + [C.c_int32]),
+    "mbir_hip_pc_admm_target": (C.c_int, [C.c_void_p] * 3),
+    "mbir_hip_pc_admm_update": (C.c_int, [C.c_void_p] * 4),
+    "mbir_hip_pc_admm_state": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     "mbir_guard_init": (None, [C.POINTER(GuardState), C.c_double, C.c_double, C.c_int32]),
     "mbir_guard_check": (C.c_int32, [C.POINTER(GuardState), C.c_double, C.c_int32]),
     "mbir_guard_alpha": (C.c_double, [C.c_double]),

@@ -484,6 +484,39 @@ class TomoStage(Stage):
         _lib.check(self.L.mbir_hip_pc_cost(self.h, _p(out)), "pc_cost")
         return out
 
+    # -- the ADMM head loop's state (mbir_hip_pc_admm_*), sinograms in the
+    #    internal layout [N_p][N_r][N_t]
+    def admm_init(self, z_delta, z_beta, from_object=False):
+        zd = np.ascontiguousarray(z_delta, np.float64).reshape(self.shape_e)
+        zb = np.ascontiguousarray(z_beta, np.float64).reshape(self.shape_e)
+        _lib.check(self.L.mbir_hip_pc_admm_init(self.h, _p(zd), _p(zb), int(bool(from_object))),
+                   "pc_admm_init")
+
+    def admm_target(self):
+        """The phase retrieval's proximal point a = k A x + V, (delta, beta)."""
+        ad = np.zeros(self.shape_e, np.float64)
+        ab = np.zeros(self.shape_e, np.float64)
+        _lib.check(self.L.mbir_hip_pc_admm_target(self.h, _p(ad), _p(ab)), "pc_admm_target")
+        return ad, ab
+
+    def admm_update(self, z_delta, z_beta):
+        """Dual update with the retrieval's new estimate; returns [beta
+        primal, beta dual, delta primal, delta dual] residuals."""
+        zd = np.ascontiguousarray(z_delta, np.float64).reshape(self.shape_e)
+        zb = np.ascontiguousarray(z_beta, np.float64).reshape(self.shape_e)
+        res = np.zeros(4, np.float64)
+        _lib.check(self.L.mbir_hip_pc_admm_update(self.h, _p(zd), _p(zb), _p(res)),
+                   "pc_admm_update")
+        return res
+
+    def admm_state(self, which):
+        """which 0: Z, 1: V, 2: P; returns (delta, beta) FP32."""
+        d = np.zeros(self.shape_e, np.float32)
+        b = np.zeros(self.shape_e, np.float32)
+        _lib.check(self.L.mbir_hip_pc_admm_state(self.h, int(which), _p(d), _p(b)),
+                   "pc_admm_state")
+        return d, b
+
 
 def substitution_batch(g, V, th1, th2, nb, nbf, tn, tf):
     """Batched functional substitution on the GPU (FP32)."""

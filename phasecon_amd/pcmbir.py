@@ -28,6 +28,7 @@ _SIGS = {
     "pcmbir_hip_estimate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double,
                                       C.c_int32, C.c_double] + [C.c_void_p] * 6),
     "pcmbir_hip_get_dual": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pcmbir_hip_set_tomo_target": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]),
     "pcmbir_hip_paganin": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
                                      C.c_void_p, C.c_void_p]),
 }
@@ -165,6 +166,22 @@ class PhaseRetrieval:
         _check(self.L.pcmbir_hip_set_nms_prior(self.h, phase_sig, phase_c, mag_sig, mag_c,
                                                delta_over_beta, int(decorrelate), p, nu),
                "pcmbir_hip_set_nms_prior")
+
+    def set_tomo_target(self, mu, a):
+        """The ENABLE_TOMO_RECONS coupling of the NMS and the pretcost:
+        mu/2 |z - a|^2 with a complex [n_views][rows][cols] (real: magnitude,
+        imag: phase) in place of the projection-domain prior; a = None
+        switches it off."""
+        if a is None:
+            _check(self.L.pcmbir_hip_set_tomo_target(self.h, 0.0, None, None),
+                   "pcmbir_hip_set_tomo_target")
+            return
+        a = np.asarray(a).reshape(self.shape)
+        self._tomo = (np.ascontiguousarray(a.real, np.float64),
+                      np.ascontiguousarray(a.imag, np.float64))
+        _check(self.L.pcmbir_hip_set_tomo_target(self.h, float(mu), _p(self._tomo[0]),
+                                                 _p(self._tomo[1])),
+               "pcmbir_hip_set_tomo_target")
 
     def nms_pass(self, zr, zi, b):
         zr = np.ascontiguousarray(zr, np.float64).reshape(self.shape).copy()
