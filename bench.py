@@ -622,7 +622,7 @@ def calibration_copy(nbytes=1 << 30):
     return nbytes
 
 
-def pcmbir_bench(views, n, sd_steps, nms):
+def pcmbir_bench(views, n, sd_steps, nms, fft_reg=1):
     """Phase-retrieval throughput on one GPU at the PCMBIR workload's shape
     (views x n x n transforms, n = 512 the padded 256^2 detector): seconds
     per steepest-descent iteration over all views (6 Fresnel transforms of
@@ -636,6 +636,7 @@ def pcmbir_bench(views, n, sd_steps, nms):
     shape = (views, n, n)
     t0 = time.time()
     ctx = PhaseRetrieval(views, n, n, 1.0, 1.0, lam, dist, win)
+    ctx.option("fft_reg", fft_reg)
     obj = np.exp(-0.01 * rng.random(shape) - 0.2j * rng.random(shape))
     y = np.abs(ctx.fresnel(obj)).astype(np.complex64)
     D = np.ones(shape, np.complex64)
@@ -653,7 +654,8 @@ def pcmbir_bench(views, n, sd_steps, nms):
     per_transform_bytes = 3 * 2 * n * n * 8          # 3 passes, read + write, complex64
     transforms = 6 * views * sd_steps                # 5 batched calls, one over 2 x views
     gbps = per_transform_bytes * transforms / (fft_ms * 1e-3) / 1e9
-    out = {"pcmbir": {"views": views, "n": n, "s_per_sd_iter": dt,
+    out = {"pcmbir": {"views": views, "n": n, "fft": "register" if fft_reg else "lds-radix2",
+                      "s_per_sd_iter": dt,
                       "fft_ms_per_sd_iter": fft_ms / sd_steps,
                       "fft_hbm_gbps": gbps, "fft_frac": gbps / PEAK_HBM_GBPS}}
     ctx.close()
@@ -745,6 +747,8 @@ def main():
                     help="phase-retrieval throughput line instead of the MBIR metric")
     ap.add_argument("--pcmbir-views", type=int, default=360)
     ap.add_argument("--pcmbir-n", type=int, default=512)
+    ap.add_argument("--pcmbir-fft", choices=("register", "lds"), default="register",
+                    help="FFT passes: register-resident radix-8 (default) or in-LDS radix-2")
     args = ap.parse_args()
     global GRAPH, NH, NH_STAGES, PREDICT_CHECKS
     GRAPH = not args.no_graph
@@ -760,7 +764,8 @@ def main():
         config4(args)
         return
     if args.pcmbir and args.gpus == 1:
-        pcmbir_bench(args.pcmbir_views, args.pcmbir_n, max(args.steps, 1), nms=True)
+        pcmbir_bench(args.pcmbir_views, args.pcmbir_n, max(args.steps, 1), nms=True,
+                     fft_reg=int(args.pcmbir_fft == "register"))
         return
     if args.gpus != 1:
         print("bench.py: the MBIR sweep runs as one process on one GPU; --gpus must be 1 "

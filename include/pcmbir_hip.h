@@ -41,6 +41,11 @@ void pcmbir_hip_destroy(pcmbir_hip_ctx* ctx);
 int pcmbir_hip_fresnel(pcmbir_hip_ctx* ctx, int32_t herm, const float* x,
                        float* y);
 
+/* Options: "fft_reg" 1 (default) runs the transforms' passes with each
+ * sequence in registers (radix-8 Stockham, LDS only between passes) where a
+ * sequence has at least 8 values, 0 the in-LDS radix-2 passes. */
+int pcmbir_hip_set_option(pcmbir_hip_ctx* ctx, const char* key, double value);
+
 /* Phase-retrieval problem of every view (XT_PhaseRet.c): measurements y, the
  * factor D, weights lambda [n_views][rows][cols] (real), proximal target v,
  * initial estimate w and phase Omega, penalty nu.  omega = NULL sets Omega to

@@ -3,6 +3,10 @@
 //
 // Device layout: complex arrays float2 [n_views][rows][cols], cols fastest.
 // Kernels:
+//   k_fft_rows_reg   batched FFTs along cols, each sequence in registers
+//                    (radix-8 Stockham, LDS between passes)
+//   k_fresnel_cols_reg  the column pass likewise, rows/8 threads per column,
+//                    up to 64 columns per workgroup (16 at 512 rows)
 //   k_fft_rows       batched in-LDS FFTs along cols; forward = bit-reversed
 //                    load + radix-2 DIT, inverse = radix-2 DIF + bit-reversed
 //                    store (both unnormalised, FFTW's conventions)
@@ -163,6 +167,184 @@ __global__ __launch_bounds__(kThreads) void k_fresnel_cols(float2* __restrict__ 
   for (int j = threadIdx.x; j < tot; j += blockDim.x) {
     const int r = j >> ltc, c = j & (TC - 1);
     base[((size_t)r << logc) + c] = s_buf[c * stride + brev(r, logr)];
+  }
+}
+
+// ------------------------------------------- register-resident FFTs
+// Stockham autosort FFT of length N = 2^logn >= 8 (radix-8 passes, then one
+// radix-2 or radix-4 pass for the remaining factor): a sequence is held by
+// N/8 threads, thread j holding elements j + (N/8) s, s = 0..7, in
+// registers, in natural order on entry and on exit.  A pass of radix R with
+// Ns = product of the earlier radices runs 8/R butterflies per thread: the
+// virtual thread vt = j + (N/8) q takes elements vt + (N/R) r, multiplies
+// them by exp(-+2 pi i (vt mod Ns) r / (Ns R)), transforms them (DFT_R) and
+// writes output r at (vt / Ns) Ns R + vt mod Ns + r Ns; between passes the
+// sequence goes through LDS (element i at lds[i ls + lo]).  The last pass's
+// outputs land where the next read would fetch them, so they stay in
+// registers.  inv = 1: exp(+...) (the inverse transform, unnormalised).
+
+// exp(-+2 pi i m / N) from tw[m] = exp(-2 pi i m / N), m < N/2.
+template <bool kInv>
+__device__ __forceinline__ float2 twid(const float2* __restrict__ tw, int m, int N) {
+  const int h = N >> 1;
+  float2 w = m < h ? tw[m] : tw[m - h];
+  if (m >= h) w = make_float2(-w.x, -w.y);
+  if (kInv) w.y = -w.y;
+  return w;
+}
+
+// times -i (forward) or +i (inverse)
+template <bool kInv>
+__device__ __forceinline__ float2 rot90(float2 a) {
+  return kInv ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+template <bool kInv>
+__device__ __forceinline__ void dft2(float2* a) {
+  const float2 t = a[0];
+  a[0] = cadd(t, a[1]);
+  a[1] = csub(t, a[1]);
+}
+
+template <bool kInv>
+__device__ __forceinline__ void dft4(float2* a) {
+  const float2 t0 = cadd(a[0], a[2]), t1 = csub(a[0], a[2]);
+  const float2 t2 = cadd(a[1], a[3]), t3 = rot90<kInv>(csub(a[1], a[3]));
+  a[0] = cadd(t0, t2);
+  a[2] = csub(t0, t2);
+  a[1] = cadd(t1, t3);
+  a[3] = csub(t1, t3);
+}
+
+// 8-point DFT: even / odd 4-point DFTs, odd outputs times exp(-+2 pi i k / 8).
+template <bool kInv>
+__device__ __forceinline__ void dft8(float2* a) {
+  float2 e[4] = {a[0], a[2], a[4], a[6]}, o[4] = {a[1], a[3], a[5], a[7]};
+  dft4<kInv>(e);
+  dft4<kInv>(o);
+  const float h = 0.70710678118654752f;  // 1 / sqrt(2)
+  o[1] = cmul(o[1], make_float2(h, kInv ? h : -h));
+  o[2] = rot90<kInv>(o[2]);
+  o[3] = cmul(o[3], make_float2(-h, kInv ? h : -h));
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    a[k] = cadd(e[k], o[k]);
+    a[k + 4] = csub(e[k], o[k]);
+  }
+}
+
+template <int R, bool kInv>
+__device__ __forceinline__ void dft_r(float2* a) {
+  if constexpr (R == 8) dft8<kInv>(a);
+  else if constexpr (R == 4) dft4<kInv>(a);
+  else dft2<kInv>(a);
+}
+
+template <int R, bool kInv>
+__device__ __forceinline__ void stockham_pass(float2 v[8], int j, int logn, int Ns,
+                                              const float2* __restrict__ tw, float2* lds, int ls,
+                                              int lo, bool last) {
+  constexpr int Q = 8 / R;
+  const int N = 1 << logn, n8 = N >> 3;
+  const int tstep = N / (Ns * R);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int vt = j + n8 * q;
+    const int k = vt & (Ns - 1);
+    float2 a[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) a[r] = v[q + Q * r];
+    if (Ns > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) a[r] = cmul(a[r], twid<kInv>(tw, k * tstep * r, N));
+    }
+    dft_r<R, kInv>(a);
+    if (last) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q + Q * r] = a[r];
+    } else {
+      const int base = (vt - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds[(base + r * Ns) * ls + lo] = a[r];
+    }
+  }
+}
+
+// The whole transform; every thread of the workgroup calls it with the same
+// logn (it holds workgroup barriers).
+template <bool kInv>
+__device__ void fft_reg(float2 v[8], int j, int logn, const float2* __restrict__ tw,
+                        float2* lds, int ls, int lo) {
+  const int p8 = logn / 3, rem = logn - 3 * p8, n8 = 1 << (logn - 3);
+  int Ns = 1;
+  for (int p = 0; p < p8; ++p) {
+    const bool last = p == p8 - 1 && rem == 0;
+    stockham_pass<8, kInv>(v, j, logn, Ns, tw, lds, ls, lo, last);
+    Ns <<= 3;
+    if (!last) {
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < 8; ++s) v[s] = lds[(j + n8 * s) * ls + lo];
+      __syncthreads();
+    }
+  }
+  if (rem == 2) stockham_pass<4, kInv>(v, j, logn, Ns, tw, lds, ls, lo, true);
+  else if (rem == 1) stockham_pass<2, kInv>(v, j, logn, Ns, tw, lds, ls, lo, true);
+}
+
+// Columns per workgroup of k_fresnel_cols_reg: 8 values per thread, rows / 8
+// threads per column, at most 1024 threads and 64 columns, at most cols.
+__host__ __device__ inline int col_tile_reg(int logr, int logc) {
+  int ltc = 0;
+  while (ltc < logc && ltc < 6 && (1 << (ltc + 1 + logr - 3)) <= 1024) ++ltc;
+  return ltc;
+}
+
+// Rows >= 8: per (view, tile of TC = 2^ltc columns) the forward FFT along
+// rows, times fac [rows][cols], the inverse FFT along rows, each column's
+// data in its threads' registers, lanes over the columns of a row (one
+// 8 TC-byte segment per row and load); LDS [rows][TC] for the exchanges.
+__global__ __launch_bounds__(1024) void k_fresnel_cols_reg(float2* __restrict__ data, int logr,
+                                                           int logc, int ltc,
+                                                           const float2* __restrict__ twg,
+                                                           const float2* __restrict__ fac) {
+  extern __shared__ float2 s_ex[];
+  const int TC = 1 << ltc, n8 = 1 << (logr - 3);
+  const int tiles = 1 << (logc - ltc);
+  const int view = blockIdx.x >> (logc - ltc), c0 = (blockIdx.x & (tiles - 1)) << ltc;
+  const int col = threadIdx.x & (TC - 1), j = threadIdx.x >> ltc;
+  float2* base = data + ((size_t)view << (logr + logc)) + c0 + col;
+  const float2* fb = fac + c0 + col;
+  float2 v[8];
+#pragma unroll
+  for (int s = 0; s < 8; ++s) v[s] = base[(size_t)(j + n8 * s) << logc];
+  fft_reg<false>(v, j, logr, twg, s_ex, TC, col);
+#pragma unroll
+  for (int s = 0; s < 8; ++s) v[s] = cmul(v[s], fb[(size_t)(j + n8 * s) << logc]);
+  fft_reg<true>(v, j, logr, twg, s_ex, TC, col);
+#pragma unroll
+  for (int s = 0; s < 8; ++s) base[(size_t)(j + n8 * s) << logc] = v[s];
+}
+
+// Rows of length 2^logn >= 8 (contiguous): 256 threads, 2048 / 2^logn rows
+// per workgroup, a row's data in 2^logn / 8 threads' registers.
+template <bool kInv>
+__global__ __launch_bounds__(256) void k_fft_rows_reg(float2* __restrict__ data, long long nrows,
+                                                      int logn,
+                                                      const float2* __restrict__ twg) {
+  __shared__ float2 s_ex[2048];
+  const int n8 = 1 << (logn - 3), S = 256 >> (logn - 3);
+  const int seq = threadIdx.x >> (logn - 3), j = threadIdx.x & (n8 - 1);
+  const long long row = (long long)blockIdx.x * S + seq;
+  const bool live = row < nrows;
+  float2* src = data + ((live ? row : 0) << logn) + j;
+  float2 v[8];
+#pragma unroll
+  for (int s = 0; s < 8; ++s) v[s] = live ? src[n8 * s] : make_float2(0.f, 0.f);
+  fft_reg<kInv>(v, j, logn, twg, s_ex, 1, seq << logn);
+  if (live) {
+#pragma unroll
+    for (int s = 0; s < 8; ++s) src[n8 * s] = v[s];
   }
 }
 
@@ -659,6 +841,7 @@ struct pcmbir_hip_ctx {
   size_t ev_used = 0;
   double fft_ms = 0.0;
   int64_t fft_launches = 0;
+  bool fft_reg = true;  // register-resident FFT passes (option; else the LDS radix-2 ones)
 };
 
 namespace {
@@ -684,13 +867,31 @@ int filter_dev(pcmbir_hip_ctx* c, float2* buf, int nbatch, const float2* fac) {
   HIP_OK(hipEventRecord(c->ev[c->ev_used].first, c->stream));
   const long long nrows = (long long)nbatch * c->rows;
   const unsigned rb = (unsigned)((nrows + (kRowElems >> c->logc) - 1) / (kRowElems >> c->logc));
-  hipLaunchKernelGGL(k_fft_rows, dim3(rb), dim3(kThreads), 0, c->stream, buf, nrows, c->logc,
-                     c->tw_c, 0);
-  const int tiles = c->cols / col_tile(c->logr, c->logc);
-  hipLaunchKernelGGL(k_fresnel_cols, dim3((unsigned)(nbatch * tiles)), dim3(kThreads), 0,
-                     c->stream, buf, c->logr, c->logc, c->tw_r, fac);
-  hipLaunchKernelGGL(k_fft_rows, dim3(rb), dim3(kThreads), 0, c->stream, buf, nrows, c->logc,
-                     c->tw_c, 1);
+  // register-resident passes where a sequence holds at least 8 values
+  const bool rows_reg = c->fft_reg && c->logc >= 3, cols_reg = c->fft_reg && c->logr >= 3;
+  if (rows_reg)
+    hipLaunchKernelGGL(k_fft_rows_reg<false>, dim3(rb), dim3(256), 0, c->stream, buf, nrows,
+                       c->logc, c->tw_c);
+  else
+    hipLaunchKernelGGL(k_fft_rows, dim3(rb), dim3(kThreads), 0, c->stream, buf, nrows, c->logc,
+                       c->tw_c, 0);
+  if (cols_reg) {
+    const int ltc = col_tile_reg(c->logr, c->logc);
+    const unsigned threads = 1u << (ltc + c->logr - 3);
+    const size_t lds = sizeof(float2) << (ltc + c->logr);
+    hipLaunchKernelGGL(k_fresnel_cols_reg, dim3((unsigned)nbatch << (c->logc - ltc)),
+                       dim3(threads), lds, c->stream, buf, c->logr, c->logc, ltc, c->tw_r, fac);
+  } else {
+    const int tiles = c->cols / col_tile(c->logr, c->logc);
+    hipLaunchKernelGGL(k_fresnel_cols, dim3((unsigned)(nbatch * tiles)), dim3(kThreads), 0,
+                       c->stream, buf, c->logr, c->logc, c->tw_r, fac);
+  }
+  if (rows_reg)
+    hipLaunchKernelGGL(k_fft_rows_reg<true>, dim3(rb), dim3(256), 0, c->stream, buf, nrows,
+                       c->logc, c->tw_c);
+  else
+    hipLaunchKernelGGL(k_fft_rows, dim3(rb), dim3(kThreads), 0, c->stream, buf, nrows, c->logc,
+                       c->tw_c, 1);
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(c->ev[c->ev_used].second, c->stream));
   c->ev_used++;
@@ -1193,6 +1394,15 @@ int pcmbir_hip_paganin(pcmbir_hip_ctx* c, const float* y, const float* D,
   HIP_OK(hipMemcpyAsync(zr, c->zr, db, hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipMemcpyAsync(zi, c->zi, db, hipMemcpyDeviceToHost, c->stream));
   return collect_events(c);
+}
+
+int pcmbir_hip_set_option(pcmbir_hip_ctx* c, const char* key, double value) {
+  if (!c || !key) return fail("null argument");
+  if (!strcmp(key, "fft_reg")) {
+    c->fft_reg = value != 0.0;
+    return 0;
+  }
+  return fail(std::string("unknown option ") + key);
 }
 
 int pcmbir_hip_fft_time(pcmbir_hip_ctx* c, double* ms, int64_t* launches) {

@@ -29,6 +29,7 @@ _SIGS = {
                                       C.c_int32, C.c_double] + [C.c_void_p] * 6),
     "pcmbir_hip_get_dual": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pcmbir_hip_set_tomo_target": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]),
+    "pcmbir_hip_set_option": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),
     "pcmbir_hip_paganin": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
                                      C.c_void_p, C.c_void_p]),
 }
@@ -118,6 +119,10 @@ class PhaseRetrieval:
             self.close()
         except Exception:
             pass
+
+    def option(self, key, value):
+        _check(self.L.pcmbir_hip_set_option(self.h, key.encode(), float(value)),
+               "pcmbir_hip_set_option")
 
     def fresnel(self, x, herm=False):
         xi = _c64(x, self.shape)

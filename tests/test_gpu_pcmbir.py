@@ -24,20 +24,27 @@ def rel_rms(a, b):
     return float(np.sqrt(np.mean(np.abs(a - b) ** 2) / np.mean(np.abs(b) ** 2)))
 
 
+@pytest.mark.parametrize("fft_reg", [1, 0])
 @pytest.mark.parametrize("views,rows,cols,dr,dc,windowed", [
     (3, 16, 32, 1.0, 1.0, False), (2, 64, 64, 0.7, 1.3, True), (2, 256, 256, 1.0, 1.0, True),
-    (1, 512, 512, 1.0, 1.0, True), (2, 2, 1024, 1.0, 0.5, False)])
-def test_fresnel_matches_oracle(views, rows, cols, dr, dc, windowed):
+    (1, 512, 512, 1.0, 1.0, True), (2, 2, 1024, 1.0, 0.5, False), (2, 1024, 8, 1.0, 1.0, True),
+    (1, 128, 4, 1.0, 1.0, False), (2, 32, 128, 0.5, 1.0, True)])
+def test_fresnel_matches_oracle(views, rows, cols, dr, dc, windowed, fft_reg):
+    """Both FFT paths: the register-resident Stockham passes (radix 8, then
+    2 or 4; every length from 8 to 1024 along rows or columns among the
+    shapes) and the in-LDS radix-2 passes."""
     from phasecon_amd.pcmbir import PhaseRetrieval
     dist = 1e5
     win = P.window(rows, cols, dr, dc, 2.0 / np.sqrt(LAM * dist)) if windowed else None
     ctx = PhaseRetrieval(views, rows, cols, dr, dc, LAM, dist, win)
+    ctx.option("fft_reg", fft_reg)
     x = rnd((views, rows, cols), rows + cols)
     for herm in (False, True):
         got = ctx.fresnel(x, herm)
         want = P.fresnel(x, dr, dc, LAM, dist, win=win, herm=herm)
         err = rel_rms(got, want)
-        print(f"{views}x{rows}x{cols} herm={herm}: rel rms {err:.3g}", flush=True)
+        print(f"{views}x{rows}x{cols} herm={herm} fft_reg={fft_reg}: rel rms {err:.3g}",
+              flush=True)
         assert err < 5e-6
     ctx.close()
 
