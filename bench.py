@@ -76,10 +76,14 @@ METRIC = "sec to MAP cost ≤1e-5 rel of FP64 oracle; equits/s; HBM GB/s; learne
 PEAK_HBM_GBPS = 8000.0
 REL_TOL = 1e-5
 
-# HBM bytes per k_sweep launch measured with rocprofv3 on this code:
-# FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (KiB x 1024), the factors read off a
-# cold 1 GiB device copy (bench.py --profile-equits 3 --calibrate).
-MEASURED_TRAFFIC = {("ch4d", 8, False): 145060.6 * 1024 * 2.0 + 120815.1 * 1024 * 1.0}
+# HBM bytes per k_sweep launch (ch4d, full resolution, tile 8) measured with
+# rocprofv3 on this code: FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (KiB per
+# dispatch x 1024), the factors read off a cold 1 GiB device copy (both passes
+# of profiles/k_sweep_ch4d_counters.md: bench.py --profile-equits 3
+# --calibrate).  A counter run cannot happen inside the timed bench, so the
+# line carries this figure with its source.
+MEASURED_TRAFFIC = {("ch4d", 8, False): 145609.5 * 1024 * 2.0 + 120815.1 * 1024 * 1.0}
+TRAFFIC_SOURCE = "profiles/k_sweep_ch4d_counters.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
 
 
 def scan_digest(scan):
@@ -964,6 +968,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBPS,
             "traffic": MEASURED_TRAFFIC.get((args.workload, tile_of(st), args.afp16)),
+            "traffic_source": TRAFFIC_SOURCE,
             "kernel": "k_sweep",
             "bytes_per_launch": bytes_launch,
             "avg_launch_ms": avg_launch_ms,
