@@ -528,11 +528,14 @@ __device__ __forceinline__ double nms_value(double d, const NmsParams& P, int t)
 }
 
 // Cost_NMS (XT_Search.c:8-43), FP64, terms in the reference's order.
-__device__ double nms_cost(double zr, double zi, const NmsPix& X, const NmsParams& P) {
+__device__ __forceinline__ double nms_cost(double zr, double zi, const NmsPix& X,
+                                          const NmsParams& P) {
   const double e = exp(-zr);
   const double cv = e * cos(-zi), sv = e * sin(-zi);
   double cost = ((cv - X.br) * (cv - X.br) + (sv - X.bi) * (sv - X.bi)) * P.nu / 2;
   if (P.tomo) return cost + ((zr - X.ar) * (zr - X.ar) + (zi - X.ai) * (zi - X.ai)) * P.mu / 2;
+  // unrolled, so that the neighbourhood stays in registers
+#pragma unroll
   for (int k = 0; k < 9; ++k) {
     if (!((X.flags >> k) & 1u)) continue;
     const double db = zr - X.nr[k], dd = zi - X.ni[k];
@@ -562,13 +565,25 @@ struct Simplex {
 };
 
 // Nelder_Mead_Simplex_2DSearch (XT_Search.c:100-215) from (zr, zi); returns
-// the iterations run and leaves the best vertex in (zr, zi).
+// the iterations run and leaves the best vertex in (zr, zi).  Each iteration
+// evaluates the reflection, then at most one more point -- the expansion
+// (reflection below the best), the outside contraction (between the second
+// and the worst) or the inside contraction (at or above the worst) -- chosen
+// per pixel before a single cost evaluation, and the shrink's two points:
+// the reference's decisions and arithmetic in an order where the lanes of a
+// wavefront that took different branches share one evaluation.
 __device__ int nms_search(double& zr, double& zi, const NmsPix& X, const NmsParams& P) {
   Simplex S;
   S.r[1] = zr; S.i[1] = zi;
   S.r[2] = 2 * zr; S.i[2] = zi;
   S.r[3] = zr; S.i[3] = 2 * zi;
-  for (int k = 1; k < 4; ++k) S.c[k] = nms_cost(S.r[k], S.i[k], X, P);
+#pragma unroll 1
+  for (int k = 1; k < 4; ++k) {
+    const double c = nms_cost(k == 2 ? 2 * zr : zr, k == 3 ? 2 * zi : zi, X, P);
+    if (k == 1) S.c[1] = c;
+    else if (k == 2) S.c[2] = c;
+    else S.c[3] = c;
+  }
   S.sort();
   S.r[0] = (S.r[1] + S.r[2]) / 2.0;
   S.i[0] = (S.i[1] + S.i[2]) / 2.0;
@@ -579,32 +594,50 @@ __device__ int nms_search(double& zr, double& zi, const NmsPix& X, const NmsPara
     const double rr = S.r[0] + P.rho * (S.r[0] - S.r[3]);
     const double ri = S.i[0] + P.rho * (S.i[0] - S.i[3]);
     const double cr = nms_cost(rr, ri, X, P);
+    // -1 none (incomparable costs), 0 accept the reflection, 1 expand,
+    // 2 outside contraction, 3 inside contraction
+    int kind = -1;
+    if (cr < S.c[2] && cr >= S.c[1]) kind = 0;
+    else if (cr < S.c[1]) kind = 1;
+    else if (cr >= S.c[2] && cr < S.c[3]) kind = 2;
+    else if (cr >= S.c[3]) kind = 3;
+    double pr = rr, pi = ri;
+    if (kind == 1) {
+      pr = S.r[0] + P.chi * (rr - S.r[0]);
+      pi = S.i[0] + P.chi * (ri - S.i[0]);
+    } else if (kind == 2) {
+      pr = S.r[0] + P.gamma * (rr - S.r[0]);
+      pi = S.i[0] + P.gamma * (ri - S.i[0]);
+    } else if (kind == 3) {
+      pr = S.r[0] - P.gamma * (S.r[0] - S.r[3]);
+      pi = S.i[0] - P.gamma * (S.i[0] - S.i[3]);
+    }
+    double cp = 0.0;
+    if (kind >= 1) cp = nms_cost(pr, pi, X, P);
     bool shrink = false;
-    if (cr < S.c[2] && cr >= S.c[1]) {
+    if (kind == 0) {
       S.accept(rr, ri, cr);
-    } else if (cr < S.c[1]) {
-      const double er = S.r[0] + P.chi * (rr - S.r[0]), ei = S.i[0] + P.chi * (ri - S.i[0]);
-      const double ce = nms_cost(er, ei, X, P);
-      if (ce < cr) S.accept(er, ei, ce);
+    } else if (kind == 1) {
+      if (cp < cr) S.accept(pr, pi, cp);
       else S.accept(rr, ri, cr);
-    } else if (cr >= S.c[2] && cr < S.c[3]) {
-      const double orr = S.r[0] + P.gamma * (rr - S.r[0]), ori = S.i[0] + P.gamma * (ri - S.i[0]);
-      const double co = nms_cost(orr, ori, X, P);
-      if (co <= cr) S.accept(orr, ori, co);
+    } else if (kind == 2) {
+      if (cp <= cr) S.accept(pr, pi, cp);
       else shrink = true;
-    } else if (cr >= S.c[3]) {
-      const double ir = S.r[0] - P.gamma * (S.r[0] - S.r[3]), ii = S.i[0] - P.gamma * (S.i[0] - S.i[3]);
-      const double ci = nms_cost(ir, ii, X, P);
-      if (ci < S.c[3]) S.accept(ir, ii, ci);
+    } else if (kind == 3) {
+      if (cp < S.c[3]) S.accept(pr, pi, cp);
       else shrink = true;
     }
     if (shrink) {  // NMS_Shrink
       S.r[2] = S.r[1] + P.sigma * (S.r[2] - S.r[1]);
       S.i[2] = S.i[1] + P.sigma * (S.i[2] - S.i[1]);
-      S.c[2] = nms_cost(S.r[2], S.i[2], X, P);
       S.r[3] = S.r[1] + P.sigma * (S.r[3] - S.r[1]);
       S.i[3] = S.i[1] + P.sigma * (S.i[3] - S.i[1]);
-      S.c[3] = nms_cost(S.r[3], S.i[3], X, P);
+#pragma unroll 1
+      for (int k = 2; k < 4; ++k) {
+        const double c = nms_cost(k == 2 ? S.r[2] : S.r[3], k == 2 ? S.i[2] : S.i[3], X, P);
+        if (k == 2) S.c[2] = c;
+        else S.c[3] = c;
+      }
       S.sort();
     }
     S.r[0] = (S.r[1] + S.r[2]) / 2.0;
@@ -624,15 +657,11 @@ __device__ int nms_search(double& zr, double& zi, const NmsPix& X, const NmsPara
 // One colour (j%2, k%2) = (cj, ck) of the four-colour NMS pass: every pixel
 // of that colour of every active view, one thread each; pixels of a colour
 // are not neighbours, so they update independently.  b = w - v from w, vd.
-__global__ __launch_bounds__(kThreads) void k_nms_colour(NmsParams P, double* __restrict__ zr,
-                                                         double* __restrict__ zi,
-                                                         const float2* __restrict__ w,
-                                                         const float2* __restrict__ vd,
-                                                         const int* __restrict__ active,
-                                                         const double* __restrict__ ar,
-                                                         const double* __restrict__ ai,
-                                                         int n_views, int rows, int cols, int cj,
-                                                         int ck, int* __restrict__ iters) {
+__global__ __launch_bounds__(kThreads) void k_nms_colour(
+    NmsParams P, double* __restrict__ zr, double* __restrict__ zi, const float2* __restrict__ w,
+    const float2* __restrict__ vd, const int* __restrict__ active, const double* __restrict__ ar,
+    const double* __restrict__ ai, int n_views, int rows, int cols, int cj, int ck,
+    int* __restrict__ iters) {
   const int hr = (rows - cj + 1) / 2, hc = (cols - ck + 1) / 2;
   const long long per = (long long)hr * hc;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
