@@ -8,8 +8,9 @@ the reference does) within 1e-5 relative, at each tile the config-4
 autotuner may pick (8, 16) with the benchmark's over-relaxation; the error
 of each against the FP64-A oracle run is printed (cost and image RMSE).
 Tile 4 is left out as in the benchmark: on config 4's coarse stages it
-diverges at alpha >= 1.5 (the mechanism of the 4D benchmark's coarsest
-stage, reproduced by the sweep model in tests/test_sweep_model.py).
+diverges at alpha >= 1.5 (staleness across concurrently updated tiles; the
+sweep model reproduces the tile-4 / tile-8 contrast on config 4's coarsest
+stage in tests/test_sweep_model.py).
 """
 import numpy as np
 import pytest

@@ -82,3 +82,62 @@ def test_guard_recovers_the_model_divergence(coarse_stage):
           + " ".join(f"{v:.4g}" for v in after), flush=True)
     assert recovered_at is not None and abs(alpha - 1.485) < 1e-12
     assert len(after) >= 8 and np.all(np.diff(after) < 0)
+
+
+def _cfg4_coarse_scan(frames=8, seed=0):
+    """Config 4's geometry at its coarsest stage on the CPU: a 512-channel,
+    32-row detector, 28-32 views per frame (K = 8 interlaced), 8 frames, the
+    workload's drifting-ellipsoid phantom at the 64x64x4 stage's resolution
+    (voxels of 8 x 8 channels and 8 rows) projected by the oracle, Poisson
+    counts at I0 = 5000."""
+    from oracle.oracle import Oracle
+    from phasecon_amd.mbir import interlaced_views
+    from phasecon_amd.synth import drifting_ellipsoids, uneven_frames
+    views, cols, rows = 30 * frames, 512, 32
+    angles, times = interlaced_views(8, views, views)
+    per = uneven_frames(views, frames, 28, 32, seed + 77)
+    first = np.concatenate([[0], np.cumsum(per)[:-1]])
+    recon_times = np.concatenate([times[first], [float(views)]]).astype(np.float32)
+    recon_times[0] = 0.0
+    scan = dict(name="cfg4_coarse", projections=np.zeros((views, rows, cols), np.float32),
+                weights=np.ones((views, rows, cols), np.float32), angles=angles,
+                proj_times=times, recon_times=recon_times, proj_rows=rows, proj_cols=cols,
+                proj_num=views, recon_num=frames, vox_wid=1.0, rot_center=cols / 2.0,
+                sig_s=0.01, sig_t=0.001, c_s=0.01, c_t=0.01, p_s=1.2, p_t=1.1, seed=seed)
+    o = Oracle(scan, mult_xy=8, mult_z=8)
+    line = o.forward_project(drifting_ellipsoids(64, 4, frames, seed).astype(np.float64))
+    line = np.ascontiguousarray(line.transpose(0, 2, 1))
+    counts = np.maximum(np.random.default_rng(seed + 1000).poisson(5e3 * np.exp(-line)), 1.0)
+    scan["projections"] = np.log(5e3 / counts).astype(np.float32)
+    scan["weights"] = counts.astype(np.float32)
+    return scan
+
+
+def test_config4_coarse_stage_tile4_diverges_where_tile8_converges():
+    """Config 4's 64x64x4 stage, where the GPU sweep diverges with tile 4 at
+    alpha >= 1.5 and converges with tile 8 at 1.9.  In the model (frames on
+    their own cache groups, tiles seeing each other's updates 4 columns
+    late), tile 4 at alpha 1.5 (64 concurrent tiles per frame, each column
+    17 channels x 32 rows of ~30 views) diverges within 8 equits, while tile
+    8 at alpha 1.9 (16 tiles per frame) lowers the cost at every equit; with
+    1 or 2 columns of lag tile 4 converges too, so on this geometry the GPU's
+    effective staleness exceeds the one column that reproduces the 4D
+    benchmark's coarsest stage above."""
+    from oracle.oracle import Oracle
+    scan = _cfg4_coarse_scan()
+    runs = {}
+    for T, alpha, n in ((4, 1.5, 8), (8, 1.9, 10)):
+        o = Oracle(scan, mult_xy=8, mult_z=8)
+        x = np.zeros(o.shape_x)
+        e = o.init_error(x)
+        c = [o.cost(e, x)[0]]
+        for it in range(1, n + 1):
+            o.icd_equit_async(x, e, it, T, alpha, 761521 + 7919 * it, 0, 4, -1, 8, 1)
+            c.append(o.cost(e, x)[0])
+            if not np.isfinite(c[-1]) or c[-1] > 10 * c[0]:
+                break
+        runs[T] = np.array(c)
+        print(f"tile {T}, alpha {alpha}: " + " ".join(f"{v:.4g}" for v in c), flush=True)
+    c4, c8 = runs[4], runs[8]
+    assert np.any(np.diff(c4) > 0) and c4[-1] > c4[0]
+    assert len(c8) == 11 and np.all(np.diff(c8) < 0)
