@@ -1108,8 +1108,14 @@ constexpr int kNhMax = 2 * kSweepThreads;  // tile columns a non-homogeneous pas
 // then wavefront 0 sends the column's error update with atomics and moves on
 // to the next column's theta while wavefront 1 starts its terms.
 // kW = W, the A window width.
+// The small-frame variants with window widths up to 5 are held to 128 VGPRs
+// (four wavefronts per SIMD, two spilled registers at W = 5): at 128x128x4
+// with tile 4 a launch's 2048 workgroups then fit the chip at once instead
+// of in two rounds.
 template <int kW, bool kSmall, bool kHalf>
-__global__ __launch_bounds__(kSweepThreads) void k_sweep(SwGeom G, SweepArgs A) {
+__global__ __launch_bounds__(kSweepThreads)
+__attribute__((amdgpu_waves_per_eu((kSmall && kW <= 5) ? 4 : 1))) void k_sweep(SwGeom G,
+                                                                              SweepArgs A) {
   extern __shared__ __align__(16) float smem[];
   __shared__ float s_tn[kWinBufs][2 * kNZMax];
   __shared__ float s_row1[kNTMax], s_row2[kNTMax];
