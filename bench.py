@@ -208,7 +208,10 @@ def to_target(st, target, max_equits, seed, refresh):
                 break
         check_at = it + 1
         gap = cost - cstar
-        if PREDICT_CHECKS and 0 < gap < prev[1]:
+        if target <= 0:
+            # no target (a reference run to max_equits): the guard's checks only
+            check_at = it + COARSE_CHECK
+        elif PREDICT_CHECKS and 0 < gap < prev[1]:
             rate = (gap / prev[1]) ** (1.0 / (it - prev[0]))
             need = math.log((target - cstar) / gap) / math.log(rate)
             check_at = it + max(1, int(0.5 * need))
@@ -428,7 +431,8 @@ def config4(args):
     for afp16 in (False, True):
         coarse = [(Stage(scan, mult_xy=mx, mult_z=mz, tile=8, alpha=a, afp16=afp16), n)
                   for (mx, mz), n, a in zip(sched[:-1], counts,
-                                            coarse_alphas(args.coarse_alpha, len(sched) - 1))]
+                                            coarse_alphas(args.config4_coarse_alpha,
+                                                          len(sched) - 1))]
         st = Stage(scan, tile=8, alpha=args.alpha, afp16=afp16)
         tiles = {}
         for s_ in [cs for cs, _ in coarse] + [st]:
@@ -724,6 +728,9 @@ def main():
     ap.add_argument("--config4", action="store_true",
                     help="the mixed-precision 512^2 x 32 x 24 workload, FP32 vs FP16 A")
     ap.add_argument("--config4-coarse", default="40,60,100")
+    ap.add_argument("--config4-coarse-alpha", default="1.97",
+                    help="over-relaxation of config 4's coarse stages (one value or one per "
+                         "stage; tiles 8 and 16 there)")
     ap.add_argument("--config4-ref-equits", type=int, default=300)
     ap.add_argument("--profile-unet", type=int, default=0,
                     help="only time N forward passes of the warm-start network")
