@@ -82,7 +82,10 @@ REL_TOL = 1e-5
 # of profiles/k_sweep_ch4d_counters.md: bench.py --profile-equits 3
 # --calibrate).  A counter run cannot happen inside the timed bench, so the
 # line carries this figure with its source.
-MEASURED_TRAFFIC = {("ch4d", 8, False): 145609.5 * 1024 * 2.0 + 120815.1 * 1024 * 1.0}
+# Keyed by (workload, tile, FP16 A, paired parities): a paired launch holds
+# two (colour, frame parity) sets.
+MEASURED_TRAFFIC = {("ch4d", 8, False, False): 145609.5 * 1024 * 2.0 + 120815.1 * 1024 * 1.0,
+                    ("ch4d", 8, False, True): 301410.4 * 1024 * 2.0 + 241630.7 * 1024 * 1.0}
 TRAFFIC_SOURCE = "profiles/k_sweep_ch4d_counters.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
 
 
@@ -741,6 +744,9 @@ def main():
                     help="store A in FP16 with a per-column FP32 scale")
     ap.add_argument("--placement", type=int, choices=(0, 1), default=0,
                     help="sweep work items on the XCDs: 0 tile-major, 1 each frame on one XCD")
+    ap.add_argument("--pair-parities", default="1",
+                    help="paired-parity launches (4 per equit instead of 8): one value for "
+                         "every stage or a comma list, coarsest first, full resolution last")
     ap.add_argument("--xcd-check", action="store_true",
                     help="count sweep work items that ran off their intended XCD")
     ap.add_argument("--no-cpu", action="store_true")
@@ -825,7 +831,15 @@ def main():
                 coarse.append((Stage(scan, mult_xy=mxy, mult_z=mz, tile=args.coarse_tile,
                                      afp16=args.afp16, alpha=a), n))
 
-    for s_ in [cs for cs, _ in coarse] + [st]:
+    stages_all = [cs for cs, _ in coarse] + [st]
+    pairs = [int(v) for v in str(args.pair_parities).split(",")]
+    if len(pairs) == 1:
+        pairs = pairs * len(stages_all)
+    if len(pairs) != len(stages_all):
+        sys.exit(f"bench.py: --pair-parities needs 1 or {len(stages_all)} values")
+    for s_, pp in zip(stages_all, pairs):
+        s_.option("pair_parities", pp)
+    for s_ in stages_all:
         s_.option("placement", args.placement)
         if args.xcd_check:
             s_.option("xcd_check", 1)
@@ -959,6 +973,8 @@ def main():
         },
         "reached_target": reached,
         "equits_to_target": equits,
+        "pair_parities": {f"{s_.g.N_x}x{s_.g.N_y}x{s_.g.N_z}": pp
+                          for s_, pp in zip(stages_all, pairs)},
         "placement": {"frames_on_xcds": args.placement == 1,
                       "xcd_check": ({f"{s_.g.N_x}x{s_.g.N_y}x{s_.g.N_z}": s_.xcd_check()
                                      for s_ in [cs for cs, _ in coarse] + [st]}
@@ -974,7 +990,8 @@ def main():
             "peak": PEAK_HBM_GBPS,
             "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBPS,
-            "traffic": MEASURED_TRAFFIC.get((args.workload, tile_of(st), args.afp16)),
+            "traffic": MEASURED_TRAFFIC.get((args.workload, tile_of(st), args.afp16,
+                                             bool(pairs[-1]))),
             "traffic_source": TRAFFIC_SOURCE,
             "kernel": "k_sweep",
             "bytes_per_launch": bytes_launch,

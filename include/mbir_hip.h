@@ -86,8 +86,16 @@ void mbir_hip_destroy(mbir_hip_ctx* ctx);
  * traffic then stays in one L2; with other frame counts a frame spans
  * 8 / gcd(frames, 8) XCDs; measured slower on the 4D benchmark).
  * "xcd_check" (0 or 1): the sweep counts its work items and those that ran
- * on another XCD than workgroup id mod 8 (mbir_hip_xcd_check).  None of
- * these has a counterpart in the reference. */
+ * on another XCD than workgroup id mod 8 (mbir_hip_xcd_check).
+ * "pair_parities" (0 or 1, default 0; placement 0): an equit's 8 (colour,
+ * frame parity) launches become 4, launch k running colour k on the even
+ * frames together with colour k + 1 on the odd ones -- sets with no
+ * neighbouring voxels and disjoint views, so one launch holds twice the
+ * work and the visiting order stays a sequential one.  With 4x4 tiles on
+ * small grids the paired sweep needs a lower over-relaxation than the
+ * colour-major one (measured: 64x64x4 x 4 frames diverges at alpha >= 1.8
+ * paired, converges at 1.9 unpaired).  None of these has a counterpart in
+ * the reference. */
 int mbir_hip_set_option(mbir_hip_ctx* ctx, const char* key, double value);
 
 /* Counters of option "xcd_check" since it was set: sweep work items run and

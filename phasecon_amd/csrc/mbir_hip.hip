@@ -1060,6 +1060,12 @@ struct SweepArgs {
   const int* tile_cols;
   const int* launch_tiles;
   int n_launch_tiles;
+  // paired launches (option pair_parities): workgroups from nwg1 on run a
+  // second (colour, frame parity) set, its tiles, count, parity and seed
+  int nwg1;
+  const int* launch_tiles2;
+  int n_launch_tiles2, fparity2;
+  unsigned seed2;
   int tiles_x;  // tiles per row of the tile grid
   int tile;     // tile edge T
   int nvmax;    // most views in one frame (sizes the LDS windows)
@@ -1128,27 +1134,38 @@ __attribute__((amdgpu_waves_per_eu((kSmall && kW <= 5) ? 4 : 1))) void k_sweep(S
   // scalar register, so the role branch is a uniform (scalar) branch and each
   // wavefront executes only its own loop's barriers.
   const bool w0 = __builtin_amdgcn_readfirstlane(tid) < kWave;
-  const int wid = blockIdx.x;
+  const int wid_all = blockIdx.x;
   const int iter = A.iter_dev ? *A.iter_dev + A.iter : A.iter;
   load_prior(G, &s_prior);
+  // this workgroup's (colour, frame parity) set
+  int wid = wid_all, ntl = A.n_launch_tiles, fpar = A.fparity;
+  const int* ltiles = A.launch_tiles;
+  unsigned lseed = A.seed;
+  if (wid_all >= A.nwg1) {
+    wid = wid_all - A.nwg1;
+    ntl = A.n_launch_tiles2;
+    fpar = A.fparity2;
+    ltiles = A.launch_tiles2;
+    lseed = A.seed2;
+  }
   int tl, fl;
   if (A.place_r == 0) {
-    tl = wid % A.n_launch_tiles;
-    fl = wid / A.n_launch_tiles;
+    tl = wid % ntl;
+    fl = wid / ntl;
   } else {
     const int s = wid / kXcds, u = wid % kXcds + kXcds * (s / A.unit_slots);
     fl = u % A.nfl;
     tl = u / A.nfl + A.place_r * (s % A.unit_slots);
-    if (tl >= A.n_launch_tiles) return;
+    if (tl >= ntl) return;
   }
-  const int tile = A.launch_tiles[tl];
-  const int f = A.fparity + 2 * fl;
+  const int tile = ltiles[tl];
+  const int f = fpar + 2 * fl;
   if (f >= G.N_time) return;
   if (A.xcd_check && tid == 0) {
     // the XCD this workgroup runs on (hardware register XCC_ID, bits 3:0)
     const unsigned xcc = (unsigned)__builtin_amdgcn_s_getreg((3 << 11) | kHwRegXccId) & 15u;
     atomicAdd(&A.xcd_check[0], 1u);
-    if (xcc != (unsigned)(wid % kXcds)) atomicAdd(&A.xcd_check[1], 1u);
+    if (xcc != (unsigned)(wid_all % kXcds)) atomicAdd(&A.xcd_check[1], 1u);
   }
   const int c0 = A.tile_ptr[tile];
   const int n_all = A.tile_ptr[tile + 1] - c0;
@@ -1168,7 +1185,7 @@ __attribute__((amdgpu_waves_per_eu((kSmall && kW <= 5) ? 4 : 1))) void k_sweep(S
   const size_t vol = (size_t)G.N_y * G.N_x * Nz;
   float* xf = A.x + (size_t)f * vol;
   // seeded random visiting order: k -> (pa k + pb) mod n, gcd(pa, n) = 1
-  const unsigned h = mix32(A.seed ^ mix32((unsigned)tile * 131071u + (unsigned)f * 524287u));
+  const unsigned h = mix32(lseed ^ mix32((unsigned)tile * 131071u + (unsigned)f * 524287u));
   int pa = 1, pb = 0;
   if (n > 1) {
     pa = 1 + (int)(h % (unsigned)(n - 1));
@@ -2440,6 +2457,7 @@ struct mbir_hip_ctx {
   // to XCDs
   int placement = 0;
   unsigned* d_xcd = nullptr;  // [items run, items off their XCD] (option "xcd_check")
+  bool pair = false;          // option "pair_parities"
   float* d_snap = nullptr;    // saved object (mbir_hip_object_snapshot)
   bool snap_valid = false;
   // PCMBIR tomographic mode (mbir_hip_pc_enable): d_x holds delta, d_x2
@@ -2756,6 +2774,11 @@ int mbir_hip_set_option(mbir_hip_ctx* c, const char* key, double value) {
   } else if (k == "placement") {
     if (value != 0 && value != 1) return fail("placement must be 0 or 1");
     c->placement = (int)value;
+  } else if (k == "pair_parities") {
+    // four launches per equit, (colour k, even frames) with (colour k + 1,
+    // odd frames) in each (placement 0 only)
+    if (value != 0 && value != 1) return fail("pair_parities must be 0 or 1");
+    c->pair = value == 1;
   } else if (k == "xcd_check") {
     // count the sweep's work items and those that ran off their XCD
     if (value != 0 && value != 1) return fail("xcd_check must be 0 or 1");
@@ -3027,16 +3050,42 @@ int sweep_launches(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t seed
   // register-light prefetch for frames of at most 64 views and 256 A values
   const bool small = c->nvmax <= kWave && c->nvmax * c->G.W <= 4 * kWave;
   *launches = 0;
+  // the equit's (colour, frame parity) sets in launch order: colour-major,
+  // or with pair_parities (placement 0) (colour k, even frames) together
+  // with (colour k + 1, odd frames) -- no voxel of one set neighbours a
+  // voxel of the other (other frames, other columns) and their frames own
+  // disjoint views, so the two run in one launch
+  struct Set {
+    int colour, fpar;
+  };
+  std::vector<std::pair<Set, Set>> plan;
+  const bool pair = c->pair && c->placement == 0;
+  for (int k = 0; k < 4; ++k) {
+    if (pair) {
+      plan.push_back({{k, 0}, {(k + 1) % 4, 1}});
+    } else {
+      plan.push_back({{k, 0}, {-1, 0}});
+      plan.push_back({{k, 1}, {-1, 0}});
+    }
+  }
+  auto set_ntl = [&](const Set& q) {
+    return q.colour < 0 ? 0 : c->launch_off[q.colour + 1] - c->launch_off[q.colour];
+  };
+  auto set_frames = [&](const Set& q) { return (nf - q.fpar + 1) / 2; };
   for (int it = 0; it < n; ++it) {
     A.iter = (timed ? first_iter : 0) + it;
-    for (int colour = 0; colour < 4; ++colour) {
-      const int ntl = c->launch_off[colour + 1] - c->launch_off[colour];
-      if (ntl == 0) continue;
-      A.launch_tiles = c->d_launch_tiles + c->launch_off[colour];
-      A.n_launch_tiles = ntl;
-      for (int fpar = 0; fpar < 2; ++fpar) {
-        const int frames = (nf - fpar + 1) / 2;
-        if (frames <= 0) continue;
+    for (const auto& pr : plan) {
+      Set a = pr.first, b = pr.second;
+      bool ha = set_ntl(a) > 0 && set_frames(a) > 0, hb = set_ntl(b) > 0 && set_frames(b) > 0;
+      if (!ha && !hb) continue;
+      if (!ha) {
+        a = b;
+        hb = false;
+      }
+      {
+        const int colour = a.colour, fpar = a.fpar, ntl = set_ntl(a), frames = set_frames(a);
+        A.launch_tiles = c->d_launch_tiles + c->launch_off[colour];
+        A.n_launch_tiles = ntl;
         A.fparity = fpar;
         A.seed = seed * 2654435761u + (unsigned)(it * 8 + colour * 2 + fpar) * 40503u;
         unsigned nwg = (unsigned)(ntl * frames);
@@ -3045,15 +3094,28 @@ int sweep_launches(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t seed
         A.unit_slots = 0;
         if (c->placement == 1) {
           // frames bound to XCDs: r parts per frame, lcm(frames, 8) units
-          int a = frames, b = kXcds;
-          while (b) {
-            const int t = a % b;
-            a = b;
-            b = t;
+          int ga = frames, gb = kXcds;
+          while (gb) {
+            const int t = ga % gb;
+            ga = gb;
+            gb = t;
           }
-          A.place_r = kXcds / a;
+          A.place_r = kXcds / ga;
           A.unit_slots = (ntl + A.place_r - 1) / A.place_r;
           nwg = (unsigned)(frames * A.place_r * A.unit_slots);
+        }
+        A.nwg1 = 0x7fffffff;
+        A.launch_tiles2 = nullptr;
+        A.n_launch_tiles2 = 1;
+        A.fparity2 = 0;
+        A.seed2 = 0;
+        if (hb) {
+          A.nwg1 = (int)nwg;
+          A.launch_tiles2 = c->d_launch_tiles + c->launch_off[b.colour];
+          A.n_launch_tiles2 = set_ntl(b);
+          A.fparity2 = b.fpar;
+          A.seed2 = seed * 2654435761u + (unsigned)(it * 8 + b.colour * 2 + b.fpar) * 40503u;
+          nwg += (unsigned)(set_ntl(b) * set_frames(b));
         }
         if (timed && ev_begin(c, MBIR_K_SWEEP)) return -1;
         const dim3 grid(nwg), block(kSweepThreads);
@@ -3316,7 +3378,8 @@ int mbir_hip_icd_graph(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t 
   if (!c->d_iter && dev_alloc(&c->d_iter, 1)) return -1;
   const long long key[7] = {n, (long long)seed, c->tile, c->afp16 ? 1 : 0,
                             (long long)((c->alpha > 0 ? c->alpha : c->g.alpha) * 1e9),
-                            (c->nh ? 1 : 0) | (c->placement << 1) | (c->d_xcd ? 4 : 0),
+                            (c->nh ? 1 : 0) | (c->placement << 1) | (c->d_xcd ? 4 : 0) |
+                                (c->pair ? 8 : 0),
                             (long long)(uintptr_t)c->d_xcd};
   if (!c->gexec || memcmp(key, c->gkey, sizeof(key)) != 0) {
     if (c->gexec) HIP_OK(hipGraphExecDestroy(c->gexec));

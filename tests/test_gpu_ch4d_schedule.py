@@ -4,7 +4,8 @@ schedule and settings: 64x64x2 for 60 equits (tile 4, alpha 1.8),
 128x128x4 for 100 equits (tile 4, alpha 1.97), then full resolution (tile
 8, alpha 1.97) until the FP64 MAP cost is within 1e-5 of C*, the FP64
 oracle's converged cost (tests/golden/cstar_ch4d.json, recorded with the
-scan's SHA-256).  Three seeds of the visiting order, including the seed
+scan's SHA-256), with the benchmark's paired-parity launches.  Five seeds
+of the visiting order, including the seed
 offset at which alpha 1.97 on the coarsest stage diverged: every run must
 reach the target with no recovery of the cost-rise guard (the guard checks
 every 20 coarse equits and at every scheduled full-resolution check, so no
@@ -32,13 +33,15 @@ def ch4d_stages():
     coarse = [(Stage(scan, mult_xy=4, mult_z=4, tile=4, alpha=1.8), 60),
               (Stage(scan, mult_xy=2, mult_z=2, tile=4, alpha=1.97), 100)]
     st = Stage(scan, tile=8, alpha=1.97)
+    for s_ in [cs for cs, _ in coarse] + [st]:
+        s_.option("pair_parities", 1)   # the benchmark's launches
     yield bench, coarse, st, rec["cstar"]
     for s_, _ in coarse:
         s_.close()
     st.close()
 
 
-@pytest.mark.parametrize("offset", [0, 4, 9])
+@pytest.mark.parametrize("offset", [0, 4, 9, 13, 17])
 def test_benchmark_schedule_reaches_fp64_target(ch4d_stages, offset):
     bench, coarse, st, cstar = ch4d_stages
     target = cstar * (1.0 + bench.REL_TOL)

@@ -68,6 +68,7 @@ def test_sweep_matches_oracle_schedule(name, placement, alpha, p_t, n0, block):
     x0 = _start(o, n0, block)
     st = Stage(scan, tile=T, alpha=alpha, p_t=p_t)
     st.option("placement", placement)
+    st.option("pair_parities", 0)  # the launch order the oracle's model follows
     st.set_object(x0.astype(np.float32))
     st.build_amatrix()
     st.init_error()
