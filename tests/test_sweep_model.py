@@ -114,19 +114,20 @@ def _cfg4_coarse_scan(frames=8, seed=0):
 
 
 def test_config4_coarse_stage_tile4_diverges_where_tile8_converges():
-    """Config 4's 64x64x4 stage, where the GPU sweep diverges with tile 4 at
-    alpha >= 1.5 and converges with tile 8 at 1.9.  In the model (frames on
-    their own cache groups, tiles seeing each other's updates 4 columns
-    late), tile 4 at alpha 1.5 (64 concurrent tiles per frame, each column
-    17 channels x 32 rows of ~30 views) diverges within 8 equits, while tile
-    8 at alpha 1.9 (16 tiles per frame) lowers the cost at every equit; with
+    """Config 4's 64x64x4 stage (4 of its frames), where the GPU sweep
+    diverges with tile 4 at alpha >= 1.5 and converges with tile 8 at 1.9.
+    In the model (frames on their own cache groups, tiles seeing each
+    other's updates 4 columns late), tile 4 at alpha 1.5 (64 concurrent
+    tiles per frame, each column 17 channels x 32 rows of ~30 views)
+    diverges within 8 equits, while tile 8 at alpha 1.9 (16 tiles per
+    frame) lowers the cost at each of 6 equits; with
     1 or 2 columns of lag tile 4 converges too, so on this geometry the GPU's
     effective staleness exceeds the one column that reproduces the 4D
     benchmark's coarsest stage above."""
     from oracle.oracle import Oracle
-    scan = _cfg4_coarse_scan()
+    scan = _cfg4_coarse_scan(frames=4)
     runs = {}
-    for T, alpha, n in ((4, 1.5, 8), (8, 1.9, 10)):
+    for T, alpha, n in ((4, 1.5, 8), (8, 1.9, 6)):
         o = Oracle(scan, mult_xy=8, mult_z=8)
         x = np.zeros(o.shape_x)
         e = o.init_error(x)
@@ -140,4 +141,4 @@ def test_config4_coarse_stage_tile4_diverges_where_tile8_converges():
         print(f"tile {T}, alpha {alpha}: " + " ".join(f"{v:.4g}" for v in c), flush=True)
     c4, c8 = runs[4], runs[8]
     assert np.any(np.diff(c4) > 0) and c4[-1] > c4[0]
-    assert len(c8) == 11 and np.all(np.diff(c8) < 0)
+    assert len(c8) == 7 and np.all(np.diff(c8) < 0)
