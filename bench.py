@@ -708,6 +708,8 @@ def main():
     ap.add_argument("--nh-stages", choices=("full", "coarse", "all"), default="full")
     ap.add_argument("--check-every-equit", action="store_true",
                     help="evaluate the stopping cost after every full-resolution equit")
+    ap.add_argument("--tile-candidates", default="4,6,8,12,16",
+                    help="tiles the autotuner times per stage")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep --tile / --coarse-tile instead of timing candidates per stage")
     ap.add_argument("--stages", default="",
@@ -856,7 +858,8 @@ def main():
         for s_ in [cs for cs, n in coarse if n > 0] + [st]:
             s_.set_object(np.zeros(s_.shape_x, np.float32))
             s_.build_amatrix()
-            best, tms = tune_tile(s_)
+            best, tms = tune_tile(s_, candidates=tuple(int(v) for v in
+                                                       args.tile_candidates.split(",")))
             tuned[f"{s_.g.N_x}x{s_.g.N_y}x{s_.g.N_z}"] = {
                 "tile": best, "ms_per_launch": {str(k): round(v, 4) for k, v in tms.items()}}
         print(f"[bench] autotuned tiles: {tuned}", file=sys.stderr, flush=True)
