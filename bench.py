@@ -25,8 +25,12 @@ cost-rise guard (a confirmed rise restores the object of the lowest cost
 and lowers alpha; recoveries are reported in the line).  Every stage's A
 build and error sinogram are inside the timed region, as are the parallel
 ICD equits and the cost checks (maintained error sinogram, FP32 prior
-potentials summed in FP64).  Each equit's 8 sweep launches are one replay of a captured
-HIP graph (mbir_hip_icd_graph; --no-graph launches them directly).  Inputs are resident in HBM before the timed region.
+potentials summed in FP64).  Each equit's sweep launches -- 4 with the
+paired-parity launches of every stage (--pair-parities, default 1), 8
+colour-major -- are one replay of a captured HIP graph (mbir_hip_icd_graph;
+--no-graph launches them directly); the coarse stages run 60 and 80 equits
+(--coarse-equits) and each stage's tile comes from the autotuner
+(--tile-candidates).  Inputs are resident in HBM before the timed region.
 Prints one JSON line.
 
 Learned warm start (skipped with --no-learned): training scans of the same
@@ -85,7 +89,8 @@ REL_TOL = 1e-5
 # Keyed by (workload, tile, FP16 A, paired parities): a paired launch holds
 # two (colour, frame parity) sets.
 MEASURED_TRAFFIC = {("ch4d", 8, False, False): 145609.5 * 1024 * 2.0 + 120815.1 * 1024 * 1.0,
-                    ("ch4d", 8, False, True): 301410.4 * 1024 * 2.0 + 241630.7 * 1024 * 1.0}
+                    ("ch4d", 8, False, True): 301410.4 * 1024 * 2.0 + 241630.7 * 1024 * 1.0,
+                    ("ch4d", 12, False, True): 310021.1 * 1024 * 2.0 + 241499.1 * 1024 * 1.0}
 TRAFFIC_SOURCE = "profiles/k_sweep_ch4d_counters.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
 
 
@@ -695,7 +700,7 @@ def main():
     ap.add_argument("--workload", default="ch4d")
     ap.add_argument("--tile", type=int, default=8)
     ap.add_argument("--schedule", choices=("multires", "single"), default="multires")
-    ap.add_argument("--coarse-equits", default="60,100",
+    ap.add_argument("--coarse-equits", default="60,80",
                     help="equits of the coarse stages of the multi-resolution schedule, "
                          "coarsest first: one count for every stage or a comma list")
     ap.add_argument("--coarse-tile", type=int, default=4)

@@ -1,7 +1,7 @@
 """The 4D benchmark workload at full size (256 x 256 x 8 voxels x 16 frames,
 256 views interlaced K = 4, temporal p 1.1) through the benchmark's own
 schedule and settings: 64x64x2 for 60 equits (tile 4, alpha 1.8),
-128x128x4 for 100 equits (tile 6, alpha 1.97), then full resolution (tile
+128x128x4 for 80 equits (tile 6, alpha 1.97), then full resolution (tile
 8 or 12, the two its autotuner picks between, alpha 1.97) until the FP64
 MAP cost is within 1e-5 of C*, the FP64 oracle's converged cost
 (tests/golden/cstar_ch4d.json, recorded with the scan's SHA-256), with the
@@ -32,7 +32,7 @@ def ch4d_stages():
     sched = multires_schedule(scan["proj_rows"], scan["proj_cols"])
     assert sched[:-1] == [(4, 4), (2, 2)]
     coarse = [(Stage(scan, mult_xy=4, mult_z=4, tile=4, alpha=1.8), 60),
-              (Stage(scan, mult_xy=2, mult_z=2, tile=6, alpha=1.97), 100)]
+              (Stage(scan, mult_xy=2, mult_z=2, tile=6, alpha=1.97), 80)]
     st = Stage(scan, tile=8, alpha=1.97)
     for s_ in [cs for cs, _ in coarse] + [st]:
         s_.option("pair_parities", 1)   # the benchmark's launches
