@@ -757,6 +757,8 @@ def main():
     ap.add_argument("--xcd-check", action="store_true",
                     help="count sweep work items that ran off their intended XCD")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-only", action="store_true",
+                    help="time only the reference's CPU path to its target (--cpu-budget)")
     ap.add_argument("--cpu-budget", type=float, default=120.0,
                     help="seconds of reference CPU time for the CPU baseline's run to its target")
     ap.add_argument("--p-t", type=float, default=None,
@@ -799,6 +801,12 @@ def main():
     from phasecon_amd.synth import make_scan, WORKLOADS
     from phasecon_amd.mbir import Stage
     from phasecon_amd.lib import K_SWEEP, K_COST, K_AMATRIX, K_FWDPROJ
+
+    if args.cpu_only:
+        # the reference's CPU time to its own target, nothing on the GPU
+        scan = make_scan(args.workload, seed=0)
+        print(json.dumps({"cpu_baseline": cpu_baseline(scan, args.cpu_budget)}))
+        return
 
     learned = not (args.no_learned or args.profile_equits or args.longrun)
     train_scans = []
