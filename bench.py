@@ -610,6 +610,11 @@ def cpu_baseline(scan, budget_s=120.0, coarse_cap=100, convg=1.0):
             out["s_to_target_est"] = spent + need * (t2 - t1) / 2
             out["note"] = ("budget spent before the target: s_to_target_est extrapolates the "
                            "last equits' geometric cost decrease")
+            if scan.get("name") == "ch4d":
+                out["full_run"] = {"value": 178.96, "unit": "s", "cores": 16,
+                                   "source": "profiles/cpu_reference_ch4d.md (bench.py "
+                                             "--cpu-only --cpu-budget 600, the same path "
+                                             "run to its target)"}
     return out
 
 
