@@ -43,7 +43,10 @@ int pcmbir_hip_fresnel(pcmbir_hip_ctx* ctx, int32_t herm, const float* x,
 
 /* Options: "fft_reg" 1 (default) runs the transforms' passes with each
  * sequence in registers (radix-8 Stockham, LDS only between passes) where a
- * sequence has at least 8 values, 0 the in-LDS radix-2 passes. */
+ * sequence has at least 8 values, 0 the in-LDS radix-2 passes.
+ * "fft_col_threads" (a power of two, 64-1024, default 512): the most
+ * threads of one workgroup of the register-resident column pass, rows / 8
+ * per column (512: 8 columns of 512 rows). */
 int pcmbir_hip_set_option(pcmbir_hip_ctx* ctx, const char* key, double value);
 
 /* Phase-retrieval problem of every view (XT_PhaseRet.c): measurements y, the

@@ -4,9 +4,10 @@
 // Device layout: complex arrays float2 [n_views][rows][cols], cols fastest.
 // Kernels:
 //   k_fft_rows_reg   batched FFTs along cols, each sequence in registers
-//                    (radix-8 Stockham, LDS between passes)
+//                    (radix-8 Stockham, LDS between passes, twiddles in LDS)
 //   k_fresnel_cols_reg  the column pass likewise, rows/8 threads per column,
-//                    up to 64 columns per workgroup (16 at 512 rows)
+//                    up to 64 columns and 512 threads per workgroup (option
+//                    fft_col_threads; 8 columns at 512 rows)
 //   k_fft_rows       batched in-LDS FFTs along cols; forward = bit-reversed
 //                    load + radix-2 DIT, inverse = radix-2 DIF + bit-reversed
 //                    store (both unnormalised, FFTW's conventions)
@@ -22,6 +23,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "pcmbir_hip.h"
@@ -183,11 +185,19 @@ __global__ __launch_bounds__(kThreads) void k_fresnel_cols(float2* __restrict__ 
 // outputs land where the next read would fetch them, so they stay in
 // registers.  inv = 1: exp(+...) (the inverse transform, unnormalised).
 
+// p advanced by a 32-bit byte offset: a wave-uniform base plus a 32-bit
+// lane offset, so that an address costs one register, not a 64-bit pair.
+template <typename T>
+__device__ __forceinline__ T* at_bytes(T* p, unsigned off) {
+  using C = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+  return reinterpret_cast<T*>(reinterpret_cast<C*>(p) + off);
+}
+
 // exp(-+2 pi i m / N) from tw[m] = exp(-2 pi i m / N), m < N/2.
 template <bool kInv>
 __device__ __forceinline__ float2 twid(const float2* __restrict__ tw, int m, int N) {
   const int h = N >> 1;
-  float2 w = m < h ? tw[m] : tw[m - h];
+  float2 w = *at_bytes(tw, (unsigned)(m < h ? m : m - h) * 8u);
   if (m >= h) w = make_float2(-w.x, -w.y);
   if (kInv) w.y = -w.y;
   return w;
@@ -292,11 +302,16 @@ __device__ void fft_reg(float2 v[8], int j, int logn, const float2* __restrict__
   else if (rem == 1) stockham_pass<2, kInv>(v, j, logn, Ns, tw, lds, ls, lo, true);
 }
 
-// Columns per workgroup of k_fresnel_cols_reg: 8 values per thread, rows / 8
-// threads per column, at most 1024 threads and 64 columns, at most cols.
-__host__ __device__ inline int col_tile_reg(int logr, int logc) {
+// Columns per workgroup of k_fresnel_cols_reg (log2): 8 values per thread,
+// rows / 8 threads per column, at most max_threads threads (a power of two,
+// 64 to 1024), 64 columns and cols, and at most 64 KiB of LDS (the exchange
+// buffer of 8 bytes per value and the rows / 2 twiddles).
+__host__ __device__ inline int col_tile_reg(int logr, int logc, int max_threads) {
+  const auto lds = [logr](int l) { return ((1 << (l + logr)) + (1 << (logr - 1))) * 8; };
   int ltc = 0;
-  while (ltc < logc && ltc < 6 && (1 << (ltc + 1 + logr - 3)) <= 1024) ++ltc;
+  while (ltc < logc && ltc < 6 && (1 << (ltc + 1 + logr - 3)) <= max_threads &&
+         lds(ltc + 1) <= 65536)
+    ++ltc;
   return ltc;
 }
 
@@ -313,17 +328,25 @@ __global__ __launch_bounds__(1024) void k_fresnel_cols_reg(float2* __restrict__ 
   const int tiles = 1 << (logc - ltc);
   const int view = blockIdx.x >> (logc - ltc), c0 = (blockIdx.x & (tiles - 1)) << ltc;
   const int col = threadIdx.x & (TC - 1), j = threadIdx.x >> ltc;
-  float2* base = data + ((size_t)view << (logr + logc)) + c0 + col;
-  const float2* fb = fac + c0 + col;
+  // wave-uniform bases and 32-bit byte offsets (a view is far below 4 GiB)
+  float2* vb = data + ((size_t)view << (logr + logc)) + c0;
+  const float2* fb = fac + c0;
+  const unsigned o0 = ((unsigned)col + ((unsigned)j << logc)) * 8u;
+  const unsigned os = ((unsigned)n8 << logc) * 8u;
   float2 v[8];
 #pragma unroll
-  for (int s = 0; s < 8; ++s) v[s] = base[(size_t)(j + n8 * s) << logc];
-  fft_reg<false>(v, j, logr, twg, s_ex, TC, col);
+  for (int s = 0; s < 8; ++s) v[s] = *at_bytes(vb, o0 + s * os);
+  // the twiddles in LDS behind the exchange buffer (LDS latency, not an L2
+  // round trip, in the passes' dependent twiddle reads)
+  float2* s_tw = s_ex + ((size_t)TC << logr);
+  for (int i = threadIdx.x; i < (1 << (logr - 1)); i += blockDim.x) s_tw[i] = twg[i];
+  __syncthreads();
+  fft_reg<false>(v, j, logr, s_tw, s_ex, TC, col);
 #pragma unroll
-  for (int s = 0; s < 8; ++s) v[s] = cmul(v[s], fb[(size_t)(j + n8 * s) << logc]);
-  fft_reg<true>(v, j, logr, twg, s_ex, TC, col);
+  for (int s = 0; s < 8; ++s) v[s] = cmul(v[s], *at_bytes(fb, o0 + s * os));
+  fft_reg<true>(v, j, logr, s_tw, s_ex, TC, col);
 #pragma unroll
-  for (int s = 0; s < 8; ++s) base[(size_t)(j + n8 * s) << logc] = v[s];
+  for (int s = 0; s < 8; ++s) *at_bytes(vb, o0 + s * os) = v[s];
 }
 
 // Rows of length 2^logn >= 8 (contiguous): 256 threads, 2048 / 2^logn rows
@@ -338,10 +361,13 @@ __global__ __launch_bounds__(256) void k_fft_rows_reg(float2* __restrict__ data,
   const long long row = (long long)blockIdx.x * S + seq;
   const bool live = row < nrows;
   float2* src = data + ((live ? row : 0) << logn) + j;
+  __shared__ float2 s_tw[kFftMax / 2];  // the twiddles, read from LDS in the passes
   float2 v[8];
 #pragma unroll
   for (int s = 0; s < 8; ++s) v[s] = live ? src[n8 * s] : make_float2(0.f, 0.f);
-  fft_reg<kInv>(v, j, logn, twg, s_ex, 1, seq << logn);
+  for (int i = threadIdx.x; i < (1 << (logn - 1)); i += 256) s_tw[i] = twg[i];
+  __syncthreads();
+  fft_reg<kInv>(v, j, logn, s_tw, s_ex, 1, seq << logn);
   if (live) {
 #pragma unroll
     for (int s = 0; s < 8; ++s) src[n8 * s] = v[s];
@@ -871,6 +897,7 @@ struct pcmbir_hip_ctx {
   double fft_ms = 0.0;
   int64_t fft_launches = 0;
   bool fft_reg = true;  // register-resident FFT passes (option; else the LDS radix-2 ones)
+  int col_threads = 512;  // most threads per workgroup of the register column pass
 };
 
 namespace {
@@ -905,9 +932,9 @@ int filter_dev(pcmbir_hip_ctx* c, float2* buf, int nbatch, const float2* fac) {
     hipLaunchKernelGGL(k_fft_rows, dim3(rb), dim3(kThreads), 0, c->stream, buf, nrows, c->logc,
                        c->tw_c, 0);
   if (cols_reg) {
-    const int ltc = col_tile_reg(c->logr, c->logc);
+    const int ltc = col_tile_reg(c->logr, c->logc, c->col_threads);
     const unsigned threads = 1u << (ltc + c->logr - 3);
-    const size_t lds = sizeof(float2) << (ltc + c->logr);
+    const size_t lds = (sizeof(float2) << (ltc + c->logr)) + (sizeof(float2) << (c->logr - 1));
     hipLaunchKernelGGL(k_fresnel_cols_reg, dim3((unsigned)nbatch << (c->logc - ltc)),
                        dim3(threads), lds, c->stream, buf, c->logr, c->logc, ltc, c->tw_r, fac);
   } else {
@@ -1429,6 +1456,13 @@ int pcmbir_hip_set_option(pcmbir_hip_ctx* c, const char* key, double value) {
   if (!c || !key) return fail("null argument");
   if (!strcmp(key, "fft_reg")) {
     c->fft_reg = value != 0.0;
+    return 0;
+  }
+  if (!strcmp(key, "fft_col_threads")) {
+    const int v = (int)value;
+    if ((double)v != value || v < 64 || v > 1024 || (v & (v - 1)))
+      return fail("fft_col_threads must be a power of two in [64, 1024]");
+    c->col_threads = v;
     return 0;
   }
   return fail(std::string("unknown option ") + key);

@@ -49,6 +49,29 @@ def test_fresnel_matches_oracle(views, rows, cols, dr, dc, windowed, fft_reg):
     ctx.close()
 
 
+@pytest.mark.parametrize("views,rows,cols", [(2, 512, 64), (2, 1024, 32), (3, 8, 128)])
+def test_fresnel_column_workgroup_widths(views, rows, cols):
+    """The register column pass at every workgroup width the option allows
+    (64 to 1024 threads: 1 to 64 columns per workgroup, at most 64 KiB of
+    LDS) gives the oracle's transform; widths that are not a power of two in
+    [64, 1024] are refused."""
+    from phasecon_amd.pcmbir import PhaseRetrieval
+    dist = 1e5
+    win = P.window(rows, cols, 1.0, 1.0, 2.0 / np.sqrt(LAM * dist))
+    ctx = PhaseRetrieval(views, rows, cols, 1.0, 1.0, LAM, dist, win)
+    x = rnd((views, rows, cols), 7 * rows + cols)
+    want = P.fresnel(x, 1.0, 1.0, LAM, dist, win=win)
+    for threads in (64, 128, 256, 512, 1024):
+        ctx.option("fft_col_threads", threads)
+        err = rel_rms(ctx.fresnel(x), want)
+        print(f"{views}x{rows}x{cols} col threads {threads}: rel rms {err:.3g}", flush=True)
+        assert err < 5e-6
+    for bad in (0, 32, 96, 2048, 512.5):
+        with pytest.raises(RuntimeError, match="fft_col_threads"):
+            ctx.option("fft_col_threads", bad)
+    ctx.close()
+
+
 def phase_problems(views, rows, cols, seed, windowed):
     r = np.random.default_rng(seed)
     win = P.window(rows, cols, 1.0, 1.0, 2.0 / np.sqrt(LAM * 1e5)) if windowed else None
