@@ -687,9 +687,13 @@ def pcmbir_bench(views, n, sd_steps, nms, fft_reg=1):
         cn.set_nms_prior(2e-2, 1e-2, 2e-2, 1e-2, 50.0, True, 1.2, 0.5)
         zr = rng.random(sub) * 0.05 + 0.02
         zi = rng.random(sub) * 0.5 + 0.2
-        t0 = time.perf_counter()
-        _, _, its = cn.nms_pass(zr, zi, obj[:nv])
-        dt_nms = time.perf_counter() - t0
+        cn.nms_pass(zr, zi, obj[:nv])  # warm-up (first launches load the code object)
+        passes = []
+        for _ in range(3):  # the same pass three times, the median reported
+            t0 = time.perf_counter()
+            _, _, its = cn.nms_pass(zr, zi, obj[:nv])
+            passes.append(time.perf_counter() - t0)
+        dt_nms = float(np.median(passes))
         out["pcmbir"].update({"nms_views": nv, "nms_pass_s": dt_nms,
                               "nms_pixels_per_s": nv * n * n / dt_nms,
                               "nms_avg_iters": its / (nv * n * n)})

@@ -441,15 +441,30 @@ __device__ void block_sum2(double& a, double& b) {
   b = rb[0];
 }
 
-// Per view (one block each): num = |g|^2, den = Re <g, b1 conj(D)> + nu num,
-// alpha = num / den (compute_stepsize, :133-170).  b1 holds H^H(...) of the
-// step's second pass before its conj(D) factor.
-__global__ __launch_bounds__(kThreads) void k_pr_step(PrArgs P) {
-  const int view = blockIdx.x;
+// The per-view reductions of the step run kPrParts blocks per view, each
+// over a contiguous chunk of the view's pixels, into FP64 partial sums
+// part[(view kPrParts + block) 2 + {0, 1}]; one thread per view then adds a
+// view's partials in block order (deterministic), so a launch has enough
+// blocks to fill the chip whatever the number of views.
+constexpr int kPrParts = 16;
+
+__device__ __forceinline__ void pr_chunk(long long npix, int b, long long& j0, long long& j1) {
+  const long long ch = (npix + kPrParts - 1) / kPrParts;
+  j0 = b * ch;
+  j1 = j0 + ch < npix ? j0 + ch : npix;
+}
+
+// Per view: num = |g|^2, den = Re <g, b1 conj(D)> (compute_stepsize,
+// :133-170), partials.  b1 holds H^H(...) of the step's second pass before
+// its conj(D) factor.
+__global__ __launch_bounds__(kThreads) void k_pr_step(PrArgs P, double* __restrict__ part) {
+  const int view = blockIdx.x / kPrParts, blk = blockIdx.x % kPrParts;
   if (!P.active[view]) return;
   const long long o = (long long)view * P.npix;
+  long long j0, j1;
+  pr_chunk(P.npix, blk, j0, j1);
   double num = 0.0, den = 0.0;
-  for (long long j = threadIdx.x; j < P.npix; j += kThreads) {
+  for (long long j = j0 + threadIdx.x; j < j1; j += kThreads) {
     const float2 g = P.g[o + j];
     const float2 b = cmulc(P.b1[o + j], P.D[o + j]);
     num += (double)g.x * g.x + (double)g.y * g.y;
@@ -457,22 +472,37 @@ __global__ __launch_bounds__(kThreads) void k_pr_step(PrArgs P) {
   }
   block_sum2(num, den);
   if (threadIdx.x == 0) {
-    den += (double)P.nu * num;
-    P.scal[4 * view + 0] = num;
-    P.scal[4 * view + 1] = den;
-    P.scal[4 * view + 2] = num / (den + DBL_MIN);
+    part[2 * (size_t)blockIdx.x] = num;
+    part[2 * (size_t)blockIdx.x + 1] = den;
   }
 }
 
-// Per view: w -= alpha g; upd = 100 sum|alpha g| / sum|w_new|
+// alpha = num / (den + nu num) per view from the partials.
+__global__ void k_pr_step_fin(PrArgs P, const double* __restrict__ part, int n_views) {
+  const int view = blockIdx.x * blockDim.x + threadIdx.x;
+  if (view >= n_views || !P.active[view]) return;
+  double num = 0.0, den = 0.0;
+  for (int b = 0; b < kPrParts; ++b) {
+    num += part[2 * ((size_t)view * kPrParts + b)];
+    den += part[2 * ((size_t)view * kPrParts + b) + 1];
+  }
+  den += (double)P.nu * num;
+  P.scal[4 * view + 0] = num;
+  P.scal[4 * view + 1] = den;
+  P.scal[4 * view + 2] = num / (den + DBL_MIN);
+}
+
+// Per view: w -= alpha g, partial sums of |alpha g| and |w_new|
 // (steepest_descent_iter, :228-237).
-__global__ __launch_bounds__(kThreads) void k_pr_update(PrArgs P) {
-  const int view = blockIdx.x;
+__global__ __launch_bounds__(kThreads) void k_pr_update(PrArgs P, double* __restrict__ part) {
+  const int view = blockIdx.x / kPrParts, blk = blockIdx.x % kPrParts;
   if (!P.active[view]) return;
   const long long o = (long long)view * P.npix;
+  long long j0, j1;
+  pr_chunk(P.npix, blk, j0, j1);
   const float alpha = (float)P.scal[4 * view + 2];
   double up = 0.0, val = 0.0;
-  for (long long j = threadIdx.x; j < P.npix; j += kThreads) {
+  for (long long j = j0 + threadIdx.x; j < j1; j += kThreads) {
     const float2 g = P.g[o + j];
     const float2 w = csub(P.w[o + j], cscale(g, alpha));
     P.w[o + j] = w;
@@ -480,7 +510,22 @@ __global__ __launch_bounds__(kThreads) void k_pr_update(PrArgs P) {
     val += sqrt((double)w.x * w.x + (double)w.y * w.y);
   }
   block_sum2(up, val);
-  if (threadIdx.x == 0) P.scal[4 * view + 3] = 100.0 * up / val;
+  if (threadIdx.x == 0) {
+    part[2 * (size_t)blockIdx.x] = up;
+    part[2 * (size_t)blockIdx.x + 1] = val;
+  }
+}
+
+// upd = 100 sum|alpha g| / sum|w_new| per view from the partials.
+__global__ void k_pr_update_fin(PrArgs P, const double* __restrict__ part, int n_views) {
+  const int view = blockIdx.x * blockDim.x + threadIdx.x;
+  if (view >= n_views || !P.active[view]) return;
+  double up = 0.0, val = 0.0;
+  for (int b = 0; b < kPrParts; ++b) {
+    up += part[2 * ((size_t)view * kPrParts + b)];
+    val += part[2 * ((size_t)view * kPrParts + b) + 1];
+  }
+  P.scal[4 * view + 3] = 100.0 * up / val;
 }
 
 // Omega = y conj(h) / (|h|^2 + eps), normalised (compute_phase_projection,
@@ -529,6 +574,7 @@ struct NmsParams {
   double T[2][2];     // decorrelating transform: row 0 phase, row 1 magnitude
   double filt[9];     // in-plane 3x3 neighbour weights
   double c[2], sq[2], sqp[2];  // per prior (0 phase, 1 magnitude): c, sigma^2, sigma^(2-p)
+  double sqc[2], sqr[2];       // c sigma^2 and sigma^2 / sigma^(2-p)
   double qmp;         // q - p
   double nu;
   double rho, chi, gamma, sigma, thresh;
@@ -546,18 +592,25 @@ struct NmsPix {
   unsigned flags;
 };
 
-// qGGMRF value with q = 2 (Mag_/Phase_QGGMRF_Spatial_Value, XT_Prior.c:7-15).
+// qGGMRF value with q = 2 (Mag_/Phase_QGGMRF_Spatial_Value, XT_Prior.c:7-15):
+// (|d|^2 / s^q) / (c + |d|^(q-p) / s^(q-p)), written with one division as
+// |d|^2 / (c s^q + |d|^(q-p) s^q / s^(q-p)) (sqc, sqr set on the host), and
+// |d|^(q-p) as exp2((q - p) log2 |d|): two of the device library's FP64
+// functions in place of its general pow.  Within a few ulp of the
+// reference's form; the search paths of the parity test are unchanged.
 __device__ __forceinline__ double nms_value(double d, const NmsParams& P, int t) {
   const double ad = fabs(d);
-  const double r = ad == 0.0 ? 0.0 : pow(ad, P.qmp);
-  return (ad * ad / P.sq[t]) / (P.c[t] + r / P.sqp[t]);
+  const double r = ad == 0.0 ? 0.0 : exp2(P.qmp * log2(ad));
+  return (ad * ad) / (P.sqc[t] + r * P.sqr[t]);
 }
 
 // Cost_NMS (XT_Search.c:8-43), FP64, terms in the reference's order.
 __device__ __forceinline__ double nms_cost(double zr, double zi, const NmsPix& X,
                                           const NmsParams& P) {
   const double e = exp(-zr);
-  const double cv = e * cos(-zi), sv = e * sin(-zi);
+  double sn, cs;
+  sincos(-zi, &sn, &cs);
+  const double cv = e * cs, sv = e * sn;
   double cost = ((cv - X.br) * (cv - X.br) + (sv - X.bi) * (sv - X.bi)) * P.nu / 2;
   if (P.tomo) return cost + ((zr - X.ar) * (zr - X.ar) + (zi - X.ai) * (zi - X.ai)) * P.mu / 2;
   // unrolled, so that the neighbourhood stays in registers
@@ -880,6 +933,7 @@ struct pcmbir_hip_ctx {
   float* lam = nullptr;
   int *active = nullptr, *iters = nullptr;
   double *scal = nullptr, *cost = nullptr;
+  double* part = nullptr;  // [n_views][kPrParts][2] partial sums of the step's reductions
   float nu = 0.f;
   bool problem = false;
   // NMS / ADMM state (estimate_complex_projection)
@@ -1035,8 +1089,12 @@ int sd_loop(pcmbir_hip_ctx* c, int max_iters, double thresh, const int* init_act
     if (fresnel_dev(c, P.b1, c->nv, 0)) return -1;
     hipLaunchKernelGGL(k_pr_mid, dim3(eb), dim3(kThreads), 0, c->stream, P, 0);
     if (fresnel_dev(c, P.b1, c->nv, 1)) return -1;
-    hipLaunchKernelGGL(k_pr_step, dim3(c->nv), dim3(kThreads), 0, c->stream, P);
-    hipLaunchKernelGGL(k_pr_update, dim3(c->nv), dim3(kThreads), 0, c->stream, P);
+    hipLaunchKernelGGL(k_pr_step, dim3(c->nv * kPrParts), dim3(kThreads), 0, c->stream, P,
+                       c->part);
+    hipLaunchKernelGGL(k_pr_step_fin, dim3(vb), dim3(256), 0, c->stream, P, c->part, c->nv);
+    hipLaunchKernelGGL(k_pr_update, dim3(c->nv * kPrParts), dim3(kThreads), 0, c->stream, P,
+                       c->part);
+    hipLaunchKernelGGL(k_pr_update_fin, dim3(vb), dim3(256), 0, c->stream, P, c->part, c->nv);
     // phase projection
     hipLaunchKernelGGL(k_pr_dmul, dim3(eb), dim3(kThreads), 0, c->stream, P, c->w, P.b1);
     if (fresnel_dev(c, P.b1, c->nv, 0)) return -1;
@@ -1155,7 +1213,7 @@ void pcmbir_hip_destroy(pcmbir_hip_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void* bufs[] = {c->tw_r, c->tw_c, c->fac_h, c->fac_hh, c->fac_pag, c->y, c->D, c->v, c->w, c->om,
-                  c->g, c->b, c->lam, c->active, c->iters, c->scal, c->cost,
+                  c->g, c->b, c->lam, c->active, c->iters, c->scal, c->cost, c->part,
                   c->zr, c->zi, c->res, c->vd, c->wold, c->pix_iters, c->admm, c->ar, c->ai};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -1185,7 +1243,7 @@ int pcmbir_hip_set_phase_problem(pcmbir_hip_ctx* c, const float* y, const float*
         dev_alloc(&c->w, c->nall) || dev_alloc(&c->om, c->nall) || dev_alloc(&c->g, c->nall) ||
         dev_alloc(&c->lam, c->nall) || dev_alloc(&c->active, c->nv) ||
         dev_alloc(&c->iters, c->nv) || dev_alloc(&c->scal, 4 * (size_t)c->nv) ||
-        dev_alloc(&c->cost, c->nv))
+        dev_alloc(&c->cost, c->nv) || dev_alloc(&c->part, 2 * (size_t)kPrParts * c->nv))
       return -1;
   }
   const size_t cb = (size_t)c->nall * sizeof(float2);
@@ -1275,6 +1333,10 @@ int pcmbir_hip_set_nms_prior(pcmbir_hip_ctx* c, double phase_sig, double phase_c
   P.sq[1] = pow(mag_sig, q);
   P.sqp[0] = pow(phase_sig, q - p);
   P.sqp[1] = pow(mag_sig, q - p);
+  for (int t = 0; t < 2; ++t) {
+    P.sqc[t] = P.c[t] * P.sq[t];
+    P.sqr[t] = P.sq[t] / P.sqp[t];
+  }
   P.qmp = q - p;
   P.nu = nu;
   // simplex search constants (XT_Init.c:507-519)
