@@ -14,7 +14,7 @@
  * Differences from the reference, by design of this library:
  *   - single process, one GPU: no MPI; proj_rows is not split across ranks;
  *   - the ICD voxel order differs (parallel ICD on the GPU), so results are
- *     judged on the converged MAP cost, not per-iteration equality;
+ *     compared on the converged MAP cost, not per-iteration equality;
  *   - remove_rings 1-3 (ring offset estimation) and, with quad_convex = 0,
  *     the noise-variance estimate are updated once per equit from the
  *     second stage on (the reference: the offset after every half-sweep of

@@ -2,7 +2,7 @@
 
 Tolerances (FP32 kernels vs FP64 restatement): A values and projections
 within 1e-6 relative, theta1/theta2 and the functional substitution within
-1e-5 relative (north star: 1e-5 in FP32), FP64 costs within 1e-9 relative,
+1e-5 relative (the FP32 bar for these terms), FP64 costs within 1e-9 relative,
 converged ICD within 1e-5 relative MAP cost and 0.5 % image RMSE.
 """
 import os
