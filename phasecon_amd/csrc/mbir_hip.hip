@@ -40,6 +40,7 @@ namespace {
 constexpr int kNB = 65;        // detector response entries per view
 constexpr int kRespBins = 64;  // XT_Constants.h:79
 constexpr int kNZMax = 128;    // voxels per column (N_z) supported by k_sweep
+constexpr int kNZPc = 256;     // voxels per column supported by the tomographic sweep
 constexpr int kNTMax = 256;    // detector rows (N_t) supported by k_sweep
 constexpr int kWMax = 33;      // window width 2*mult_xy+1 (mult_xy <= 16)
 constexpr int kWave = 64;
@@ -1452,8 +1453,8 @@ __device__ __forceinline__ float wave_sum(float v) {
 // both error sinograms updated with atomics (XT_VoxUpdate.c:57-79).
 __global__ __launch_bounds__(kWave) void k_pc_sweep(SwGeom G, PcTab P, PcArgs A) {
   extern __shared__ __align__(16) float smem[];
-  __shared__ float s_t1d[kNZMax], s_t1b[kNZMax], s_dxd[kNZMax], s_dxb[kNZMax];
-  __shared__ float s_tn[4][kNZMax];  // delta t-1, delta t+1, beta t-1, beta t+1
+  __shared__ float s_t1d[kNZPc], s_t1b[kNZPc], s_dxd[kNZPc], s_dxb[kNZPc];
+  __shared__ float s_tn[4][kNZPc];  // delta t-1, delta t+1, beta t-1, beta t+1
   const int lane = threadIdx.x;
   const int wid = blockIdx.x;
   const int tile = A.launch_tiles[wid % A.n_launch_tiles];
@@ -1527,33 +1528,28 @@ __global__ __launch_bounds__(kWave) void k_pc_sweep(SwGeom G, PcTab P, PcArgs A)
       t2 += a * a;
     }
     const float th2 = P.mu * (float)Wd * wave_sum(t2);
-    // theta1 of each voxel from both error sinograms
-    const int E = nv * W * Wd;
-    for (int z = 0; z < Nz; ++z) {
+    // theta1 of each voxel from both error sinograms: a lane per voxel, its
+    // Wd detector rows of every (view, bin) entry (consecutive lanes read
+    // consecutive rows)
+    for (int z = lane; z < Nz; z += kWave) {
       float a1 = 0.f, a2 = 0.f;
-      for (int i = lane; i < E; i += kWave) {
-        const int sl = i / (W * Wd), rem = i % (W * Wd);
-        const int jj = rem / Wd, r = rem % Wd;
-        const int base = wbase[sl];
-        const float a = base >= 0 ? wav[sl * W + jj] : 0.f;
-        if (a != 0.f) {
-          const int o = base + jj * G.N_t + z * Wd + r;
-          a1 -= A.ed[o] * a;
-          a2 -= A.eb[o] * a;
+      for (int i = 0; i < nv * W; ++i) {
+        const int base = wbase[i / W];
+        const float a = base >= 0 ? wav[i] : 0.f;
+        if (a == 0.f) continue;
+        const int o = base + (i % W) * G.N_t + z * Wd;
+        for (int r = 0; r < Wd; ++r) {
+          a1 -= A.ed[o + r] * a;
+          a2 -= A.eb[o + r] * a;
         }
       }
-      a1 = wave_sum(a1);
-      a2 = wave_sum(a2);
-      if (lane == 0) {
-        s_t1d[z] = P.mu * a1;
-        s_t1b[z] = P.mu * a2;
-      }
+      s_t1d[z] = P.mu * a1;
+      s_t1b[z] = P.mu * a2;
     }
     __syncthreads();
-    // the two z-parity halves
+    // the two z-parity halves (voxels of one parity are not neighbours)
     for (int par = 0; par < 2; ++par) {
-      const int z = par + 2 * lane;
-      if (z < Nz) {
+      for (int z = par + 2 * lane; z < Nz; z += 2 * kWave) {
         const int ctr = centre0 + z;
         const float Vd = sd[ctr], Vb = sb[ctr];
         double u[2] = {0.0, 0.0}, M[3] = {0.0, 0.0, 0.0};
@@ -1598,19 +1594,20 @@ __global__ __launch_bounds__(kWave) void k_pc_sweep(SwGeom G, PcTab P, PcArgs A)
       }
       __syncthreads();
     }
-    // both error sinograms: e -= k a dx along the column
-    for (int i = lane; i < Nz * E; i += kWave) {
-      const int z = i / E, e = i % E;
-      const float dd = s_dxd[z], db = s_dxb[z];
-      if (dd == 0.f && db == 0.f) continue;
-      const int sl = e / (W * Wd), rem = e % (W * Wd);
-      const int jj = rem / Wd, r = rem % Wd;
-      const int base = wbase[sl];
-      const float a = base >= 0 ? wav[sl * W + jj] : 0.f;
+    // both error sinograms: e -= k a dx along the column, a lane per voxel
+    // as for theta1
+    for (int i = 0; i < nv * W; ++i) {
+      const int base = wbase[i / W];
+      const float a = base >= 0 ? wav[i] : 0.f;
       if (a == 0.f) continue;
-      const int o = base + jj * G.N_t + z * Wd + r;
-      if (dd != 0.f) e_add(A.ed + o, -a * dd);
-      if (db != 0.f) e_add(A.eb + o, -a * db);
+      const int o = base + (i % W) * G.N_t;
+      for (int z = lane; z < Nz; z += kWave) {
+        const float dd = s_dxd[z], db = s_dxb[z];
+        for (int r = 0; r < Wd; ++r) {
+          if (dd != 0.f) e_add(A.ed + o + z * Wd + r, -a * dd);
+          if (db != 0.f) e_add(A.eb + o + z * Wd + r, -a * db);
+        }
+      }
     }
     __syncthreads();
   }
@@ -2621,7 +2618,7 @@ int check_geom(const mbir_hip_geom* g) {
   if (g->N_r <= 0 || g->N_t <= 0 || g->N_p <= 0 || g->N_x <= 0 || g->N_y <= 0 ||
       g->N_z <= 0 || g->N_time <= 0 || g->mult_xy <= 0 || g->mult_z <= 0)
     return fail("non-positive geometry size");
-  if (g->N_z > kNZMax) return fail("N_z exceeds the sweep kernel's LDS column (128)");
+  if (g->N_z > kNZPc) return fail("N_z exceeds 256 voxels per column");
   if (g->N_t > kNTMax) return fail("N_t exceeds the sweep kernel's LDS rows (256)");
   if (2 * g->mult_xy + 1 > kWMax) return fail("mult_xy exceeds 16");
   if (g->q != 2.0) return fail("q must be 2 (the qGGMRF of the sweep kernels)");
@@ -3013,6 +3010,7 @@ namespace {
 // records).  Returns the number of launches.
 int sweep_launches(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t seed,
                    const int* iter_dev, int* launches, float nh_frac = 0.f) {
+  if (c->g.N_z > kNZMax) return fail("the MBIR sweep takes at most 128 voxels per column");
   SweepArgs A;
   A.mag = c->nh ? c->d_mag : nullptr;
   A.nh_rank = c->d_vsc;
@@ -3575,6 +3573,7 @@ int mbir_hip_voxel_thetas(mbir_hip_ctx* c, int32_t n, const int32_t* tzyx, float
                           float* theta2) {
   if (!c || n < 0 || (n > 0 && (!tzyx || !theta1 || !theta2))) return fail("bad argument");
   if (!c->a_built) return fail("A not built");
+  if (c->g.N_z > kNZMax) return fail("the theta kernel takes at most 128 voxels per column");
   if (n == 0) return 0;
   for (int i = 0; i < n; ++i) {
     const int32_t* q = tzyx + 4 * i;
@@ -3611,7 +3610,7 @@ int mbir_hip_pc_enable(mbir_hip_ctx* c, const mbir_hip_pc_prior* pr) {
   if (!c || !pr) return fail("null argument");
   if (pr->q != 2.0) return fail("q must be 2 (the qGGMRF of the tomographic kernels)");
   if (pr->mu <= 0 || pr->wavenumber <= 0) return fail("mu and the wavenumber must be positive");
-  if (c->g.N_z > kWave) return fail("the tomographic sweep takes at most 64 voxels per column");
+  if (c->g.N_z > kNZPc) return fail("the tomographic sweep takes at most 256 voxels per column");
   if (!c->d_x2 && (dev_alloc(&c->d_x2, c->nvox) || dev_alloc(&c->d_e2, c->nsino))) return -1;
   HIP_OK(hipMemset(c->d_x2, 0, c->nvox * sizeof(float)));
   HIP_OK(hipMemset(c->d_e2, 0, c->nsino * sizeof(float)));
@@ -3710,7 +3709,7 @@ int mbir_hip_pc_equits(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t 
   if (build_tiles(c)) return -1;
   const size_t nx = (size_t)(c->tile + 2) * (c->tile + 2) * c->g.N_z;
   const size_t words = 2 * nx + (size_t)c->nvmax * (2 + c->G.W) + (size_t)c->tile * c->tile;
-  if (words * 4 + 4 * 1024 + 6 * kNZMax * 4 > c->lds_max)
+  if (words * 4 + 4 * 1024 + 8 * kNZPc * 4 > c->lds_max)
     return fail("tile slabs of both objects exceed the LDS of one workgroup");
   HIP_OK(hipMemsetAsync(c->d_stats, 0, 4 * sizeof(double), c->stream));
   PcArgs A;

@@ -28,11 +28,11 @@ PHASE = (0.02, 1e-3, 0.01, 0.01)
 MAG = (0.005, 1e-3, 0.01, 0.01)
 
 
-def _problem(name, wavenumber=30.0, dob=50.0, alpha=1.5, seed=0):
+def _problem(name, wavenumber=30.0, dob=50.0, alpha=1.5, seed=0, **over):
     from oracle.oracle import Oracle, pc_prior
     from phasecon_amd.mbir import tomo_prior
     from phasecon_amd.synth import make_scan
-    scan = make_scan(name, seed=seed)
+    scan = make_scan(name, seed=seed, **over)
     o = Oracle(scan)
     Po = pc_prior(PHASE, MAG, 1, o.g.delta_recon, wavenumber, delta_over_beta=dob, alpha=alpha)
     Pg = tomo_prior(PHASE, MAG, 1, o.g.delta_recon, wavenumber, delta_over_beta=dob, alpha=alpha)
@@ -55,18 +55,21 @@ def _start(o, P, td, tb, n0):
     return xd.astype(np.float32).astype(np.float64), xb.astype(np.float32).astype(np.float64)
 
 
-@pytest.mark.parametrize("name,dob,n0", [("tiny3d", 50.0, 10), ("tiny3d", None, 10),
-                                         ("ch4d_small", 50.0, 20)])
-def test_tomo_equit_matches_oracle(name, dob, n0):
+# (workload, delta/beta, n0, overrides): the last case has 200 voxels per
+# column (a lane runs several voxels of a column, the slab 58 KiB of LDS)
+@pytest.mark.parametrize("name,dob,n0,over", [
+    ("tiny3d", 50.0, 10, {}), ("tiny3d", None, 10, {}), ("ch4d_small", 50.0, 20, {}),
+    ("tiny3d", 50.0, 10, {"rows": 200, "cols": 8})])
+def test_tomo_equit_matches_oracle(name, dob, n0, over):
     from phasecon_amd.mbir import TomoStage
-    scan, o, Po, Pg, td, tb = _problem(name, dob=dob or 50.0)
+    scan, o, Po, Pg, td, tb = _problem(name, dob=dob or 50.0, **over)
     if dob is None:
         from oracle.oracle import pc_prior
         from phasecon_amd.mbir import tomo_prior
         Po = pc_prior(PHASE, MAG, 1, o.g.delta_recon, 30.0, alpha=1.5)
         Pg = tomo_prior(PHASE, MAG, 1, o.g.delta_recon, 30.0, alpha=1.5)
     T = max(o.g.N_x, o.g.N_y) // 2
-    print(f"{name}: tile {T}, delta/beta {dob}", flush=True)
+    print(f"{name} {over}: tile {T}, delta/beta {dob}", flush=True)
     xd0, xb0 = _start(o, Po, td, tb, n0)
     st = TomoStage(scan, Pg, tile=T)
     try:
