@@ -701,6 +701,65 @@ def pcmbir_bench(views, n, sd_steps, nms, fft_reg=1):
     print(json.dumps(out))
 
 
+def pctomo_bench(n, views, equits, tile, seed=761521):
+    """The PCMBIR tomographic ICD (k_pc_sweep) on one GPU at the PCMBIR
+    workload's object size: an n^3 (delta, beta) object from seeded
+    ellipsoids (delta up to 1e-6, beta = delta / 1000), views parallel-beam
+    views over 180 degrees on an n x n detector, wavenumber of 17 keV per
+    1 um voxel.  The targets are k A x of that object (on the GPU); the
+    equits run from zero.  Reports seconds per equit, the sweep's launches
+    (library events) and its algorithmic HBM bytes against 8 TB/s."""
+    from phasecon_amd.synth import geometry_scan, drifting_ellipsoids
+    from phasecon_amd.mbir import TomoStage, tomo_prior
+    from phasecon_amd.lib import K_SWEEP
+    t0 = time.time()
+    scan = geometry_scan(n, n, views, seed=0)
+    dob = 1000.0
+    kw = 2.0 * np.pi * 1e-6 / (1.23984e-9 / 17.0)   # 2 pi / lambda, per 1 um voxel
+    prior = tomo_prior((2e-7, 1e-3, 0.01, 0.01), (2e-10, 1e-3, 0.01, 0.01), 1,
+                       float(views), kw, delta_over_beta=dob, alpha=1.5)
+    st = TomoStage(scan, prior, tile=tile)
+    xd = drifting_ellipsoids(n, n, 1, seed=3, lo=2e-7, hi=1e-6, count=20)
+    xb = xd / dob
+    zero = np.zeros(st.shape_e, np.float32)
+    st.set_objects(xd, xb)
+    st.init_errors(zero, zero)
+    ed, eb = st.get_errors()
+    td, tb = -ed, -eb                                 # k A x of the object
+    st.set_objects(np.zeros(st.shape_x, np.float32), np.zeros(st.shape_x, np.float32))
+    st.init_errors(td, tb)
+    c0 = st.tomo_cost()[0]
+    print(f"[bench] pctomo setup {time.time() - t0:.1f}s (tile {tile})", file=sys.stderr,
+          flush=True)
+    st.equits(1, 1, seed)                              # warm-up
+    st.synchronize()
+    st.reset_kernel_time()
+    t0 = time.perf_counter()
+    st.equits(equits, 2, seed)
+    st.synchronize()
+    dt = (time.perf_counter() - t0) / equits
+    ms, launches = st.kernel_time(K_SWEEP)
+    c1 = st.tomo_cost()[0]
+    # algorithmic bytes of one equit: per column visit the A window (first
+    # index + W values per view); per voxel both error sinograms' read +
+    # write (16 B) on each nonzero (view, bin, row) entry and both objects'
+    # read + write (16 B)
+    first, val = st.get_amatrix()
+    nnz = int((val != 0).sum())
+    ncol = int(st.mask.sum())
+    g = st.g
+    bytes_equit = ncol * int(st.frame_ptr[-1]) * (4 * st.W + 4) + \
+        g.N_z * (16.0 * nnz * g.mult_z + 16.0 * ncol)
+    bytes_launch = bytes_equit * equits / launches
+    gbps = bytes_launch / (ms / launches * 1e-3) / 1e9
+    print(json.dumps({"pctomo": {
+        "object": f"{n}^3 (delta, beta)", "views": views, "tile": tile, "equits": equits,
+        "s_per_equit": dt, "sweep_launches": launches, "sweep_avg_ms": ms / launches,
+        "bytes_per_launch": bytes_launch, "hbm_gbps": gbps, "frac": gbps / PEAK_HBM_GBPS,
+        "cost_start": c0, "cost_end": c1, "data": "synthetic (seeded ellipsoids, targets k A x)"}}))
+    st.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -782,6 +841,11 @@ def main():
                     help="phase-retrieval throughput line instead of the MBIR metric")
     ap.add_argument("--pcmbir-views", type=int, default=360)
     ap.add_argument("--pcmbir-n", type=int, default=512)
+    ap.add_argument("--pctomo", action="store_true",
+                    help="the PCMBIR tomographic ICD at the PCMBIR object size instead")
+    ap.add_argument("--pctomo-n", type=int, default=256)
+    ap.add_argument("--pctomo-views", type=int, default=360)
+    ap.add_argument("--pctomo-tile", type=int, default=4)
     ap.add_argument("--pcmbir-fft", choices=("register", "lds"), default="register",
                     help="FFT passes: register-resident radix-8 (default) or in-LDS radix-2")
     args = ap.parse_args()
@@ -797,6 +861,9 @@ def main():
         return
     if args.config4 and args.gpus == 1:
         config4(args)
+        return
+    if args.pctomo and args.gpus == 1:
+        pctomo_bench(args.pctomo_n, args.pctomo_views, max(args.steps, 1), args.pctomo_tile)
         return
     if args.pcmbir and args.gpus == 1:
         pcmbir_bench(args.pcmbir_views, args.pcmbir_n, max(args.steps, 1), nms=True,

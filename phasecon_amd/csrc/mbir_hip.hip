@@ -1451,11 +1451,13 @@ __device__ __forceinline__ float wave_sum(float v) {
 // voxel (PCMBIR_3D/XT_VoxUpdate.c:23-52), then the two z-parity halves of
 // coupled substitutions with zero skipping (XT_VoxUpdate.c:208-228), then
 // both error sinograms updated with atomics (XT_VoxUpdate.c:57-79).
-__global__ __launch_bounds__(kWave) void k_pc_sweep(SwGeom G, PcTab P, PcArgs A) {
+__global__ __launch_bounds__(4 * kWave) void k_pc_sweep(SwGeom G, PcTab P, PcArgs A) {
   extern __shared__ __align__(16) float smem[];
   __shared__ float s_t1d[kNZPc], s_t1b[kNZPc], s_dxd[kNZPc], s_dxb[kNZPc];
   __shared__ float s_tn[4][kNZPc];  // delta t-1, delta t+1, beta t-1, beta t+1
-  const int lane = threadIdx.x;
+  // up to four wavefronts, over the voxels of a column (pc_threads); the
+  // wavefronts of a workgroup share its slab
+  const int tid = threadIdx.x, nth = blockDim.x, lane = tid & (kWave - 1);
   const int wid = blockIdx.x;
   const int tile = A.launch_tiles[wid % A.n_launch_tiles];
   const int f = A.fparity + 2 * (wid / A.n_launch_tiles);
@@ -1475,7 +1477,7 @@ __global__ __launch_bounds__(kWave) void k_pc_sweep(SwGeom G, PcTab P, PcArgs A)
   const size_t vol = (size_t)G.N_y * G.N_x * Nz;
   float* xdf = A.xd + (size_t)f * vol;
   float* xbf = A.xb + (size_t)f * vol;
-  for (int j = lane; j < nx; j += kWave) {
+  for (int j = tid; j < nx; j += nth) {
     const int z = j % Nz, b = j / Nz;
     const int yy = oy + b / BW, xx = ox + b % BW;
     const bool in = yy >= 0 && yy < G.N_y && xx >= 0 && xx < G.N_x;
@@ -1483,8 +1485,8 @@ __global__ __launch_bounds__(kWave) void k_pc_sweep(SwGeom G, PcTab P, PcArgs A)
     sd[j] = in ? xdf[o] : 0.f;
     sb[j] = in ? xbf[o] : 0.f;
   }
-  for (int j = lane; j < n; j += kWave) cols[j] = A.tile_cols[c0 + j];
-  for (int j = lane; j < nv; j += kWave) views[j] = A.frame_views[s0 + j];
+  for (int j = tid; j < n; j += nth) cols[j] = A.tile_cols[c0 + j];
+  for (int j = tid; j < nv; j += nth) views[j] = A.frame_views[s0 + j];
   // seeded visiting order k -> (pb + k pa) mod n, as k_sweep
   const unsigned h = mix32(A.seed ^ mix32((unsigned)tile * 131071u + (unsigned)f * 524287u));
   int pa = 1, pb = 0;
@@ -1505,15 +1507,15 @@ __global__ __launch_bounds__(kWave) void k_pc_sweep(SwGeom G, PcTab P, PcArgs A)
     const int ly = cy - oy, lx = cx - ox;
     const int centre0 = (ly * BW + lx) * Nz;
     // the column's A window (values times the wavenumber) and e offsets
-    for (int sl = lane; sl < nv; sl += kWave) {
+    for (int sl = tid; sl < nv; sl += nth) {
       const int first = A.afirst[(size_t)c * G.nslots + s0 + sl];
       wbase[sl] = first < 0 ? -1 : (views[sl] * G.N_r + first) * G.N_t;
     }
-    for (int i = lane; i < nv * W; i += kWave) {
+    for (int i = tid; i < nv * W; i += nth) {
       const int sl = i / W, jj = i % W;
       wav[i] = A.aval[((size_t)c * G.nslots + s0 + sl) * W + jj] * P.kw;
     }
-    for (int z = lane; z < Nz; z += kWave) {
+    for (int z = tid; z < Nz; z += nth) {
       const size_t tc = (size_t)c * Nz + z;
       s_tn[0][z] = f > 0 ? A.xd[(size_t)(f - 1) * vol + tc] : 0.f;
       s_tn[1][z] = f + 1 < G.N_time ? A.xd[(size_t)(f + 1) * vol + tc] : 0.f;
@@ -1521,7 +1523,8 @@ __global__ __launch_bounds__(kWave) void k_pc_sweep(SwGeom G, PcTab P, PcArgs A)
       s_tn[3][z] = f + 1 < G.N_time ? A.xb[(size_t)(f + 1) * vol + tc] : 0.f;
     }
     __syncthreads();
-    // theta2, shared by all voxels of the column and both objects
+    // theta2, shared by all voxels of the column and both objects (each
+    // wavefront sums it)
     float t2 = 0.f;
     for (int i = lane; i < nv * W; i += kWave) {
       const float a = wbase[i / W] >= 0 ? wav[i] : 0.f;
@@ -1531,16 +1534,30 @@ __global__ __launch_bounds__(kWave) void k_pc_sweep(SwGeom G, PcTab P, PcArgs A)
     // theta1 of each voxel from both error sinograms: a lane per voxel, its
     // Wd detector rows of every (view, bin) entry (consecutive lanes read
     // consecutive rows)
-    for (int z = lane; z < Nz; z += kWave) {
+    for (int z = tid; z < Nz; z += nth) {
       float a1 = 0.f, a2 = 0.f;
-      for (int i = 0; i < nv * W; ++i) {
-        const int base = wbase[i / W];
-        const float a = base >= 0 ? wav[i] : 0.f;
-        if (a == 0.f) continue;
-        const int o = base + (i % W) * G.N_t + z * Wd;
-        for (int r = 0; r < Wd; ++r) {
-          a1 -= A.ed[o + r] * a;
-          a2 -= A.eb[o + r] * a;
+      // every entry loads, so that the loads of several entries are in
+      // flight together: an entry with a = 0 (the column misses the detector
+      // in that view, or a window bin past the footprint, which may lie past
+      // the detector's last channel) reads the view-0, channel-0 row instead
+      if (Wd == 1) {
+#pragma unroll 8
+        for (int i = 0; i < nv * W; ++i) {
+          const int base = wbase[i / W];
+          const float a = base >= 0 ? wav[i] : 0.f;
+          const int o = (a != 0.f ? base + (i % W) * G.N_t : 0) + z;
+          a1 -= A.ed[o] * a;
+          a2 -= A.eb[o] * a;
+        }
+      } else {
+        for (int i = 0; i < nv * W; ++i) {
+          const int base = wbase[i / W];
+          const float a = base >= 0 ? wav[i] : 0.f;
+          const int o = (a != 0.f ? base + (i % W) * G.N_t : 0) + z * Wd;
+          for (int r = 0; r < Wd; ++r) {
+            a1 -= A.ed[o + r] * a;
+            a2 -= A.eb[o + r] * a;
+          }
         }
       }
       s_t1d[z] = P.mu * a1;
@@ -1549,7 +1566,7 @@ __global__ __launch_bounds__(kWave) void k_pc_sweep(SwGeom G, PcTab P, PcArgs A)
     __syncthreads();
     // the two z-parity halves (voxels of one parity are not neighbours)
     for (int par = 0; par < 2; ++par) {
-      for (int z = par + 2 * lane; z < Nz; z += 2 * kWave) {
+      for (int z = par + 2 * tid; z < Nz; z += 2 * nth) {
         const int ctr = centre0 + z;
         const float Vd = sd[ctr], Vb = sb[ctr];
         double u[2] = {0.0, 0.0}, M[3] = {0.0, 0.0, 0.0};
@@ -1601,7 +1618,7 @@ __global__ __launch_bounds__(kWave) void k_pc_sweep(SwGeom G, PcTab P, PcArgs A)
       const float a = base >= 0 ? wav[i] : 0.f;
       if (a == 0.f) continue;
       const int o = base + (i % W) * G.N_t;
-      for (int z = lane; z < Nz; z += kWave) {
+      for (int z = tid; z < Nz; z += nth) {
         const float dd = s_dxd[z], db = s_dxb[z];
         for (int r = 0; r < Wd; ++r) {
           if (dd != 0.f) e_add(A.ed + o + z * Wd + r, -a * dd);
@@ -1612,7 +1629,7 @@ __global__ __launch_bounds__(kWave) void k_pc_sweep(SwGeom G, PcTab P, PcArgs A)
     __syncthreads();
   }
   // the tile back (interior of the box, inside the object)
-  for (int j = lane; j < nx; j += kWave) {
+  for (int j = tid; j < nx; j += nth) {
     const int z = j % Nz, b = j / Nz;
     const int by = b / BW, bx = b % BW;
     if (by == 0 || bx == 0 || by == BW - 1 || bx == BW - 1) continue;
@@ -2659,10 +2676,6 @@ int mbir_hip_create(mbir_hip_ctx** out, const mbir_hip_geom* g, const float* H,
     }
   for (int f = 0; f < g->N_time; ++f)
     c->nvmax = std::max(c->nvmax, (int)(frame_ptr[f + 1] - frame_ptr[f]));
-  if (c->nvmax > kMaxViews || c->nvmax * (2 * g->mult_xy + 1) > kMaxAV) {
-    delete c;
-    return fail("too many views per frame for the sweep's LDS window (512 views, 1024 A values)");
-  }
   std::vector<int> view_frame(g->N_p, -1);
   for (int f = 0; f < g->N_time; ++f)
     for (int s = frame_ptr[f]; s < frame_ptr[f + 1]; ++s) {
@@ -3004,13 +3017,23 @@ int mbir_hip_fbp(mbir_hip_ctx* c, float* out) {
 
 namespace {
 
+// The MBIR sweep's per-column limits (its LDS column, window and register
+// staging; the tomographic sweep has its own): 0 when the stage fits.
+int mbir_sweep_fits(const mbir_hip_ctx* c) {
+  if (c->g.N_z > kNZMax) return fail("the MBIR sweep takes at most 128 voxels per column");
+  if (c->nvmax > kMaxViews || c->nvmax * (2 * c->g.mult_xy + 1) > kMaxAV)
+    return fail("too many views per frame for the MBIR sweep's LDS window (512 views, "
+                "1024 A values)");
+  return 0;
+}
+
 // The 8 launches per equit (2x2 tile colours x 2 frame parities) of n equits.
 // iter_dev null: iterations first_iter .. first_iter + n - 1, each launch
 // timed; otherwise the iteration is *iter_dev + it (graph capture: no event
 // records).  Returns the number of launches.
 int sweep_launches(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t seed,
                    const int* iter_dev, int* launches, float nh_frac = 0.f) {
-  if (c->g.N_z > kNZMax) return fail("the MBIR sweep takes at most 128 voxels per column");
+  if (mbir_sweep_fits(c)) return -1;
   SweepArgs A;
   A.mag = c->nh ? c->d_mag : nullptr;
   A.nh_rank = c->d_vsc;
@@ -3573,7 +3596,7 @@ int mbir_hip_voxel_thetas(mbir_hip_ctx* c, int32_t n, const int32_t* tzyx, float
                           float* theta2) {
   if (!c || n < 0 || (n > 0 && (!tzyx || !theta1 || !theta2))) return fail("bad argument");
   if (!c->a_built) return fail("A not built");
-  if (c->g.N_z > kNZMax) return fail("the theta kernel takes at most 128 voxels per column");
+  if (mbir_sweep_fits(c)) return -1;
   if (n == 0) return 0;
   for (int i = 0; i < n; ++i) {
     const int32_t* q = tzyx + 4 * i;
@@ -3712,6 +3735,8 @@ int mbir_hip_pc_equits(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t 
   if (words * 4 + 4 * 1024 + 8 * kNZPc * 4 > c->lds_max)
     return fail("tile slabs of both objects exceed the LDS of one workgroup");
   HIP_OK(hipMemsetAsync(c->d_stats, 0, 4 * sizeof(double), c->stream));
+  // a wavefront per 64 voxels of a column, at most four
+  const int threads = std::min(4, (c->g.N_z + kWave - 1) / kWave) * kWave;
   PcArgs A;
   A.aval = c->d_aval;
   A.afirst = c->d_afirst;
@@ -3742,7 +3767,7 @@ int mbir_hip_pc_equits(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t 
         A.fparity = fpar;
         A.seed = seed * 2654435761u + (unsigned)(it * 8 + colour * 2 + fpar) * 40503u;
         if (ev_begin(c, MBIR_K_SWEEP)) return -1;
-        hipLaunchKernelGGL(k_pc_sweep, dim3((unsigned)(ntl * frames)), dim3(kWave),
+        hipLaunchKernelGGL(k_pc_sweep, dim3((unsigned)(ntl * frames)), dim3(threads),
                            words * sizeof(float), c->stream, c->SG, c->ptab, A);
         if (ev_end(c, MBIR_K_SWEEP)) return -1;
       }

@@ -217,6 +217,22 @@ def make_scan(name, seed=0, **over):
     )
 
 
+def geometry_scan(rows, cols, views, seed=0):
+    """A parallel-beam 3D scan's geometry and reconstruct() parameters with
+    no measurement (projections 0, weights 1): views uniform over 180
+    degrees, one frame, a rows x cols detector and a cols^2 x rows grid.
+    For the PCMBIR tomographic step, whose targets come from the phase
+    retrieval rather than from the scan's projections."""
+    angles, times = uniform_views(views)
+    return dict(
+        name="geometry", projections=np.zeros((views, rows, cols), np.float32),
+        weights=np.ones((views, rows, cols), np.float32), angles=angles, proj_times=times,
+        recon_times=np.array([0.0, float(views)], np.float32), proj_rows=rows, proj_cols=cols,
+        proj_num=views, recon_num=1, vox_wid=1.0, rot_center=cols / 2.0, sig_s=0.005,
+        sig_t=1.0, c_s=0.01, c_t=0.01, p_s=1.2, p_t=1.2, phantom=None, seed=seed,
+    )
+
+
 def drifting_ellipsoids(n, nz, nt, seed, lo=0.005, hi=0.02, count=40):
     """[nt][nz][n][n] two-phase phantom: value lo inside a disk of radius
     0.85 (of the half-width), hi inside seeded ellipsoids that drift a little
