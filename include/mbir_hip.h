@@ -61,7 +61,10 @@ const char* mbir_hip_last_error(void);
  * [N_time+1] / frame_views [frame_ptr[N_time]] the view->frame map
  * (XT_Init.c:69-96); y, w the projections and weights (w already divided by
  * var_est); d the ring offset [N_r][N_t] or NULL for zero; mask the update
- * mask [N_y][N_x] (XT_ICD_update.c:1127-1136). */
+ * mask [N_y][N_x] (XT_ICD_update.c:1127-1136).  N_z is at most 256; the
+ * MBIR sweep (mbir_hip_icd_*) takes at most 128 voxels per column and 512
+ * views (1024 A values) per column and frame, and refuses more when it
+ * runs; the tomographic sweep (mbir_hip_pc_equits) takes the full 256. */
 int mbir_hip_create(mbir_hip_ctx** out, const mbir_hip_geom* g,
                     const float* H, const double* cosv, const double* sinv,
                     const int32_t* frame_ptr, const int32_t* frame_views,
