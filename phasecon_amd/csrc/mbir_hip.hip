@@ -1465,8 +1465,11 @@ __global__ __launch_bounds__(4 * kWave) void k_pc_sweep(SwGeom G, PcTab P, PcArg
   const int c0 = A.tile_ptr[tile], n = A.tile_ptr[tile + 1] - c0;
   const int s0 = A.frame_ptr[f], nv = A.frame_ptr[f + 1] - s0;
   if (n == 0 || nv == 0) return;
-  const int Nz = G.N_z, T = A.tile, BW = T + 2, W = A.W;
-  const int nx = BW * BW * Nz;
+  // the slabs hold the tile's own T x T columns of both objects; the
+  // neighbouring columns belong to tiles of other colours, which no
+  // workgroup of this launch updates, so they are read from the objects
+  const int Nz = G.N_z, T = A.tile, W = A.W;
+  const int nx = T * T * Nz;
   float* sd = smem;
   float* sb = sd + nx;
   int* views = reinterpret_cast<int*>(sb + nx);
@@ -1479,8 +1482,8 @@ __global__ __launch_bounds__(4 * kWave) void k_pc_sweep(SwGeom G, PcTab P, PcArg
   float* xbf = A.xb + (size_t)f * vol;
   for (int j = tid; j < nx; j += nth) {
     const int z = j % Nz, b = j / Nz;
-    const int yy = oy + b / BW, xx = ox + b % BW;
-    const bool in = yy >= 0 && yy < G.N_y && xx >= 0 && xx < G.N_x;
+    const int yy = oy + 1 + b / T, xx = ox + 1 + b % T;
+    const bool in = yy < G.N_y && xx < G.N_x;
     const size_t o = in ? ((size_t)yy * G.N_x + xx) * Nz + z : 0;
     sd[j] = in ? xdf[o] : 0.f;
     sb[j] = in ? xbf[o] : 0.f;
@@ -1505,7 +1508,7 @@ __global__ __launch_bounds__(4 * kWave) void k_pc_sweep(SwGeom G, PcTab P, PcArg
     if (pos >= n) pos -= n;
     const int cy = c / G.N_x, cx = c % G.N_x;
     const int ly = cy - oy, lx = cx - ox;
-    const int centre0 = (ly * BW + lx) * Nz;
+    const int centre0 = ((ly - 1) * T + (lx - 1)) * Nz;
     // the column's A window (values times the wavenumber) and e offsets
     for (int sl = tid; sl < nv; sl += nth) {
       const int first = A.afirst[(size_t)c * G.nslots + s0 + sl];
@@ -1577,8 +1580,17 @@ __global__ __launch_bounds__(4 * kWave) void k_pc_sweep(SwGeom G, PcTab P, PcArg
           const bool ok = z + dz >= 0 && z + dz < Nz && cy + dy >= 0 && cy + dy < G.N_y &&
                           cx + dx >= 0 && cx + dx < G.N_x;
           if (!ok) continue;
-          const int o = ctr + (dy * BW + dx) * Nz + dz;
-          const float nd = sd[o], nb = sb[o];
+          const int by = ly + dy, bx = lx + dx;
+          float nd, nb;
+          if (by >= 1 && by <= T && bx >= 1 && bx <= T) {
+            const int o = ctr + (dy * T + dx) * Nz + dz;
+            nd = sd[o];
+            nb = sb[o];
+          } else {
+            const size_t o = ((size_t)(cy + dy) * G.N_x + (cx + dx)) * Nz + z + dz;
+            nd = xdf[o];
+            nb = xbf[o];
+          }
           pos_nb |= nd > 0.f || nb > 0.f;
           pc_pair(P, 0, P.filt[j], Vd, Vb, nd, nb, u, M);
         }
@@ -1628,12 +1640,10 @@ __global__ __launch_bounds__(4 * kWave) void k_pc_sweep(SwGeom G, PcTab P, PcArg
     }
     __syncthreads();
   }
-  // the tile back (interior of the box, inside the object)
+  // the tile's columns back (those inside the object)
   for (int j = tid; j < nx; j += nth) {
     const int z = j % Nz, b = j / Nz;
-    const int by = b / BW, bx = b % BW;
-    if (by == 0 || bx == 0 || by == BW - 1 || bx == BW - 1) continue;
-    const int yy = oy + by, xx = ox + bx;
+    const int yy = oy + 1 + b / T, xx = ox + 1 + b % T;
     if (yy < G.N_y && xx < G.N_x) {
       const size_t o = ((size_t)yy * G.N_x + xx) * Nz + z;
       xdf[o] = sd[j];
@@ -3730,7 +3740,7 @@ int mbir_hip_pc_equits(mbir_hip_ctx* c, int32_t n, int32_t first_iter, uint32_t 
   if (!c->a_built) return fail("A not built");
   if (c->afp16) return fail("the tomographic sweep reads the FP32 A");
   if (build_tiles(c)) return -1;
-  const size_t nx = (size_t)(c->tile + 2) * (c->tile + 2) * c->g.N_z;
+  const size_t nx = (size_t)c->tile * c->tile * c->g.N_z;
   const size_t words = 2 * nx + (size_t)c->nvmax * (2 + c->G.W) + (size_t)c->tile * c->tile;
   if (words * 4 + 4 * 1024 + 8 * kNZPc * 4 > c->lds_max)
     return fail("tile slabs of both objects exceed the LDS of one workgroup");
