@@ -760,6 +760,87 @@ def pctomo_bench(n, views, equits, tile, seed=761521):
     st.close()
 
 
+def pcmbir_admm_bench(n, views, heads, tile, n_icd=2, pret_iters=2, sd_iters=3,
+                      seed=761521):
+    """The PCMBIR reconstruction with its tomographic step, end to end on one
+    GPU at the PCMBIR workload's size: an n^3 (delta, beta) object from
+    seeded ellipsoids (delta up to 1e-6, beta = delta / 1000, 17 keV, 1 um
+    voxels, 0.1 m propagation), views parallel-beam views of n x n, the
+    intensity measurements |H exp(-z)| of its projections z = k A x
+    (simulated on the GPU, D = 1, unit weights).  From zero objects and the
+    Paganin estimate of z it runs `heads` head iterations (n_icd ICD equits, the coupled
+    phase retrieval of every view with pret_iters Nelder-Mead passes and
+    sd_iters steepest-descent iterations each, the dual update) and reports
+    seconds per head iteration and the part of each stage."""
+    from phasecon_amd.synth import geometry_scan, drifting_ellipsoids
+    from phasecon_amd.mbir import TomoStage, tomo_prior
+    from phasecon_amd.pcmbir import PhaseRetrieval, wavelength, fresnel_window
+    from phasecon_amd.pctomo import TomoADMM
+    t0 = time.time()
+    scan = geometry_scan(n, n, views, seed=0)
+    dob, lam, dist = 1000.0, wavelength(17.0), 1e5
+    kw = 2.0 * np.pi / lam
+    prior = tomo_prior((2e-7, 1e-3, 0.01, 0.01), (2e-10, 1e-3, 0.01, 0.01), 1,
+                       float(views), kw, delta_over_beta=dob, mu=0.5, alpha=1.5)
+    # projections z = k A x of the object, by the tomographic stage
+    st = TomoStage(scan, prior, tile=tile)
+    xd = drifting_ellipsoids(n, n, 1, seed=3, lo=2e-7, hi=1e-6, count=20)
+    zero = np.zeros(st.shape_e, np.float32)
+    st.set_objects(xd, xd / dob)
+    st.init_errors(zero, zero)
+    ed, eb = st.get_errors()
+    st.close()
+    n_p, n_r, n_t = ed.shape
+    win = fresnel_window(n_r, n_t, 1.0, 1.0, 2.0 / np.sqrt(lam * dist))
+    sim = PhaseRetrieval(n_p, n_r, n_t, 1.0, 1.0, lam, dist, win)
+    zb, zd = -eb.astype(np.float64), -ed.astype(np.float64)     # magnitude, phase
+    meas = np.abs(sim.fresnel(np.exp(-zb) * (np.cos(-zd) + 1j * np.sin(-zd)))).astype(np.complex64)
+    sim.close()
+    del ed, eb, zb, zd
+    nms = dict(phase_sig=2e-1, phase_c=1e-2, mag_sig=2e-4, mag_c=1e-2, delta_over_beta=dob,
+               decorrelate=True)
+    ones = np.ones(meas.shape, np.complex64)
+    run = TomoADMM(scan, prior, meas, ones, 1.0, (1.0, 1.0, lam, dist, win), 0.5, nms,
+                   tile=tile)
+    # the reference's start: the Paganin estimate of z, zero objects (the
+    # simplex search cannot leave z = 0: its start simplex (z, 2 z) collapses)
+    z_mag, z_phase = run.pr.paganin(meas, ones, dob, kw)
+    zeros = np.zeros(run.st.shape_x, np.float32)
+    run.start(zeros, zeros, z_phase, z_mag)
+    print(f"[bench] pcmbir admm setup {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    rows = []
+    for h in range(heads):
+        t_h = time.perf_counter()
+        c0 = run.st.tomo_cost()[0]
+        t_i = time.perf_counter()
+        for i in range(n_icd):
+            run.st.equits(1, i + 1, seed + run.equits)
+            run.equits += 1
+        run.st.synchronize()
+        t_icd = time.perf_counter() - t_i
+        c1 = run.st.tomo_cost()[0]
+        t_r = time.perf_counter()
+        ad, ab = run.st.admm_target()
+        run.pr.set_tomo_target(prior.mu, ab + 1j * ad)
+        out = run.pr.estimate(run.zb, run.zd, pret_iters, 0.0, sd_iters, 0.0)
+        run.zb, run.zd = out["zr"], out["zi"]
+        t_ret = time.perf_counter() - t_r
+        t_u = time.perf_counter()
+        res = run.st.admm_update(run.zd, run.zb)
+        t_upd = time.perf_counter() - t_u
+        dt = time.perf_counter() - t_h
+        rows.append(dict(s=dt, icd_s=t_icd, retrieval_s=t_ret, dual_s=t_upd,
+                         icd_cost=[c0, c1], residuals=[float(v) for v in np.ravel(res)]))
+        print(f"[bench] head {h}: {dt:.3f}s (ICD {t_icd:.3f}, retrieval {t_ret:.3f}, "
+              f"dual {t_upd:.3f})", file=sys.stderr, flush=True)
+    run.close()
+    print(json.dumps({"pcmbir_admm": {
+        "object": f"{n}^3 (delta, beta)", "views": views, "view": f"{n_r}x{n_t}", "tile": tile,
+        "icd_equits": n_icd, "pret_iters": pret_iters, "sd_iters": sd_iters,
+        "s_per_head": float(np.mean([r["s"] for r in rows[1:] or rows])), "heads": rows,
+        "data": "synthetic (seeded ellipsoids; measurements |H exp(-k A x)| on the GPU)"}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -845,7 +926,10 @@ def main():
                     help="the PCMBIR tomographic ICD at the PCMBIR object size instead")
     ap.add_argument("--pctomo-n", type=int, default=256)
     ap.add_argument("--pctomo-views", type=int, default=360)
-    ap.add_argument("--pctomo-tile", type=int, default=4)
+    ap.add_argument("--pctomo-tile", type=int, default=8,
+                    help="tile of the tomographic sweep (4 diverges at 256^3 x 360 views: too many concurrent columns per slice stack)")
+    ap.add_argument("--pcmbir-admm", action="store_true",
+                    help="the PCMBIR reconstruction with its tomographic step, end to end")
     ap.add_argument("--pcmbir-fft", choices=("register", "lds"), default="register",
                     help="FFT passes: register-resident radix-8 (default) or in-LDS radix-2")
     args = ap.parse_args()
@@ -861,6 +945,9 @@ def main():
         return
     if args.config4 and args.gpus == 1:
         config4(args)
+        return
+    if args.pcmbir_admm and args.gpus == 1:
+        pcmbir_admm_bench(args.pctomo_n, args.pctomo_views, max(args.steps, 1), args.pctomo_tile)
         return
     if args.pctomo and args.gpus == 1:
         pctomo_bench(args.pctomo_n, args.pctomo_views, max(args.steps, 1), args.pctomo_tile)
