@@ -33,6 +33,7 @@ class TomoADMM:
     def __init__(self, scan, prior, meas, D, Lam, geo, nu, nms, tile=16):
         self.prior = prior
         self.st = TomoStage(scan, prior, tile=tile)
+        self.tile = tile
         n_p, n_r, n_t = self.st.shape_e
         d_rows, d_cols, lam, dist, win = geo
         self.pr = PhaseRetrieval(n_p, n_r, n_t, d_rows, d_cols, lam, dist, win)
@@ -41,6 +42,7 @@ class TomoADMM:
         self.D = np.asarray(D).reshape(n_p, n_r, n_t)
         self.Lam = np.broadcast_to(Lam, (n_p, n_r, n_t))
         self.nu = nu
+        self.rise_tol = 1e-6  # relative ICD cost rise over a head iteration that stops the run
         self.zd = self.zb = None
         self.equits = 0  # ICD equits run, over all head iterations
 
@@ -73,6 +75,12 @@ class TomoADMM:
             self.st.equits(1, i + 1, seed + self.equits)
             self.equits += 1
         c1 = self.st.tomo_cost()[0]
+        if not c1 <= c0 * (1.0 + self.rise_tol):
+            # the reference stops on a cost increase (XT_ICD_update.c:1171);
+            # here a rise means the parallel sweep overshot (too many columns
+            # updated at once on one sinogram: take a larger tile)
+            raise RuntimeError(f"tomographic ICD cost rose from {c0:.10g} to {c1:.10g} over "
+                               f"{n_icd} equits (tile {self.tile})")
         ad, ab = self.st.admm_target()
         self.pr.set_tomo_target(self.prior.mu, ab + 1j * ad)
         out = self.pr.estimate(self.zb, self.zd, pret_iters, pret_thresh, sd_iters, sd_thresh)
