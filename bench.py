@@ -60,6 +60,17 @@ over a bounded sample of equits of the same workload.
                                                 kernel k_to_colmajor reads and
                                                 writes 4 B per voxel: the
                                                 traffic calibration copy)
+    python bench.py --pcmbir                    PCMBIR phase retrieval, 360
+                                                views x 512^2: s per SD
+                                                iteration, FFT HBM rate,
+                                                Nelder-Mead pixels/s
+    python bench.py --pctomo [--steps K]        PCMBIR tomographic ICD, 256^3
+                                                (delta, beta) x 360 views
+                                                (tile 8): s per equit, sweep
+                                                HBM rate
+    python bench.py --pcmbir-admm [--steps H]   PCMBIR with its tomographic
+                                                step, H head iterations end
+                                                to end at that size
 """
 from __future__ import annotations
 
