@@ -2771,10 +2771,11 @@ int mbir_hip_set_option(mbir_hip_ctx* c, const char* key, double value) {
   if (!c || !key) return fail("null argument");
   const std::string k(key);
   if (k == "tile") {
-    // tiles of 2x2 columns make the sweep diverge (a defect not yet found:
-    // the tiled schedule converges at that size in the CPU simulation, and
-    // neither the theta pipeline, its correction nor the load / atomic
-    // ordering causes it), so they are refused
+    // tiles of 2x2 columns make the sweep diverge: that many concurrent
+    // columns (a quarter of the tiles, 4 columns each), each reading an
+    // error sinogram that misses the others' latest updates, overshoot even
+    // at alpha 1 (the oracle's asynchronous model shows it with 1-3 columns
+    // of staleness, and convergence at tile 4), so they are refused
     if (value < 4 || value > 1024) return fail("tile must be in [4, 1024]");
     c->tile = (int)value;
   } else if (k == "alpha") {

@@ -142,3 +142,33 @@ def test_config4_coarse_stage_tile4_diverges_where_tile8_converges():
     c4, c8 = runs[4], runs[8]
     assert np.any(np.diff(c4) > 0) and c4[-1] > c4[0]
     assert len(c8) == 7 and np.all(np.diff(c8) < 0)
+
+
+def test_two_by_two_tiles_diverge_with_staleness_where_tile4_converges():
+    """Tiles of 2x2 columns, which the GPU sweep diverges with (and the tile
+    option therefore refuses), on ch4d_small: every tile of a launch in
+    lock-step, tiles seeing each other's updates one column late.  Tile 2
+    at alpha 1.5 (256 concurrent tiles of 4 columns per frame) diverges
+    within 10 equits while tile 4 at the same alpha and lag lowers the cost
+    at every equit; without the lag tile 2 converges too (the synchronous
+    tiled schedule), so the divergence is the asynchronous staleness of that
+    many concurrent columns, not the tile size itself."""
+    from oracle.oracle import Oracle
+    from phasecon_amd.synth import make_scan
+    o = Oracle(make_scan("ch4d_small", seed=0))
+    runs = {}
+    for T, lag in ((2, 1), (4, 1), (2, 0)):
+        x = np.zeros(o.shape_x)
+        e = o.init_error(x)
+        c = [o.cost(e, x)[0]]
+        for it in range(1, 11):
+            o.icd_equit_async(x, e, it, T, 1.5, 1000 + it, 0, lag, lag, 1, 0)
+            c.append(o.cost(e, x)[0])
+            if not np.isfinite(c[-1]) or c[-1] > 10 * c[0]:
+                break
+        runs[(T, lag)] = np.array(c)
+        print(f"tile {T}, lag {lag}: " + " ".join(f"{v:.4g}" for v in c), flush=True)
+    c2, c4, c2s = runs[(2, 1)], runs[(4, 1)], runs[(2, 0)]
+    assert np.any(np.diff(c2) > 0) and c2[-1] > c2[1]
+    assert len(c4) == 11 and np.all(np.diff(c4) < 0)
+    assert len(c2s) == 11 and np.all(np.diff(c2s) < 0)
