@@ -1544,7 +1544,7 @@ __global__ __launch_bounds__(4 * kWave) void k_pc_sweep(SwGeom G, PcTab P, PcArg
       // in that view, or a window bin past the footprint, which may lie past
       // the detector's last channel) reads the view-0, channel-0 row instead
       if (Wd == 1) {
-#pragma unroll 8
+#pragma unroll 16
         for (int i = 0; i < nv * W; ++i) {
           const int base = wbase[i / W];
           const float a = base >= 0 ? wav[i] : 0.f;
