@@ -71,6 +71,8 @@ over a bounded sample of equits of the same workload.
     python bench.py --pcmbir-admm [--steps H]   PCMBIR with its tomographic
                                                 step, H head iterations end
                                                 to end at that size
+    python bench.py --aux                       ring offsets, variance,
+                                                Paganin, phantom simulator
 """
 from __future__ import annotations
 
@@ -852,6 +854,55 @@ def pcmbir_admm_bench(n, views, heads, tile, n_icd=2, pret_iters=2, sd_iters=3,
         "data": "synthetic (seeded ellipsoids; measurements |H exp(-k A x)| on the GPU)"}}))
 
 
+def aux_bench():
+    """Timings of the stages around the sweeps, each after one untimed call:
+    the ring-offset update (modes 1-3) and the noise-variance re-estimate on
+    the ch4d workload (once per equit when enabled; device only), the
+    Paganin start over 360 views of 512^2 and the phase-contrast phantom
+    simulator over a 256^3 object and 360 views of 256^2 (once per scan;
+    host arrays in and out, so the PCIe copies are included)."""
+    from phasecon_amd.synth import make_scan, drifting_ellipsoids
+    from phasecon_amd.mbir import Stage, phantom_project
+    from phasecon_amd.pcmbir import PhaseRetrieval, wavelength, fresnel_window
+    out = {}
+    st = Stage(make_scan("ch4d", seed=0), tile=12)
+    st.build_amatrix()
+    st.init_error()
+    for mode in (1, 2, 3):
+        st.offset_update(mode)
+        st.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(10):
+            st.offset_update(mode)
+        st.synchronize()
+        out[f"ring_offset_mode{mode}_ms"] = (time.perf_counter() - t0) / 10 * 1e3
+    st.variance_update()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        st.variance_update()
+    st.synchronize()
+    out["variance_ms"] = (time.perf_counter() - t0) / 10 * 1e3
+    st.close()
+    lam, dist, n, nv = wavelength(17.0), 1e5, 512, 360
+    win = fresnel_window(n, n, 1.0, 1.0, 2.0 / np.sqrt(lam * dist))
+    pr = PhaseRetrieval(nv, n, n, 1.0, 1.0, lam, dist, win)
+    y = (1.0 + 0.01 * np.random.default_rng(0).random((nv, n, n))).astype(np.complex64)
+    D = np.ones((nv, n, n), np.complex64)
+    pr.paganin(y, D, 1000.0, 2 * np.pi / lam)
+    t0 = time.perf_counter()
+    pr.paganin(y, D, 1000.0, 2 * np.pi / lam)
+    out["paganin_360x512sq_s"] = time.perf_counter() - t0
+    pr.close()
+    obj = drifting_ellipsoids(256, 256, 1, seed=3, lo=2e-7, hi=1e-6, count=20)[0]
+    ang = (np.arange(360) * np.pi / 360).astype(np.float32)
+    args = (obj / 1000, obj, ang, 256, 256, 1.0, 128.0, 2 * np.pi / lam)
+    phantom_project(*args)
+    t0 = time.perf_counter()
+    phantom_project(*args)
+    out["phantom_project_256cubed_x360_s"] = time.perf_counter() - t0
+    print(json.dumps({"aux": out}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -939,6 +990,8 @@ def main():
     ap.add_argument("--pctomo-views", type=int, default=360)
     ap.add_argument("--pctomo-tile", type=int, default=8,
                     help="tile of the tomographic sweep (4 diverges at 256^3 x 360 views: too many concurrent columns per slice stack)")
+    ap.add_argument("--aux", action="store_true",
+                    help="timings of ring offsets, variance, Paganin and the simulator")
     ap.add_argument("--pcmbir-admm", action="store_true",
                     help="the PCMBIR reconstruction with its tomographic step, end to end")
     ap.add_argument("--pcmbir-fft", choices=("register", "lds"), default="register",
@@ -956,6 +1009,9 @@ def main():
         return
     if args.config4 and args.gpus == 1:
         config4(args)
+        return
+    if args.aux and args.gpus == 1:
+        aux_bench()
         return
     if args.pcmbir_admm and args.gpus == 1:
         pcmbir_admm_bench(args.pctomo_n, args.pctomo_views, max(args.steps, 1), args.pctomo_tile)
