@@ -2455,6 +2455,11 @@ struct mbir_hip_ctx {
   uint8_t* d_mask = nullptr;  // update mask [N_y * N_x]
   float* d_ws = nullptr;      // warm-start sinogram scratch (allocated on first use)
   std::vector<float> offset;  // ring offset d [N_r][N_t] (host copy of d_d)
+  // ring-offset windows (mode 3): per sinogram entry the windows covering it
+  // and their weights, [off_cover_ptr[k], off_cover_ptr[k + 1])
+  int off_windows = 0;
+  std::vector<int> off_cover_ptr, off_cover_win;
+  std::vector<double> off_cover_val;
   double* d_rt = nullptr;     // [3][N_r N_t] FP64 sums and step of the offset update
   // captured equits (mbir_hip_icd_graph): the graph, its key and launches
   int* d_iter = nullptr;
@@ -3288,22 +3293,38 @@ int mbir_hip_offset_update(mbir_hip_ctx* c, int32_t mode) {
     const double gamma = ns / ds;
     for (int i = 0; i < nrt; ++i) step[i] = (num[i] - gamma) / lam[i];
   } else {  // windowed zero-sum constraints, XT_ICD_update.c:749-810
-    std::vector<double> A;
-    const int M = offset_windows(g.N_r, g.N_t, &A);
+    // The windows overlap only their neighbours, so every sum runs over the
+    // windows covering each entry (cached: they depend on the detector
+    // only), in the same order as the full sums over all entries and
+    // windows, whose other terms are zero.
+    if (c->off_cover_ptr.empty()) {
+      std::vector<double> A;
+      c->off_windows = offset_windows(g.N_r, g.N_t, &A);
+      c->off_cover_ptr.assign((size_t)nrt + 1, 0);
+      for (int k = 0; k < nrt; ++k) {
+        for (int m = 0; m < c->off_windows; ++m) {
+          const double a = A[(size_t)m * nrt + k];
+          if (a != 0.0) {
+            c->off_cover_win.push_back(m);
+            c->off_cover_val.push_back(a);
+          }
+        }
+        c->off_cover_ptr[k + 1] = (int)c->off_cover_win.size();
+      }
+    }
+    const int M = c->off_windows;
     if (M == 0) return fail("detector too small for the offset windows");
-    std::vector<double> D((size_t)M * M), v(M), cm(M);
-    for (int i = 0; i < M; ++i)
-      for (int j = 0; j < M; ++j) {
-        double s = 0.0;
-        for (int k = 0; k < nrt; ++k) s += A[(size_t)i * nrt + k] * A[(size_t)j * nrt + k] / lam[k];
-        D[(size_t)i * M + j] = s;
+    const std::vector<int>& cp = c->off_cover_ptr;
+    const std::vector<int>& cw = c->off_cover_win;
+    const std::vector<double>& cv = c->off_cover_val;
+    std::vector<double> D((size_t)M * M, 0.0), v(M, 0.0), cm(M);
+    for (int k = 0; k < nrt; ++k)
+      for (int p = cp[k]; p < cp[k + 1]; ++p) {
+        v[cw[p]] += cv[p] * num[k] / lam[k];
+        for (int q = cp[k]; q < cp[k + 1]; ++q)
+          D[(size_t)cw[p] * M + cw[q]] += cv[p] * cv[q] / lam[k];
       }
     if (!invert_in_place(D, M)) return fail("offset constraint matrix is singular");
-    for (int i = 0; i < M; ++i) {
-      double s = 0.0;
-      for (int k = 0; k < nrt; ++k) s += A[(size_t)i * nrt + k] * num[k] / lam[k];
-      v[i] = s;
-    }
     for (int i = 0; i < M; ++i) {
       double s = 0.0;
       for (int j = 0; j < M; ++j) s += D[(size_t)i * M + j] * v[j];
@@ -3311,7 +3332,7 @@ int mbir_hip_offset_update(mbir_hip_ctx* c, int32_t mode) {
     }
     for (int k = 0; k < nrt; ++k) {
       double s = 0.0;
-      for (int m = 0; m < M; ++m) s += A[(size_t)m * nrt + k] * cm[m];
+      for (int p = cp[k]; p < cp[k + 1]; ++p) s += cv[p] * cm[cw[p]];
       step[k] = (num[k] - s) / lam[k] - d[k];  // new offset minus old
     }
   }
