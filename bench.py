@@ -859,7 +859,8 @@ def aux_bench():
     the ring-offset update (modes 1-3) and the noise-variance re-estimate on
     the ch4d workload (once per equit when enabled; device only), the
     Paganin start over 360 views of 512^2 and the phase-contrast phantom
-    simulator over a 256^3 object and 360 views of 256^2 (once per scan;
+    simulator over a 256^3 object and 360 views of 256^2, and the APS
+    flat-field normalisation of 720 raw frames of 64 x 1024 (once per scan;
     host arrays in and out, so the PCIe copies are included)."""
     from phasecon_amd.synth import make_scan, drifting_ellipsoids
     from phasecon_amd.mbir import Stage, phantom_project
@@ -900,6 +901,18 @@ def aux_bench():
     t0 = time.perf_counter()
     phantom_project(*args)
     out["phantom_project_256cubed_x360_s"] = time.perf_counter() - t0
+    # APS input stage at the mixed-precision workload's detector: 720 raw
+    # uint16 frames of 64 x 1024 (down-sampled by 2 to 32 x 512), 20 white
+    # and 20 dark frames
+    from phasecon_amd.mbir import aps_normalize
+    rng = np.random.default_rng(1)
+    raw = rng.integers(1000, 9000, (720, 64, 1024), dtype=np.uint16)
+    white = rng.integers(9000, 10000, (20, 64, 1024), dtype=np.uint16)
+    dark = rng.integers(0, 100, (20, 64, 1024), dtype=np.uint16)
+    aps_normalize(raw, white, dark, 2)
+    t0 = time.perf_counter()
+    aps_normalize(raw, white, dark, 2)
+    out["aps_normalize_720x64x1024_s"] = time.perf_counter() - t0
     print(json.dumps({"aux": out}))
 
 
