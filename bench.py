@@ -861,7 +861,8 @@ def aux_bench():
     Paganin start over 360 views of 512^2 and the phase-contrast phantom
     simulator over a 256^3 object and 360 views of 256^2, and the APS
     flat-field normalisation of 720 raw frames of 64 x 1024 (once per scan;
-    host arrays in and out, so the PCIe copies are included)."""
+    host arrays in and out, so the PCIe copies are included); the drop-in
+    reconstruct() on ch4d with the reference's schedule and stopping rule."""
     from phasecon_amd.synth import make_scan, drifting_ellipsoids
     from phasecon_amd.mbir import Stage, phantom_project
     from phasecon_amd.pcmbir import PhaseRetrieval, wavelength, fresnel_window
@@ -913,6 +914,19 @@ def aux_bench():
     t0 = time.perf_counter()
     aps_normalize(raw, white, dark, 2)
     out["aps_normalize_720x64x1024_s"] = time.perf_counter() - t0
+    # the drop-in reconstruct() on ch4d with the reference's own schedule and
+    # stopping rule (average update below 1 % per stage), host arrays in and
+    # out, against the FP64 oracle's C* of the problem it solves (the
+    # reference has one p: 1.2 for both priors)
+    from phasecon_amd.mbir import reconstruct
+    scan = make_scan("ch4d", seed=0)
+    t0 = time.perf_counter()
+    _, eq, cost = reconstruct(scan, convg_thresh=1.0)
+    out["reconstruct_ch4d_s"] = time.perf_counter() - t0
+    out["reconstruct_ch4d_last_stage_equits"] = eq
+    rec = load_cstar("ch4d", 1.2)
+    if rec is not None:
+        out["reconstruct_ch4d_cost_rel_cstar_p1.2"] = cost / rec["cstar"] - 1.0
     print(json.dumps({"aux": out}))
 
 
