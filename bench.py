@@ -568,9 +568,11 @@ def cpu_baseline(scan, budget_s=120.0, coarse_cap=100, convg=1.0):
     if rec is None or rec.get("digest") != scan_digest(scan):
         return {"value": None, "unit": "s", "cores": threads, "kind": "reference",
                 "sample": "no FP64 C* for the reference's p = 1.2 problem"}
-    target = rec["cstar"] * (1.0 + REL_TOL)
+    cstar_ref = rec["cstar"]
+    target = cstar_ref * (1.0 + REL_TOL)
     stages = multires_schedule(s["proj_rows"], s["proj_cols"])
     spent, x, per_stage, reached, trace = 0.0, None, [], False, []
+    own_stop = None
     for k, (mxy, mz) in enumerate(stages):
         last = k == len(stages) - 1
         t0 = time.perf_counter()
@@ -595,6 +597,10 @@ def cpu_baseline(scan, budget_s=120.0, coarse_cap=100, convg=1.0):
                   file=sys.stderr, flush=True)
             if last:
                 trace.append((round(spent, 3), float(costs[0])))
+                if own_stop is None and conv[0] and n > 1:
+                    # where the reference's own stopping rule ends its run
+                    own_stop = {"s": round(spent, 3), "full_res_equits": n,
+                                "cost_rel_cstar": float(costs[0]) / cstar_ref - 1.0}
                 if costs[0] <= target:
                     reached = True
                     break
@@ -613,6 +619,11 @@ def cpu_baseline(scan, budget_s=120.0, coarse_cap=100, convg=1.0):
                       "from zero through its multi-resolution stages to (1 + 1e-5) C* of the "
                       "same scan with p = 1.2 for both priors (the reference has one p), "
                       f"within a budget of {budget_s:g} s")}
+    if own_stop is not None:
+        # the same run where the reference's own convergence test (average
+        # update below convg % at full resolution) would have ended it: the
+        # stopping rule of the drop-in reconstruct()
+        out["own_stop"] = own_stop
     if not reached and len(trace) >= 3:
         # geometric extrapolation of the last full-resolution equits (labelled)
         (t1, c1), (t2, c2) = trace[-3], trace[-1]
