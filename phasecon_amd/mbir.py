@@ -440,7 +440,7 @@ class TomoStage(Stage):
     wavenumber.  The scan supplies the geometry; its projections and
     weights are not used."""
 
-    def __init__(self, scan, prior, mult_xy=1, mult_z=1, tile=16):
+    def __init__(self, scan, prior, mult_xy=1, mult_z=1, tile=8):
         super().__init__(scan, mult_xy=mult_xy, mult_z=mult_z, tile=tile, alpha=prior.alpha,
                          full_mask=True)
         self.prior = prior

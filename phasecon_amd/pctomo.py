@@ -28,9 +28,11 @@ class TomoADMM:
     (d_rows, d_cols, wavelength, distance, window); nu: the retrieval's
     penalty; nms: the projection-domain prior of pcmbir_hip_set_nms_prior
     (phase_sig, phase_c, mag_sig, mag_c, delta_over_beta, decorrelate), kept
-    for the uncoupled parts of the retrieval's set-up."""
+    for the uncoupled parts of the retrieval's set-up.  tile: the sweep's
+    tile edge; 8 fits columns of up to 256 voxels in LDS and stays stable on
+    the PCMBIR workload's 256^3 object, where tile 4 diverges."""
 
-    def __init__(self, scan, prior, meas, D, Lam, geo, nu, nms, tile=16):
+    def __init__(self, scan, prior, meas, D, Lam, geo, nu, nms, tile=8):
         self.prior = prior
         self.st = TomoStage(scan, prior, tile=tile)
         self.tile = tile
